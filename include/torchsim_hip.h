@@ -1,0 +1,107 @@
+/*
+ * torchsim_hip.h -- C ABI of the gfx950 EPG library (libtorchsim_hip.so).
+ *
+ * Plain pointers and sizes only. Every pointer to array data is a DEVICE
+ * pointer unless stated otherwise; `stream` is a hipStream_t passed as void*.
+ * Every entry point returns 0 on success or a nonzero status:
+ *   1 bad argument, 2 unsupported, 3 HIP failure;
+ * the message of the last failure on the calling thread is returned by
+ * tsim_last_error(). No entry point throws across the ABI, and none allocates
+ * caller-visible memory: the caller owns every input and output buffer.
+ * Launches are asynchronous on `stream`; entry points are safe to call from
+ * several host threads on distinct streams.
+ *
+ * Packed buffer order (26 pointers) is the reference's ABI:
+ *   tissue (17, FP32, one value per atom, or one value in total where the
+ *   caller says so):  t1 t2 m0 b1 b1_phase b0 inversion_efficiency diffusion
+ *   velocity bound_fraction bound_exchange t1_bound pool_b_fraction
+ *   pool_b_exchange t1_pool_b t2_pool_b pool_b_shift
+ *   events (9, one entry per event): duration(f32, s) kind(i32: 0 wait,
+ *   1 rf, 2 adc) flip(f32, rad) phase(f32, rad) action(u8) output_index(i32,
+ *   -1 when not recorded) shim_index(i32) saturation(f32) rf_frequency(f32)
+ * (reference src/torchsim/_epg_cpu.cpp:613-631, sequence/_parameters.py:105-195).
+ * Feature bits: 1 off_axis, 2 moving, 4 diffusing, 8 transmit, 16 density,
+ * 32 inverting (reference _epg_cpu.cpp:89-96).
+ */
+#ifndef TORCHSIM_HIP_H
+#define TORCHSIM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char* tsim_last_error(void);
+
+/* ABI revision of the loaded library. */
+int tsim_version(void);
+
+/*
+ * Forward EPG of one train for every atom, into the reference's two output
+ * planes. Replaces the reference's CPython entry `_epg_cpu.simulate`
+ * (src/torchsim/_epg_cpu.cpp:10740-10926) and the Triton launcher
+ * `_epg_triton.simulate` (sequence/_epg_triton.py:17120).
+ *
+ * pointers: 26 packed buffers (order above) then output_real, output_imag,
+ *           each (atom_count, output_count) FP32; pointer_count must be 28.
+ * real_axis: 1 when every pulse and sample share one phase and the tissue has
+ *           no off-resonance (the reference's verdict, sequence/
+ *           _accelerators.py:1953-2017); then the real-subspace kernel runs.
+ *           Otherwise the complex kernel runs.
+ * pool_kind: 0 single pool, 2 one chemically exchanging pool.
+ * tissue_stride: 1 if every optional tissue buffer holds one value per atom,
+ *           0 if each holds one value for all atoms (t1, t2 are per atom).
+ */
+int tsim_simulate(void* const* pointers, int64_t pointer_count, int64_t atom_count,
+                  int64_t train_count, int64_t event_count, int64_t state_count,
+                  int64_t output_count, int real_axis, int64_t shim_count, int pool_kind,
+                  int features, int64_t tissue_stride, void* stream);
+
+/*
+ * Complex forward EPG (single pool or one exchanging pool, diffusion,
+ * off-resonance, RF phase). Same pointer layout as tsim_simulate.
+ * Replaces `_epg_cpu.simulate` with pool_kind 0/2 and real_axis -1
+ * (src/torchsim/_epg_cpu.cpp:4216-4566, sequence/_epg_triton.py:14676-15370).
+ */
+int tsim_complex_simulate(void* const* pointers, int64_t pointer_count, int64_t atom_count,
+                          int64_t event_count, int64_t state_count, int64_t output_count,
+                          int pool_kind, int features, int64_t tissue_stride, void* stream);
+
+/*
+ * Real-subspace signal and Jacobian in ONE pass, written as complex64 in the
+ * public layout. Replaces the P pairs of `_epg_cpu.simulate` +
+ * `_epg_cpu.simulate_jvp` (src/torchsim/_epg_cpu.cpp:10740, 11017) that
+ * `Simulator.jacobian` issues through `torch.func.jvp`, one per property
+ * (reference model/_signal.py:230-238).
+ *
+ * packed:   the 26 packed buffers.
+ * columns:  HOST array of column_count (<= 5) tissue indices among
+ *           {0 t1, 1 t2, 2 m0, 3 b1, 6 inversion_efficiency}; column p of the
+ *           Jacobian is the derivative along property columns[p].
+ * factor:   complex factor (factor_re + i factor_im) every recorded sample
+ *           i*m0*a0 is multiplied by (MRF: i, reference simulators/mrf.py:84).
+ * sample_sign: DEVICE (output_count) signs from bringing pulses onto one axis
+ *           (reference sequence/_accelerators.py:549-571), or NULL.
+ * signal:   complex64 (atom_count, output_count), or NULL.
+ * jacobian: complex64 (atom_count, column_count, output_count), or NULL when
+ *           column_count is 0.
+ * tile:     state tile index (tsim_tile_shape), or -1 for the library's choice.
+ */
+int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
+                           int64_t state_count, int64_t output_count, int features,
+                           int64_t tissue_stride, const int32_t* columns, int64_t column_count,
+                           float factor_re, float factor_im, const float* sample_sign,
+                           void* signal, void* jacobian, int tile, void* stream);
+
+/* Number of state tiles compiled into the library, and the shape of one:
+ * orders per lane and lanes per atom. No reference counterpart (tuning). */
+int tsim_tile_count(void);
+int tsim_tile_shape(int tile, int* orders_per_lane, int* lanes_per_atom);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TORCHSIM_HIP_H */

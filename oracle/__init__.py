@@ -1,0 +1,238 @@
+"""FP64 CPU oracle for the EPG hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this package, and only as the checker or the timed CPU
+baseline. The product (``torchsim_amd``) never imports it.
+
+* :func:`real` -- ctypes binding of ``epg_oracle.c``: the real-subspace state
+  machine in double precision with forward-mode tangents (reference
+  src/torchsim/_epg_cpu.cpp:3041-3285).
+* :mod:`oracle.packed` -- the complex state machine in torch float64
+  (reference tests/utils/packed_reference.py:36-389), differentiable by
+  autograd, for the complex, exchanging-pool and diffusion paths and for
+  sequence gradients.
+* :mod:`oracle.isochromat` -- a Bloch isochromat brute force, independent of
+  EPG, used to pin the oracle itself.
+* :func:`reference_module` -- the reference's own CPU extension built from its
+  sources into ``oracle/_ref`` (a second checker; CPU baseline "reference").
+
+Parity status of each function is stated in its docstring.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import importlib.machinery
+import importlib.util
+import subprocess
+from pathlib import Path
+from typing import Any
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_LIB = HERE / "libepg_oracle.so"
+_lib = None
+
+
+def build() -> None:
+    """Compile the C oracle (and the reference extension where its sources exist)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    if Path("/root/reference/src/torchsim/_epg_cpu.cpp").exists():
+        subprocess.run(["make", "-s", "-C", str(HERE), "ref"], check=True)
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not _LIB.exists():
+            subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+        lib = ctypes.CDLL(str(_LIB))
+        d = ctypes.POINTER(ctypes.c_double)
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        u8 = ctypes.POINTER(ctypes.c_uint8)
+        i64 = ctypes.c_int64
+        lib.epg_oracle_real.restype = ctypes.c_int
+        lib.epg_oracle_real.argtypes = [i64, i64, i64, i64, d, d, d, d, d, d, i32, d, i32, d, u8, i32, i64, d, d, d, d]
+        _lib = lib
+    return _lib
+
+
+def _f64(value: Any, atoms: int | None = None) -> np.ndarray | None:
+    if value is None:
+        return None
+    array = np.ascontiguousarray(np.asarray(value, dtype=np.float64).reshape(-1))
+    return array
+
+
+def _ptr(array: np.ndarray | None, ctype=ctypes.c_double):
+    if array is None:
+        return None
+    return array.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+#: Tissue tangent rows of the C oracle.
+TANGENT_ROWS = ("t1", "t2", "m0", "b1", "ie", "diffusion")
+
+
+def real(
+    events: dict[str, Any],
+    *,
+    t1: Any,
+    t2: Any,
+    m0: Any = None,
+    b1: Any = None,
+    ie: Any = None,
+    diffusion: Any = None,
+    states: int,
+    outputs: int,
+    tangents: list[dict[str, Any]] | None = None,
+) -> tuple[np.ndarray, np.ndarray | None]:
+    """Real-subspace signal ``m0 * a0`` (atoms, outputs) and its tangents.
+
+    ``events`` holds numpy arrays ``duration, kind, flip, action,
+    output_index`` (the packed stream). Each tangent direction is a dict with
+    optional per-atom rows ``t1, t2, m0, b1, ie, diffusion`` and per-event rows
+    ``duration, flip``. Returns ``(signal, tangents)`` with tangents shaped
+    (atoms, directions, outputs).
+
+    Parity: pinned -- benchmark checksums, the reference's own CPU kernel,
+    isochromat brute force, finite differences, closed forms
+    (tests/test_oracle.py).
+    """
+    lib = _load()
+    t1 = _f64(t1)
+    t2 = _f64(t2)
+    atoms = int(max(t1.size, t2.size, *(np.size(v) for v in (m0, b1, ie, diffusion) if v is not None)))
+    bufs = [t1, t2, _f64(m0), _f64(b1), _f64(ie), _f64(diffusion)]
+    per_atom = np.array([0 if b is None else int(b.size == atoms and atoms > 1) for b in bufs], dtype=np.int32)
+    for k, b in enumerate(bufs):
+        if b is not None and b.size not in (1, atoms):
+            raise ValueError(f"tissue row {k} has {b.size} entries for {atoms} atoms")
+    duration = _f64(events["duration"])
+    kind = np.ascontiguousarray(np.asarray(events["kind"], dtype=np.int32))
+    flip = _f64(events["flip"])
+    action = np.ascontiguousarray(np.asarray(events["action"], dtype=np.uint8))
+    output_index = np.ascontiguousarray(np.asarray(events["output_index"], dtype=np.int32))
+    count = kind.size
+    directions = tangents or []
+    ntan = len(directions)
+    tan_tissue = tan_events = None
+    if ntan:
+        tan_tissue = np.zeros((ntan, len(TANGENT_ROWS), atoms))
+        tan_events = np.zeros((ntan, 2, count))
+        for k, direction in enumerate(directions):
+            for q, name in enumerate(TANGENT_ROWS):
+                if name in direction:
+                    tan_tissue[k, q] = np.broadcast_to(np.asarray(direction[name], dtype=np.float64), (atoms,))
+            for q, name in enumerate(("duration", "flip")):
+                if name in direction:
+                    tan_events[k, q] = np.asarray(direction[name], dtype=np.float64)
+    signal = np.zeros((atoms, outputs))
+    tangent_out = np.zeros((atoms, ntan, outputs)) if ntan else None
+    status = lib.epg_oracle_real(
+        atoms, count, states, outputs,
+        *[_ptr(b) for b in bufs],
+        _ptr(per_atom, ctypes.c_int32),
+        _ptr(duration), _ptr(kind, ctypes.c_int32), _ptr(flip), _ptr(action, ctypes.c_uint8),
+        _ptr(output_index, ctypes.c_int32),
+        ntan, _ptr(tan_tissue), _ptr(tan_events),
+        _ptr(signal), _ptr(tangent_out),
+    )
+    if status != 0:
+        raise RuntimeError(f"oracle failed with status {status}")
+    return signal, tangent_out
+
+
+def events_of(packed: Any) -> dict[str, np.ndarray]:
+    """The packed stream of a ``PackedEvents`` as numpy arrays."""
+    return {
+        "duration": packed.duration.detach().cpu().numpy(),
+        "kind": packed.kind.cpu().numpy(),
+        "flip": packed.flip.detach().cpu().numpy(),
+        "phase": packed.phase.detach().cpu().numpy(),
+        "action": packed.action.cpu().numpy(),
+        "output_index": packed.output_index.cpu().numpy(),
+    }
+
+
+def reference_module():
+    """The reference's CPU extension built from source into ``oracle/_ref``, or None."""
+    matches = sorted((HERE / "_ref").glob("_epg_cpu*.so"))
+    if not matches:
+        return None
+    loader = importlib.machinery.ExtensionFileLoader("_epg_cpu", str(matches[0]))
+    spec = importlib.util.spec_from_loader("_epg_cpu", loader)
+    module = importlib.util.module_from_spec(spec)
+    loader.exec_module(module)
+    return module
+
+
+_TISSUE_ORDER = (
+    "t1", "t2", "m0", "b1", "b1_phase", "b0", "ie", "diffusion", "velocity",
+    "bound_fraction", "bound_exchange", "t1_bound", "pool_b_fraction",
+    "pool_b_exchange", "t1_pool_b", "t2_pool_b", "pool_b_shift",
+)
+_TISSUE_DEFAULT = {"m0": 1.0, "b1": 1.0, "ie": 1.0, "t1_bound": 1.0, "t1_pool_b": 1.0, "t2_pool_b": 1.0}
+
+
+def reference_real(
+    events: dict[str, Any],
+    tissue: dict[str, Any],
+    *,
+    states: int,
+    outputs: int,
+    features: int,
+    tangent: dict[str, Any] | None = None,
+    threads: int = 1,
+    real_axis: int = 1,
+) -> np.ndarray:
+    """Run the reference's own FP32 CPU kernel (built from source) on packed buffers.
+
+    Returns the imaginary output plane (atoms, outputs): the recorded
+    ``m0 * a0`` for the forward, or its tangent along ``tangent`` (per-atom
+    rows keyed like ``tissue``) through ``simulate_jvp``.
+    """
+    module = reference_module()
+    if module is None:
+        raise RuntimeError("oracle/_ref is not built")
+    atoms = max(np.size(v) for v in tissue.values())
+    keep = []
+
+    def f32(value, size):
+        array = np.ascontiguousarray(np.broadcast_to(np.asarray(value, dtype=np.float32), (size,)).copy())
+        keep.append(array)
+        return array.ctypes.data
+
+    tissue_ptrs = [f32(tissue.get(name, _TISSUE_DEFAULT.get(name, 0.0)), atoms) for name in _TISSUE_ORDER]
+    count = int(np.size(events["kind"]))
+
+    def arr(value, dtype):
+        array = np.ascontiguousarray(np.asarray(value, dtype=dtype))
+        keep.append(array)
+        return array.ctypes.data
+
+    event_ptrs = [
+        arr(events["duration"], np.float32),
+        arr(events["kind"], np.int32),
+        arr(events["flip"], np.float32),
+        arr(events.get("phase", np.zeros(count)), np.float32),
+        arr(events["action"], np.uint8),
+        arr(events["output_index"], np.int32),
+        arr(np.zeros(count), np.int32),
+        arr(np.zeros(count), np.float32),
+        arr(np.zeros(count), np.float32),
+    ]
+    real = np.zeros((atoms, outputs), dtype=np.float32)
+    imag = np.zeros((atoms, outputs), dtype=np.float32)
+    tail = [0] * 13
+    dims = (atoms, 1, count, states, outputs, threads, real_axis, 0.0, 0.0, 1, 1, 0, 1.0, 0, 1.0, 0, features, 0, 0, 0)
+    if tangent is None:
+        module.simulate((*tissue_ptrs, *event_ptrs, real.ctypes.data, imag.ctypes.data, *tail), *dims)
+    else:
+        tangent_ptrs = [f32(tangent.get(name, 0.0), atoms) for name in _TISSUE_ORDER]
+        tangent_ptrs += [f32(0.0, count), f32(0.0, count), f32(0.0, count)]
+        module.simulate_jvp(
+            (*tissue_ptrs, *event_ptrs, *tangent_ptrs, real.ctypes.data, imag.ctypes.data, *tail), *dims
+        )
+    return imag.astype(np.float64)

@@ -1,0 +1,56 @@
+"""Workloads shared by the tests: the MRF trains and tissue draws of the
+benchmark configurations, at sizes the FP64 oracle finishes in seconds."""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+def mrf_flips() -> np.ndarray:
+    """880 pulses: 5->60 deg over 300, 60->2 over 300, then 280 at 2 deg."""
+    return np.concatenate([np.linspace(5, 60, 300), np.linspace(60, 2, 300), np.full(280, 2.0)])
+
+
+def benchmark_flips(length: int) -> np.ndarray:
+    """The reference benchmark's train (benchmarks/_common.py:25-35)."""
+    up = np.linspace(5.0, 60.0, round(0.375 * length))
+    down = np.linspace(60.0, 2.0, round(0.375 * length))
+    tail = np.full(length - up.size - down.size, 2.0)
+    return np.concatenate((up, down, tail))
+
+
+def benchmark_grid(atoms: int) -> tuple[np.ndarray, np.ndarray]:
+    """Paired geomspace T1/T2 of the reference benchmark (benchmarks/_common.py:38-45)."""
+    return np.geomspace(200.0, 3000.0, atoms), np.geomspace(10.0, 300.0, atoms)
+
+
+def random_tissue(atoms: int, seed: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """T1 ~ U[200, 3000] ms, T2 ~ U[20, 300] ms redrawn until T2 < T1."""
+    generator = torch.Generator().manual_seed(seed)
+    t1 = torch.empty(atoms, dtype=torch.float64).uniform_(200.0, 3000.0, generator=generator)
+    t2 = torch.empty(atoms, dtype=torch.float64).uniform_(20.0, 300.0, generator=generator)
+    bad = t2 >= t1
+    while bool(bad.any()):
+        t2[bad] = torch.empty(int(bad.sum()), dtype=torch.float64).uniform_(20.0, 300.0, generator=generator)
+        bad = t2 >= t1
+    return t1.numpy().astype(np.float32), t2.numpy().astype(np.float32)
+
+
+def dictionary_grid() -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """T1 log 100-5000 (100) x T2 log 10-2000 (80), T2 < T1, x B1 0.60:0.05:1.40."""
+    t1 = np.geomspace(100.0, 5000.0, 100)
+    t2 = np.geomspace(10.0, 2000.0, 80)
+    b1 = np.round(np.arange(0.60, 1.40 + 1e-9, 0.05), 2)
+    T1, T2 = np.meshgrid(t1, t2, indexing="ij")
+    keep = T2 < T1
+    pairs_t1, pairs_t2 = T1[keep], T2[keep]
+    T1b = np.repeat(pairs_t1[:, None], b1.size, 1).reshape(-1)
+    T2b = np.repeat(pairs_t2[:, None], b1.size, 1).reshape(-1)
+    B1b = np.tile(b1, pairs_t1.size)
+    return T1b.astype(np.float32), T2b.astype(np.float32), B1b.astype(np.float32)
+
+
+def rel_err(x: np.ndarray, ref: np.ndarray) -> float:
+    """max|x - ref| / max|ref|, the reference's convention (test_cuda_parity.py:131-133)."""
+    return float(np.abs(x - ref).max() / max(np.abs(ref).max(), 1e-300))

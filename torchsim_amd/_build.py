@@ -1,0 +1,101 @@
+"""Build the gfx950 HIP library in-tree.
+
+Every HIP translation unit is compiled by ``hipcc --offload-arch=gfx950`` into
+``torchsim_amd/csrc/_build/`` and linked into ``torchsim_amd/libtorchsim_hip.so``,
+which is what :mod:`torchsim_amd._native` loads. The real-subspace kernel is
+compiled once per state tile so the tiles build in parallel.
+"""
+
+from __future__ import annotations
+
+import concurrent.futures
+import os
+import re
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PACKAGE = Path(__file__).resolve().parent
+CSRC = PACKAGE / "csrc"
+BUILD = CSRC / "_build"
+LIBRARY = PACKAGE / "libtorchsim_hip.so"
+ARCH = os.environ.get("TSIM_OFFLOAD_ARCH", "gfx950")
+
+# Packed FP32 instructions are kept out of every kernel on purpose.
+FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Xclang",
+    "-target-feature",
+    "-Xclang",
+    "-packed-fp32-ops",
+]
+
+
+def _hipcc() -> str:
+    found = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not Path(found).exists():
+        raise RuntimeError("hipcc not found: the HIP library cannot be built")
+    return found
+
+
+def _tiles() -> list[tuple[int, int, int]]:
+    text = (CSRC / "epg_real_impl.h").read_text()
+    return [
+        (int(i), int(sl), int(lanes))
+        for i, sl, lanes in re.findall(r"X\((\d+), (\d+), (\d+)\)", text)
+    ]
+
+
+def _units() -> list[tuple[Path, list[str], Path]]:
+    units = []
+    for source in sorted(CSRC.glob("*.hip")):
+        if source.name == "epg_real_tile.hip":
+            for ident, sl, lanes in _tiles():
+                defines = [
+                    f"-DTSIM_TILE_ID={ident}",
+                    f"-DTSIM_TILE_SL={sl}",
+                    f"-DTSIM_TILE_L={lanes}",
+                ]
+                units.append((source, defines, BUILD / f"epg_real_tile_{ident}.o"))
+        else:
+            units.append((source, [], BUILD / f"{source.stem}.o"))
+    return units
+
+
+def _compile(hipcc: str, source: Path, defines: list[str], target: Path) -> None:
+    command = [hipcc, *FLAGS, *defines, "-c", str(source), "-o", str(target)]
+    done = subprocess.run(command, capture_output=True, text=True)
+    if done.returncode != 0:
+        raise RuntimeError(
+            f"hipcc failed on {source.name} {' '.join(defines)}:\n{done.stderr[-4000:]}"
+        )
+
+
+def build(jobs: int | None = None, verbose: bool = False) -> Path:
+    """Compile every HIP source for gfx950 and link the shared library."""
+    hipcc = _hipcc()
+    BUILD.mkdir(parents=True, exist_ok=True)
+    units = _units()
+    jobs = jobs or min(16, max(1, len(os.sched_getaffinity(0))))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
+        futures = [pool.submit(_compile, hipcc, *unit) for unit in units]
+        for future in futures:
+            future.result()
+    objects = [str(target) for _, _, target in units]
+    partial = LIBRARY.with_suffix(".so.tmp")
+    command = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objects, "-o", str(partial)]
+    done = subprocess.run(command, capture_output=True, text=True)
+    if done.returncode != 0:
+        raise RuntimeError(f"link failed:\n{done.stderr[-4000:]}")
+    os.replace(partial, LIBRARY)
+    if verbose:
+        print(f"built {LIBRARY} from {len(units)} units", file=sys.stderr)
+    return LIBRARY
+
+
+if __name__ == "__main__":
+    build(verbose=True)

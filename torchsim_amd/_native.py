@@ -1,0 +1,165 @@
+"""The C ABI of ``libtorchsim_hip.so``, loaded through ctypes.
+
+The library is the only compute path: there is no CPU fallback. If it is
+missing, or no GPU is visible, every simulation raises rather than answering
+from somewhere else.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+from typing import Any
+
+import torch
+
+LIBRARY = Path(__file__).resolve().parent / "libtorchsim_hip.so"
+
+#: Tissue property order of the packed ABI (reference sequence/_parameters.py:105-133).
+TISSUE_NAMES = (
+    "t1_ms",
+    "t2_ms",
+    "m0",
+    "b1",
+    "b1_phase_rad",
+    "b0_hz",
+    "inversion_efficiency",
+    "diffusion_um2_per_ms",
+    "velocity_m_per_s",
+    "bound_fraction",
+    "bound_exchange_rate",
+    "t1_bound_ms",
+    "pool_b_fraction",
+    "pool_b_exchange_rate",
+    "t1_pool_b_ms",
+    "t2_pool_b_ms",
+    "pool_b_shift_hz",
+)
+TISSUE_INDEX = {name: index for index, name in enumerate(TISSUE_NAMES)}
+TISSUE_COUNT = len(TISSUE_NAMES)
+EVENT_COUNT = 9
+PACKED_COUNT = TISSUE_COUNT + EVENT_COUNT
+
+# Feature bits (reference src/torchsim/_epg_cpu.cpp:89-96).
+FEATURE_OFF_AXIS = 1 << 0
+FEATURE_MOVING = 1 << 1
+FEATURE_DIFFUSING = 1 << 2
+FEATURE_TRANSMIT = 1 << 3
+FEATURE_DENSITY = 1 << 4
+FEATURE_INVERTING = 1 << 5
+
+_STATUS = {1: ValueError, 2: NotImplementedError, 3: RuntimeError}
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int
+_F32 = ctypes.c_float
+
+#: Every symbol include/torchsim_hip.h declares, with its ctypes signature.
+SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
+    "tsim_last_error": (ctypes.c_char_p, []),
+    "tsim_version": (_I32, []),
+    "tsim_simulate": (
+        _I32,
+        [ctypes.POINTER(_P), _I64, _I64, _I64, _I64, _I64, _I64, _I32, _I64, _I32, _I32, _I64, _P],
+    ),
+    "tsim_complex_simulate": (
+        _I32,
+        [ctypes.POINTER(_P), _I64, _I64, _I64, _I64, _I64, _I32, _I32, _I64, _P],
+    ),
+    "tsim_simulate_jacobian": (
+        _I32,
+        [
+            ctypes.POINTER(_P),
+            _I64,
+            _I64,
+            _I64,
+            _I64,
+            _I32,
+            _I64,
+            ctypes.POINTER(ctypes.c_int32),
+            _I64,
+            _F32,
+            _F32,
+            _P,
+            _P,
+            _P,
+            _I32,
+            _P,
+        ],
+    ),
+    "tsim_tile_count": (_I32, []),
+    "tsim_tile_shape": (_I32, [_I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
+}
+
+_lib: ctypes.CDLL | None = None
+
+
+def load(path: str | os.PathLike | None = None) -> ctypes.CDLL:
+    """Load the library (once) and bind every declared signature.
+
+    Raises
+    ------
+    RuntimeError
+        If the library has not been built.
+    """
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    where = Path(path) if path is not None else LIBRARY
+    if not where.exists():
+        raise RuntimeError(
+            f"{where} is missing: build it with `python -m torchsim_amd._build` "
+            "(the HIP library is the only compute path; there is no fallback)"
+        )
+    lib = ctypes.CDLL(str(where))
+    for name, (restype, argtypes) in SIGNATURES.items():
+        function = getattr(lib, name)
+        function.restype = restype
+        function.argtypes = argtypes
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    """Raise the exception a nonzero status stands for."""
+    if status == 0:
+        return
+    message = load().tsim_last_error().decode(errors="replace")
+    raise _STATUS.get(status, RuntimeError)(message)
+
+
+def require_device(device: torch.device) -> None:
+    """Refuse anything but a visible HIP device."""
+    if device.type != "cuda":
+        raise RuntimeError(
+            f"torchsim_amd computes on the GPU only; got tensors on {device}"
+        )
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU is visible: torchsim_amd has no CPU path")
+
+
+def stream_of(device: torch.device) -> int:
+    """The raw hipStream_t of torch's current stream on ``device``."""
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def pointers(tensors: list[torch.Tensor | None]) -> ctypes.Array:
+    """A C array of device pointers (null for ``None``)."""
+    array = (_P * len(tensors))()
+    for index, tensor in enumerate(tensors):
+        array[index] = None if tensor is None else tensor.data_ptr()
+    return array
+
+
+def tiles() -> list[tuple[int, int]]:
+    """The (orders per lane, lanes per atom) of every compiled state tile."""
+    lib = load()
+    shapes = []
+    for tile in range(lib.tsim_tile_count()):
+        sl, lanes = _I32(), _I32()
+        check(lib.tsim_tile_shape(tile, ctypes.byref(sl), ctypes.byref(lanes)))
+        shapes.append((sl.value, lanes.value))
+    return shapes

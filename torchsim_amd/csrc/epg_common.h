@@ -1,0 +1,85 @@
+// Shared definitions for the gfx950 EPG kernels: the event bits of the packed
+// stream, the compact per-event record the kernels walk, and the status
+// plumbing of the C ABI.
+//
+// Event bits follow the reference's packed action word
+// (src/torchsim/_epg_cpu.cpp:36-41, sequence/_accelerators.py _pack_events).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tsim {
+
+constexpr uint32_t PRE_SHIFT = 1;
+constexpr uint32_t POST_SHIFT = 2;
+constexpr uint32_t INVERSION = 4;
+constexpr uint32_t SPOIL_AFTER = 8;
+constexpr uint32_t SHIFT_AFTER = 16;
+constexpr uint32_t RECORD = 32;
+
+// Event kinds of the packed stream (reference sequence/_description.py:31-36).
+constexpr int32_t KIND_WAIT = 0;
+constexpr int32_t KIND_RF = 1;
+constexpr int32_t KIND_ADC = 2;
+
+// Feature bits, in the order the reference packs them
+// (src/torchsim/_epg_cpu.cpp:89-96, sequence/_parameters.py FEATURE_BITS).
+constexpr int FEATURE_OFF_AXIS = 1 << 0;
+constexpr int FEATURE_MOVING = 1 << 1;
+constexpr int FEATURE_DIFFUSING = 1 << 2;
+constexpr int FEATURE_TRANSMIT = 1 << 3;
+constexpr int FEATURE_DENSITY = 1 << 4;
+constexpr int FEATURE_INVERTING = 1 << 5;
+
+// Tissue property order of the packed ABI (reference _epg_cpu.cpp:613-620).
+enum TissueIndex : int {
+  T_T1 = 0, T_T2, T_M0, T_B1, T_B1_PHASE, T_B0, T_INV_EFF, T_DIFFUSION,
+  T_VELOCITY, T_BOUND_FRACTION, T_BOUND_EXCHANGE, T_T1_BOUND,
+  T_POOL_B_FRACTION, T_POOL_B_EXCHANGE, T_T1_POOL_B, T_T2_POOL_B,
+  T_POOL_B_SHIFT, TISSUE_COUNT
+};
+// Event buffers after the tissue block (reference _epg_cpu.cpp:622-631).
+enum EventIndex : int {
+  E_DURATION = 0, E_KIND, E_FLIP, E_PHASE, E_ACTION, E_OUTPUT_INDEX,
+  E_SHIM_INDEX, E_SATURATION, E_RF_FREQUENCY, EVENT_COUNT
+};
+constexpr int PACKED_COUNT = TISSUE_COUNT + EVENT_COUNT;
+
+// Control word of a compiled event: the action bits in the low byte, the
+// kind above them.
+constexpr uint32_t CTRL_KIND_SHIFT = 8;
+
+// One event as the kernels read it: 32 bytes, a single scalar load. Built on
+// the device from the reference-format buffers by a one-thread-per-event pass.
+struct alignas(32) EventCode {
+  float dt;        // interval before the event, seconds
+  float flip;      // radians
+  float cosine;    // cos(flip), for a launch whose flip carries no B1 map
+  float sine;      // sin(flip)
+  float phase;     // radians
+  uint32_t ctrl;   // action | kind << 8
+  int32_t out;     // output index, -1 when not recorded
+  int32_t event;   // position in the packed stream (adjoint bookkeeping)
+};
+
+// Status codes every entry point returns. Zero is success; the message of the
+// last failure on the calling thread is available from tsim_last_error().
+enum Status : int {
+  OK = 0,
+  BAD_ARGUMENT = 1,
+  UNSUPPORTED = 2,
+  HIP_FAILURE = 3,
+};
+
+void set_error(const char* fmt, ...);
+const char* last_error();
+int hip_status(hipError_t err, const char* where);
+
+}  // namespace tsim
+
+extern "C" int tsim_complex_simulate(void* const* pointers, int64_t pointer_count,
+                                     int64_t atom_count, int64_t event_count,
+                                     int64_t state_count, int64_t output_count, int pool_kind,
+                                     int features, int64_t tissue_stride, void* stream);

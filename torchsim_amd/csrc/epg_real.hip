@@ -1,0 +1,400 @@
+// Real-subspace EPG forward fused with forward-mode tangents, for gfx950.
+//
+// What it computes is the reference's real-subspace state machine
+// (src/torchsim/_epg_cpu.cpp:3041-3128 forward, :3136-3285 one-tangent JVP,
+// sequence/_epg_triton.py:14221-14620), carrying the value and up to four
+// tangent directions (T1, T2, B1, inversion efficiency) through ONE walk of
+// the event stream, where the reference runs one forward and one JVP kernel
+// per differentiated property. The M0 column needs no recursion: it is the
+// recorded a0 itself.
+//
+// Layout. An atom's S configuration orders are split into L contiguous blocks
+// of SL = S/L registers, one block per lane, so an atom occupies L adjacent
+// lanes of a wave (64/L atoms per wave). Relaxation and the RF T-matrix are
+// per-state register arithmetic; the gradient shift moves registers within a
+// lane and crosses block boundaries with one DPP row shift per plane
+// (row_shr:1 for F+ moving up, row_shl:1 for F- moving down). Every per-event
+// quantity is uniform across the launch and lives in scalar registers; the
+// event stream is read one 32-byte record at a time through the scalar cache.
+//
+// Representation (reference _epg_cpu.cpp:3029-3038): F+ = e^{i phi} i a,
+// F- = e^{-i phi} i b, Z = c, all real; the recorded sample is i * m0 * a0.
+#include "epg_real_impl.h"
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+namespace tsim {
+
+namespace {
+
+thread_local char g_error[512] = "";
+
+}  // namespace
+
+const char* last_error() { return g_error; }
+
+void set_error(const char* fmt, ...) {
+  va_list args;
+  va_start(args, fmt);
+  vsnprintf(g_error, sizeof(g_error), fmt, args);
+  va_end(args);
+}
+
+int hip_status(hipError_t err, const char* where) {
+  if (err == hipSuccess) return OK;
+  set_error("%s: %s", where, hipGetErrorString(err));
+  return HIP_FAILURE;
+}
+
+// ---------------------------------------------------------------------------
+// Event compilation: reference-format buffers -> one EventCode per event.
+// ---------------------------------------------------------------------------
+
+__global__ void compile_events(const float* __restrict__ duration,
+                               const int32_t* __restrict__ kind,
+                               const float* __restrict__ flip,
+                               const float* __restrict__ phase,
+                               const uint8_t* __restrict__ action,
+                               const int32_t* __restrict__ output_index,
+                               int64_t events, EventCode* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= events) return;
+  EventCode code;
+  code.dt = duration[e];
+  code.flip = flip[e];
+  float s, c;
+  sincosf(code.flip, &s, &c);
+  code.cosine = c;
+  code.sine = s;
+  code.phase = phase ? phase[e] : 0.0f;
+  code.ctrl = (uint32_t)action[e] | ((uint32_t)kind[e] << CTRL_KIND_SHIFT);
+  code.out = output_index[e];
+  code.event = (int32_t)e;
+  out[e] = code;
+}
+
+int compile_stream(const float* duration, const int32_t* kind, const float* flip,
+                   const float* phase, const uint8_t* action,
+                   const int32_t* output_index, int64_t events, EventCode* out,
+                   hipStream_t stream) {
+  if (events <= 0) return OK;
+  const int threads = 256;
+  const int64_t blocks = (events + threads - 1) / threads;
+  hipLaunchKernelGGL(compile_events, dim3((unsigned)blocks), dim3(threads), 0, stream,
+                     duration, kind, flip, phase, action, output_index, events, out);
+  return hip_status(hipGetLastError(), "compile_events launch");
+}
+
+// ---------------------------------------------------------------------------
+// The real-subspace kernel.
+// ---------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------
+// Dispatch over the instantiated tiles.
+// ---------------------------------------------------------------------------
+
+// A state tile: SL orders per lane, L lanes per atom.
+struct Tile {
+  int sl;
+  int l;
+};
+
+#define TSIM_TILE_ENTRY(ID, SL, L) {SL, L},
+constexpr Tile kTiles[] = {TSIM_TILES(TSIM_TILE_ENTRY)};
+#undef TSIM_TILE_ENTRY
+constexpr int kTileCount = sizeof(kTiles) / sizeof(kTiles[0]);
+
+RealLaunch pick_real(int tile, int tm, bool diff, bool b1map) {
+  switch (tile) {
+#define TSIM_TILE_CASE(ID, SL, L) \
+  case ID:                        \
+    return pick_real_tile_##ID(tm, diff, b1map);
+    TSIM_TILES(TSIM_TILE_CASE)
+#undef TSIM_TILE_CASE
+    default:
+      return nullptr;
+  }
+}
+
+int default_tile(int states, int tangents) {
+  // Among the tiles covering S orders, weigh the padded orders (work spent on
+  // zeros), the in-lane register moves and DPP crossings of a wider atom, and
+  // the occupancy the register file leaves: 3 planes * SL orders *
+  // (1 + tangents) floats of state per lane.
+  int best = -1;
+  float best_cost = 1e30f;
+  for (int t = 0; t < kTileCount; ++t) {
+    const Tile tile = kTiles[t];
+    const int cover = tile.sl * tile.l;
+    if (cover < states) continue;
+    const int regs = 3 * tile.sl * (1 + tangents);
+    const float occupancy = regs <= 96 ? 1.0f : (regs <= 150 ? 1.12f : (regs <= 200 ? 1.3f : 1.8f));
+    const float cost = (float)cover / (float)states * (1.0f + 0.06f * (float)(tile.l - 1)) * occupancy;
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = t;
+    }
+  }
+  return best;
+}
+
+int run_real(RealArgs& args, int tm, bool diff, bool b1map, int tile, hipStream_t stream) {
+  if (tile < 0) tile = default_tile(args.states, popcount4(tm == 0 ? 0 : (tm > 7 ? 15 : tm)));
+  if (tile < 0 || tile >= kTileCount) {
+    set_error("no state tile covers %d orders (at most %d)", args.states,
+              kTiles[kTileCount - 1].sl * kTiles[kTileCount - 1].l);
+    return UNSUPPORTED;
+  }
+  if (kTiles[tile].sl * kTiles[tile].l < args.states) {
+    set_error("tile %d covers %d orders, fewer than the %d asked for", tile,
+              kTiles[tile].sl * kTiles[tile].l, args.states);
+    return BAD_ARGUMENT;
+  }
+  if (args.atoms == 0 || args.outputs == 0) return OK;
+  const int L = kTiles[tile].l;
+  const int threads = 256;
+  const int64_t lanes = args.atoms * L;
+  const int64_t blocks = (lanes + threads - 1) / threads;
+  if (blocks > 0x7FFFFFFF) {
+    set_error("too many atoms for one launch: %lld", (long long)args.atoms);
+    return BAD_ARGUMENT;
+  }
+  RealLaunch launch = pick_real(tile, tm, diff, b1map);
+  launch(args, dim3((unsigned)blocks), dim3(threads), stream);
+  return hip_status(hipGetLastError(), "real_kernel launch");
+}
+
+}  // namespace tsim
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+using namespace tsim;
+
+namespace {
+
+bool tangent_mask(const int32_t* columns, int64_t column_count, int& tm, int cols[5]) {
+  tm = 0;
+  for (int i = 0; i < 5; ++i) cols[i] = -1;
+  for (int64_t c = 0; c < column_count; ++c) {
+    const int prop = columns[c];
+    int slot;
+    switch (prop) {
+      case T_T1: tm |= TAN_T1; slot = 0; break;
+      case T_T2: tm |= TAN_T2; slot = 1; break;
+      case T_B1: tm |= TAN_B1; slot = 2; break;
+      case T_INV_EFF: tm |= TAN_IE; slot = 3; break;
+      case T_M0: slot = 4; break;
+      default:
+        set_error("column %lld: property %d carries no real-subspace tangent", (long long)c, prop);
+        return false;
+    }
+    if (cols[slot] >= 0) {
+      set_error("property %d appears twice among the Jacobian columns", prop);
+      return false;
+    }
+    cols[slot] = (int)c;
+  }
+  // Masks carrying the inversion efficiency run the full kernel; the unused
+  // directions are computed and not written.
+  if (tm & TAN_IE) tm = 15;
+  return true;
+}
+
+}  // namespace
+
+namespace {
+
+// Stream-ordered scratch for the compiled event stream: allocated and freed on
+// the caller's stream, so overlapping calls on distinct streams never share it.
+struct Compiled {
+  EventCode* codes = nullptr;
+  hipStream_t stream = nullptr;
+  ~Compiled() {
+    if (codes) (void)hipFreeAsync(codes, stream);
+  }
+};
+
+int compile_packed(void* const* packed, int64_t event_count, hipStream_t stream, Compiled& out) {
+  out.stream = stream;
+  if (event_count <= 0) return OK;
+  for (int i : {E_DURATION, E_KIND, E_FLIP, E_ACTION, E_OUTPUT_INDEX}) {
+    if (packed[TISSUE_COUNT + i] == nullptr) {
+      set_error("event buffer %d is null", i);
+      return BAD_ARGUMENT;
+    }
+  }
+  int st = hip_status(hipMallocAsync((void**)&out.codes, sizeof(EventCode) * (size_t)event_count, stream),
+                      "hipMallocAsync(event codes)");
+  if (st != OK) return st;
+  return compile_stream((const float*)packed[TISSUE_COUNT + E_DURATION],
+                        (const int32_t*)packed[TISSUE_COUNT + E_KIND],
+                        (const float*)packed[TISSUE_COUNT + E_FLIP],
+                        (const float*)packed[TISSUE_COUNT + E_PHASE],
+                        (const uint8_t*)packed[TISSUE_COUNT + E_ACTION],
+                        (const int32_t*)packed[TISSUE_COUNT + E_OUTPUT_INDEX], event_count, out.codes,
+                        stream);
+}
+
+int fill_real_args(RealArgs& args, void* const* packed, int64_t atom_count, int64_t event_count,
+                   int64_t state_count, int64_t output_count, int features, int64_t tissue_stride) {
+  if (atom_count < 0 || event_count < 0 || state_count < 1 || output_count < 0) {
+    set_error("invalid EPG buffer dimensions");
+    return BAD_ARGUMENT;
+  }
+  if (state_count > 64) {
+    set_error("at most 64 configuration orders are carried, %lld asked", (long long)state_count);
+    return UNSUPPORTED;
+  }
+  if (tissue_stride != 0 && tissue_stride != 1) {
+    set_error("tissue_stride must be 0 or 1");
+    return BAD_ARGUMENT;
+  }
+  if (packed[T_T1] == nullptr || packed[T_T2] == nullptr) {
+    set_error("t1 and t2 buffers are required");
+    return BAD_ARGUMENT;
+  }
+  memset(&args, 0, sizeof(args));
+  args.event_count = event_count;
+  args.t1 = (const float*)packed[T_T1];
+  args.t2 = (const float*)packed[T_T2];
+  args.m0 = (const float*)packed[T_M0];
+  args.b1 = (const float*)packed[T_B1];
+  args.inv_eff = (const float*)packed[T_INV_EFF];
+  args.diffusion = (const float*)packed[T_DIFFUSION];
+  args.stride_t1 = 1;
+  args.stride_t2 = 1;
+  args.stride_opt = tissue_stride;
+  args.atoms = atom_count;
+  args.outputs = output_count;
+  args.states = (int)state_count;
+  args.density = (features & FEATURE_DENSITY) && args.m0 ? 1 : 0;
+  args.inverting = (features & FEATURE_INVERTING) && args.inv_eff ? 1 : 0;
+  args.factor_re = 1.0f;
+  args.factor_im = 0.0f;
+  args.col_t1 = args.col_t2 = args.col_b1 = args.col_ie = args.col_m0 = -1;
+  if ((features & FEATURE_TRANSMIT) && args.b1 == nullptr) {
+    set_error("transmit feature set but the b1 buffer is null");
+    return BAD_ARGUMENT;
+  }
+  if ((features & FEATURE_DIFFUSING) && args.diffusion == nullptr) {
+    set_error("diffusing feature set but the diffusion buffer is null");
+    return BAD_ARGUMENT;
+  }
+  return OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tsim_last_error(void) { return tsim::last_error(); }
+
+int tsim_version(void) { return 1; }
+
+int tsim_simulate(void* const* pointers, int64_t pointer_count, int64_t atom_count,
+                  int64_t train_count, int64_t event_count, int64_t state_count,
+                  int64_t output_count, int real_axis, int64_t shim_count, int pool_kind,
+                  int features, int64_t tissue_stride, void* stream) {
+  if (pointers == nullptr || pointer_count != PACKED_COUNT + 2) {
+    set_error("wrong number of buffer pointers: %lld, expected %d", (long long)pointer_count,
+              PACKED_COUNT + 2);
+    return BAD_ARGUMENT;
+  }
+  if (train_count != 1) {
+    set_error("one train per launch (batched trains run one launch each)");
+    return UNSUPPORTED;
+  }
+  if (real_axis != 1 || pool_kind != 0 || shim_count != 1) {
+    return tsim_complex_simulate(pointers, pointer_count, atom_count, event_count, state_count,
+                                 output_count, pool_kind, features, tissue_stride, stream);
+  }
+  RealArgs args;
+  int st = fill_real_args(args, pointers, atom_count, event_count, state_count, output_count,
+                          features, tissue_stride);
+  if (st != OK) return st;
+  args.out_real = (float*)pointers[PACKED_COUNT];
+  args.out_imag = (float*)pointers[PACKED_COUNT + 1];
+  if (args.out_real == nullptr || args.out_imag == nullptr) {
+    set_error("output planes are required");
+    return BAD_ARGUMENT;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Compiled compiled;
+  st = compile_packed(pointers, event_count, s, compiled);
+  if (st != OK) return st;
+  args.events = compiled.codes;
+  const bool diff = (features & FEATURE_DIFFUSING) != 0;
+  const bool b1map = (features & FEATURE_TRANSMIT) != 0;
+  return run_real(args, 0, diff, b1map, -1, s);
+}
+
+int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
+                           int64_t state_count, int64_t output_count, int features,
+                           int64_t tissue_stride, const int32_t* columns, int64_t column_count,
+                           float factor_re, float factor_im, const float* sample_sign,
+                           void* signal, void* jacobian, int tile, void* stream) {
+  if (packed == nullptr) {
+    set_error("packed buffers are null");
+    return BAD_ARGUMENT;
+  }
+  if (column_count < 0 || column_count > 5 || (column_count > 0 && columns == nullptr)) {
+    set_error("between 0 and 5 Jacobian columns, got %lld", (long long)column_count);
+    return BAD_ARGUMENT;
+  }
+  if (column_count > 0 && jacobian == nullptr) {
+    set_error("Jacobian columns asked for but the Jacobian buffer is null");
+    return BAD_ARGUMENT;
+  }
+  RealArgs args;
+  int st = fill_real_args(args, packed, atom_count, event_count, state_count, output_count,
+                          features, tissue_stride);
+  if (st != OK) return st;
+  int tm = 0;
+  int cols[5];
+  if (!tangent_mask(columns, column_count, tm, cols)) return BAD_ARGUMENT;
+  if ((tm & TAN_B1) && !(features & FEATURE_TRANSMIT)) {
+    set_error("a B1 column needs the transmit feature and a b1 buffer");
+    return BAD_ARGUMENT;
+  }
+  if (cols[3] >= 0 && !args.inverting) {
+    set_error("an inversion-efficiency column needs the inverting feature");
+    return BAD_ARGUMENT;
+  }
+  args.col_t1 = cols[0];
+  args.col_t2 = cols[1];
+  args.col_b1 = cols[2];
+  args.col_ie = cols[3];
+  args.col_m0 = cols[4];
+  args.columns = (int)column_count;
+  args.signal = (float2*)signal;
+  args.jacobian = column_count > 0 ? (float2*)jacobian : nullptr;
+  args.sample_sign = sample_sign;
+  args.factor_re = factor_re;
+  args.factor_im = factor_im;
+  hipStream_t s = (hipStream_t)stream;
+  Compiled compiled;
+  st = compile_packed(packed, event_count, s, compiled);
+  if (st != OK) return st;
+  args.events = compiled.codes;
+  const bool diff = (features & FEATURE_DIFFUSING) != 0;
+  const bool b1map = (features & FEATURE_TRANSMIT) != 0;
+  return run_real(args, tm, diff, b1map, tile, s);
+}
+
+int tsim_tile_count(void) { return kTileCount; }
+
+int tsim_tile_shape(int tile, int* orders_per_lane, int* lanes_per_atom) {
+  if (tile < 0 || tile >= kTileCount) {
+    set_error("no tile %d", tile);
+    return BAD_ARGUMENT;
+  }
+  *orders_per_lane = kTiles[tile].sl;
+  *lanes_per_atom = kTiles[tile].l;
+  return OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,304 @@
+"""Signal models: declared properties, a layout of operators, and the two
+derivative modes over them.
+
+Mirrors the reference's public model API (model/_signal.py:158-240
+``simulate``/``jacobian``; model/_state_machine.py:356-916 ``Simulator``):
+properties and sequence arguments are given together and told apart by the
+model's declared properties; a scalar-only call returns one signal, a call with
+maps returns one per voxel; ``jacobian(diff)`` returns the signal and its
+derivative along each named property (a single name collapses that axis).
+
+What differs is below the API: a model whose evaluate is the recorded sample
+times a constant (``signal_factor``) runs its signal and every Jacobian column
+in ONE fused gfx950 launch, where the reference issues one forward-mode pass
+per property.
+"""
+
+from __future__ import annotations
+
+from collections import OrderedDict
+from collections.abc import Mapping, Sequence
+from copy import copy as shallow_copy
+from dataclasses import dataclass, field
+from typing import Any
+
+import numpy as np
+import torch
+
+from ..sequence import _engine
+from ..sequence._events import EventOperators, PackedEvents, compose, pack
+
+#: Public property names a call may give beside a model's own (reference
+#: sequence/_parameters.py PUBLIC_PROPERTIES, the subset this library carries).
+PUBLIC_PROPERTIES = {
+    "T1": "t1_ms",
+    "T2": "t2_ms",
+    "M0": "m0",
+    "B1": "b1",
+    "inv_efficiency": "inversion_efficiency",
+    "D": "diffusion_um2_per_ms",
+    "B0": "b0_hz",
+    "pool_b_fraction": "pool_b_fraction",
+    "pool_b_exchange": "pool_b_exchange_rate",
+    "T1_pool_b": "t1_pool_b_ms",
+    "T2_pool_b": "t2_pool_b_ms",
+    "pool_b_shift": "pool_b_shift_hz",
+}
+
+#: Run settings a call may name (reference model/_state_machine.py:97-106).
+RUN_SETTINGS = ("states", "repetitions", "device", "tile")
+
+
+_PACKED_CACHE: OrderedDict = OrderedDict()
+_PACKED_CACHE_SIZE = 64
+
+
+def _protocol_key(kind: type, sequence: Mapping[str, Any]) -> tuple | None:
+    """A hashable key for a protocol, or None if some argument carries a graph."""
+    parts = [kind.__qualname__]
+    for name in sorted(sequence):
+        if name in ("device", "tile"):
+            continue
+        value = sequence[name]
+        if isinstance(value, torch.Tensor):
+            if value.requires_grad:
+                return None
+            held = value.detach().cpu().contiguous()
+            parts.append((name, str(held.dtype), tuple(held.shape), held.numpy().tobytes()))
+        elif isinstance(value, np.ndarray):
+            parts.append((name, str(value.dtype), value.shape, np.ascontiguousarray(value).tobytes()))
+        elif isinstance(value, (int, float, complex, str, bool, type(None))):
+            parts.append((name, type(value).__name__, value))
+        else:
+            return None
+    return tuple(parts)
+
+
+@dataclass(frozen=True)
+class SpinPhysics:
+    """Which properties a voxel exposes and what each kind of event plays."""
+
+    properties: Mapping[str, str | None] = field(default_factory=dict)
+    operators: EventOperators = field(default_factory=EventOperators)
+
+
+def _as_torch(value: Any) -> Any:
+    if isinstance(value, torch.Tensor):
+        return value
+    if isinstance(value, np.ndarray):
+        return torch.from_numpy(value)
+    if isinstance(value, (list, tuple)):
+        return torch.as_tensor(value)
+    return value
+
+
+def _backend(values: Mapping[str, Any]) -> Any:
+    """numpy if the first array the caller brought is numpy, else None."""
+    for value in values.values():
+        if isinstance(value, np.ndarray):
+            return np
+        if isinstance(value, torch.Tensor):
+            return None
+    return None
+
+
+def _like(result: torch.Tensor, backend: Any) -> Any:
+    if backend is np:
+        return result.detach().cpu().numpy()
+    return result
+
+
+def _target_device(values: Mapping[str, Any], device: Any) -> torch.device:
+    if device is not None:
+        device = torch.device(device)
+        if device.type == "cuda" and device.index is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        return device
+    for value in values.values():
+        if isinstance(value, torch.Tensor) and value.device.type == "cuda":
+            return value.device
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU is visible: torchsim_amd has no CPU path")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+class Simulator:
+    """A signal model made of an event layout run through the EPG engine.
+
+    Subclasses declare ``model`` (a :class:`SpinPhysics`), ``states`` (the
+    default number of configuration orders) and ``layout(**sequence)``; they
+    may override ``evaluate`` to post-process the recorded samples. A model
+    whose recorded signal is the sample times a constant declares that
+    constant as ``signal_factor`` and gets the fused Jacobian.
+    """
+
+    model: SpinPhysics = SpinPhysics()
+    states: int | None = None
+    repetitions: int = 1
+    signal_factor: complex | None = 1.0
+
+    def __init__(self, **values: Any) -> None:
+        self.bound = {name: _as_torch(value) if name in self.exposes else value for name, value in values.items()}
+
+    # -- what is fixed on the model -------------------------------------------
+
+    @property
+    def exposes(self) -> tuple[str, ...]:
+        return tuple(self.model.properties)
+
+    @property
+    def accepts(self) -> tuple[str, ...]:
+        return tuple(dict.fromkeys((*self.exposes, *PUBLIC_PROPERTIES)))
+
+    def bind(self, **values: Any) -> Simulator:
+        held = shallow_copy(self)
+        held.bound = {**self.bound, **{name: _as_torch(value) if name in self.exposes else value for name, value in values.items()}}
+        return held
+
+    def to(self, device: torch.device | str) -> Simulator:
+        moved = shallow_copy(self)
+        moved.bound = {name: value.to(device) if isinstance(value, torch.Tensor) else value for name, value in self.bound.items()}
+        return moved
+
+    # -- layout ------------------------------------------------------------------
+
+    def layout(self, **sequence: Any) -> list:  # pragma: no cover - abstract
+        raise NotImplementedError
+
+    def played(self, **sequence: Any) -> dict[str, Any]:
+        """The sequence arguments the layout takes (run settings removed)."""
+        return {name: value for name, value in sequence.items() if name not in RUN_SETTINGS}
+
+    def repetition_us(self, played_us: Any, **sequence: Any) -> Any:
+        """How long one playing of the train lasts, for repeated playings."""
+        return played_us
+
+    def packed(self, **sequence: Any) -> PackedEvents:
+        """The packed event stream of this protocol (host tensors).
+
+        A protocol whose arguments carry no autograd graph is resolved once and
+        remembered (the reference resolves a structure once per protocol too,
+        model/_binding.py:94-139); later calls with equal arguments reuse it.
+        """
+        key = _protocol_key(type(self), sequence)
+        if key is not None:
+            cached = _PACKED_CACHE.get(key)
+            if cached is not None:
+                _PACKED_CACHE.move_to_end(key)
+                return cached
+        built = self._packed(**sequence)
+        if key is not None:
+            _PACKED_CACHE[key] = built
+            while len(_PACKED_CACHE) > _PACKED_CACHE_SIZE:
+                _PACKED_CACHE.popitem(last=False)
+        return built
+
+    def _packed(self, **sequence: Any) -> PackedEvents:
+        played = self.played(**sequence)
+        events, end_s = compose(*self.layout(**played))
+        repetitions = int(sequence.get("repetitions", self.repetitions) or 1)
+        tr_us = self.repetition_us(1e6 * end_s, **played) if repetitions > 1 else 0.0
+        return pack(events, repetitions=repetitions, tr_duration_us=tr_us)
+
+    # -- the two derivative modes -----------------------------------------------
+
+    def _split(self, values: Mapping[str, Any]) -> tuple[dict, dict]:
+        declared = self.accepts
+        held = {name: _as_torch(values[name]) for name in declared if name in values}
+        sequence = {name: value for name, value in values.items() if name not in declared}
+        return held, sequence
+
+    def _tissue_fields(self, held: Mapping[str, Any]) -> dict[str, Any]:
+        fields: dict[str, Any] = {}
+        for name, value in held.items():
+            target = self.model.properties.get(name, PUBLIC_PROPERTIES.get(name)) if name in self.model.properties else PUBLIC_PROPERTIES.get(name)
+            if target is not None:
+                fields[target] = value
+        return fields
+
+    @staticmethod
+    def _batch(held: Mapping[str, Any]) -> tuple[int, ...]:
+        shapes = [torch.as_tensor(value).shape for value in held.values()]
+        if not shapes:
+            return ()
+        return tuple(torch.broadcast_shapes(*shapes))
+
+    def _run(
+        self,
+        held: Mapping[str, Any],
+        sequence: Mapping[str, Any],
+        diff: tuple[str, ...] = (),
+    ) -> tuple[torch.Tensor, torch.Tensor | None]:
+        """Signal (atoms, N) and Jacobian (atoms, P, N) of the recorded samples."""
+        device = _target_device({**held, **sequence}, sequence.get("device"))
+        packed = self.packed(**sequence)
+        fields = self._tissue_fields(held)
+        differentiated = tuple(self._tissue_fields({name: held[name] for name in diff}).keys())
+        tissue = _engine.prepare_tissue(fields, device, differentiated)
+        states = int(sequence.get("states", self.states) or 10)
+        real, flip, samples = _engine.real_subspace(packed, tissue)
+        factor = self.signal_factor if self.signal_factor is not None else 1.0
+        if not real:
+            from ..sequence import _complex
+
+            return _complex.simulate_complex(packed, tissue, states=states, columns=differentiated, factor=factor)
+        return _engine.simulate_real(
+            packed,
+            tissue,
+            states=states,
+            columns=differentiated,
+            factor=factor,
+            flip=flip,
+            samples=samples,
+            tile=int(sequence.get("tile", -1)),
+        )
+
+    def evaluate(self, properties: Mapping[str, Any], **sequence: Any) -> torch.Tensor:
+        """The recorded signal, (voxels, samples)."""
+        signal, _ = self._run(properties, sequence)
+        return signal
+
+    def simulate(self, **values: Any) -> Any:
+        """Return the recorded signal, shaped like the caller's property maps."""
+        backend = _backend(values)
+        held, sequence = self._split({**self.bound, **values})
+        batch = self._batch(held)
+        signal = self.evaluate(held, **sequence)
+        return _like(signal.reshape(*batch, signal.shape[-1]), backend)
+
+    def jacobian(self, diff: str | Sequence[str], **values: Any) -> tuple[Any, Any]:
+        """Return the signal and its derivative along each name in ``diff``.
+
+        Raises
+        ------
+        ValueError
+            If ``diff`` names a property the call did not give, or one that is
+            not floating point.
+        """
+        names = (diff,) if isinstance(diff, str) else tuple(diff)
+        backend = _backend(values)
+        held, sequence = self._split({**self.bound, **values})
+        for name in names:
+            if name not in held:
+                raise ValueError(
+                    f"{name!r} is not among the properties this call gave "
+                    f"{type(self).__name__}: {sorted(held)}. Forward mode "
+                    f"differentiates a property the tissue carries; for a "
+                    f"sequence argument, build a cost and call backward()."
+                )
+            if not torch.is_floating_point(torch.as_tensor(held[name])):
+                raise ValueError(f"{name!r} does not carry a derivative")
+        batch = self._batch(held)
+        signal, jacobian = self.linearize(held, names, **sequence)
+        signal = signal.reshape(*batch, signal.shape[-1])
+        jacobian = jacobian.reshape(*batch, len(names), jacobian.shape[-1])
+        if isinstance(diff, str):
+            jacobian = jacobian[..., 0, :]
+        return _like(signal, backend), _like(jacobian, backend)
+
+    def linearize(self, held: Mapping[str, Any], names: tuple[str, ...], **sequence: Any) -> tuple[torch.Tensor, torch.Tensor]:
+        """Signal (voxels, N) and Jacobian (voxels, P, N) along ``names``."""
+        if self.signal_factor is None:
+            raise NotImplementedError(f"{type(self).__name__} must define linearize")
+        signal, jacobian = self._run(held, sequence, names)
+        return signal, jacobian

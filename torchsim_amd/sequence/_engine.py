@@ -1,0 +1,237 @@
+"""Tissue preparation, the real-subspace verdict, and the native launches.
+
+This is the host side of the reference's engine and dispatch
+(sequence/_simulation.py:359-484 ``EpgEngine.simulate``, :661-737
+``_prepare_tissue``; sequence/_accelerators.py:549-571 ``_one_axis``,
+:574-746 ``simulate_native``, :1953-2017 the real-subspace verdict), reduced to
+what the gfx950 library runs: one train per launch, single pool or one
+exchanging pool, instantaneous pulses.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Any
+
+import torch
+
+from .. import _native
+from ._events import INVERSION, RECORD, PackedEvents
+
+#: Relaxation times are clamped here before they are used as denominators
+#: (reference sequence/_simulation.py:59, :734-736).
+MINIMUM_RELAXATION_TIME_MS = 1e-6
+
+#: How far from one axis a pulse or sample phase may sit and still count as on
+#: it (reference sequence/_accelerators.py ``_PHASE_TOLERANCE``).
+PHASE_TOLERANCE = 1e-5
+
+#: The identity of each optional property: at it, the term is absent.
+_IDENTITY = {
+    "m0": 1.0,
+    "b1": 1.0,
+    "b1_phase_rad": 0.0,
+    "b0_hz": 0.0,
+    "inversion_efficiency": 1.0,
+    "diffusion_um2_per_ms": 0.0,
+    "velocity_m_per_s": 0.0,
+    "bound_fraction": 0.0,
+    "pool_b_fraction": 0.0,
+}
+_FEATURE_OF = {
+    "m0": _native.FEATURE_DENSITY,
+    "b1": _native.FEATURE_TRANSMIT,
+    "inversion_efficiency": _native.FEATURE_INVERTING,
+    "diffusion_um2_per_ms": _native.FEATURE_DIFFUSING,
+    "b0_hz": _native.FEATURE_OFF_AXIS,
+    "b1_phase_rad": _native.FEATURE_OFF_AXIS,
+}
+
+
+@dataclass
+class PreparedTissue:
+    """The 17 FP32 tissue buffers in ABI order, on the device."""
+
+    buffers: list[torch.Tensor | None]
+    shape: torch.Size
+    atoms: int
+    features: int
+    stride: int  # 1: optional buffers hold one value per atom; 0: one in total
+    exchanging: bool
+
+    @property
+    def device(self) -> torch.device:
+        return self.buffers[0].device
+
+
+def _at_identity(name: str, value: Any) -> bool:
+    if name not in _IDENTITY:
+        return False
+    if isinstance(value, torch.Tensor):
+        if value.requires_grad:
+            return False
+        if value.numel() == 1:
+            return float(value.reshape(())) == _IDENTITY[name]
+        return False
+    return float(value) == _IDENTITY[name]
+
+
+def prepare_tissue(
+    properties: dict[str, Any],
+    device: torch.device,
+    differentiated: tuple[str, ...] = (),
+) -> PreparedTissue:
+    """Lay the tissue out as the kernels read it.
+
+    ``properties`` maps tissue field names (``t1_ms``, ``t2_ms``, ``m0``, ...)
+    to numbers or tensors. A property left at its identity and not
+    differentiated is absent: its feature bit stays off and its buffer is not
+    read. Fields being differentiated are always laid out per atom.
+    """
+    given = {name: value for name, value in properties.items() if value is not None}
+    for name in ("t1_ms", "t2_ms"):
+        if name not in given:
+            raise ValueError(f"{name} is required")
+    features = 0
+    live: dict[str, torch.Tensor] = {}
+    for name, value in given.items():
+        if name not in _native.TISSUE_INDEX:
+            raise ValueError(f"unknown tissue property {name!r}")
+        tensor = torch.as_tensor(value, device=device).to(torch.float32)
+        if name in ("t1_ms", "t2_ms") or name in differentiated or not _at_identity(name, value):
+            live[name] = tensor
+            features |= _FEATURE_OF.get(name, 0)
+    exchanging = "pool_b_fraction" in live
+    shape = torch.broadcast_shapes(*(value.shape for value in live.values()))
+    atoms = 1
+    for size in shape:
+        atoms *= int(size)
+    optional = [name for name in live if name not in ("t1_ms", "t2_ms")]
+    narrowed = all(live[name].numel() == 1 and name not in differentiated for name in optional)
+    stride = 0 if narrowed else 1
+    buffers: list[torch.Tensor | None] = [None] * _native.TISSUE_COUNT
+    for name, tensor in live.items():
+        if stride == 0 and name in optional:
+            flat = tensor.reshape(-1)[:1].contiguous()
+        else:
+            flat = tensor.expand(shape).reshape(-1).contiguous()
+        if name in ("t1_ms", "t2_ms", "t1_bound_ms", "t1_pool_b_ms", "t2_pool_b_ms"):
+            flat = flat.clamp_min(MINIMUM_RELAXATION_TIME_MS)
+        buffers[_native.TISSUE_INDEX[name]] = flat
+    # Buffers a live feature reads but the caller left unset take their identity.
+    for name, bit in _FEATURE_OF.items():
+        index = _native.TISSUE_INDEX[name]
+        if features & bit and buffers[index] is None:
+            fill = torch.full((1 if stride == 0 else atoms,), _IDENTITY[name], dtype=torch.float32, device=device)
+            buffers[index] = fill
+    return PreparedTissue(buffers, torch.Size(shape), atoms, features, stride, exchanging)
+
+
+def half_turns(packed: PackedEvents) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor | None] | None:
+    """Bring pulses and samples onto one phase axis, or say it cannot be done.
+
+    Returns ``(phase_offsets, flip_signs, sample_signs)`` where subtracting a
+    whole number of half turns from a phase is compensated by a sign on the
+    flip (pulses) or on the recorded sample (samples); ``None`` if some pulse
+    or sample sits off that axis (reference sequence/_accelerators.py:500-546).
+    """
+    kind = packed.kind.cpu()
+    action = packed.action.cpu().to(torch.int32)
+    phase = packed.phase.detach().cpu().to(torch.float64)
+    turning = (kind == 1) & ((action & INVERSION) == 0)
+    recording = (kind == 2) & ((action & RECORD) != 0)
+    if not bool(turning.any()):
+        return None
+    anchor = recording if bool(recording.any()) else turning
+    delta = phase - phase[anchor][0]
+    on_axis = turning | recording
+    out_by = torch.where(on_axis, torch.round(delta / torch.pi), torch.zeros_like(delta))
+    residue = torch.where(on_axis, delta - out_by * torch.pi, torch.zeros_like(delta))
+    if float(residue.abs().max()) > PHASE_TOLERANCE:
+        return None
+    sign = torch.cos(out_by * torch.pi)
+    flips = torch.where(turning, sign, torch.ones_like(sign))
+    samples = sign[recording] if bool(recording.any()) else None
+    return out_by, flips, samples
+
+
+def real_subspace(packed: PackedEvents, tissue: PreparedTissue) -> tuple[bool, torch.Tensor | None, torch.Tensor | None]:
+    """Whether the real-subspace kernel applies; the flip and sample signs if so."""
+    if tissue.exchanging or tissue.features & (_native.FEATURE_OFF_AXIS | _native.FEATURE_MOVING):
+        return False, None, None
+    turns = half_turns(packed)
+    if turns is None:
+        return False, None, None
+    _, flips, samples = turns
+    if bool((flips != 1).any()):
+        flip = (packed.flip.detach().cpu() * flips.to(torch.float32)).to(packed.flip.device)
+    else:
+        flip = None
+    if samples is not None and bool((samples == 1).all()):
+        samples = None
+    return True, flip, samples
+
+
+_COLUMN_INDEX = {
+    "t1_ms": 0,
+    "t2_ms": 1,
+    "m0": 2,
+    "b1": 3,
+    "inversion_efficiency": 6,
+}
+
+
+def simulate_real(
+    packed: PackedEvents,
+    tissue: PreparedTissue,
+    *,
+    states: int,
+    columns: tuple[str, ...] = (),
+    factor: complex = 1.0,
+    flip: torch.Tensor | None = None,
+    samples: torch.Tensor | None = None,
+    tile: int = -1,
+) -> tuple[torch.Tensor, torch.Tensor | None]:
+    """Signal and (optionally) Jacobian columns in one real-subspace launch.
+
+    Returns complex64 ``(atoms, outputs)`` and ``(atoms, len(columns), outputs)``
+    already multiplied by ``factor`` (and by the sample signs).
+    """
+    device = tissue.device
+    _native.require_device(device)
+    lib = _native.load()
+    events = list(packed.on(device))
+    if flip is not None:
+        events[2] = flip.to(device=device, dtype=torch.float32).contiguous()
+    atoms, outputs = tissue.atoms, packed.output_count
+    signal = torch.empty((atoms, outputs), dtype=torch.complex64, device=device)
+    jacobian = None
+    if columns:
+        jacobian = torch.empty((atoms, len(columns), outputs), dtype=torch.complex64, device=device)
+    ptrs = _native.pointers([*tissue.buffers, *events])
+    cols = (ctypes.c_int32 * max(1, len(columns)))(*[_COLUMN_INDEX[name] for name in columns])
+    sign_buffer = None
+    if samples is not None:
+        sign_buffer = samples.to(device=device, dtype=torch.float32).contiguous()
+    with torch.cuda.device(device):
+        status = lib.tsim_simulate_jacobian(
+            ptrs,
+            atoms,
+            packed.event_count,
+            states,
+            outputs,
+            tissue.features,
+            tissue.stride,
+            cols,
+            len(columns),
+            float(complex(factor).real),
+            float(complex(factor).imag),
+            None if sign_buffer is None else sign_buffer.data_ptr(),
+            signal.data_ptr(),
+            None if jacobian is None else jacobian.data_ptr(),
+            tile,
+            _native.stream_of(device),
+        )
+    _native.check(status)
+    return signal, jacobian
