@@ -291,20 +291,28 @@ __global__ void __launch_bounds__(256) real_kernel(const RealArgs p) {
         if (p.jacobian) {
           float2* row = p.jacobian + atom * (int64_t)p.columns * p.outputs + o;
           if constexpr ((TM & TAN_T1) != 0) {
-            const float d = m0 * ta[ST1][0];
-            row[(int64_t)p.col_t1 * p.outputs] = make_float2(-fi * d, fr * d);
+            if (p.col_t1 >= 0) {
+              const float d = m0 * ta[ST1][0];
+              row[(int64_t)p.col_t1 * p.outputs] = make_float2(-fi * d, fr * d);
+            }
           }
           if constexpr ((TM & TAN_T2) != 0) {
-            const float d = m0 * ta[ST2][0];
-            row[(int64_t)p.col_t2 * p.outputs] = make_float2(-fi * d, fr * d);
+            if (p.col_t2 >= 0) {
+              const float d = m0 * ta[ST2][0];
+              row[(int64_t)p.col_t2 * p.outputs] = make_float2(-fi * d, fr * d);
+            }
           }
           if constexpr ((TM & TAN_B1) != 0) {
-            const float d = m0 * ta[SB1][0];
-            row[(int64_t)p.col_b1 * p.outputs] = make_float2(-fi * d, fr * d);
+            if (p.col_b1 >= 0) {
+              const float d = m0 * ta[SB1][0];
+              row[(int64_t)p.col_b1 * p.outputs] = make_float2(-fi * d, fr * d);
+            }
           }
           if constexpr ((TM & TAN_IE) != 0) {
-            const float d = m0 * ta[SIE][0];
-            row[(int64_t)p.col_ie * p.outputs] = make_float2(-fi * d, fr * d);
+            if (p.col_ie >= 0) {
+              const float d = m0 * ta[SIE][0];
+              row[(int64_t)p.col_ie * p.outputs] = make_float2(-fi * d, fr * d);
+            }
           }
           if (p.col_m0 >= 0) {
             const float d = a[0];
