@@ -356,7 +356,7 @@ int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t even
   int tm = 0;
   int cols[5];
   if (!tangent_mask(columns, column_count, tm, cols)) return BAD_ARGUMENT;
-  if ((tm & TAN_B1) && !(features & FEATURE_TRANSMIT)) {
+  if (cols[2] >= 0 && !(features & FEATURE_TRANSMIT)) {
     set_error("a B1 column needs the transmit feature and a b1 buffer");
     return BAD_ARGUMENT;
   }
