@@ -345,7 +345,7 @@ int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t even
     set_error("between 0 and 5 Jacobian columns, got %lld", (long long)column_count);
     return BAD_ARGUMENT;
   }
-  if (column_count > 0 && jacobian == nullptr) {
+  if (column_count > 0 && jacobian == nullptr && atom_count > 0 && output_count > 0) {
     set_error("Jacobian columns asked for but the Jacobian buffer is null");
     return BAD_ARGUMENT;
   }

@@ -209,6 +209,8 @@ def simulate_real(
     jacobian = None
     if columns:
         jacobian = torch.empty((atoms, len(columns), outputs), dtype=torch.complex64, device=device)
+    if atoms == 0 or outputs == 0:
+        return signal, jacobian
     ptrs = _native.pointers([*tissue.buffers, *events])
     cols = (ctypes.c_int32 * max(1, len(columns)))(*[_COLUMN_INDEX[name] for name in columns])
     sign_buffer = None
