@@ -120,6 +120,13 @@ def main() -> int:
     parser.add_argument("--tiles", type=int, default=TILES, help="copies of the 106,182-atom grid")
     parser.add_argument("--tile", type=int, default=-1, help="state tile index (-1: library choice)")
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument(
+        "--calibrate-copy",
+        type=int,
+        default=0,
+        metavar="BYTES",
+        help="first run one device-to-device copy of BYTES (a known-size kernel for counter scaling)",
+    )
     args = parser.parse_args()
     if args.gpus != 1:
         print(
@@ -134,6 +141,12 @@ def main() -> int:
     from torchsim_amd.simulators import MRFSimulator
 
     device = torch.device("cuda", 0)
+    if args.calibrate_copy:
+        source = torch.ones(args.calibrate_copy // 4, dtype=torch.float32, device=device)
+        target = torch.empty_like(source)
+        target.copy_(source)
+        torch.cuda.synchronize()
+        del source, target
     T1, T2, B1 = dictionary(args.tiles)
     atoms = T1.size
     t1 = torch.tensor(T1, device=device)
