@@ -157,6 +157,7 @@ def main() -> int:
     def step():
         return seq.jacobian(NAMES, T1=t1, T2=t2, B1=b1, tile=args.tile)
 
+    out = None
     for _ in range(args.warmup):
         out = step()
     del out
