@@ -70,6 +70,19 @@ int tsim_complex_simulate(void* const* pointers, int64_t pointer_count, int64_t 
                           int pool_kind, int features, int64_t tissue_stride, void* stream);
 
 /*
+ * Complex signal and Jacobian in ONE pass, complex64 in the public layout.
+ * Replaces `_epg_cpu.simulate` + `_epg_cpu.simulate_jvp` per property for
+ * the complex kernels (src/torchsim/_epg_cpu.cpp:4216-4566, 2469-3005).
+ * columns: HOST array of column_count (<= 4) indices among {0 t1, 1 t2,
+ * 2 m0, 3 b1}; pool_kind 0 single pool, 2 one exchanging pool.
+ */
+int tsim_complex_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
+                          int64_t state_count, int64_t output_count, int pool_kind, int features,
+                          int64_t tissue_stride, const int32_t* columns, int64_t column_count,
+                          float factor_re, float factor_im, void* signal, void* jacobian,
+                          void* stream);
+
+/*
  * Real-subspace signal and Jacobian in ONE pass, written as complex64 in the
  * public layout. Replaces the P pairs of `_epg_cpu.simulate` +
  * `_epg_cpu.simulate_jvp` (src/torchsim/_epg_cpu.cpp:10740, 11017) that

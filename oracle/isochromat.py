@@ -38,6 +38,7 @@ def simulate(
     m0: float = 1.0,
     b1: float = 1.0,
     ie: float = 1.0,
+    b0: float = 0.0,
     isochromats: int = 512,
 ) -> np.ndarray:
     """Complex samples (outputs,) for one tissue."""
@@ -61,6 +62,12 @@ def simulate(
         e2 = np.exp(-1000.0 / t2 * dt)
         m[:, 0:2] *= e2
         m[:, 2] = m[:, 2] * e1 + (1.0 - e1)
+        if b0:
+            # free precession in the rotating frame: Mxy turns by -2 pi b0 dt
+            c, s = np.cos(-2 * np.pi * b0 * dt), np.sin(-2 * np.pi * b0 * dt)
+            x, y = m[:, 0].copy(), m[:, 1].copy()
+            m[:, 0] = c * x - s * y
+            m[:, 1] = s * x + c * y
         act = int(events["action"][e])
         kind = int(events["kind"][e])
         if act & _PRE_SHIFT:

@@ -61,6 +61,9 @@ def _units() -> list[tuple[Path, list[str], Path]]:
                     f"-DTSIM_TILE_L={lanes}",
                 ]
                 units.append((source, defines, BUILD / f"epg_real_tile_{ident}.o"))
+        elif source.name == "epg_complex_tile.hip":
+            for lanes in (4, 8, 16, 32, 64):
+                units.append((source, [f"-DTSIM_COMPLEX_L={lanes}"], BUILD / f"epg_complex_tile_{lanes}.o"))
         else:
             units.append((source, [], BUILD / f"{source.stem}.o"))
     return units

@@ -73,6 +73,17 @@ enum Status : int {
   HIP_FAILURE = 3,
 };
 
+// Stream-ordered scratch holding a compiled event stream: allocated and
+// freed on the caller's stream, so calls on distinct streams never share it.
+struct Compiled {
+  EventCode* codes = nullptr;
+  hipStream_t stream = nullptr;
+  ~Compiled();
+};
+
+// Compile the reference-format event buffers (packed[17..22]) into EventCodes.
+int compile_packed(void* const* packed, int64_t event_count, hipStream_t stream, Compiled& out);
+
 void set_error(const char* fmt, ...);
 const char* last_error();
 int hip_status(hipError_t err, const char* where);

@@ -1,8 +1,195 @@
-// Complex EPG kernels (placeholder until the complex path lands).
-#include "epg_common.h"
+// Host side of the complex EPG kernel: argument checks, lane-count choice and
+// the C ABI entry points (declared in include/torchsim_hip.h).
+#include "epg_complex_impl.h"
 
-extern "C" int tsim_complex_simulate(void* const*, int64_t, int64_t, int64_t, int64_t, int64_t, int,
-                                     int, int64_t, void*) {
-  tsim::set_error("the complex (off-axis / exchanging-pool) kernel is not built yet");
-  return tsim::UNSUPPORTED;
+#include <cstring>
+
+namespace tsim {
+namespace {
+
+ComplexLaunch pick_complex(int lanes, int tm, bool exch) {
+  switch (lanes) {
+#define TSIM_COMPLEX_CASE(LANES) \
+  case LANES:                    \
+    return pick_complex_##LANES(tm, exch);
+    TSIM_COMPLEX_LANES(TSIM_COMPLEX_CASE)
+#undef TSIM_COMPLEX_CASE
+    default:
+      return nullptr;
+  }
 }
+
+int lanes_for(int64_t states) {
+  for (int l : {4, 8, 16, 32, 64})
+    if (states <= l) return l;
+  return -1;
+}
+
+int fill_complex(ComplexArgs& a, void* const* packed, int64_t atoms, int64_t events, int64_t states,
+                 int64_t outputs, int pool_kind, int features, int64_t stride) {
+  if (atoms < 0 || events < 0 || states < 1 || outputs < 0) {
+    set_error("invalid EPG buffer dimensions");
+    return BAD_ARGUMENT;
+  }
+  if (states > 64) {
+    set_error("at most 64 configuration orders are carried, %lld asked", (long long)states);
+    return UNSUPPORTED;
+  }
+  if (pool_kind != 0 && pool_kind != 2) {
+    set_error("pool kind %d is not built (single pool 0 or one exchanging pool 2)", pool_kind);
+    return UNSUPPORTED;
+  }
+  if (stride != 0 && stride != 1) {
+    set_error("tissue_stride must be 0 or 1");
+    return BAD_ARGUMENT;
+  }
+  if (packed[T_T1] == nullptr || packed[T_T2] == nullptr) {
+    set_error("t1 and t2 buffers are required");
+    return BAD_ARGUMENT;
+  }
+  memset(&a, 0, sizeof(a));
+  a.event_count = events;
+  a.t1 = (const float*)packed[T_T1];
+  a.t2 = (const float*)packed[T_T2];
+  a.m0 = (const float*)packed[T_M0];
+  a.b1 = (const float*)packed[T_B1];
+  a.b1_phase = (const float*)packed[T_B1_PHASE];
+  a.b0 = (const float*)packed[T_B0];
+  a.inv_eff = (const float*)packed[T_INV_EFF];
+  a.diffusion = (const float*)packed[T_DIFFUSION];
+  a.pool_fraction = (const float*)packed[T_POOL_B_FRACTION];
+  a.pool_exchange = (const float*)packed[T_POOL_B_EXCHANGE];
+  a.t1_pool = (const float*)packed[T_T1_POOL_B];
+  a.t2_pool = (const float*)packed[T_T2_POOL_B];
+  a.pool_shift = (const float*)packed[T_POOL_B_SHIFT];
+  a.stride_opt = stride;
+  a.atoms = atoms;
+  a.outputs = outputs;
+  a.states = (int)states;
+  a.density = (features & FEATURE_DENSITY) && a.m0 ? 1 : 0;
+  a.inverting = (features & FEATURE_INVERTING) && a.inv_eff ? 1 : 0;
+  a.transmit = (features & FEATURE_TRANSMIT) && a.b1 ? 1 : 0;
+  a.off_axis = (features & FEATURE_OFF_AXIS) ? 1 : 0;
+  a.diffusing = (features & FEATURE_DIFFUSING) && a.diffusion ? 1 : 0;
+  a.factor_re = 1.0f;
+  a.col_t1 = a.col_t2 = a.col_b1 = a.col_m0 = -1;
+  if (pool_kind == 2 && (!a.pool_fraction || !a.pool_exchange || !a.t1_pool || !a.t2_pool)) {
+    set_error("an exchanging pool needs its fraction, exchange rate, T1 and T2 buffers");
+    return BAD_ARGUMENT;
+  }
+  return OK;
+}
+
+int run_complex(ComplexArgs& a, int tm, bool exch, hipStream_t stream) {
+  if (a.atoms == 0 || a.outputs == 0) return OK;
+  const int lanes = lanes_for(a.states);
+  ComplexLaunch launch = pick_complex(lanes, tm, exch);
+  if (!launch) {
+    set_error("no complex kernel for %d orders", a.states);
+    return UNSUPPORTED;
+  }
+  const int threads = 256;
+  const int64_t blocks = (a.atoms * lanes + threads - 1) / threads;
+  if (blocks > 0x7FFFFFFF) {
+    set_error("too many atoms for one launch: %lld", (long long)a.atoms);
+    return BAD_ARGUMENT;
+  }
+  launch(a, dim3((unsigned)blocks), dim3(threads), stream);
+  return hip_status(hipGetLastError(), "complex_kernel launch");
+}
+
+}  // namespace
+}  // namespace tsim
+
+using namespace tsim;
+
+extern "C" {
+
+int tsim_complex_simulate(void* const* pointers, int64_t pointer_count, int64_t atom_count,
+                          int64_t event_count, int64_t state_count, int64_t output_count,
+                          int pool_kind, int features, int64_t tissue_stride, void* stream) {
+  if (pointers == nullptr || pointer_count != PACKED_COUNT + 2) {
+    set_error("wrong number of buffer pointers: %lld, expected %d", (long long)pointer_count,
+              PACKED_COUNT + 2);
+    return BAD_ARGUMENT;
+  }
+  ComplexArgs a;
+  int st = fill_complex(a, pointers, atom_count, event_count, state_count, output_count, pool_kind,
+                        features, tissue_stride);
+  if (st != OK) return st;
+  a.out_real = (float*)pointers[PACKED_COUNT];
+  a.out_imag = (float*)pointers[PACKED_COUNT + 1];
+  if ((a.out_real == nullptr || a.out_imag == nullptr) && atom_count > 0 && output_count > 0) {
+    set_error("output planes are required");
+    return BAD_ARGUMENT;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Compiled compiled;
+  st = compile_packed(pointers, event_count, s, compiled);
+  if (st != OK) return st;
+  a.events = compiled.codes;
+  return run_complex(a, 0, pool_kind == 2, s);
+}
+
+int tsim_complex_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
+                          int64_t state_count, int64_t output_count, int pool_kind, int features,
+                          int64_t tissue_stride, const int32_t* columns, int64_t column_count,
+                          float factor_re, float factor_im, void* signal, void* jacobian,
+                          void* stream) {
+  if (packed == nullptr) {
+    set_error("packed buffers are null");
+    return BAD_ARGUMENT;
+  }
+  if (column_count < 0 || column_count > 4 || (column_count > 0 && columns == nullptr)) {
+    set_error("between 0 and 4 Jacobian columns, got %lld", (long long)column_count);
+    return BAD_ARGUMENT;
+  }
+  if (column_count > 0 && jacobian == nullptr && atom_count > 0 && output_count > 0) {
+    set_error("Jacobian columns asked for but the Jacobian buffer is null");
+    return BAD_ARGUMENT;
+  }
+  ComplexArgs a;
+  int st = fill_complex(a, packed, atom_count, event_count, state_count, output_count, pool_kind,
+                        features, tissue_stride);
+  if (st != OK) return st;
+  int tm = 0;
+  int cols[4] = {-1, -1, -1, -1};
+  for (int64_t c = 0; c < column_count; ++c) {
+    int slot;
+    switch (columns[c]) {
+      case T_T1: tm |= CT1; slot = 0; break;
+      case T_T2: tm |= CT2; slot = 1; break;
+      case T_B1: tm |= CB1; slot = 2; break;
+      case T_M0: slot = 3; break;
+      default:
+        set_error("column %lld: property %d carries no complex-kernel tangent", (long long)c, columns[c]);
+        return BAD_ARGUMENT;
+    }
+    if (cols[slot] >= 0) {
+      set_error("property %d appears twice among the Jacobian columns", columns[c]);
+      return BAD_ARGUMENT;
+    }
+    cols[slot] = (int)c;
+  }
+  if (cols[2] >= 0 && !a.transmit) {
+    set_error("a B1 column needs the transmit feature and a b1 buffer");
+    return BAD_ARGUMENT;
+  }
+  a.col_t1 = cols[0];
+  a.col_t2 = cols[1];
+  a.col_b1 = cols[2];
+  a.col_m0 = cols[3];
+  a.columns = (int)column_count;
+  a.signal = (float2*)signal;
+  a.jacobian = column_count > 0 ? (float2*)jacobian : nullptr;
+  a.factor_re = factor_re;
+  a.factor_im = factor_im;
+  hipStream_t s = (hipStream_t)stream;
+  Compiled compiled;
+  st = compile_packed(packed, event_count, s, compiled);
+  if (st != OK) return st;
+  a.events = compiled.codes;
+  return run_complex(a, tm, pool_kind == 2, s);
+}
+
+}  // extern "C"

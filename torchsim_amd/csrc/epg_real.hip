@@ -206,22 +206,16 @@ bool tangent_mask(const int32_t* columns, int64_t column_count, int& tm, int col
 
 }  // namespace
 
-namespace {
+namespace tsim {
 
-// Stream-ordered scratch for the compiled event stream: allocated and freed on
-// the caller's stream, so overlapping calls on distinct streams never share it.
-struct Compiled {
-  EventCode* codes = nullptr;
-  hipStream_t stream = nullptr;
-  ~Compiled() {
-    if (codes) (void)hipFreeAsync(codes, stream);
-  }
-};
+Compiled::~Compiled() {
+  if (codes) (void)hipFreeAsync(codes, stream);
+}
 
 int compile_packed(void* const* packed, int64_t event_count, hipStream_t stream, Compiled& out) {
   out.stream = stream;
   if (event_count <= 0) return OK;
-  for (int i : {E_DURATION, E_KIND, E_FLIP, E_ACTION, E_OUTPUT_INDEX}) {
+  for (int i : {E_DURATION, E_KIND, E_FLIP, E_PHASE, E_ACTION, E_OUTPUT_INDEX}) {
     if (packed[TISSUE_COUNT + i] == nullptr) {
       set_error("event buffer %d is null", i);
       return BAD_ARGUMENT;
@@ -238,6 +232,10 @@ int compile_packed(void* const* packed, int64_t event_count, hipStream_t stream,
                         (const int32_t*)packed[TISSUE_COUNT + E_OUTPUT_INDEX], event_count, out.codes,
                         stream);
 }
+
+}  // namespace tsim
+
+namespace {
 
 int fill_real_args(RealArgs& args, void* const* packed, int64_t atom_count, int64_t event_count,
                    int64_t state_count, int64_t output_count, int features, int64_t tissue_stride) {

@@ -45,8 +45,30 @@ PUBLIC_PROPERTIES = {
     "pool_b_shift": "pool_b_shift_hz",
 }
 
-#: Run settings a call may name (reference model/_state_machine.py:97-106).
-RUN_SETTINGS = ("states", "repetitions", "device", "tile")
+#: Run settings a call may name (reference model/_state_machine.py:97-106),
+#: and the gradient geometry a diffusion coefficient is read through
+#: (reference sequence/_accelerators.py:371-420).
+RUN_SETTINGS = ("states", "repetitions", "device", "tile", "crusher_dephasing_rad", "voxel_size_m")
+
+#: The apparent diffusion coefficient is given in um^2/ms; the b-factor wants m^2/s.
+_DIFFUSION_UNIT = 1e-9
+
+
+def damping_scale(sequence: Mapping[str, Any]) -> float:
+    """What a diffusion coefficient is multiplied by to give the kernels' rate:
+    (crusher dephasing / voxel size)^2 * 1e-9, zero without a declared crusher
+    (reference sequence/_accelerators.py:374-398)."""
+    crusher = sequence.get("crusher_dephasing_rad")
+    if not crusher:
+        return 0.0
+    voxel = sequence.get("voxel_size_m")
+    if voxel is None:
+        raise ValueError(
+            "a crusher winds its dephasing across a voxel, so a sequence that declares "
+            "crusher_dephasing_rad must declare voxel_size_m too"
+        )
+    per_m = float(crusher) / float(voxel)
+    return per_m * per_m * _DIFFUSION_UNIT
 
 
 _PACKED_CACHE: OrderedDict = OrderedDict()
@@ -233,6 +255,10 @@ class Simulator:
         device = _target_device({**held, **sequence}, sequence.get("device"))
         packed = self.packed(**sequence)
         fields = self._tissue_fields(held)
+        if "diffusion_um2_per_ms" in fields:
+            scale = damping_scale(sequence)
+            value = fields["diffusion_um2_per_ms"]
+            fields["diffusion_um2_per_ms"] = value * scale if scale else 0.0
         differentiated = tuple(self._tissue_fields({name: held[name] for name in diff}).keys())
         tissue = _engine.prepare_tissue(fields, device, differentiated)
         states = int(sequence.get("states", self.states) or 10)
