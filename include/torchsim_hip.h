@@ -108,6 +108,29 @@ int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t even
                            float factor_re, float factor_im, const float* sample_sign,
                            void* signal, void* jacobian, int tile, void* stream);
 
+/*
+ * Adjoint of the fused real-subspace system: gradients of a scalar loss with
+ * respect to every event's flip angle and duration, given the loss's real
+ * cotangents on the recorded m0*a0 and on the Jacobian columns. Replaces the
+ * reference's `_epg_cpu.simulate_vjp_jvp` / `simulate_vjp`
+ * (src/torchsim/_epg_cpu.cpp:11150-11755; kernels :7028-7314, :5985-6227)
+ * that `_NativeEpgJvp.backward` / `_NativeEpg.backward` reach
+ * (sequence/_accelerators.py:1544-1586, 1787-1850). The state is
+ * checkpointed every segment_events events and recomputed per segment.
+ *
+ * columns:  HOST array of column_count (<= 3) indices among {0 t1, 1 t2, 2 m0}.
+ * signal_cotangent:   DEVICE (atom_count, output_count) FP32, or NULL.
+ * jacobian_cotangent: DEVICE (atom_count, column_count, output_count), or NULL.
+ * grad_flip, grad_duration: DEVICE (event_count) FP32, summed over atoms in a
+ *           fixed order (deterministic).
+ */
+int tsim_simulate_vjp_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
+                               int64_t state_count, int64_t output_count, int features,
+                               int64_t tissue_stride, const int32_t* columns, int64_t column_count,
+                               const float* signal_cotangent, const float* jacobian_cotangent,
+                               int64_t segment_events, float* grad_flip, float* grad_duration,
+                               void* stream);
+
 /* Number of state tiles compiled into the library, and the shape of one:
  * orders per lane and lanes per atom. No reference counterpart (tuning). */
 int tsim_tile_count(void);

@@ -109,6 +109,26 @@ SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
             _P,
         ],
     ),
+    "tsim_simulate_vjp_jacobian": (
+        _I32,
+        [
+            ctypes.POINTER(_P),
+            _I64,
+            _I64,
+            _I64,
+            _I64,
+            _I32,
+            _I64,
+            ctypes.POINTER(ctypes.c_int32),
+            _I64,
+            _P,
+            _P,
+            _I64,
+            _P,
+            _P,
+            _P,
+        ],
+    ),
     "tsim_tile_count": (_I32, []),
     "tsim_tile_shape": (_I32, [_I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
 }

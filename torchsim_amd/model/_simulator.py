@@ -48,7 +48,9 @@ PUBLIC_PROPERTIES = {
 #: Run settings a call may name (reference model/_state_machine.py:97-106),
 #: and the gradient geometry a diffusion coefficient is read through
 #: (reference sequence/_accelerators.py:371-420).
-RUN_SETTINGS = ("states", "repetitions", "device", "tile", "crusher_dephasing_rad", "voxel_size_m")
+RUN_SETTINGS = (
+    "states", "repetitions", "device", "tile", "segment_events", "crusher_dephasing_rad", "voxel_size_m",
+)
 
 #: The apparent diffusion coefficient is given in um^2/ms; the b-factor wants m^2/s.
 _DIFFUSION_UNIT = 1e-9
@@ -268,6 +270,18 @@ class Simulator:
             from ..sequence import _complex
 
             return _complex.simulate_complex(packed, tissue, states=states, columns=differentiated, factor=factor)
+        if packed.flip.requires_grad or packed.duration.requires_grad:
+            held_grad = [name for name, value in held.items() if isinstance(value, torch.Tensor) and value.requires_grad]
+            if held_grad:
+                raise NotImplementedError(
+                    f"gradients reach the sequence (flips, timing) only; {held_grad} also require grad"
+                )
+            flip_in = packed.flip if flip is None else flip
+            signal, jacobian = _engine.FusedRealJacobian.apply(
+                flip_in.to(device), packed.duration.to(device), packed, tissue, states, differentiated, factor,
+                samples, int(sequence.get("segment_events", 256)),
+            )
+            return signal, (jacobian if differentiated else None)
         return _engine.simulate_real(
             packed,
             tissue,

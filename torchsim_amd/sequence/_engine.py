@@ -192,6 +192,7 @@ def simulate_real(
     flip: torch.Tensor | None = None,
     samples: torch.Tensor | None = None,
     tile: int = -1,
+    duration: torch.Tensor | None = None,
 ) -> tuple[torch.Tensor, torch.Tensor | None]:
     """Signal and (optionally) Jacobian columns in one real-subspace launch.
 
@@ -203,7 +204,9 @@ def simulate_real(
     lib = _native.load()
     events = list(packed.on(device))
     if flip is not None:
-        events[2] = flip.to(device=device, dtype=torch.float32).contiguous()
+        events[2] = flip.detach().to(device=device, dtype=torch.float32).contiguous()
+    if duration is not None:
+        events[0] = duration.detach().to(device=device, dtype=torch.float32).contiguous()
     atoms, outputs = tissue.atoms, packed.output_count
     signal = torch.empty((atoms, outputs), dtype=torch.complex64, device=device)
     jacobian = None
@@ -237,3 +240,103 @@ def simulate_real(
         )
     _native.check(status)
     return signal, jacobian
+
+
+def adjoint_real(
+    packed: PackedEvents,
+    tissue: PreparedTissue,
+    *,
+    states: int,
+    columns: tuple[str, ...],
+    flip: torch.Tensor,
+    duration: torch.Tensor,
+    signal_cotangent: torch.Tensor | None,
+    jacobian_cotangent: torch.Tensor | None,
+    segment_events: int = 256,
+) -> tuple[torch.Tensor, torch.Tensor]:
+    """d(loss)/d(flip) and d(loss)/d(duration), one entry per event, from the
+    loss's real cotangents on the recorded m0*a0 (atoms, outputs) and on the
+    Jacobian columns (atoms, P, outputs)."""
+    device = tissue.device
+    _native.require_device(device)
+    lib = _native.load()
+    events = list(packed.on(device))
+    events[2] = flip.detach().to(device=device, dtype=torch.float32).contiguous()
+    events[0] = duration.detach().to(device=device, dtype=torch.float32).contiguous()
+    count = packed.event_count
+    grad_flip = torch.empty(count, dtype=torch.float32, device=device)
+    grad_duration = torch.empty(count, dtype=torch.float32, device=device)
+    ptrs = _native.pointers([*tissue.buffers, *events])
+    cols = (ctypes.c_int32 * max(1, len(columns)))(*[_COLUMN_INDEX[name] for name in columns])
+    ws = None if signal_cotangent is None else signal_cotangent.to(device=device, dtype=torch.float32).contiguous()
+    wj = None if jacobian_cotangent is None else jacobian_cotangent.to(device=device, dtype=torch.float32).contiguous()
+    with torch.cuda.device(device):
+        status = lib.tsim_simulate_vjp_jacobian(
+            ptrs,
+            tissue.atoms,
+            count,
+            states,
+            packed.output_count,
+            tissue.features,
+            tissue.stride,
+            cols,
+            len(columns),
+            None if ws is None else ws.data_ptr(),
+            None if wj is None else wj.data_ptr(),
+            segment_events,
+            grad_flip.data_ptr(),
+            grad_duration.data_ptr(),
+            _native.stream_of(device),
+        )
+    _native.check(status)
+    return grad_flip, grad_duration
+
+
+class FusedRealJacobian(torch.autograd.Function):
+    """Signal and Jacobian of the real subspace, differentiable in the event
+    flips and durations (reference sequence/_accelerators.py:1402-1858: the
+    forward of ``_NativeEpg``/``_NativeEpgJvp`` and their backward, here one
+    fused forward kernel and one checkpointed adjoint kernel)."""
+
+    @staticmethod
+    def forward(ctx, flip, duration, packed, tissue, states, columns, factor, samples, segment_events):
+        signal, jacobian = simulate_real(
+            packed, tissue, states=states, columns=columns, factor=factor, flip=flip,
+            samples=samples, duration=duration,
+        )
+        ctx.save_for_backward(flip, duration)
+        ctx.packed = packed
+        ctx.tissue = tissue
+        ctx.states = states
+        ctx.columns = columns
+        ctx.factor = complex(factor)
+        ctx.samples = samples
+        ctx.segment_events = segment_events
+        if jacobian is None:
+            jacobian = torch.empty((tissue.atoms, 0, packed.output_count), dtype=torch.complex64, device=signal.device)
+        return signal, jacobian
+
+    @staticmethod
+    def backward(ctx, grad_signal, grad_jacobian):
+        flip, duration = ctx.saved_tensors
+        # every output is c * v with v real and c = factor * i (* sample sign):
+        # the loss's derivative along v is Re(conj(c) * grad).
+        c = torch.tensor(ctx.factor * 1j, dtype=torch.complex64, device=flip.device)
+        scale = c.conj()
+        if ctx.samples is not None:
+            scale = scale * ctx.samples.to(device=flip.device, dtype=torch.complex64)
+        ws = None if grad_signal is None else (grad_signal * scale).real
+        wj = None
+        if grad_jacobian is not None and len(ctx.columns):
+            wj = (grad_jacobian * scale).real
+        if ws is None and wj is None:
+            return (None,) * 9
+        grad_flip, grad_duration = adjoint_real(
+            ctx.packed, ctx.tissue, states=ctx.states, columns=ctx.columns, flip=flip, duration=duration,
+            signal_cotangent=ws, jacobian_cotangent=wj, segment_events=ctx.segment_events,
+        )
+        return (
+            grad_flip.to(flip.dtype) if ctx.needs_input_grad[0] else None,
+            grad_duration.to(duration.dtype) if ctx.needs_input_grad[1] else None,
+            None, None, None, None, None, None, None,
+        )
