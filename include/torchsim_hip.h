@@ -131,6 +131,45 @@ int tsim_simulate_vjp_jacobian(void* const* packed, int64_t atom_count, int64_t 
                                int64_t segment_events, float* grad_flip, float* grad_duration,
                                void* stream);
 
+/*
+ * Gauss-Newton normal equations, fused: for each voxel, run the real-subspace
+ * signal and its Jacobian along `columns` and accumulate, at every sample, in
+ * FP64: the cost sum |model - data|^2, the gradient J^T r and the upper
+ * triangle of J^T J -- the Jacobian never reaches memory. Replaces the
+ * reference's per-iteration `operator.A(x)` + `operator.jacobian(x)` (one
+ * forward + one JVP kernel per property) and the lstsq on the stacked blocks
+ * (recon/_gauss_newton.py:247-307, 633-651; recon/_operator.py:342-361).
+ *
+ * columns: HOST array of 1..4 indices among {0 t1, 1 t2, 2 m0, 3 b1}.
+ * data:    DEVICE complex64 (atom_count, output_count), the measured samples.
+ * normal:  DEVICE FP64 (atom_count, tsim_normal_stride()): [cost, g0..g3,
+ *          H00 H01 H02 H03 H11 H12 H13 H22 H23 H33] (unused columns zero).
+ * The model sample is factor * i * m0 * a0 (times sample_sign), as in
+ * tsim_simulate_jacobian.
+ */
+int tsim_gauss_newton_normal(void* const* packed, int64_t atom_count, int64_t event_count,
+                             int64_t state_count, int64_t output_count, int features,
+                             int64_t tissue_stride, const int32_t* columns, int64_t column_count,
+                             float factor_re, float factor_im, const float* sample_sign,
+                             const void* data, double* normal, int tile, void* stream);
+
+/*
+ * The damped Gauss-Newton step of every voxel in FP64, one thread per voxel:
+ * H = D H D, g = D g with D the per-voxel chain-rule scale of the bound
+ * transform (DEVICE FP64 (voxels, columns), NULL for none); then
+ * (H + damping I) step = -g + offset by Cholesky (NaN where not positive
+ * definite; offset DEVICE FP64 (voxels, columns) or NULL -- a damping that
+ * pulls toward a reference passes damping * (reference - x)),
+ * predicted = step . (damping step - g), gradient = g, curvature = max diag H.
+ * Any output may be NULL. Replaces `direct` (recon/_gauss_newton.py:247-307).
+ */
+int tsim_solve_damped(const double* normal, const double* scale, const double* damping, const double* offset,
+                      int64_t voxels, int columns, double* step, double* predicted, double* gradient,
+                      double* curvature, void* stream);
+
+/* Doubles per voxel in the normal-equation buffer. */
+int tsim_normal_stride(void);
+
 /* Number of state tiles compiled into the library, and the shape of one:
  * orders per lane and lanes per atom. No reference counterpart (tuning). */
 int tsim_tile_count(void);

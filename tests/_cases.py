@@ -54,3 +54,41 @@ def dictionary_grid() -> tuple[np.ndarray, np.ndarray, np.ndarray]:
 def rel_err(x: np.ndarray, ref: np.ndarray) -> float:
     """max|x - ref| / max|ref|, the reference's convention (test_cuda_parity.py:131-133)."""
     return float(np.abs(x - ref).max() / max(np.abs(ref).max(), 1e-300))
+
+
+def phantom(shape=(256, 256, 8), *, classes=6, seed=0, device="cpu"):
+    """Piecewise-smooth T1/T2/M0 maps: ``classes`` random ellipsoidal tissue
+    classes (the first fills the field of view) with T1 300-2500 ms, T2
+    30-250 ms, M0 0.5-1.0, each map times (1 + 5% Gaussian-smoothed noise)."""
+    generator = torch.Generator().manual_seed(seed)
+
+    def draw(low, high, n):
+        return low + (high - low) * torch.rand(n, generator=generator, dtype=torch.float64)
+
+    t1c, t2c, m0c = draw(300, 2500, classes), draw(30, 250, classes), draw(0.5, 1.0, classes)
+    axes = [torch.linspace(-1, 1, n, dtype=torch.float64) for n in shape]
+    grid = torch.stack(torch.meshgrid(*axes, indexing="ij"), -1)
+    label = torch.zeros(shape, dtype=torch.int64)
+    for k in range(1, classes):
+        center = draw(-0.5, 0.5, 3)
+        radii = draw(0.15, 0.5, 3)
+        inside = (((grid - center) / radii) ** 2).sum(-1) <= 1.0
+        label[inside] = k
+    maps = []
+    for values in (t1c, t2c, m0c):
+        base = values[label]
+        noise = torch.randn(shape, generator=generator, dtype=torch.float64)
+        kernel = torch.exp(-0.5 * (torch.arange(-6, 7, dtype=torch.float64) / 2.0) ** 2)
+        kernel = kernel / kernel.sum()
+        smooth = noise[None, None]
+        for axis in range(3):
+            view = [1, 1, 1]
+            view[axis] = kernel.numel()
+            pad = [0, 0, 0, 0, 0, 0]
+            pad[2 * (2 - axis)] = pad[2 * (2 - axis) + 1] = 6
+            smooth = torch.nn.functional.conv3d(torch.nn.functional.pad(smooth, pad, mode="replicate"),
+                                                kernel.reshape(1, 1, *view))
+        smooth = smooth[0, 0]
+        smooth = smooth / smooth.abs().max()
+        maps.append((base * (1.0 + 0.05 * smooth)).to(torch.float32).to(device))
+    return tuple(maps)

@@ -1,0 +1,172 @@
+"""Gauss-Newton parameter mapping on the fused normal-equation kernel.
+
+The kernel's per-voxel cost, J^T r and J^T J are checked against the FP64
+oracle; the FP64 damped solve against numpy; a whole Levenberg-Marquardt fit
+against the same algorithm run in FP64 on the oracle's Jacobian."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from tests import _cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _mrf(states=10):
+    from torchsim_amd.simulators import MRFSimulator
+
+    return MRFSimulator(flip=torch.tensor(_cases.mrf_flips(), dtype=torch.float32), TR=10.0, states=states)
+
+
+def _oracle_normal(t1, t2, m0, data, states=10):
+    seq = _mrf(states)
+    events = oracle.events_of(seq.packed(**seq.bound))
+    sig, tan = oracle.real(events, t1=t1, t2=t2, m0=m0, states=states, outputs=880,
+                           tangents=[{"t1": 1.0}, {"t2": 1.0}, {"m0": 1.0}])
+    model = -sig  # 1j * (i m0 a0)
+    jac = -tan
+    r = model - data  # complex residual (model real)
+    cost = (np.abs(r) ** 2).sum(-1)
+    g = np.einsum("vps,vs->vp", jac, r.real)
+    h = np.einsum("vps,vqs->vpq", jac, jac)
+    return cost, g, h, model
+
+
+def test_normal_equations_match_fp64_oracle():
+    from torchsim_amd.recon import FusedModel
+
+    t1, t2 = _cases.random_tissue(200, seed=21)
+    m0 = np.linspace(0.5, 1.0, 200).astype(np.float32)
+    rng = np.random.default_rng(2)
+    _, _, _, model = _oracle_normal(t1, t2, m0, np.zeros((200, 880)))
+    data = (model + 0.01 * (rng.normal(size=model.shape) + 1j * rng.normal(size=model.shape))).astype(np.complex64)
+    cost, g, h, _ = _oracle_normal(t1, t2, m0, data.astype(np.complex128))
+    op = FusedModel(_mrf(), ("T1", "T2", "M0"))
+    x = torch.tensor(np.stack([t1, t2, m0], -1), dtype=torch.float64, device="cuda")
+    normal = op.normal(x, torch.tensor(data).cuda(), torch.arange(200, device="cuda")).cpu().numpy()
+    assert _cases.rel_err(normal[:, 0], cost) < 1e-3  # cost of a small residual: cancellation-limited
+    for p in range(3):
+        assert _cases.rel_err(normal[:, 1 + p], g[:, p]) < 1e-3
+    upper = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    index = {(0, 0): 5, (0, 1): 6, (0, 2): 7, (1, 1): 9, (1, 2): 10, (2, 2): 12}
+    for pq in upper:
+        assert _cases.rel_err(normal[:, index[pq]], h[:, pq[0], pq[1]]) < 1e-4, pq
+
+
+def test_damped_solve_matches_numpy():
+    from torchsim_amd.recon import solve_damped
+
+    rng = np.random.default_rng(3)
+    voxels = 1000
+    a = rng.normal(size=(voxels, 3, 5))
+    hm = a @ a.transpose(0, 2, 1)
+    g = rng.normal(size=(voxels, 3))
+    scale = rng.uniform(0.5, 2.0, size=(voxels, 3))
+    lam = rng.uniform(0.01, 1.0, size=voxels)
+    normal = np.zeros((voxels, 15))
+    normal[:, 1:4] = g
+    tri = [(0, 0), (0, 1), (0, 2), (0, 3), (1, 1), (1, 2), (1, 3), (2, 2), (2, 3), (3, 3)]
+    for k, (p, q) in enumerate(tri):
+        if p < 3 and q < 3:
+            normal[:, 5 + k] = hm[:, p, q]
+    step, predicted, gradient, curvature = solve_damped(
+        torch.tensor(normal).cuda(), torch.tensor(scale).cuda(), torch.tensor(lam).cuda(), 3)
+    hf = scale[:, :, None] * hm * scale[:, None, :]
+    gf = scale * g
+    ref = np.linalg.solve(hf + lam[:, None, None] * np.eye(3), -gf[..., None])[..., 0]
+    assert np.abs(step.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-12
+    assert np.abs(gradient.cpu().numpy() - gf).max() < 1e-12
+    pred = (ref * (lam[:, None] * ref - gf)).sum(-1)
+    assert np.abs(predicted.cpu().numpy() - pred).max() / np.abs(pred).max() < 1e-12
+    assert np.abs(curvature.cpu().numpy() - np.einsum("vii->vi", hf).max(-1)).max() < 1e-12
+
+
+def test_noiseless_recovery():
+    from torchsim_amd.estimators import NonlinearLeastSquares
+    from torchsim_amd.recon import GaussNewton, TrustRegion
+
+    seq = _mrf()
+    t1, t2 = _cases.random_tissue(300, seed=5)
+    t1 = np.clip(t1, 300, 2500).astype(np.float32)
+    t2 = np.clip(t2, 30, 250).astype(np.float32)
+    m0 = np.linspace(0.5, 1.0, 300).astype(np.float32)
+    data = seq.simulate(T1=torch.tensor(t1).cuda(), T2=torch.tensor(t2).cuda(), M0=torch.tensor(m0).cuda())
+    fit = NonlinearLeastSquares(seq, bounds={"T1": (100.0, 5000.0), "T2": (10.0, 2000.0), "M0": (0.01, 2.0)},
+                                loop=GaussNewton(TrustRegion(), max_iterations=40))
+    maps = fit.fit(T1=(300.0, 2500.0), T2=(30.0, 250.0), M0=(0.5, 1.0)).map(data)
+    for name, truth in (("T1", t1), ("T2", t2), ("M0", m0)):
+        err = np.abs(maps[name].cpu().numpy() - truth) / truth
+        assert np.median(err) < 1e-4, name
+        assert np.percentile(err, 99) < 1e-2, name
+
+
+def _oracle_lm(t1, t2, m0, data, start, bounds, iterations):
+    """The same Levenberg-Marquardt (TrustRegion, Nielsen) in FP64 on the C oracle."""
+    names = ("T1", "T2", "M0")
+    lo = np.array([bounds[n][0] for n in names])
+    hi = np.array([bounds[n][1] for n in names])
+
+    def natural(x):
+        return lo + (hi - lo) / (1 + np.exp(-x))
+
+    def scale(x):
+        s = 1 / (1 + np.exp(-x))
+        return (hi - lo) * s * (1 - s)
+
+    def normal(x):
+        nat = natural(x)
+        cost, g, h, _ = _oracle_normal(nat[:, 0], nat[:, 1], nat[:, 2], data)
+        d = scale(x)
+        return cost, d * g, d[:, :, None] * h * d[:, None, :]
+
+    inside = (start - lo) / (hi - lo)
+    x = np.log(inside) - np.log1p(-inside)
+    x = np.broadcast_to(x, (data.shape[0], 3)).copy()
+    cost, g, h = normal(x)
+    lam = 1e-2 * np.einsum("vii->vi", h).max(-1)
+    rising = np.full_like(lam, 2.0)
+    for _ in range(iterations):
+        step = np.linalg.solve(h + lam[:, None, None] * np.eye(3), -g[..., None])[..., 0]
+        pred = (step * (lam[:, None] * step - g)).sum(-1)
+        cand = x + step
+        tc, tg, th = normal(cand)
+        gain = np.where(pred > 0, (cost - tc) / np.maximum(pred, 1e-38), 0.0)
+        better = gain > 0
+        lowered = lam * np.maximum(1 - (2 * gain - 1) ** 3, 1 / 3)
+        rising_new = np.where(better, 2.0, rising * 2.0)
+        lam = np.where(better, lowered, lam * rising)
+        rising = rising_new
+        x = np.where(better[:, None], cand, x)
+        cost = np.where(better, tc, cost)
+        g = np.where(better[:, None], tg, g)
+        h = np.where(better[:, None, None], th, h)
+    return natural(x)
+
+
+def test_fit_agrees_with_fp64_levenberg_marquardt_on_noisy_voxels():
+    from torchsim_amd.estimators import NonlinearLeastSquares
+    from torchsim_amd.recon import GaussNewton, TrustRegion
+
+    seq = _mrf()
+    rng = np.random.default_rng(8)
+    t1 = rng.uniform(400, 2200, 24).astype(np.float32)
+    t2 = rng.uniform(40, 200, 24).astype(np.float32)
+    m0 = rng.uniform(0.5, 1.0, 24).astype(np.float32)
+    clean = seq.simulate(T1=torch.tensor(t1).cuda(), T2=torch.tensor(t2).cuda(), M0=torch.tensor(m0).cuda())
+    sigma = float(clean.abs().pow(2).mean().sqrt()) / 50.0
+    noise = sigma / np.sqrt(2) * (rng.normal(size=(24, 880)) + 1j * rng.normal(size=(24, 880)))
+    data = clean + torch.tensor(noise, dtype=torch.complex64).cuda()
+    bounds = {"T1": (100.0, 5000.0), "T2": (10.0, 2000.0), "M0": (0.01, 2.0)}
+    fit = NonlinearLeastSquares(seq, bounds=bounds, initial={"T1": 1000.0, "T2": 100.0, "M0": 0.8},
+                                loop=GaussNewton(TrustRegion(), max_iterations=30, gradient_tolerance=0.0,
+                                                 step_tolerance=0.0))
+    maps = fit.fit(T1=(300.0, 2500.0), T2=(30.0, 250.0), M0=(0.5, 1.0)).map(data)
+    ours = np.stack([maps[n].cpu().numpy() for n in ("T1", "T2", "M0")], -1)
+    ref = _oracle_lm(t1, t2, m0, data.cpu().numpy().astype(np.complex128), np.array([1000.0, 100.0, 0.8]),
+                     bounds, 30)
+    assert np.abs(ours - ref).max(0).max() / 1.0 < 5.0  # ms-scale absolute agreement
+    assert (np.abs(ours - ref) / np.abs(ref)).max() < 1e-3

@@ -129,6 +129,13 @@ SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
             _P,
         ],
     ),
+    "tsim_gauss_newton_normal": (
+        _I32,
+        [ctypes.POINTER(_P), _I64, _I64, _I64, _I64, _I32, _I64, ctypes.POINTER(ctypes.c_int32), _I64,
+         _F32, _F32, _P, _P, _P, _I32, _P],
+    ),
+    "tsim_solve_damped": (_I32, [_P, _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P]),
+    "tsim_normal_stride": (_I32, []),
     "tsim_tile_count": (_I32, []),
     "tsim_tile_shape": (_I32, [_I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
 }

@@ -166,6 +166,83 @@ int run_real(RealArgs& args, int tm, bool diff, bool b1map, int tile, hipStream_
   return hip_status(hipGetLastError(), "real_kernel launch");
 }
 
+RealLaunch pick_normal(int tile, int tm, bool b1map) {
+  switch (tile) {
+#define TSIM_NORMAL_CASE(ID, SL, L) \
+  case ID:                          \
+    return pick_normal_tile_##ID(tm, b1map);
+    TSIM_TILES(TSIM_NORMAL_CASE)
+#undef TSIM_NORMAL_CASE
+    default:
+      return nullptr;
+  }
+}
+
+// One thread per voxel: H_free = D H D, g_free = D g from the accumulated
+// normal equations and the chain-rule scale D of the bound transform, then
+// the damped step (H_free + lambda I) d = -g_free by Cholesky in FP64, and
+// the reduction that step predicts, d . (lambda d - g_free).
+__global__ void solve_damped(const double* __restrict__ normal, const double* __restrict__ scale,
+                             const double* __restrict__ damping, const double* __restrict__ offset,
+                             int64_t voxels, int cols,
+                             double* __restrict__ step, double* __restrict__ predicted,
+                             double* __restrict__ gradient, double* __restrict__ curvature) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= voxels) return;
+  const double* n = normal + v * NORMAL_STRIDE;
+  double D[NORMAL_COLUMNS], g[NORMAL_COLUMNS], H[NORMAL_COLUMNS][NORMAL_COLUMNS];
+  for (int c = 0; c < NORMAL_COLUMNS; ++c) D[c] = (c < cols) ? (scale ? scale[v * cols + c] : 1.0) : 0.0;
+  int h = 0;
+  for (int c = 0; c < NORMAL_COLUMNS; ++c) {
+    g[c] = D[c] * n[1 + c];
+    for (int k = c; k < NORMAL_COLUMNS; ++k) {
+      const double x = D[c] * n[1 + NORMAL_COLUMNS + h++] * D[k];
+      H[c][k] = x;
+      H[k][c] = x;
+    }
+  }
+  double peak = 0.0;
+  for (int c = 0; c < cols; ++c) peak = fmax(peak, H[c][c]);
+  if (curvature) curvature[v] = peak;
+  if (gradient)
+    for (int c = 0; c < cols; ++c) gradient[v * cols + c] = g[c];
+  if (!step) return;
+  const double lam = damping ? damping[v] : 0.0;
+  // Cholesky of H + lam I (cols <= 4)
+  double Lm[NORMAL_COLUMNS][NORMAL_COLUMNS] = {};
+  bool ok = true;
+  for (int i = 0; i < cols; ++i) {
+    for (int j = 0; j <= i; ++j) {
+      double sum = H[i][j] + (i == j ? lam : 0.0);
+      for (int k = 0; k < j; ++k) sum -= Lm[i][k] * Lm[j][k];
+      if (i == j) {
+        if (!(sum > 0.0)) ok = false;
+        Lm[i][i] = sqrt(fmax(sum, 0.0));
+      } else {
+        Lm[i][j] = sum / Lm[j][j];
+      }
+    }
+  }
+  double y[NORMAL_COLUMNS], d[NORMAL_COLUMNS];
+  for (int i = 0; i < cols; ++i) {
+    double sum = -g[i] + (offset ? offset[v * cols + i] : 0.0);
+    for (int k = 0; k < i; ++k) sum -= Lm[i][k] * y[k];
+    y[i] = sum / Lm[i][i];
+  }
+  for (int i = cols - 1; i >= 0; --i) {
+    double sum = y[i];
+    for (int k = i + 1; k < cols; ++k) sum -= Lm[k][i] * d[k];
+    d[i] = sum / Lm[i][i];
+  }
+  double pred = 0.0;
+  for (int c = 0; c < cols; ++c) {
+    const double dc = ok ? d[c] : __builtin_nan("");
+    step[v * cols + c] = dc;
+    pred += dc * (lam * dc - g[c]);
+  }
+  if (predicted) predicted[v] = pred;
+}
+
 }  // namespace tsim
 
 // ---------------------------------------------------------------------------
@@ -394,5 +471,92 @@ int tsim_tile_shape(int tile, int* orders_per_lane, int* lanes_per_atom) {
   *lanes_per_atom = kTiles[tile].l;
   return OK;
 }
+
+}  // extern "C"
+
+extern "C" {
+
+int tsim_gauss_newton_normal(void* const* packed, int64_t atom_count, int64_t event_count,
+                             int64_t state_count, int64_t output_count, int features,
+                             int64_t tissue_stride, const int32_t* columns, int64_t column_count,
+                             float factor_re, float factor_im, const float* sample_sign,
+                             const void* data, double* normal, int tile, void* stream) {
+  if (packed == nullptr) {
+    set_error("packed buffers are null");
+    return BAD_ARGUMENT;
+  }
+  if (column_count < 1 || column_count > NORMAL_COLUMNS || columns == nullptr) {
+    set_error("between 1 and %d fitted columns, got %lld", NORMAL_COLUMNS, (long long)column_count);
+    return BAD_ARGUMENT;
+  }
+  if (atom_count > 0 && output_count > 0 && (data == nullptr || normal == nullptr)) {
+    set_error("measured data and the normal-equation buffer are required");
+    return BAD_ARGUMENT;
+  }
+  if (features & FEATURE_DIFFUSING) {
+    set_error("the normal-equation kernel carries no diffusion term");
+    return UNSUPPORTED;
+  }
+  RealArgs args;
+  int st = fill_real_args(args, packed, atom_count, event_count, state_count, output_count, features,
+                          tissue_stride);
+  if (st != OK) return st;
+  int tm = 0;
+  int cols[5];
+  if (!tangent_mask(columns, column_count, tm, cols)) return BAD_ARGUMENT;
+  if (cols[3] >= 0 || tm > 7) {
+    set_error("the normal-equation kernel fits t1, t2, b1 and m0");
+    return UNSUPPORTED;
+  }
+  if (cols[2] >= 0 && !(features & FEATURE_TRANSMIT)) {
+    set_error("a B1 column needs the transmit feature and a b1 buffer");
+    return BAD_ARGUMENT;
+  }
+  args.col_t1 = cols[0];
+  args.col_t2 = cols[1];
+  args.col_b1 = cols[2];
+  args.col_m0 = cols[4];
+  args.columns = (int)column_count;
+  args.sample_sign = sample_sign;
+  args.factor_re = factor_re;
+  args.factor_im = factor_im;
+  args.data = (const float2*)data;
+  args.normal = normal;
+  if (atom_count == 0 || output_count == 0) return OK;
+  if (tile < 0) tile = default_tile(args.states, popcount4(tm));
+  if (tile < 0 || tile >= kTileCount || kTiles[tile].sl * kTiles[tile].l < args.states) {
+    set_error("no state tile covers %d orders", args.states);
+    return UNSUPPORTED;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Compiled compiled;
+  st = compile_packed(packed, event_count, s, compiled);
+  if (st != OK) return st;
+  args.events = compiled.codes;
+  const int L = kTiles[tile].l;
+  const int64_t blocks = (atom_count * L + 255) / 256;
+  pick_normal(tile, tm, (features & FEATURE_TRANSMIT) != 0)(args, dim3((unsigned)blocks), dim3(256), s);
+  return hip_status(hipGetLastError(), "normal-equation kernel launch");
+}
+
+int tsim_solve_damped(const double* normal, const double* scale, const double* damping, const double* offset,
+                      int64_t voxels, int columns, double* step, double* predicted, double* gradient,
+                      double* curvature, void* stream) {
+  if (columns < 1 || columns > NORMAL_COLUMNS || voxels < 0) {
+    set_error("between 1 and %d columns and a non-negative voxel count", NORMAL_COLUMNS);
+    return BAD_ARGUMENT;
+  }
+  if (voxels == 0) return OK;
+  if (normal == nullptr) {
+    set_error("normal-equation buffer is null");
+    return BAD_ARGUMENT;
+  }
+  const int64_t blocks = (voxels + 255) / 256;
+  hipLaunchKernelGGL(solve_damped, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, normal, scale,
+                     damping, offset, voxels, columns, step, predicted, gradient, curvature);
+  return hip_status(hipGetLastError(), "solve_damped launch");
+}
+
+int tsim_normal_stride(void) { return NORMAL_STRIDE; }
 
 }  // extern "C"

@@ -34,7 +34,17 @@ struct RealArgs {
   float factor_im;
   int columns;
   int col_t1, col_t2, col_b1, col_ie, col_m0;
+  // Normal-equation mode: measured samples and the per-atom accumulators
+  // (cost, gradient[4], upper triangle of J^T J [10]) in FP64.
+  const float2* data;  // (atoms, outputs) complex64
+  double* normal;      // (atoms, NORMAL_STRIDE)
 };
+
+// Accumulators one atom hands back in normal-equation mode: the cost
+// sum |model - data|^2, J^T r (4), and the upper triangle of J^T J (10),
+// columns in the order the caller named them.
+constexpr int NORMAL_COLUMNS = 4;
+constexpr int NORMAL_STRIDE = 1 + NORMAL_COLUMNS + NORMAL_COLUMNS * (NORMAL_COLUMNS + 1) / 2;
 
 // Tangent directions a kernel carries, as bits of its template mask.
 constexpr int TAN_T1 = 1;
@@ -86,7 +96,7 @@ __device__ __forceinline__ void clear_padding(float (&a)[SL], unsigned keep) {
   for (int r = 0; r < SL; ++r) a[r] = ((keep >> r) & 1u) ? a[r] : 0.0f;
 }
 
-template <int SL, int L, int TM, bool DIFF, bool B1MAP>
+template <int SL, int L, int TM, bool DIFF, bool B1MAP, bool ACCUM = false>
 __global__ void __launch_bounds__(256) real_kernel(const RealArgs p) {
   constexpr int NT = popcount4(TM);
   constexpr int NTA = NT > 0 ? NT : 1;
@@ -136,6 +146,14 @@ __global__ void __launch_bounds__(256) real_kernel(const RealArgs p) {
     }
   }
   c[0] = first ? 1.0f : 0.0f;
+
+  double acc_cost = 0.0;
+  double acc_g[ACCUM ? NORMAL_COLUMNS : 1];
+  double acc_h[ACCUM ? NORMAL_COLUMNS * (NORMAL_COLUMNS + 1) / 2 : 1];
+#pragma unroll
+  for (int i = 0; i < (ACCUM ? NORMAL_COLUMNS : 1); ++i) acc_g[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < (ACCUM ? NORMAL_COLUMNS * (NORMAL_COLUMNS + 1) / 2 : 1); ++i) acc_h[i] = 0.0;
 
   float last_dt = -1.0f;
   float e1 = 1.0f, e2 = 1.0f, de1 = 0.0f, de2 = 0.0f;
@@ -277,7 +295,38 @@ __global__ void __launch_bounds__(256) real_kernel(const RealArgs p) {
         }
       }
     } else if (kind == KIND_ADC && (act & RECORD) && ev.out >= 0 && ev.out < p.outputs) {
-      if (first && live) {
+      if constexpr (ACCUM) {
+        if (first && live) {
+          const int64_t o = ev.out;
+          const float sign = p.sample_sign ? p.sample_sign[o] : 1.0f;
+          const float fr = p.factor_re * sign, fi = p.factor_im * sign;
+          const float v = m0 * a[0];
+          const float2 y = p.data[atom * p.outputs + o];
+          const double rx = (double)(-fi * v) - (double)y.x;
+          const double ry = (double)(fr * v) - (double)y.y;
+          acc_cost += rx * rx + ry * ry;
+          // J_c = (-fi d_c, fr d_c): J_c . r = d_c q, J_c . J_k = |factor|^2 d_c d_k
+          const double q = -(double)fi * rx + (double)fr * ry;
+          const double ff = (double)fr * fr + (double)fi * fi;
+          double d[NORMAL_COLUMNS];
+#pragma unroll
+          for (int c = 0; c < NORMAL_COLUMNS; ++c) {
+            float x = 0.0f;
+            if constexpr ((TM & TAN_T1) != 0) x = (p.col_t1 == c) ? m0 * ta[ST1][0] : x;
+            if constexpr ((TM & TAN_T2) != 0) x = (p.col_t2 == c) ? m0 * ta[ST2][0] : x;
+            if constexpr ((TM & TAN_B1) != 0) x = (p.col_b1 == c) ? m0 * ta[SB1][0] : x;
+            x = (p.col_m0 == c) ? a[0] : x;
+            d[c] = (double)x;
+          }
+          int h = 0;
+#pragma unroll
+          for (int c = 0; c < NORMAL_COLUMNS; ++c) {
+            acc_g[c] += d[c] * q;
+#pragma unroll
+            for (int k = c; k < NORMAL_COLUMNS; ++k) acc_h[h++] += ff * d[c] * d[k];
+          }
+        }
+      } else if (first && live) {
         const int64_t o = ev.out;
         const float v = m0 * a[0];
         if (p.out_real) {
@@ -352,6 +401,16 @@ __global__ void __launch_bounds__(256) real_kernel(const RealArgs p) {
       }
     }
   }
+  if constexpr (ACCUM) {
+    if (first && live) {
+      double* out = p.normal + atom * NORMAL_STRIDE;
+      out[0] = acc_cost;
+#pragma unroll
+      for (int c = 0; c < NORMAL_COLUMNS; ++c) out[1 + c] = acc_g[c];
+#pragma unroll
+      for (int h = 0; h < NORMAL_COLUMNS * (NORMAL_COLUMNS + 1) / 2; ++h) out[1 + NORMAL_COLUMNS + h] = acc_h[h];
+    }
+  }
 }
 
 
@@ -360,6 +419,30 @@ using RealLaunch = void (*)(const RealArgs&, dim3, dim3, hipStream_t);
 template <int SL, int L, int TM, bool DIFF, bool B1MAP>
 void launch_real(const RealArgs& args, dim3 grid, dim3 block, hipStream_t stream) {
   hipLaunchKernelGGL((real_kernel<SL, L, TM, DIFF, B1MAP>), grid, block, 0, stream, args);
+}
+
+template <int SL, int L, int TM, bool B1MAP>
+void launch_normal(const RealArgs& args, dim3 grid, dim3 block, hipStream_t stream) {
+  hipLaunchKernelGGL((real_kernel<SL, L, TM, false, B1MAP, true>), grid, block, 0, stream, args);
+}
+
+template <int SL, int L, int TM>
+RealLaunch pick_normal_b1(bool b1map) {
+  return b1map ? &launch_normal<SL, L, TM, true> : &launch_normal<SL, L, TM, false>;
+}
+
+template <int SL, int L>
+RealLaunch pick_normal_mask(int tm, bool b1map) {
+  switch (tm) {
+    case 0: return pick_normal_b1<SL, L, 0>(b1map);
+    case 1: return pick_normal_b1<SL, L, 1>(b1map);
+    case 2: return pick_normal_b1<SL, L, 2>(b1map);
+    case 3: return pick_normal_b1<SL, L, 3>(b1map);
+    case 4: return pick_normal_b1<SL, L, 4>(b1map);
+    case 5: return pick_normal_b1<SL, L, 5>(b1map);
+    case 6: return pick_normal_b1<SL, L, 6>(b1map);
+    default: return pick_normal_b1<SL, L, 7>(b1map);
+  }
 }
 
 template <int SL, int L, int TM, bool DIFF>
@@ -404,7 +487,9 @@ RealLaunch pick_mask(int tm, bool diff, bool b1map) {
   X(12, 12, 4)        \
   X(13, 16, 4)
 
-#define TSIM_DECLARE_TILE(ID, SL, L) RealLaunch pick_real_tile_##ID(int tm, bool diff, bool b1map);
+#define TSIM_DECLARE_TILE(ID, SL, L)                                   \
+  RealLaunch pick_real_tile_##ID(int tm, bool diff, bool b1map); \
+  RealLaunch pick_normal_tile_##ID(int tm, bool b1map);
 TSIM_TILES(TSIM_DECLARE_TILE)
 #undef TSIM_DECLARE_TILE
 

@@ -16,4 +16,8 @@ RealLaunch TSIM_CAT(pick_real_tile_, TSIM_TILE_ID)(int tm, bool diff, bool b1map
   return pick_mask<TSIM_TILE_SL, TSIM_TILE_L>(tm, diff, b1map);
 }
 
+RealLaunch TSIM_CAT(pick_normal_tile_, TSIM_TILE_ID)(int tm, bool b1map) {
+  return pick_normal_mask<TSIM_TILE_SL, TSIM_TILE_L>(tm, b1map);
+}
+
 }  // namespace tsim

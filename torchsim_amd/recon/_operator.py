@@ -1,0 +1,94 @@
+"""A signal model over bounded unknowns, as fused normal equations per voxel.
+
+The role of the reference's ModelOperator (recon/_operator.py:342-483) for a
+model with nothing encoding the voxels together: hold the simulator and the
+unknowns, map free variables to the properties, and give the loop what it
+needs per voxel. Where the reference hands back the Jacobian blocks, this
+hands back cost, J^T r and J^T J from one gfx950 launch.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from collections.abc import Mapping, Sequence
+from typing import Any
+
+import torch
+
+from .. import _native
+from ..sequence import _engine
+from ._bounds import to_free, to_natural, widen
+
+_COLUMN_INDEX = {"t1_ms": 0, "t2_ms": 1, "m0": 2, "b1": 3}
+
+
+class FusedModel:
+    """``simulator`` evaluated at per-voxel unknowns ``names`` (public property
+    names, e.g. ``("T1", "T2", "M0")``), bounded by ``bounds``."""
+
+    def __init__(self, simulator: Any, names: Sequence[str], *, bounds: Mapping[str, Any] | None = None,
+                 known: Mapping[str, Any] | None = None) -> None:
+        self.simulator = simulator
+        self.names = tuple(names)
+        self.bounds = dict(bounds or {})
+        self.known = dict(known or {})
+        held, sequence = simulator._split({**simulator.bound, **self.known})
+        self.sequence = sequence
+        self.held = {name: value for name, value in held.items() if name not in self.names}
+        self.fields = tuple(simulator._tissue_fields({name: 1.0 for name in self.names}))
+        for field in self.fields:
+            if field not in _COLUMN_INDEX:
+                raise NotImplementedError(f"the fused fit carries no derivative along {field!r}")
+        self.packed = simulator.packed(**sequence)
+        self.states = int(sequence.get("states", simulator.states) or 10)
+
+    @property
+    def columns(self) -> int:
+        return len(self.names)
+
+    def natural(self, x: torch.Tensor) -> torch.Tensor:
+        return to_natural(x, self.bounds, self.names)
+
+    def free(self, natural: torch.Tensor) -> torch.Tensor:
+        return to_free(natural, self.bounds, self.names)
+
+    def scale(self, x: torch.Tensor) -> torch.Tensor:
+        return widen(self.bounds, self.names, x).to(torch.float64)
+
+    def normal(self, x: torch.Tensor, data: torch.Tensor, live: torch.Tensor) -> torch.Tensor:
+        """(voxels, stride) FP64: cost, J^T r (natural columns), J^T J."""
+        device = data.device
+        values = self.natural(x)
+        fields: dict[str, Any] = {}
+        for name, value in self.held.items():
+            tensor = torch.as_tensor(value, device=device)
+            if tensor.numel() > 1:
+                tensor = tensor.reshape(-1)[live]
+            fields.update(self.simulator._tissue_fields({name: tensor}))
+        for index, field in enumerate(self.fields):
+            fields[field] = values[:, index].to(torch.float32)
+        tissue = _engine.prepare_tissue(fields, device, self.fields)
+        real, flip, samples = _engine.real_subspace(self.packed, tissue)
+        if not real:
+            raise NotImplementedError("the fused fit runs sequences whose pulses share one phase axis")
+        lib = _native.load()
+        events = list(self.packed.on(device))
+        if flip is not None:
+            events[2] = flip.to(device=device, dtype=torch.float32).contiguous()
+        stride = lib.tsim_normal_stride()
+        normal = torch.empty((tissue.atoms, stride), dtype=torch.float64, device=device)
+        if tissue.atoms == 0:
+            return normal
+        measured = data.to(device=device, dtype=torch.complex64).contiguous()
+        cols = (ctypes.c_int32 * self.columns)(*[_COLUMN_INDEX[f] for f in self.fields])
+        sign = None if samples is None else samples.to(device=device, dtype=torch.float32).contiguous()
+        factor = complex(self.simulator.signal_factor)
+        with torch.cuda.device(device):
+            status = lib.tsim_gauss_newton_normal(
+                _native.pointers([*tissue.buffers, *events]), tissue.atoms, self.packed.event_count, self.states,
+                self.packed.output_count, tissue.features, tissue.stride, cols, self.columns, factor.real, factor.imag,
+                None if sign is None else sign.data_ptr(), measured.data_ptr(), normal.data_ptr(), -1,
+                _native.stream_of(device),
+            )
+        _native.check(status)
+        return normal
