@@ -1,0 +1,96 @@
+"""FSE (complex kernel + recovery post-processing) and balanced TrueFISP
+(real subspace through half-turn signs; complex with an exchanging pool)
+against the FP64 torch oracle."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from oracle import packed as P
+from tests import _cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _fse_reference(seq_kwargs, t1, t2, m0, b1, names):
+    from torchsim_amd.simulators import FSESimulator
+
+    sim = FSESimulator()
+    packed = sim.packed(**seq_kwargs)
+    events = oracle.events_of(packed)
+    echoes = len(np.atleast_1d(seq_kwargs["flip"]))
+    left = seq_kwargs["TR"] - seq_kwargs["ESP"] * echoes
+    base = {"t1": torch.tensor(t1, dtype=torch.float64), "t2": torch.tensor(t2, dtype=torch.float64),
+            "m0": torch.tensor(m0, dtype=torch.float64), "b1": torch.tensor(b1, dtype=torch.float64)}
+
+    def run(t1_, t2_, m0_, b1_):
+        s = P.simulate(events, {"t1": t1_, "t2": t2_, "b1": b1_}, states=10, outputs=echoes)
+        rec = torch.exp(-1e3 / t1_ * left * 1e-3)[:, None]
+        return m0_[:, None] * s * (1 - rec) / (1 - rec * s)
+
+    order = ("T1", "T2", "M0", "B1")
+    primals = tuple(base[k] for k in ("t1", "t2", "m0", "b1"))
+    out = run(*primals)
+    columns = []
+    for name in names:
+        k = order.index(name)
+        tangents = tuple(torch.ones_like(p) if j == k else torch.zeros_like(p) for j, p in enumerate(primals))
+        columns.append(torch.func.jvp(run, primals, tangents)[1])
+    return out.numpy(), torch.stack(columns, 1).numpy()
+
+
+def test_fse_signal_and_jacobian():
+    from torchsim_amd.simulators import FSESimulator
+
+    flips = np.linspace(180, 120, 32)
+    kw = dict(flip=torch.tensor(flips, dtype=torch.float32), ESP=5.0, TR=3000.0)
+    t1, t2 = _cases.random_tissue(40, seed=31)
+    m0 = np.linspace(0.5, 1.0, 40).astype(np.float32)
+    b1 = np.linspace(0.8, 1.2, 40).astype(np.float32)
+    names = ("T1", "T2", "M0", "B1")
+    sig, jac = FSESimulator(**kw).jacobian(names, T1=torch.tensor(t1).cuda(), T2=torch.tensor(t2).cuda(),
+                                           M0=torch.tensor(m0).cuda(), B1=torch.tensor(b1).cuda())
+    ref, rjac = _fse_reference(dict(flip=flips.astype(np.float32), ESP=5.0, TR=3000.0), t1, t2, m0, b1, names)
+    assert _cases.rel_err(sig.cpu().numpy(), ref) < 1e-5
+    for p in range(4):
+        assert _cases.rel_err(jac.cpu().numpy()[:, p], rjac[:, p]) < 1e-4, names[p]
+    only = FSESimulator(**kw).simulate(T1=torch.tensor(t1).cuda(), T2=torch.tensor(t2).cuda(),
+                                       M0=torch.tensor(m0).cuda(), B1=torch.tensor(b1).cuda())
+    assert torch.equal(only, sig)
+
+
+@pytest.mark.parametrize("inversion", [None, "ideal"])
+def test_truefisp_balanced_train(inversion):
+    from torchsim_amd.simulators import TrueFISPSimulator
+
+    t1, t2 = _cases.random_tissue(30, seed=32)
+    sim = TrueFISPSimulator(flip=50.0, TR=4.5, nshots=300, inversion=inversion)
+    sig, jac = sim.jacobian(("T1", "T2"), T1=torch.tensor(t1).cuda(), T2=torch.tensor(t2).cuda())
+    events = oracle.events_of(sim.packed(**sim.bound))
+    ref, rjac = P.jacobian(events, {"t1": t1, "t2": t2}, ("t1", "t2"), states=1, outputs=300)
+    assert _cases.rel_err(sig.cpu().numpy(), ref.numpy()) < 1e-5
+    for p in range(2):
+        assert _cases.rel_err(jac.cpu().numpy()[:, p], rjac[:, p].numpy()) < 1e-4
+
+
+def test_truefisp_with_exchanging_pool_and_off_resonance():
+    from torchsim_amd.simulators import TrueFISPSimulator
+
+    t1, t2 = _cases.random_tissue(20, seed=33)
+    rng = np.random.default_rng(34)
+    f = rng.uniform(0.05, 0.3, 20).astype(np.float32)
+    k = rng.uniform(5, 20, 20).astype(np.float32)
+    b0 = rng.uniform(-20, 20, 20).astype(np.float32)
+    sim = TrueFISPSimulator(flip=40.0, TR=5.0, nshots=200, states=4)
+    sig, jac = sim.jacobian(("T1", "T2"), T1=torch.tensor(t1).cuda(), T2=torch.tensor(t2).cuda(),
+                            B0=torch.tensor(b0).cuda(), pool_b_fraction=torch.tensor(f).cuda(),
+                            pool_b_exchange=torch.tensor(k).cuda(), T1_pool_b=400.0, T2_pool_b=15.0)
+    events = oracle.events_of(sim.packed(**sim.bound))
+    tissue = {"t1": t1, "t2": t2, "b0": b0, "fraction": f, "exchange": k, "t1b": 400.0, "t2b": 15.0}
+    ref, rjac = P.jacobian(events, tissue, ("t1", "t2"), states=4, outputs=200)
+    assert _cases.rel_err(sig.cpu().numpy(), ref.numpy()) < 1e-5
+    for p in range(2):
+        assert _cases.rel_err(jac.cpu().numpy()[:, p], rjac[:, p].numpy()) < 1e-4

@@ -4,7 +4,9 @@ The public API mirrors torchsim's (``mrf_sim``, ``simulators.MRFSimulator``,
 ``Simulator.simulate/jacobian``); the compute runs in ``libtorchsim_hip.so``.
 """
 
-from . import model, sequence, simulators
-from ._functional import mrf_sim
+from . import estimators, model, optim, recon, sequence, simulators
+from ._functional import bssfp_sim, fse_sim, mrf_sim
+from .optim import crlb
 
-__all__ = ["mrf_sim", "model", "sequence", "simulators"]
+__all__ = ["bssfp_sim", "crlb", "estimators", "fse_sim", "model", "mrf_sim", "optim", "recon", "sequence",
+           "simulators"]

@@ -1,5 +1,7 @@
 """Predefined EPG simulators."""
 
+from .fse import FSESimulator
 from .mrf import MRFSimulator
+from .truefisp import TrueFISPSimulator
 
-__all__ = ["MRFSimulator"]
+__all__ = ["FSESimulator", "MRFSimulator", "TrueFISPSimulator"]
