@@ -170,6 +170,50 @@ int tsim_solve_damped(const double* normal, const double* scale, const double* d
 /* Doubles per voxel in the normal-equation buffer. */
 int tsim_normal_stride(void);
 
+/*
+ * One request of an uneven batch: its 26 packed DEVICE buffers and sizes, the
+ * Jacobian columns it wants (HOST indices as tsim_simulate_jacobian), the
+ * complex factor and optional sample signs, and either the complex64 outputs
+ * (mode 0) or the measured data and normal-equation output (mode 1).
+ */
+typedef struct tsim_batch_request {
+  void* packed[26];
+  int64_t atom_count;
+  int64_t event_count;
+  int64_t state_count;
+  int64_t output_count;
+  int64_t tissue_stride;
+  int32_t features;
+  int32_t column_count;
+  int32_t columns[5];
+  float factor_re;
+  float factor_im;
+  const float* sample_sign;
+  void* signal;
+  void* jacobian;
+  const void* data;
+  double* normal;
+} tsim_batch_request;
+
+/*
+ * The uneven-batch launcher: requests with their own event streams, state
+ * counts (padded to the batch's tile), atom counts and outputs, run by ONE
+ * persistent launch of `grid_blocks` workgroups (<= 0: one per work block)
+ * walking a table of (request, first atom) work blocks. All requests of a
+ * batch must share one kernel variant: the same tangent set, diffusion and
+ * transmit features (the caller groups them). Every atom runs exactly the
+ * arithmetic its request alone would, so results are bitwise the same alone
+ * and in the mix. No reference counterpart: the reference runs one launch per
+ * call (sequence/_accelerators.py:3262-3449).
+ * mode: 0 signal + Jacobian (as tsim_simulate_jacobian), 1 normal equations
+ * (as tsim_gauss_newton_normal). tile: -1 for the library's choice.
+ */
+int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int mode, int tile,
+                        int64_t grid_blocks, void* stream);
+
+/* sizeof(tsim_batch_request) as compiled into the library. */
+int tsim_batch_request_size(void);
+
 /* Number of state tiles compiled into the library, and the shape of one:
  * orders per lane and lanes per atom. No reference counterpart (tuning). */
 int tsim_tile_count(void);

@@ -1,0 +1,136 @@
+"""The serving loop: continuous batching over the uneven-batch launcher."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from tests import _cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _mrf(pulses, states=10):
+    from torchsim_amd.simulators import MRFSimulator
+
+    return MRFSimulator(flip=torch.tensor(_cases.benchmark_flips(pulses), dtype=torch.float32), TR=10.0,
+                        states=states)
+
+
+def _simulate_request(atoms, pulses, seed, diff=("T1", "T2"), states=10):
+    from torchsim_amd.serving import SimulateRequest
+
+    t1, t2 = _cases.random_tissue(atoms, seed=seed)
+    return SimulateRequest(_mrf(pulses, states), {"T1": torch.tensor(t1).cuda(), "T2": torch.tensor(t2).cuda()},
+                           diff=diff)
+
+
+def test_batched_launch_matches_single_launches_bitwise():
+    from torchsim_amd.serving import BatchItem, launch_batch
+    from torchsim_amd.sequence import _engine
+
+    items, singles = [], []
+    for k, (atoms, pulses, states) in enumerate([(37, 120, 10), (500, 300, 7), (1, 80, 10), (2049, 200, 9)]):
+        seq = _mrf(pulses, states)
+        t1, t2 = _cases.random_tissue(atoms, seed=40 + k)
+        packed = seq.packed(**seq.bound)
+        tissue = _engine.prepare_tissue({"t1_ms": torch.tensor(t1), "t2_ms": torch.tensor(t2)},
+                                        torch.device("cuda", 0), ("t1_ms", "t2_ms"))
+        sig = torch.empty((atoms, pulses), dtype=torch.complex64, device="cuda")
+        jac = torch.empty((atoms, 2, pulses), dtype=torch.complex64, device="cuda")
+        items.append(BatchItem(packed, tissue, states, ("t1_ms", "t2_ms"), 1j, signal=sig, jacobian=jac))
+        singles.append(_engine.simulate_real(packed, tissue, states=states, columns=("t1_ms", "t2_ms"), factor=1j,
+                                             tile=5))
+    stream = torch.cuda.current_stream()
+    launch_batch(items, mode=0, stream=stream, tile=5, grid_blocks=7)  # persistent: 7 workgroups walk the table
+    torch.cuda.synchronize()
+    for item, (sig, jac) in zip(items, singles):
+        assert torch.equal(item.signal, sig)
+        assert torch.equal(item.jacobian, jac)
+
+
+def test_same_alone_and_in_the_mix():
+    from torchsim_amd.serving import FitRequest, Server
+
+    probe = _simulate_request(300, 250, seed=1)
+    with Server() as alone:
+        a_sig, a_jac = alone.submit(probe).result(timeout=120)
+    seq = _mrf(200)
+    t1, t2 = _cases.random_tissue(64, seed=7)
+    clean = seq.simulate(T1=torch.tensor(t1).cuda(), T2=torch.tensor(t2).cuda())
+    fit = FitRequest(seq, clean, bounds={"T1": (100.0, 5000.0), "T2": (10.0, 2000.0), "M0": (0.01, 2.0)},
+                     initial={"T1": 1000.0, "T2": 100.0, "M0": 0.8}, iterations=6)
+    with Server() as alone:
+        f_alone = alone.submit(fit).result(timeout=120)
+    with Server() as mixed:
+        others = [mixed.submit(_simulate_request(n, p, seed=s)) for n, p, s in
+                  [(1000, 400, 2), (17, 100, 3), (5000, 300, 4), (123, 600, 5)]]
+        f_mix = mixed.submit(fit)
+        mine = mixed.submit(probe)
+        more = [mixed.submit(_simulate_request(n, p, seed=s, diff=())) for n, p, s in [(2000, 250, 6), (9, 50, 8)]]
+        b_sig, b_jac = mine.result(timeout=120)
+        f_mixed = f_mix.result(timeout=120)
+        for f in others + more:
+            f.result(timeout=120)
+        stats = mixed.stats()
+    assert torch.equal(a_sig, b_sig) and torch.equal(a_jac, b_jac)
+    for name in ("T1", "T2", "M0"):
+        assert torch.equal(f_alone[name], f_mixed[name]), name
+    assert stats["completed"] == 8
+    assert stats["batched_jobs"] > stats["launches"]  # some launches carried several requests
+
+
+def test_served_results_equal_the_api_and_fits_recover():
+    from torchsim_amd.serving import FitRequest, Server
+
+    request = _simulate_request(700, 300, seed=12)
+    direct = request.simulator.jacobian(("T1", "T2"), **request.properties)
+    seq = _mrf(300)
+    t1 = np.linspace(400, 2000, 50).astype(np.float32)
+    t2 = np.linspace(40, 200, 50).astype(np.float32)
+    clean = seq.simulate(T1=torch.tensor(t1).cuda(), T2=torch.tensor(t2).cuda())
+    fit = FitRequest(seq, clean, bounds={"T1": (100.0, 5000.0), "T2": (10.0, 2000.0), "M0": (0.01, 2.0)},
+                     initial={"T1": 1000.0, "T2": 100.0, "M0": 0.8}, iterations=25)
+    with Server() as server:
+        served = server.submit(request).result(timeout=120)
+        maps = server.submit(fit).result(timeout=120)
+    assert torch.equal(served[0], direct[0]) and torch.equal(served[1], direct[1])
+    assert float(((maps["T1"].cpu() - torch.tensor(t1)).abs() / torch.tensor(t1)).max()) < 1e-3
+    assert float(((maps["T2"].cpu() - torch.tensor(t2)).abs() / torch.tensor(t2)).max()) < 1e-3
+
+
+def test_unbatched_sequences_are_served_too():
+    """FSE (complex kernel) simulate and fit run through the ordinary path."""
+    from torchsim_amd.serving import FitRequest, Server, SimulateRequest
+    from torchsim_amd.simulators import FSESimulator
+
+    fse = FSESimulator(flip=torch.full((64,), 150.0), ESP=5.0, TR=3000.0)
+    t1 = np.linspace(500, 2000, 20).astype(np.float32)
+    t2 = np.linspace(40, 150, 20).astype(np.float32)
+    props = {"T1": torch.tensor(t1).cuda(), "T2": torch.tensor(t2).cuda()}
+    direct = fse.simulate(**props)
+    with Server() as server:
+        served = server.submit(SimulateRequest(fse, props)).result(timeout=120)
+        maps = server.submit(FitRequest(fse, direct, bounds={"T1": (100.0, 5000.0), "T2": (10.0, 2000.0),
+                                                             "M0": (0.01, 2.0)},
+                                        initial={"T1": 1000.0, "T2": 80.0, "M0": 0.9}, iterations=30)).result(120)
+    assert torch.equal(served, direct)
+    assert float(((maps["T2"].cpu() - torch.tensor(t2)).abs() / torch.tensor(t2)).max()) < 1e-2
+
+
+def test_admission_by_budget_and_refusal():
+    from torchsim_amd.serving import Server
+
+    small = _simulate_request(4000, 300, seed=20)
+    from torchsim_amd.serving._server import _request_bytes
+
+    need = _request_bytes(small)
+    with Server(hbm_budget=int(need * 1.5)) as server:
+        futures = [server.submit(_simulate_request(4000, 300, seed=20 + k)) for k in range(6)]
+        for f in futures:
+            f.result(timeout=120)
+        huge = server.submit(_simulate_request(200000, 1000, seed=9))
+        with pytest.raises(MemoryError):
+            huge.result(timeout=10)
+        assert server.stats()["completed"] == 6

@@ -140,6 +140,32 @@ SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "tsim_tile_shape": (_I32, [_I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
 }
 
+class BatchRequest(ctypes.Structure):
+    """Mirror of ``tsim_batch_request`` (include/torchsim_hip.h)."""
+
+    _fields_ = [
+        ("packed", _P * PACKED_COUNT),
+        ("atom_count", _I64),
+        ("event_count", _I64),
+        ("state_count", _I64),
+        ("output_count", _I64),
+        ("tissue_stride", _I64),
+        ("features", ctypes.c_int32),
+        ("column_count", ctypes.c_int32),
+        ("columns", ctypes.c_int32 * 5),
+        ("factor_re", _F32),
+        ("factor_im", _F32),
+        ("sample_sign", _P),
+        ("signal", _P),
+        ("jacobian", _P),
+        ("data", _P),
+        ("normal", _P),
+    ]
+
+
+SIGNATURES["tsim_simulate_batch"] = (_I32, [ctypes.POINTER(BatchRequest), _I64, _I32, _I32, _I64, _P])
+SIGNATURES["tsim_batch_request_size"] = (_I32, [])
+
 _lib: ctypes.CDLL | None = None
 
 
@@ -165,6 +191,8 @@ def load(path: str | os.PathLike | None = None) -> ctypes.CDLL:
         function = getattr(lib, name)
         function.restype = restype
         function.argtypes = argtypes
+    if lib.tsim_batch_request_size() != ctypes.sizeof(BatchRequest):
+        raise RuntimeError("tsim_batch_request layout differs between the library and the bindings")
     if path is None:
         _lib = lib
     return lib

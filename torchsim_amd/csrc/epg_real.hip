@@ -24,6 +24,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 namespace tsim {
 
@@ -164,6 +166,18 @@ int run_real(RealArgs& args, int tm, bool diff, bool b1map, int tile, hipStream_
   RealLaunch launch = pick_real(tile, tm, diff, b1map);
   launch(args, dim3((unsigned)blocks), dim3(threads), stream);
   return hip_status(hipGetLastError(), "real_kernel launch");
+}
+
+BatchLaunch pick_batch(int tile, int tm, bool diff, bool b1map, bool accum) {
+  switch (tile) {
+#define TSIM_BATCH_CASE(ID, SL, L) \
+  case ID:                         \
+    return pick_batch_tile_##ID(tm, diff, b1map, accum);
+    TSIM_TILES(TSIM_BATCH_CASE)
+#undef TSIM_BATCH_CASE
+    default:
+      return nullptr;
+  }
 }
 
 RealLaunch pick_normal(int tile, int tm, bool b1map) {
@@ -558,5 +572,211 @@ int tsim_solve_damped(const double* normal, const double* scale, const double* d
 }
 
 int tsim_normal_stride(void) { return NORMAL_STRIDE; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Uneven-batch launch
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// Pinned staging for the request and work tables: freed once the copy that
+// reads it has completed (checked at the next batch call), so a batch launch
+// never waits on the host.
+struct Staged {
+  void* host;
+  hipEvent_t copied;
+};
+std::mutex g_staged_mutex;
+std::vector<Staged> g_staged;
+
+void reap_staged() {
+  std::lock_guard<std::mutex> lock(g_staged_mutex);
+  for (size_t i = 0; i < g_staged.size();) {
+    if (hipEventQuery(g_staged[i].copied) == hipSuccess) {
+      (void)hipHostFree(g_staged[i].host);
+      (void)hipEventDestroy(g_staged[i].copied);
+      g_staged[i] = g_staged.back();
+      g_staged.pop_back();
+    } else {
+      ++i;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+typedef struct tsim_batch_request {
+  void* packed[26];
+  int64_t atom_count;
+  int64_t event_count;
+  int64_t state_count;
+  int64_t output_count;
+  int64_t tissue_stride;
+  int32_t features;
+  int32_t column_count;
+  int32_t columns[5];
+  float factor_re;
+  float factor_im;
+  const float* sample_sign;
+  void* signal;
+  void* jacobian;
+  const void* data;
+  double* normal;
+} tsim_batch_request;
+
+int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int mode, int tile,
+                        int64_t grid_blocks, void* stream) {
+  if (count < 0 || (count > 0 && requests == nullptr)) {
+    set_error("a batch needs its request table");
+    return BAD_ARGUMENT;
+  }
+  if (mode != 0 && mode != 1) {
+    set_error("mode 0 (signal and Jacobian) or 1 (normal equations), got %d", mode);
+    return BAD_ARGUMENT;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<RealArgs> args;
+  std::vector<int64_t> code_offset;
+  args.reserve((size_t)count);
+  int key_tm = -1, key_diff = -1, key_b1 = -1, max_states = 0;
+  int64_t total_events = 0;
+  for (int64_t r = 0; r < count; ++r) {
+    const tsim_batch_request& q = requests[r];
+    if (q.atom_count == 0 || q.output_count == 0) continue;
+    RealArgs a;
+    int st = fill_real_args(a, q.packed, q.atom_count, q.event_count, q.state_count, q.output_count, q.features,
+                            q.tissue_stride);
+    if (st != OK) return st;
+    int tm = 0;
+    int cols[5];
+    if (q.column_count < 0 || q.column_count > 5 || !tangent_mask(q.columns, q.column_count, tm, cols)) {
+      if (q.column_count < 0 || q.column_count > 5) set_error("request %lld: 0 to 5 columns", (long long)r);
+      return BAD_ARGUMENT;
+    }
+    if (cols[2] >= 0 && !(q.features & FEATURE_TRANSMIT)) {
+      set_error("request %lld: a B1 column needs the transmit feature", (long long)r);
+      return BAD_ARGUMENT;
+    }
+    const int diff = (q.features & FEATURE_DIFFUSING) ? 1 : 0;
+    const int b1 = (q.features & FEATURE_TRANSMIT) ? 1 : 0;
+    if (mode == 1 && (tm > 7 || diff || q.column_count < 1 || q.column_count > NORMAL_COLUMNS || !q.data || !q.normal)) {
+      set_error("request %lld: normal-equation mode fits 1-4 of t1, t2, m0, b1 with data and output", (long long)r);
+      return BAD_ARGUMENT;
+    }
+    if (key_tm < 0) {
+      key_tm = tm;
+      key_diff = diff;
+      key_b1 = b1;
+    } else if (tm != key_tm || diff != key_diff || b1 != key_b1) {
+      set_error("request %lld runs another kernel variant than the batch (group requests by variant)",
+                (long long)r);
+      return BAD_ARGUMENT;
+    }
+    a.col_t1 = cols[0];
+    a.col_t2 = cols[1];
+    a.col_b1 = cols[2];
+    a.col_ie = cols[3];
+    a.col_m0 = cols[4];
+    a.columns = q.column_count;
+    a.sample_sign = q.sample_sign;
+    a.factor_re = q.factor_re;
+    a.factor_im = q.factor_im;
+    a.signal = (float2*)q.signal;
+    a.jacobian = q.column_count > 0 ? (float2*)q.jacobian : nullptr;
+    a.data = (const float2*)q.data;
+    a.normal = q.normal;
+    max_states = a.states > max_states ? a.states : max_states;
+    code_offset.push_back(total_events);
+    total_events += q.event_count;
+    args.push_back(a);
+  }
+  if (args.empty()) return OK;
+  if (tile < 0) tile = default_tile(max_states, popcount4(key_tm));
+  if (tile < 0 || tile >= kTileCount || kTiles[tile].sl * kTiles[tile].l < max_states) {
+    set_error("no state tile covers %d orders", max_states);
+    return UNSUPPORTED;
+  }
+  const int L = kTiles[tile].l;
+  const int64_t per_block = 256 / L;
+  std::vector<BatchWork> work;
+  for (size_t r = 0; r < args.size(); ++r) {
+    for (int64_t first = 0; first < args[r].atoms; first += per_block) {
+      BatchWork w;
+      w.request = (int32_t)r;
+      w.pad = 0;
+      w.first_atom = first;
+      work.push_back(w);
+    }
+  }
+  // one workspace: compiled events, the request table, the work table
+  const size_t codes_bytes = sizeof(EventCode) * (size_t)total_events;
+  const size_t args_bytes = sizeof(RealArgs) * args.size();
+  const size_t work_bytes = sizeof(BatchWork) * work.size();
+  const size_t a_off = (codes_bytes + 255) & ~(size_t)255;
+  const size_t w_off = (a_off + args_bytes + 255) & ~(size_t)255;
+  char* space = nullptr;
+  int st = hip_status(hipMallocAsync((void**)&space, w_off + work_bytes, s), "hipMallocAsync(batch workspace)");
+  if (st != OK) return st;
+  EventCode* codes = (EventCode*)space;
+  size_t k = 0;
+  for (int64_t r = 0; r < count && st == OK; ++r) {
+    const tsim_batch_request& q = requests[r];
+    if (q.atom_count == 0 || q.output_count == 0) continue;
+    EventCode* mine = codes + code_offset[k];
+    st = compile_stream((const float*)q.packed[TISSUE_COUNT + E_DURATION],
+                        (const int32_t*)q.packed[TISSUE_COUNT + E_KIND],
+                        (const float*)q.packed[TISSUE_COUNT + E_FLIP],
+                        (const float*)q.packed[TISSUE_COUNT + E_PHASE],
+                        (const uint8_t*)q.packed[TISSUE_COUNT + E_ACTION],
+                        (const int32_t*)q.packed[TISSUE_COUNT + E_OUTPUT_INDEX], q.event_count, mine, s);
+    args[k].events = mine;
+    ++k;
+  }
+  reap_staged();
+  char* staging = nullptr;
+  hipEvent_t copied = nullptr;
+  if (st == OK)
+    st = hip_status(hipHostMalloc((void**)&staging, w_off + work_bytes, hipHostMallocDefault), "hipHostMalloc(staging)");
+  if (st == OK) {
+    memcpy(staging + a_off, args.data(), args_bytes);
+    memcpy(staging + w_off, work.data(), work_bytes);
+    st = hip_status(hipMemcpyAsync(space + a_off, staging + a_off, w_off + work_bytes - a_off, hipMemcpyHostToDevice, s),
+                    "hipMemcpyAsync(batch tables)");
+  }
+  if (st == OK) st = hip_status(hipEventCreateWithFlags(&copied, hipEventDisableTiming), "hipEventCreate");
+  if (st == OK) st = hip_status(hipEventRecord(copied, s), "hipEventRecord");
+  if (staging) {
+    if (st == OK) {
+      std::lock_guard<std::mutex> lock(g_staged_mutex);
+      g_staged.push_back(Staged{staging, copied});
+    } else {
+      (void)hipStreamSynchronize(s);
+      (void)hipHostFree(staging);
+      if (copied) (void)hipEventDestroy(copied);
+    }
+  }
+  if (st == OK) {
+    const int64_t blocks = (int64_t)work.size();
+    int64_t grid = grid_blocks > 0 ? grid_blocks : blocks;
+    if (grid > blocks) grid = blocks;
+    BatchLaunch launch = pick_batch(tile, key_tm, key_diff != 0, key_b1 != 0, mode == 1);
+    if (!launch) {
+      set_error("no batch kernel for this variant");
+      st = UNSUPPORTED;
+    } else {
+      launch((const RealArgs*)(space + a_off), (const BatchWork*)(space + w_off), blocks, dim3((unsigned)grid),
+             dim3(256), s);
+      st = hip_status(hipGetLastError(), "real_batch_kernel launch");
+    }
+  }
+  (void)hipFreeAsync(space, s);
+  return st;
+}
+
+int tsim_batch_request_size(void) { return (int)sizeof(tsim_batch_request); }
 
 }  // extern "C"

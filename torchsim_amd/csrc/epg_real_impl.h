@@ -96,8 +96,10 @@ __device__ __forceinline__ void clear_padding(float (&a)[SL], unsigned keep) {
   for (int r = 0; r < SL; ++r) a[r] = ((keep >> r) & 1u) ? a[r] : 0.0f;
 }
 
-template <int SL, int L, int TM, bool DIFF, bool B1MAP, bool ACCUM = false>
-__global__ void __launch_bounds__(256) real_kernel(const RealArgs p) {
+// One atom group's walk of its event stream. `group` is the atom this lane's
+// group stands for (clamped by the caller's `live` test inside).
+template <int SL, int L, int TM, bool DIFF, bool B1MAP, bool ACCUM>
+__device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   constexpr int NT = popcount4(TM);
   constexpr int NTA = NT > 0 ? NT : 1;
   constexpr int ST1 = slot_of(TM, TAN_T1);
@@ -108,7 +110,6 @@ __global__ void __launch_bounds__(256) real_kernel(const RealArgs p) {
   const int sub = (int)(threadIdx.x % L);
   const bool first = sub == 0;
   const bool last = sub == L - 1;
-  const int64_t group = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L;
   const bool live = group < p.atoms;
   const int64_t atom = live ? group : p.atoms - 1;
 
@@ -414,6 +415,72 @@ __global__ void __launch_bounds__(256) real_kernel(const RealArgs p) {
 }
 
 
+template <int SL, int L, int TM, bool DIFF, bool B1MAP, bool ACCUM = false>
+__global__ void __launch_bounds__(256) real_kernel(const RealArgs p) {
+  const int64_t group = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L;
+  real_body<SL, L, TM, DIFF, B1MAP, ACCUM>(p, group);
+}
+
+// The uneven-batch launch: a table of requests, each with its own event
+// stream, tissue, outputs and state count, and a table of work blocks
+// (request, first atom). A persistent grid walks the work blocks; every block
+// runs exactly the arithmetic a launch of its request alone would, so a
+// result is bitwise the same alone and in the mix.
+struct BatchWork {
+  int32_t request;
+  int32_t pad;
+  int64_t first_atom;
+};
+
+template <int SL, int L, int TM, bool DIFF, bool B1MAP, bool ACCUM = false>
+__global__ void __launch_bounds__(256) real_batch_kernel(const RealArgs* __restrict__ requests,
+                                                        const BatchWork* __restrict__ work, int64_t blocks) {
+  for (int64_t b = blockIdx.x; b < blocks; b += gridDim.x) {
+    const BatchWork w = work[b];
+    const RealArgs p = requests[w.request];
+    const int64_t group = w.first_atom + (int64_t)threadIdx.x / L;
+    real_body<SL, L, TM, DIFF, B1MAP, ACCUM>(p, group);
+  }
+}
+
+using BatchLaunch = void (*)(const RealArgs*, const BatchWork*, int64_t, dim3, dim3, hipStream_t);
+
+template <int SL, int L, int TM, bool DIFF, bool B1MAP, bool ACCUM>
+void launch_batch(const RealArgs* r, const BatchWork* w, int64_t blocks, dim3 grid, dim3 block, hipStream_t s) {
+  hipLaunchKernelGGL((real_batch_kernel<SL, L, TM, DIFF, B1MAP, ACCUM>), grid, block, 0, s, r, w, blocks);
+}
+
+template <int SL, int L, int TM, bool ACCUM>
+BatchLaunch pick_batch_flags(bool diff, bool b1map) {
+  if constexpr (ACCUM) {
+    return b1map ? &launch_batch<SL, L, TM, false, true, true> : &launch_batch<SL, L, TM, false, false, true>;
+  } else {
+    if (diff) return b1map ? &launch_batch<SL, L, TM, true, true, false> : &launch_batch<SL, L, TM, true, false, false>;
+    return b1map ? &launch_batch<SL, L, TM, false, true, false> : &launch_batch<SL, L, TM, false, false, false>;
+  }
+}
+
+template <int SL, int L>
+BatchLaunch pick_batch_mask(int tm, bool diff, bool b1map, bool accum) {
+  switch (tm) {
+#define TSIM_BATCH_TM(M)                                                     \
+  case M:                                                                   \
+    return accum ? pick_batch_flags<SL, L, M, true>(diff, b1map)            \
+                 : pick_batch_flags<SL, L, M, false>(diff, b1map);
+    TSIM_BATCH_TM(0)
+    TSIM_BATCH_TM(1)
+    TSIM_BATCH_TM(2)
+    TSIM_BATCH_TM(3)
+    TSIM_BATCH_TM(4)
+    TSIM_BATCH_TM(5)
+    TSIM_BATCH_TM(6)
+    TSIM_BATCH_TM(7)
+#undef TSIM_BATCH_TM
+    default:
+      return accum ? nullptr : pick_batch_flags<SL, L, 15, false>(diff, b1map);
+  }
+}
+
 using RealLaunch = void (*)(const RealArgs&, dim3, dim3, hipStream_t);
 
 template <int SL, int L, int TM, bool DIFF, bool B1MAP>
@@ -489,7 +556,8 @@ RealLaunch pick_mask(int tm, bool diff, bool b1map) {
 
 #define TSIM_DECLARE_TILE(ID, SL, L)                                   \
   RealLaunch pick_real_tile_##ID(int tm, bool diff, bool b1map); \
-  RealLaunch pick_normal_tile_##ID(int tm, bool b1map);
+  RealLaunch pick_normal_tile_##ID(int tm, bool b1map);          \
+  BatchLaunch pick_batch_tile_##ID(int tm, bool diff, bool b1map, bool accum);
 TSIM_TILES(TSIM_DECLARE_TILE)
 #undef TSIM_DECLARE_TILE
 

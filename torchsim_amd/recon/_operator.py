@@ -69,8 +69,8 @@ class FusedModel:
             fields[field] = values[:, index].to(torch.float32)
         tissue = _engine.prepare_tissue(fields, device, self.fields)
         real, flip, samples = _engine.real_subspace(self.packed, tissue)
-        if not real:
-            raise NotImplementedError("the fused fit runs sequences whose pulses share one phase axis")
+        if not real or tissue.features & _native.FEATURE_DIFFUSING or self.simulator.signal_factor is None:
+            return self._normal_from_jacobian(values, data, live)
         lib = _native.load()
         events = list(self.packed.on(device))
         if flip is not None:
@@ -91,4 +91,32 @@ class FusedModel:
                 _native.stream_of(device),
             )
         _native.check(status)
+        return normal
+
+    def _normal_from_jacobian(self, values: torch.Tensor, data: torch.Tensor, live: torch.Tensor) -> torch.Tensor:
+        """Any sequence: the simulator's fused signal + Jacobian (complex
+        kernel, post-processing included), reduced to the normal equations in
+        FP64 -- the Jacobian passes through memory on this path."""
+        device = data.device
+        held = {}
+        for name, value in self.held.items():
+            tensor = torch.as_tensor(value, device=device)
+            held[name] = tensor.reshape(-1)[live] if tensor.numel() > 1 else tensor
+        for index, name in enumerate(self.names):
+            held[name] = values[:, index].to(torch.float32)
+        signal, jacobian = self.simulator.linearize(held, self.names, **self.sequence)
+        stride = _native.load().tsim_normal_stride()
+        normal = torch.zeros((signal.shape[0], stride), dtype=torch.float64, device=device)
+        residual = (signal - data.to(signal.dtype)).to(torch.complex128)
+        jac = jacobian.to(torch.complex128)
+        normal[:, 0] = (residual.real**2 + residual.imag**2).sum(-1)
+        g = (jac.conj() * residual[:, None, :]).real.sum(-1)
+        h = torch.einsum("vps,vqs->vpq", jac.conj(), jac).real
+        normal[:, 1:1 + self.columns] = g
+        k = 0
+        for c in range(4):
+            for q in range(c, 4):
+                if c < self.columns and q < self.columns:
+                    normal[:, 5 + k] = h[:, c, q]
+                k += 1
         return normal
