@@ -112,6 +112,259 @@ def cpu_baseline(budget_s: float = 12.0) -> dict:
     }
 
 
+def torch_per_op_baseline(atoms: int = 65536) -> dict:
+    """The plain per-op torch-ROCm EPG path on the GPU: the same recursion
+    (real subspace, value + T1/T2/B1 tangents) as whole-array torch
+    operations, one group of launches per event -- the way a torch program
+    without a fused kernel runs it (the reference's own test oracle,
+    tests/utils/packed_reference.py, is this pattern in complex arithmetic)."""
+    import torch
+
+    from torchsim_amd.simulators import MRFSimulator
+
+    device = torch.device("cuda", 0)
+    packed = MRFSimulator().packed(flip=torch.tensor(flips(), dtype=torch.float32), TR=10.0)
+    T1, T2, B1 = dictionary(1)
+    pick = np.random.default_rng(0).choice(T1.size, atoms, replace=False)
+    t1 = torch.tensor(T1[pick], device=device)
+    t2 = torch.tensor(T2[pick], device=device)
+    b1 = torch.tensor(B1[pick], device=device)
+    events = [(float(d), int(k), float(f), int(a), int(o)) for d, k, f, a, o in zip(
+        packed.duration.tolist(), packed.kind.tolist(), packed.flip.tolist(), packed.action.tolist(),
+        packed.output_index.tolist())]
+
+    def run():
+        S = STATES
+        a = torch.zeros((4, atoms, S), device=device)  # value, dT1, dT2, dB1
+        b = torch.zeros_like(a)
+        c = torch.zeros_like(a)
+        c[0, :, 0] = 1.0
+        out = torch.empty((atoms, 4, PULSES), device=device)
+        r1, r2 = 1000.0 / t1, 1000.0 / t2
+
+        def shift(x, y):
+            y2 = torch.cat((y[..., 1:], torch.zeros_like(y[..., :1])), -1)
+            x2 = torch.cat((-y2[..., :1], x[..., :-1]), -1)
+            return x2, y2
+
+        for dt, kind, flip, act, o in events:
+            if dt != 0.0:
+                e1 = torch.exp(-r1 * dt)[:, None]
+                e2 = torch.exp(-r2 * dt)[:, None]
+                de1 = e1 * (1000.0 * dt / t1**2)[:, None]
+                de2 = e2 * (1000.0 * dt / t2**2)[:, None]
+                a0, b0, c0 = a[0].clone(), b[0].clone(), c[0].clone()
+                a, b, c = a * e2, b * e2, c * e1
+                a[2] += a0 * de2
+                b[2] += b0 * de2
+                c[1] += c0 * de1
+                c[0, :, 0] += 1 - e1[:, 0]
+                c[1, :, 0] -= de1[:, 0]
+            if act & 1:
+                a, b = shift(a, b)
+            if kind == 1 and not act & 4:
+                alpha = flip * b1
+                cs, sn = torch.cos(alpha)[:, None], torch.sin(alpha)[:, None]
+                ch, sh, hs = 0.5 * (1 + cs), 0.5 * (1 - cs), 0.5 * sn
+                p0, m0, z0 = a[0], b[0], c[0]
+                na = ch * a + sh * b - sn * c
+                nb = sh * a + ch * b + sn * c
+                nc = hs * (a - b) + cs * c
+                dal = flip
+                na[3] += (-hs * (p0 - m0) - cs * z0) * dal
+                nb[3] -= (-hs * (p0 - m0) - cs * z0) * dal
+                nc[3] += (0.5 * cs * (p0 - m0) - sn * z0) * dal
+                a, b, c = na, nb, nc
+            elif kind == 1:
+                c = -c
+            elif kind == 2 and act & 32:
+                out[:, :, o] = -a[:, :, 0].transpose(0, 1)
+            if act & 2:
+                a, b = shift(a, b)
+            if act & 8:
+                a, b = torch.zeros_like(a), torch.zeros_like(b)
+            elif act & 16:
+                a, b = shift(a, b)
+        return out
+
+    run()
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    run()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    return {"value": atoms * PULSES / elapsed, "unit": "atom*pulses/s", "atoms": atoms, "seconds": elapsed,
+            "sample": f"{atoms} dictionary atoms x {PULSES} pulses, S={STATES}, value + T1/T2/B1 tangents, "
+                      "whole-array torch ops per event on the GPU (FP32)"}
+
+
+def design_workload(steps: int, warmup: int) -> dict:
+    """Sequence design: 1,024 atoms, 880 flips U[2,70] deg and TRs U[8,15] ms,
+    loss sum tr(F^-1(T1,T2,M0)), gradient w.r.t. every flip and TR through
+    the checkpointed adjoint (segments of 64 pulses)."""
+    import torch
+
+    from tests import _cases
+    from torchsim_amd.optim import crlb
+    from torchsim_amd.simulators import MRFSimulator
+
+    rng = np.random.default_rng(1)
+    flips_deg = rng.uniform(2.0, 70.0, PULSES)
+    trs = rng.uniform(8.0, 15.0, PULSES)
+    t1, t2 = _cases.random_tissue(1024, seed=0)
+    T1, T2 = torch.tensor(t1).cuda(), torch.tensor(t2).cuda()
+    M0 = torch.ones(1024).cuda()
+    seq = MRFSimulator(states=10, segment_events=256)
+
+    def step():
+        flip = torch.tensor(flips_deg, dtype=torch.float32, requires_grad=True)
+        tr = torch.tensor(trs, dtype=torch.float32, requires_grad=True)
+        _, jac = seq.jacobian(("T1", "T2", "M0"), flip=flip, TR=tr, T1=T1, T2=T2, M0=M0)
+        loss = crlb(jac).sum()
+        loss.backward()
+        return flip.grad, tr.grad
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - start) * 1e3 / steps
+    return {"metric": "sequence-design gradient steps/s", "value": 1e3 / ms, "unit": "steps/s", "ms_per_step": ms,
+            "config": {"workload": "crlb(T1,T2,M0) gradient w.r.t. 880 flips + 880 TRs", "atoms": 1024,
+                       "pulses": PULSES, "states": 10, "checkpoint_pulses": 64},
+            "atom_pulses_per_s": 1024 * PULSES / (ms * 1e-3)}
+
+
+def fit_workload(iterations: int = 10) -> dict:
+    """Gauss-Newton mapping of a 256x256x8 phantom (524,288 voxels) from
+    880-frame MRF at SNR 50: 10 Levenberg-Marquardt iterations of T1, T2, M0."""
+    import torch
+
+    from tests import _cases
+    from torchsim_amd.estimators import NonlinearLeastSquares
+    from torchsim_amd.recon import GaussNewton, TrustRegion
+    from torchsim_amd.simulators import MRFSimulator
+
+    t1, t2, m0 = (x.cuda() for x in _cases.phantom((256, 256, 8), seed=0))
+    seq = MRFSimulator(flip=torch.tensor(flips(), dtype=torch.float32), TR=10.0, states=10)
+    clean = seq.simulate(T1=t1, T2=t2, M0=m0)
+    generator = torch.Generator(device="cuda").manual_seed(0)
+    sigma = float(clean.abs().pow(2).mean().sqrt()) / 50.0
+    noise = torch.complex(torch.randn(clean.shape, generator=generator, device="cuda"),
+                          torch.randn(clean.shape, generator=generator, device="cuda")) * (sigma / np.sqrt(2))
+    measured = clean + noise
+    del clean, noise
+    fit = NonlinearLeastSquares(seq, bounds={"T1": (100.0, 5000.0), "T2": (10.0, 2000.0), "M0": (0.01, 2.0)},
+                                loop=GaussNewton(TrustRegion(), max_iterations=iterations))
+    fit.fit(T1=(300.0, 2500.0), T2=(30.0, 250.0), M0=(0.5, 1.0), seed=0)
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    maps = fit.map(measured)
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - start
+    errors = {name: float(((maps[name] - truth).abs() / truth).median()) for name, truth in
+              (("T1", t1), ("T2", t2), ("M0", m0))}
+    voxels = t1.numel()
+    return {"metric": "Gauss-Newton voxels/s (10 iterations)", "value": voxels / seconds, "unit": "voxels/s",
+            "seconds": seconds, "iterations": fit.iterations, "unconverged": fit.unconverged,
+            "median_relative_error": errors,
+            "config": {"workload": "256x256x8 phantom, 880-frame MRF, SNR 50, LM over T1/T2/M0", "voxels": voxels,
+                       "states": 10}}
+
+
+def serve_workload(duration_s: float = 60.0, rate: float = 50.0, seed: int = 0) -> dict:
+    """Poisson arrivals (rate/s for duration_s): half simulate, half fit;
+    voxels log-uniform 1k-200k; bSSFP 200-500 pulses, FSE 64-256 echoes or
+    MRF 500-1000 pulses; a third with a two-pool exchange model, a sixth
+    diffusion-weighted. Reports p50/p99 latency and throughput."""
+    import torch
+
+    from torchsim_amd.serving import FitRequest, Server, SimulateRequest
+    from torchsim_amd.simulators import FSESimulator, MRFSimulator, TrueFISPSimulator
+
+    rng = np.random.default_rng(seed)
+    count = int(duration_s * rate)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    geometry = dict(crusher_dephasing_rad=8 * np.pi, voxel_size_m=0.5e-3)
+    plans = []
+    for k in range(count):
+        voxels = int(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
+        family = rng.integers(3)
+        pooled = rng.uniform() < 1 / 3
+        diffusing = (not pooled) and rng.uniform() < 0.25  # one sixth of all requests
+        if family == 0:
+            sim = TrueFISPSimulator(flip=float(rng.uniform(20, 60)), TR=4.5, nshots=int(rng.integers(200, 501)),
+                                    states=4 if diffusing else 1, **(geometry if diffusing else {}))
+        elif family == 1:
+            echoes = int(rng.integers(64, 257))
+            sim = FSESimulator(flip=torch.full((echoes,), float(rng.uniform(120, 180))), ESP=5.0, TR=3000.0,
+                               **(geometry if diffusing else {}))
+        else:
+            pulses = int(rng.integers(500, 1001))
+            train = np.concatenate([np.linspace(5, 60, pulses // 2), np.linspace(60, 2, pulses - pulses // 2)])
+            sim = MRFSimulator(flip=torch.tensor(train, dtype=torch.float32), TR=10.0, states=10,
+                               **(geometry if diffusing else {}))
+        plans.append((arrivals[k], voxels, family, pooled, diffusing, sim, rng.uniform() < 0.5,
+                      int(rng.integers(1 << 30))))
+
+    def tissue(voxels, pooled, diffusing, draw_seed):
+        g = np.random.default_rng(draw_seed)
+        props = {"T1": torch.tensor(g.uniform(300, 2500, voxels), dtype=torch.float32).cuda(),
+                 "T2": torch.tensor(g.uniform(30, 250, voxels), dtype=torch.float32).cuda()}
+        if pooled:
+            props.update(pool_b_fraction=torch.tensor(g.uniform(0.05, 0.3, voxels), dtype=torch.float32).cuda(),
+                         pool_b_exchange=torch.tensor(g.uniform(5, 20, voxels), dtype=torch.float32).cuda(),
+                         T1_pool_b=400.0, T2_pool_b=15.0)
+        if diffusing:
+            props["D"] = torch.tensor(g.uniform(0.5, 3.0, voxels), dtype=torch.float32).cuda()
+        return props
+
+    bounds = {"T1": (100.0, 5000.0), "T2": (10.0, 2000.0), "M0": (0.01, 2.0)}
+    client = torch.cuda.Stream()
+    outcome = {"errors": 0, "done": 0}
+    lock = __import__("threading").Lock()
+
+    def finished(future):
+        with lock:
+            outcome["done"] += 1
+            if future.exception() is not None:
+                outcome["errors"] += 1
+
+    with Server() as server:
+        start = time.perf_counter()
+        for when, voxels, family, pooled, diffusing, sim, is_fit, draw_seed in plans:
+            delay = start + when - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            # the client builds each request as it arrives (its own stream)
+            with torch.cuda.stream(client):
+                props = tissue(voxels, pooled, diffusing, draw_seed)
+                if is_fit:
+                    measured = sim.simulate(**props)
+                    known = {k: v for k, v in props.items() if k not in ("T1", "T2")}
+                    request = FitRequest(sim, measured, bounds=bounds, known=known,
+                                         initial={"T1": 1000.0, "T2": 100.0, "M0": 0.8}, iterations=10)
+                else:
+                    request = SimulateRequest(sim, props, diff=("T1", "T2"))
+            client.synchronize()
+            server.submit(request).add_done_callback(finished)
+        while True:
+            with lock:
+                if outcome["done"] >= count:
+                    break
+            time.sleep(0.01)
+        elapsed = time.perf_counter() - start
+        stats = server.stats()
+    return {"metric": "served requests/s with p50/p99 latency", "value": count / elapsed, "unit": "req/s",
+            "p50_s": stats["p50_s"], "p99_s": stats["p99_s"], "completed": stats["completed"],
+            "errors": outcome["errors"], "launches": stats["launches"], "batched_jobs": stats["batched_jobs"],
+            "seconds": elapsed,
+            "config": {"workload": "Poisson mix", "rate_per_s": rate, "duration_s": duration_s, "requests": count}}
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__)
     parser.add_argument("--gpus", type=int, default=1)
@@ -120,6 +373,9 @@ def main() -> int:
     parser.add_argument("--tiles", type=int, default=TILES, help="copies of the 106,182-atom grid")
     parser.add_argument("--tile", type=int, default=-1, help="state tile index (-1: library choice)")
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--no-torch-baseline", action="store_true")
+    parser.add_argument("--workload", choices=("dictionary", "design", "fit", "serve"), default="dictionary")
+    parser.add_argument("--serve-seconds", type=float, default=60.0)
     parser.add_argument(
         "--calibrate-copy",
         type=int,
@@ -139,6 +395,16 @@ def main() -> int:
     import torch
 
     from torchsim_amd.simulators import MRFSimulator
+
+    if args.workload == "design":
+        print(json.dumps(design_workload(args.steps, args.warmup)), flush=True)
+        return 0
+    if args.workload == "fit":
+        print(json.dumps(fit_workload()), flush=True)
+        return 0
+    if args.workload == "serve":
+        print(json.dumps(serve_workload(duration_s=args.serve_seconds)), flush=True)
+        return 0
 
     device = torch.device("cuda", 0)
     if args.calibrate_copy:
@@ -217,6 +483,10 @@ def main() -> int:
         baseline = cpu_baseline()
         record["cpu_baseline"] = baseline
         record["speedup_vs_cpu_baseline"] = value / baseline["value"]
+    if not args.no_torch_baseline:
+        per_op = torch_per_op_baseline()
+        record["torch_per_op_baseline"] = per_op
+        record["speedup_vs_torch_per_op"] = value / per_op["value"]
     print(json.dumps(record), flush=True)
     return 0
 
