@@ -143,7 +143,7 @@ int default_tile(int states, int tangents) {
 }
 
 int run_real(RealArgs& args, int tm, bool diff, bool b1map, int tile, hipStream_t stream) {
-  if (tile < 0) tile = default_tile(args.states, popcount4(tm == 0 ? 0 : (tm > 7 ? 15 : tm)));
+  if (tile < 0) tile = default_tile(args.states, popcount4(tangent_class(tm)));
   if (tile < 0 || tile >= kTileCount) {
     set_error("no state tile covers %d orders (at most %d)", args.states,
               kTiles[kTileCount - 1].sl * kTiles[kTileCount - 1].l);
@@ -537,7 +537,7 @@ int tsim_gauss_newton_normal(void* const* packed, int64_t atom_count, int64_t ev
   args.data = (const float2*)data;
   args.normal = normal;
   if (atom_count == 0 || output_count == 0) return OK;
-  if (tile < 0) tile = default_tile(args.states, popcount4(tm));
+  if (tile < 0) tile = default_tile(args.states, popcount4(tangent_class(tm)));
   if (tile < 0 || tile >= kTileCount || kTiles[tile].sl * kTiles[tile].l < args.states) {
     set_error("no state tile covers %d orders", args.states);
     return UNSUPPORTED;
@@ -671,7 +671,7 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
       key_tm = tm;
       key_diff = diff;
       key_b1 = b1;
-    } else if (tm != key_tm || diff != key_diff || b1 != key_b1) {
+    } else if (tangent_class(tm) != tangent_class(key_tm) || diff != key_diff || b1 != key_b1) {
       set_error("request %lld runs another kernel variant than the batch (group requests by variant)",
                 (long long)r);
       return BAD_ARGUMENT;
@@ -695,7 +695,7 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
     args.push_back(a);
   }
   if (args.empty()) return OK;
-  if (tile < 0) tile = default_tile(max_states, popcount4(key_tm));
+  if (tile < 0) tile = default_tile(max_states, popcount4(tangent_class(key_tm)));
   if (tile < 0 || tile >= kTileCount || kTiles[tile].sl * kTiles[tile].l < max_states) {
     set_error("no state tile covers %d orders", max_states);
     return UNSUPPORTED;

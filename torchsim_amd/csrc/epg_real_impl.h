@@ -4,6 +4,9 @@
 
 #include "epg_common.h"
 
+#include <type_traits>
+#include <utility>
+
 namespace tsim {
 
 struct RealArgs {
@@ -96,8 +99,65 @@ __device__ __forceinline__ void clear_padding(float (&a)[SL], unsigned keep) {
   for (int r = 0; r < SL; ++r) a[r] = ((keep >> r) & 1u) ? a[r] : 0.0f;
 }
 
+// ---------------------------------------------------------------------------
+// Rotating register file.
+//
+// Logical order j of this lane's F+ block (a) lives in physical register
+// (j + R) mod SL, of F- (b) in (j - R) mod SL, of Z (c) in j, for every plane
+// set (value and tangents). A gradient shift then moves no register: R drops
+// by one, and only the one value per plane that crosses a block edge travels,
+// by DPP, into the register it lands in. The event loop runs in segments
+// that each end at a shift, with R a compile-time constant inside a segment:
+// SL segment bodies in a row make one turn of the rotation. Tiles wider than
+// ROTATE_MAX orders per lane keep R = 0 and move registers on a shift (their
+// SL-fold code would not fit the instruction cache).
+// ---------------------------------------------------------------------------
+
+constexpr int ROTATE_MAX = 8;
+
+template <int SL, int R>
+struct Rot {
+  static constexpr int a(int j) { return (j + R) % SL; }
+  static constexpr int b(int j) { return ((j - R) % SL + SL) % SL; }
+};
+
+// The shift of one plane pair at rotation R (reference shift_real,
+// _epg_cpu.cpp:3010-3027): F- moves down an order, F+ up, the top order drops
+// and F+[0] = -F-[0]. Afterwards the rotation is R - 1 (mod SL).
+template <int SL, int L, int R>
+__device__ __forceinline__ void shift_rotating(float (&a)[SL], float (&b)[SL], bool first, bool last) {
+  constexpr int RN = (R + SL - 1) % SL;  // rotation after the shift
+  constexpr int KA = RN;                 // old logical a[SL-1], new logical a[0]
+  constexpr int KB0 = (SL - R) % SL;     // old logical b[0], new logical b[SL-1]
+  constexpr int KB1 = (SL - RN) % SL;    // new logical b[0]
+  if constexpr (L > 1) {
+    const float b_in = dpp_from_upper(b[KB0]);
+    const float a_in = dpp_from_lower(a[KA]);
+    b[KB0] = last ? 0.0f : b_in;
+    a[KA] = first ? -b[KB1] : a_in;
+  } else {
+    b[KB0] = 0.0f;
+    a[KA] = -b[KB1];
+  }
+}
+
+// Zero the F+ register that holds logical order `pad` (the first order past
+// the state count in this lane, or SL when the lane has none) at rotation R.
+template <int SL, int R>
+__device__ __forceinline__ void clear_order(float (&a)[SL], int pad) {
+#pragma unroll
+  for (int j = 0; j < SL; ++j) a[Rot<SL, R>::a(j)] = (pad == j) ? 0.0f : a[Rot<SL, R>::a(j)];
+}
+
+// One turn of the rotation: segments at R = 0, SL-1, ..., 1 (a shift takes R
+// to R - 1), stopping at the first that finds the stream ended.
+template <int SL, typename F, int... I>
+__device__ __forceinline__ bool run_rotations(F& f, std::integer_sequence<int, I...>) {
+  return (f(std::integral_constant<int, (SL - I) % SL>{}) && ...);
+}
+
 // One atom group's walk of its event stream. `group` is the atom this lane's
-// group stands for (clamped by the caller's `live` test inside).
+// group stands for (lanes past the last atom run clamped and write nothing).
 template <int SL, int L, int TM, bool DIFF, bool B1MAP, bool ACCUM>
 __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   constexpr int NT = popcount4(TM);
@@ -106,6 +166,7 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   constexpr int ST2 = slot_of(TM, TAN_T2);
   constexpr int SB1 = slot_of(TM, TAN_B1);
   constexpr int SIE = slot_of(TM, TAN_IE);
+  constexpr bool ROTATE = SL <= ROTATE_MAX;
 
   const int sub = (int)(threadIdx.x % L);
   const bool first = sub == 0;
@@ -113,11 +174,10 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   const bool live = group < p.atoms;
   const int64_t atom = live ? group : p.atoms - 1;
 
-  // Which of this lane's registers stand for an order below S.
+  // The first logical order of this lane past the state count (SL if none):
+  // after a shift F+ carries a value into it, which the truncation drops.
   const int base = sub * SL;
-  unsigned keep = 0;
-#pragma unroll
-  for (int r = 0; r < SL; ++r) keep |= (base + r < p.states) ? (1u << r) : 0u;
+  const int pad = (p.states - base >= 0 && p.states - base < SL) ? (int)(p.states - base) : SL;
   const bool padded = SL * L != p.states;
 
   const float t1 = p.t1[atom * p.stride_t1];
@@ -158,6 +218,7 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
 
   float last_dt = -1.0f;
   float e1 = 1.0f, e2 = 1.0f, de1 = 0.0f, de2 = 0.0f;
+  // diffusion damping per logical order of this lane's block
   float dl[DIFF ? SL : 1], dtr[DIFF ? SL : 1];
 #pragma unroll
   for (int r = 0; r < (DIFF ? SL : 1); ++r) {
@@ -167,253 +228,314 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
 
   const EventCode* __restrict__ stream = p.events;
   const int64_t E = p.event_count;
-  for (int64_t e = 0; e < E; ++e) {
-    const EventCode ev = stream[e];
-    const float dt = ev.dt;
-    const uint32_t act = ev.ctrl & 0xFFu;
-    const int32_t kind = (int32_t)(ev.ctrl >> CTRL_KIND_SHIFT);
+  int64_t e = 0;
+  // Where the event at `e` resumes after the shift that ended a segment:
+  // 0 from its start, 1 after its pre-shift, 2 after its post-shift.
+  int phase = 0;
 
-    if (dt != 0.0f) {
-      if (dt != last_dt) {
-        last_dt = dt;
-        e1 = expf(-r1 * dt);
-        e2 = expf(-r2 * dt);
-        de1 = e1 * (k1 * dt);
-        de2 = e2 * (k2 * dt);
-        if constexpr (DIFF) {
-          const float bf = drate * dt;
-#pragma unroll
-          for (int r = 0; r < SL; ++r) {
-            const float l = (float)(base + r);
-            dl[r] = expf(-(l * l) * bf);
-            dtr[r] = expf(-(l * l + l + (1.0f / 3.0f)) * bf);
-          }
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < SL; ++r) {
-        const float tr = DIFF ? e2 * dtr[r] : e2;
-        const float lo = DIFF ? e1 * dl[r] : e1;
-        if constexpr ((TM & TAN_T1) != 0) {
-          const float dlo = DIFF ? de1 * dl[r] : de1;
-          ta[ST1][r] *= tr;
-          tb[ST1][r] *= tr;
-          tc[ST1][r] = fmaf(tc[ST1][r], lo, c[r] * dlo);
-        }
-        if constexpr ((TM & TAN_T2) != 0) {
-          const float dtv = DIFF ? de2 * dtr[r] : de2;
-          ta[ST2][r] = fmaf(ta[ST2][r], tr, a[r] * dtv);
-          tb[ST2][r] = fmaf(tb[ST2][r], tr, b[r] * dtv);
-          tc[ST2][r] *= lo;
-        }
-        if constexpr ((TM & TAN_B1) != 0) {
-          ta[SB1][r] *= tr;
-          tb[SB1][r] *= tr;
-          tc[SB1][r] *= lo;
-        }
-        if constexpr ((TM & TAN_IE) != 0) {
-          ta[SIE][r] *= tr;
-          tb[SIE][r] *= tr;
-          tc[SIE][r] *= lo;
-        }
-        a[r] *= tr;
-        b[r] *= tr;
-        c[r] *= lo;
-      }
-      // Recovery toward equilibrium lands on order zero only.
-      c[0] += first ? 1.0f - e1 : 0.0f;
-      if constexpr ((TM & TAN_T1) != 0) tc[ST1][0] -= first ? de1 : 0.0f;
-    }
-
-    if (act & PRE_SHIFT) {
-      shift_pair<SL, L>(a, b, first, last);
-      if (padded) clear_padding<SL>(a, keep);
-#pragma unroll
-      for (int k = 0; k < NT; ++k) {
-        shift_pair<SL, L>(ta[k], tb[k], first, last);
-        if (padded) clear_padding<SL>(ta[k], keep);
-      }
-    }
-
-    if (kind == KIND_RF) {
-      if (act & INVERSION) {
-        const float eff = -eta;
+  // Relaxation and recovery over dt (reference relax_real,
+  // _epg_cpu.cpp:2930-2975), register map of rotation R.
+  auto relax = [&](auto rc, float dt) __attribute__((always_inline)) {
+    constexpr int R = decltype(rc)::value;
+    if (dt != last_dt) {
+      last_dt = dt;
+      e1 = expf(-r1 * dt);
+      e2 = expf(-r2 * dt);
+      de1 = e1 * (k1 * dt);
+      de2 = e2 * (k2 * dt);
+      if constexpr (DIFF) {
+        const float bf = drate * dt;
 #pragma unroll
         for (int r = 0; r < SL; ++r) {
-#pragma unroll
-          for (int k = 0; k < NT; ++k) {
-            if ((TM & TAN_IE) != 0 && k == SIE) {
-              tc[k][r] = fmaf(tc[k][r], eff, -c[r]);
-            } else {
-              tc[k][r] *= eff;
-            }
-          }
-          c[r] *= eff;
-        }
-      } else {
-        float cs, sn;
-        float alpha = ev.flip;
-        if constexpr (B1MAP) {
-          alpha = ev.flip * b1;
-          sincosf(alpha, &sn, &cs);
-        } else {
-          cs = ev.cosine;
-          sn = ev.sine;
-        }
-        const float ch = 0.5f * (1.0f + cs);
-        const float sh = 0.5f * (1.0f - cs);
-        const float hs = 0.5f * sn;
-        // Along B1 the angle moves by the flip itself per unit B1.
-        const float dal = ev.flip;
-        const float dch = -0.5f * sn * dal;
-        const float dhs = 0.5f * cs * dal;
-        const float dsn = cs * dal;
-        const float dcs = -sn * dal;
-#pragma unroll
-        for (int r = 0; r < SL; ++r) {
-          const float pv = a[r], mv = b[r], zv = c[r];
-#pragma unroll
-          for (int k = 0; k < NT; ++k) {
-            const float dp = ta[k][r], dm = tb[k][r], dz = tc[k][r];
-            float na = ch * dp + sh * dm - sn * dz;
-            float nb = sh * dp + ch * dm + sn * dz;
-            float nc = hs * (dp - dm) + cs * dz;
-            if ((TM & TAN_B1) != 0 && k == SB1) {
-              // product rule on the angle: d(ch) = -d(sh) = dch
-              const float u = pv - mv;
-              const float xa = dch * u - dsn * zv;
-              na += xa;
-              nb -= xa;
-              nc += dhs * u + dcs * zv;
-            }
-            ta[k][r] = na;
-            tb[k][r] = nb;
-            tc[k][r] = nc;
-          }
-          a[r] = ch * pv + sh * mv - sn * zv;
-          b[r] = sh * pv + ch * mv + sn * zv;
-          c[r] = hs * (pv - mv) + cs * zv;
-        }
-      }
-    } else if (kind == KIND_ADC && (act & RECORD) && ev.out >= 0 && ev.out < p.outputs) {
-      if constexpr (ACCUM) {
-        if (first && live) {
-          const int64_t o = ev.out;
-          const float sign = p.sample_sign ? p.sample_sign[o] : 1.0f;
-          const float fr = p.factor_re * sign, fi = p.factor_im * sign;
-          const float v = m0 * a[0];
-          const float2 y = p.data[atom * p.outputs + o];
-          const double rx = (double)(-fi * v) - (double)y.x;
-          const double ry = (double)(fr * v) - (double)y.y;
-          acc_cost += rx * rx + ry * ry;
-          // J_c = (-fi d_c, fr d_c): J_c . r = d_c q, J_c . J_k = |factor|^2 d_c d_k
-          const double q = -(double)fi * rx + (double)fr * ry;
-          const double ff = (double)fr * fr + (double)fi * fi;
-          double d[NORMAL_COLUMNS];
-#pragma unroll
-          for (int c = 0; c < NORMAL_COLUMNS; ++c) {
-            float x = 0.0f;
-            if constexpr ((TM & TAN_T1) != 0) x = (p.col_t1 == c) ? m0 * ta[ST1][0] : x;
-            if constexpr ((TM & TAN_T2) != 0) x = (p.col_t2 == c) ? m0 * ta[ST2][0] : x;
-            if constexpr ((TM & TAN_B1) != 0) x = (p.col_b1 == c) ? m0 * ta[SB1][0] : x;
-            x = (p.col_m0 == c) ? a[0] : x;
-            d[c] = (double)x;
-          }
-          int h = 0;
-#pragma unroll
-          for (int c = 0; c < NORMAL_COLUMNS; ++c) {
-            acc_g[c] += d[c] * q;
-#pragma unroll
-            for (int k = c; k < NORMAL_COLUMNS; ++k) acc_h[h++] += ff * d[c] * d[k];
-          }
-        }
-      } else if (first && live) {
-        const int64_t o = ev.out;
-        const float v = m0 * a[0];
-        if (p.out_real) {
-          p.out_real[atom * p.outputs + o] = 0.0f;
-          p.out_imag[atom * p.outputs + o] = v;
-        }
-        const float sign = p.sample_sign ? p.sample_sign[o] : 1.0f;
-        const float fr = p.factor_re * sign, fi = p.factor_im * sign;
-        // factor * (0 + i v) = (-fi v, fr v)
-        if (p.signal) p.signal[atom * p.outputs + o] = make_float2(-fi * v, fr * v);
-        if (p.jacobian) {
-          float2* row = p.jacobian + atom * (int64_t)p.columns * p.outputs + o;
-          if constexpr ((TM & TAN_T1) != 0) {
-            if (p.col_t1 >= 0) {
-              const float d = m0 * ta[ST1][0];
-              row[(int64_t)p.col_t1 * p.outputs] = make_float2(-fi * d, fr * d);
-            }
-          }
-          if constexpr ((TM & TAN_T2) != 0) {
-            if (p.col_t2 >= 0) {
-              const float d = m0 * ta[ST2][0];
-              row[(int64_t)p.col_t2 * p.outputs] = make_float2(-fi * d, fr * d);
-            }
-          }
-          if constexpr ((TM & TAN_B1) != 0) {
-            if (p.col_b1 >= 0) {
-              const float d = m0 * ta[SB1][0];
-              row[(int64_t)p.col_b1 * p.outputs] = make_float2(-fi * d, fr * d);
-            }
-          }
-          if constexpr ((TM & TAN_IE) != 0) {
-            if (p.col_ie >= 0) {
-              const float d = m0 * ta[SIE][0];
-              row[(int64_t)p.col_ie * p.outputs] = make_float2(-fi * d, fr * d);
-            }
-          }
-          if (p.col_m0 >= 0) {
-            const float d = a[0];
-            row[(int64_t)p.col_m0 * p.outputs] = make_float2(-fi * d, fr * d);
-          }
+          const float l = (float)(base + r);
+          dl[r] = expf(-(l * l) * bf);
+          dtr[r] = expf(-(l * l + l + (1.0f / 3.0f)) * bf);
         }
       }
     }
+    // Every update below reads a register's old value before it overwrites
+    // that register, so the allocator can keep each plane where it lives.
+#pragma unroll
+    for (int j = 0; j < SL; ++j) {
+      const int ia = Rot<SL, R>::a(j), ib = Rot<SL, R>::b(j);
+      const float tra = DIFF ? e2 * dtr[j] : e2;
+      const float lo = DIFF ? e1 * dl[j] : e1;
+      const float dlo = DIFF ? de1 * dl[j] : de1;
+      const float dtra = DIFF ? de2 * dtr[j] : de2;
+      if constexpr ((TM & TAN_T1) != 0) {
+        ta[ST1][ia] *= tra;
+        tb[ST1][ib] *= tra;
+        tc[ST1][j] = fmaf(c[j], dlo, tc[ST1][j] * lo);
+      }
+      if constexpr ((TM & TAN_T2) != 0) {
+        ta[ST2][ia] = fmaf(a[ia], dtra, ta[ST2][ia] * tra);
+        tb[ST2][ib] = fmaf(b[ib], dtra, tb[ST2][ib] * tra);
+        tc[ST2][j] *= lo;
+      }
+      if constexpr ((TM & TAN_B1) != 0) {
+        ta[SB1][ia] *= tra;
+        tb[SB1][ib] *= tra;
+        tc[SB1][j] *= lo;
+      }
+      if constexpr ((TM & TAN_IE) != 0) {
+        ta[SIE][ia] *= tra;
+        tb[SIE][ib] *= tra;
+        tc[SIE][j] *= lo;
+      }
+      a[ia] *= tra;
+      b[ib] *= tra;
+      c[j] *= lo;
+    }
+    // Recovery toward equilibrium lands on order zero only.
+    c[0] += first ? 1.0f - e1 : 0.0f;
+    if constexpr ((TM & TAN_T1) != 0) tc[ST1][0] -= first ? de1 : 0.0f;
+  };
 
-    if (act & POST_SHIFT) {
-      shift_pair<SL, L>(a, b, first, last);
-      if (padded) clear_padding<SL>(a, keep);
-#pragma unroll
-      for (int k = 0; k < NT; ++k) {
-        shift_pair<SL, L>(ta[k], tb[k], first, last);
-        if (padded) clear_padding<SL>(ta[k], keep);
-      }
-    }
-    if (act & SPOIL_AFTER) {
+  // An RF rotation about the x axis (reference rf_real, _epg_cpu.cpp:2980-3005)
+  // or an inversion, register map of rotation R.
+  auto pulse = [&](auto rc, const EventCode& ev, uint32_t act) __attribute__((always_inline)) {
+    constexpr int R = decltype(rc)::value;
+    if (act & INVERSION) {
+      const float eff = -eta;
 #pragma unroll
       for (int r = 0; r < SL; ++r) {
-        a[r] = 0.0f;
-        b[r] = 0.0f;
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
-          ta[k][r] = 0.0f;
-          tb[k][r] = 0.0f;
+          if ((TM & TAN_IE) != 0 && k == SIE) {
+            tc[k][r] = fmaf(tc[k][r], eff, -c[r]);
+          } else {
+            tc[k][r] *= eff;
+          }
         }
+        c[r] *= eff;
       }
-    } else if (act & SHIFT_AFTER) {
-      shift_pair<SL, L>(a, b, first, last);
-      if (padded) clear_padding<SL>(a, keep);
+      return;
+    }
+    float cs, sn;
+    if constexpr (B1MAP) {
+      sincosf(ev.flip * b1, &sn, &cs);
+    } else {
+      cs = ev.cosine;
+      sn = ev.sine;
+    }
+    // With s = p + m and d = p - m the rotation reads
+    //   p' = s/2 + x,  m' = s/2 - x,  z' = (sin/2) d + cos z,
+    //   x = (cos/2) d - sin z,
+    // which consumes p and m before it writes either, so every plane is
+    // updated in the registers it lives in.
+    const float hcs = 0.5f * cs;
+    const float hsn = 0.5f * sn;
+    // Along B1 the angle moves by the flip itself per unit B1: the extra
+    // terms d x = dhcs d - dsn z and d z' = dhsn d + dcs z.
+    const float dal = ev.flip;
+    const float dhcs = -0.5f * sn * dal;
+    const float dhsn = 0.5f * cs * dal;
+    const float dsn = cs * dal;
+    const float dcs = -sn * dal;
+#pragma unroll
+    for (int j = 0; j < SL; ++j) {
+      const int ia = Rot<SL, R>::a(j), ib = Rot<SL, R>::b(j);
+      const float pv = a[ia], mv = b[ib], zv = c[j];
+      float xb = 0.0f, zb = 0.0f;
+      if constexpr ((TM & TAN_B1) != 0) {
+        const float u = pv - mv;
+        xb = fmaf(dhcs, u, -dsn * zv);
+        zb = fmaf(dhsn, u, dcs * zv);
+      }
 #pragma unroll
       for (int k = 0; k < NT; ++k) {
-        shift_pair<SL, L>(ta[k], tb[k], first, last);
-        if (padded) clear_padding<SL>(ta[k], keep);
+        const float dp = ta[k][ia], dm = tb[k][ib], dz = tc[k][j];
+        const float d = dp - dm;
+        const float sum = dp + dm;
+        float x = fmaf(hcs, d, -sn * dz);
+        float z = fmaf(hsn, d, cs * dz);
+        if ((TM & TAN_B1) != 0 && k == SB1) {
+          x += xb;
+          z += zb;
+        }
+        ta[k][ia] = fmaf(0.5f, sum, x);
+        tb[k][ib] = fmaf(0.5f, sum, -x);
+        tc[k][j] = z;
+      }
+      const float d = pv - mv;
+      const float sum = pv + mv;
+      const float x = fmaf(hcs, d, -sn * zv);
+      c[j] = fmaf(hsn, d, cs * zv);
+      a[ia] = fmaf(0.5f, sum, x);
+      b[ib] = fmaf(0.5f, sum, -x);
+    }
+  };
+
+  // The sample i m0 F+[0] and its tangents (reference record,
+  // _epg_cpu.cpp:3090-3120); F+[0] is register R of the first lane.
+  auto record = [&](auto rc, const EventCode& ev) __attribute__((always_inline)) {
+    constexpr int R = decltype(rc)::value;
+    const float a0 = a[R];
+    float t0[NTA];
+#pragma unroll
+    for (int k = 0; k < NTA; ++k) t0[k] = NT > 0 ? ta[k][R] : 0.0f;
+    if (!(first && live)) return;
+    const int64_t o = ev.out;
+    if constexpr (ACCUM) {
+      const float sign = p.sample_sign ? p.sample_sign[o] : 1.0f;
+      const float fr = p.factor_re * sign, fi = p.factor_im * sign;
+      const float v = m0 * a0;
+      const float2 y = p.data[atom * p.outputs + o];
+      const double rx = (double)(-fi * v) - (double)y.x;
+      const double ry = (double)(fr * v) - (double)y.y;
+      acc_cost += rx * rx + ry * ry;
+      // J_c = (-fi d_c, fr d_c): J_c . r = d_c q, J_c . J_k = |factor|^2 d_c d_k
+      const double q = -(double)fi * rx + (double)fr * ry;
+      const double ff = (double)fr * fr + (double)fi * fi;
+      double d[NORMAL_COLUMNS];
+#pragma unroll
+      for (int cc = 0; cc < NORMAL_COLUMNS; ++cc) {
+        float x = 0.0f;
+        if constexpr ((TM & TAN_T1) != 0) x = (p.col_t1 == cc) ? m0 * t0[ST1] : x;
+        if constexpr ((TM & TAN_T2) != 0) x = (p.col_t2 == cc) ? m0 * t0[ST2] : x;
+        if constexpr ((TM & TAN_B1) != 0) x = (p.col_b1 == cc) ? m0 * t0[SB1] : x;
+        x = (p.col_m0 == cc) ? a0 : x;
+        d[cc] = (double)x;
+      }
+      int h = 0;
+#pragma unroll
+      for (int cc = 0; cc < NORMAL_COLUMNS; ++cc) {
+        acc_g[cc] += d[cc] * q;
+#pragma unroll
+        for (int kk = cc; kk < NORMAL_COLUMNS; ++kk) acc_h[h++] += ff * d[cc] * d[kk];
+      }
+    } else {
+      const float v = m0 * a0;
+      if (p.out_real) {
+        p.out_real[atom * p.outputs + o] = 0.0f;
+        p.out_imag[atom * p.outputs + o] = v;
+      }
+      const float sign = p.sample_sign ? p.sample_sign[o] : 1.0f;
+      const float fr = p.factor_re * sign, fi = p.factor_im * sign;
+      // factor * (0 + i v) = (-fi v, fr v)
+      if (p.signal) p.signal[atom * p.outputs + o] = make_float2(-fi * v, fr * v);
+      if (p.jacobian) {
+        float2* row = p.jacobian + atom * (int64_t)p.columns * p.outputs + o;
+        if constexpr ((TM & TAN_T1) != 0) {
+          if (p.col_t1 >= 0) {
+            const float d = m0 * t0[ST1];
+            row[(int64_t)p.col_t1 * p.outputs] = make_float2(-fi * d, fr * d);
+          }
+        }
+        if constexpr ((TM & TAN_T2) != 0) {
+          if (p.col_t2 >= 0) {
+            const float d = m0 * t0[ST2];
+            row[(int64_t)p.col_t2 * p.outputs] = make_float2(-fi * d, fr * d);
+          }
+        }
+        if constexpr ((TM & TAN_B1) != 0) {
+          if (p.col_b1 >= 0) {
+            const float d = m0 * t0[SB1];
+            row[(int64_t)p.col_b1 * p.outputs] = make_float2(-fi * d, fr * d);
+          }
+        }
+        if constexpr ((TM & TAN_IE) != 0) {
+          if (p.col_ie >= 0) {
+            const float d = m0 * t0[SIE];
+            row[(int64_t)p.col_ie * p.outputs] = make_float2(-fi * d, fr * d);
+          }
+        }
+        if (p.col_m0 >= 0) row[(int64_t)p.col_m0 * p.outputs] = make_float2(-fi * a0, fr * a0);
       }
     }
+  };
+
+  // The shift that ends a segment at rotation R.
+  auto shift = [&](auto rc) __attribute__((always_inline)) {
+    constexpr int R = decltype(rc)::value;
+    if constexpr (ROTATE) {
+      constexpr int RN = (R + SL - 1) % SL;
+      shift_rotating<SL, L, R>(a, b, first, last);
+#pragma unroll
+      for (int k = 0; k < NT; ++k) shift_rotating<SL, L, R>(ta[k], tb[k], first, last);
+      if (padded) {
+        clear_order<SL, RN>(a, pad);
+#pragma unroll
+        for (int k = 0; k < NT; ++k) clear_order<SL, RN>(ta[k], pad);
+      }
+    } else {
+      shift_pair<SL, L>(a, b, first, last);
+#pragma unroll
+      for (int k = 0; k < NT; ++k) shift_pair<SL, L>(ta[k], tb[k], first, last);
+      if (padded) {
+        clear_order<SL, 0>(a, pad);
+#pragma unroll
+        for (int k = 0; k < NT; ++k) clear_order<SL, 0>(ta[k], pad);
+      }
+    }
+  };
+
+  // Events from (e, phase) up to and including the next shift, at rotation
+  // R; false once the stream has ended.
+  auto segment = [&](auto rc) __attribute__((always_inline)) -> bool {
+    for (;;) {
+      if (e >= E) return false;
+      const EventCode ev = stream[e];
+      const uint32_t act = ev.ctrl & 0xFFu;
+      const int32_t kind = (int32_t)(ev.ctrl >> CTRL_KIND_SHIFT);
+      if (phase == 0) {
+        if (ev.dt != 0.0f) relax(rc, ev.dt);
+        if (act & PRE_SHIFT) {
+          phase = 1;
+          break;
+        }
+      }
+      if (phase <= 1) {
+        if (kind == KIND_RF) {
+          pulse(rc, ev, act);
+        } else if (kind == KIND_ADC && (act & RECORD) && ev.out >= 0 && ev.out < p.outputs) {
+          record(rc, ev);
+        }
+        if (act & POST_SHIFT) {
+          phase = 2;
+          break;
+        }
+      }
+      phase = 0;
+      ++e;
+      if (act & SPOIL_AFTER) {
+#pragma unroll
+        for (int r = 0; r < SL; ++r) {
+          a[r] = 0.0f;
+          b[r] = 0.0f;
+#pragma unroll
+          for (int k = 0; k < NT; ++k) {
+            ta[k][r] = 0.0f;
+            tb[k][r] = 0.0f;
+          }
+        }
+      } else if (act & SHIFT_AFTER) {
+        break;
+      }
+    }
+    shift(rc);
+    return true;
+  };
+
+  if constexpr (ROTATE) {
+    while (run_rotations<SL>(segment, std::make_integer_sequence<int, SL>{})) {
+    }
+  } else {
+    while (segment(std::integral_constant<int, 0>{})) {
+    }
   }
+
   if constexpr (ACCUM) {
     if (first && live) {
       double* out = p.normal + atom * NORMAL_STRIDE;
       out[0] = acc_cost;
 #pragma unroll
-      for (int c = 0; c < NORMAL_COLUMNS; ++c) out[1 + c] = acc_g[c];
+      for (int cc = 0; cc < NORMAL_COLUMNS; ++cc) out[1 + cc] = acc_g[cc];
 #pragma unroll
       for (int h = 0; h < NORMAL_COLUMNS * (NORMAL_COLUMNS + 1) / 2; ++h) out[1 + NORMAL_COLUMNS + h] = acc_h[h];
     }
   }
 }
-
 
 template <int SL, int L, int TM, bool DIFF, bool B1MAP, bool ACCUM = false>
 __global__ void __launch_bounds__(256) real_kernel(const RealArgs p) {
@@ -460,24 +582,20 @@ BatchLaunch pick_batch_flags(bool diff, bool b1map) {
   }
 }
 
+// Tangent sets come in four kernels: none, (T1, T2), (T1, T2, B1) and all
+// four. A request for a subset runs the smallest superset; the extra
+// directions are carried and never written (their column index is -1).
+__host__ __device__ constexpr int tangent_class(int tm) {
+  return tm == 0 ? 0 : (tm & ~3) == 0 ? 3 : (tm & ~7) == 0 ? 7 : 15;
+}
+
 template <int SL, int L>
 BatchLaunch pick_batch_mask(int tm, bool diff, bool b1map, bool accum) {
-  switch (tm) {
-#define TSIM_BATCH_TM(M)                                                     \
-  case M:                                                                   \
-    return accum ? pick_batch_flags<SL, L, M, true>(diff, b1map)            \
-                 : pick_batch_flags<SL, L, M, false>(diff, b1map);
-    TSIM_BATCH_TM(0)
-    TSIM_BATCH_TM(1)
-    TSIM_BATCH_TM(2)
-    TSIM_BATCH_TM(3)
-    TSIM_BATCH_TM(4)
-    TSIM_BATCH_TM(5)
-    TSIM_BATCH_TM(6)
-    TSIM_BATCH_TM(7)
-#undef TSIM_BATCH_TM
-    default:
-      return accum ? nullptr : pick_batch_flags<SL, L, 15, false>(diff, b1map);
+  switch (tangent_class(tm)) {
+    case 0: return accum ? pick_batch_flags<SL, L, 0, true>(diff, b1map) : pick_batch_flags<SL, L, 0, false>(diff, b1map);
+    case 3: return accum ? pick_batch_flags<SL, L, 3, true>(diff, b1map) : pick_batch_flags<SL, L, 3, false>(diff, b1map);
+    case 7: return accum ? pick_batch_flags<SL, L, 7, true>(diff, b1map) : pick_batch_flags<SL, L, 7, false>(diff, b1map);
+    default: return accum ? nullptr : pick_batch_flags<SL, L, 15, false>(diff, b1map);
   }
 }
 
@@ -500,14 +618,9 @@ RealLaunch pick_normal_b1(bool b1map) {
 
 template <int SL, int L>
 RealLaunch pick_normal_mask(int tm, bool b1map) {
-  switch (tm) {
+  switch (tangent_class(tm)) {
     case 0: return pick_normal_b1<SL, L, 0>(b1map);
-    case 1: return pick_normal_b1<SL, L, 1>(b1map);
-    case 2: return pick_normal_b1<SL, L, 2>(b1map);
     case 3: return pick_normal_b1<SL, L, 3>(b1map);
-    case 4: return pick_normal_b1<SL, L, 4>(b1map);
-    case 5: return pick_normal_b1<SL, L, 5>(b1map);
-    case 6: return pick_normal_b1<SL, L, 6>(b1map);
     default: return pick_normal_b1<SL, L, 7>(b1map);
   }
 }
@@ -524,14 +637,9 @@ RealLaunch pick_diff(bool diff, bool b1map) {
 
 template <int SL, int L>
 RealLaunch pick_mask(int tm, bool diff, bool b1map) {
-  switch (tm) {
+  switch (tangent_class(tm)) {
     case 0: return pick_diff<SL, L, 0>(diff, b1map);
-    case 1: return pick_diff<SL, L, 1>(diff, b1map);
-    case 2: return pick_diff<SL, L, 2>(diff, b1map);
     case 3: return pick_diff<SL, L, 3>(diff, b1map);
-    case 4: return pick_diff<SL, L, 4>(diff, b1map);
-    case 5: return pick_diff<SL, L, 5>(diff, b1map);
-    case 6: return pick_diff<SL, L, 6>(diff, b1map);
     case 7: return pick_diff<SL, L, 7>(diff, b1map);
     default: return pick_diff<SL, L, 15>(diff, b1map);
   }
