@@ -128,11 +128,14 @@ struct Dual {
 // Interval factors of one atom, memoized on the interval.
 struct Relax {
   float dt = -1.0f;
-  float e1 = 1.0f, e2 = 1.0f, de1 = 0.0f, de2 = 0.0f;
+  float e1 = 1.0f, e2 = 1.0f, de1 = 0.0f, de2 = 0.0f, om1 = 0.0f;
   __device__ __forceinline__ void set(float t, float r1, float r2, float k1, float k2) {
     if (t == dt) return;
     dt = t;
-    e1 = expf(-r1 * t);
+    // 1 - e1 from expm1: a rounded e1 would cost it five digits when t << T1
+    const float e1m = expm1f(-r1 * t);
+    e1 = 1.0f + e1m;
+    om1 = -e1m;
     e2 = expf(-r2 * t);
     de1 = e1 * (k1 * t);
     de2 = e2 * (k2 * t);
@@ -159,7 +162,7 @@ __device__ __forceinline__ void relax_forward(Dual<SL, TM>& x, const Relax& f, b
     x.b[r] *= f.e2;
     x.c[r] *= f.e1;
   }
-  x.c[0] += first ? 1.0f - f.e1 : 0.0f;
+  x.c[0] += first ? f.om1 : 0.0f;
   if constexpr ((TM & TAN_T1) != 0) x.tc[ST1][0] -= first ? f.de1 : 0.0f;
 }
 
@@ -312,7 +315,7 @@ __global__ void __launch_bounds__(256) adjoint_kernel(const AdjointArgs p) {
   Relax f;
   for (int64_t e = 0; e < E; ++e) {
     if (e % p.segment == 0) x.store(p.checkpoints + (e / p.segment) * D::floats * T + tid, T);
-    const EventCode ev = p.events[e];
+    const EventCode ev = load_event(p.events, e);
     event_forward<SL, L, TM>(x, ev, f, r1, r2, k1, k2, b1, eta, transmit, first, last, padded, keep);
   }
 
@@ -326,11 +329,11 @@ __global__ void __launch_bounds__(256) adjoint_kernel(const AdjointArgs p) {
     Relax fr;
     for (int64_t e = e0; e < e1; ++e) {
       x.store(p.scratch + (e - e0) * D::floats * T + tid, T);
-      const EventCode ev = p.events[e];
+      const EventCode ev = load_event(p.events, e);
       event_forward<SL, L, TM>(x, ev, fr, r1, r2, k1, k2, b1, eta, transmit, first, last, padded, keep);
     }
     for (int64_t e = e1 - 1; e >= e0; --e) {
-      const EventCode ev = p.events[e];
+      const EventCode ev = load_event(p.events, e);
       const uint32_t act = ev.ctrl & 0xFFu;
       const int32_t kind = (int32_t)(ev.ctrl >> CTRL_KIND_SHIFT);
       float g_flip = 0.0f, g_dt = 0.0f;

@@ -64,6 +64,24 @@ struct alignas(32) EventCode {
   int32_t event;   // position in the packed stream (adjoint bookkeeping)
 };
 
+// Event codes are never written while a kernel runs. Reading them through the
+// constant address space lets the compiler turn a wave-uniform index into
+// scalar loads (one s_load per code) instead of vector loads plus readlanes.
+__device__ __forceinline__ EventCode load_event(const EventCode* codes, int64_t e) {
+  using Word = const __attribute__((address_space(4))) uint32_t;
+  Word* w = (Word*)(codes + e);
+  EventCode ev;
+  ev.dt = __uint_as_float(w[0]);
+  ev.flip = __uint_as_float(w[1]);
+  ev.cosine = __uint_as_float(w[2]);
+  ev.sine = __uint_as_float(w[3]);
+  ev.phase = __uint_as_float(w[4]);
+  ev.ctrl = w[5];
+  ev.out = (int32_t)w[6];
+  ev.event = (int32_t)w[7];
+  return ev;
+}
+
 // Status codes every entry point returns. Zero is success; the message of the
 // last failure on the calling thread is available from tsim_last_error().
 enum Status : int {
@@ -73,16 +91,22 @@ enum Status : int {
   HIP_FAILURE = 3,
 };
 
+struct MacroOp;
+
 // Stream-ordered scratch holding a compiled event stream: allocated and
 // freed on the caller's stream, so calls on distinct streams never share it.
 struct Compiled {
   EventCode* codes = nullptr;
+  MacroOp* ops = nullptr;
   hipStream_t stream = nullptr;
   ~Compiled();
 };
 
 // Compile the reference-format event buffers (packed[17..22]) into EventCodes.
 int compile_packed(void* const* packed, int64_t event_count, hipStream_t stream, Compiled& out);
+
+// Group compiled events into the real kernel's macro-ops (after compile_packed).
+int compile_ops(int64_t event_count, Compiled& out);
 
 void set_error(const char* fmt, ...);
 const char* last_error();

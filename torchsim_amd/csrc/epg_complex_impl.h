@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
   float2 turn = mk(1.0f, 0.0f);   // e^{-2 pi i b0 dt} x transverse damping of this order
   float dlong = 1.0f;             // longitudinal damping of this order
   // single pool
-  float e1 = 1.0f, e2 = 1.0f, de1 = 0.0f, de2 = 0.0f;
+  float e1 = 1.0f, e2 = 1.0f, de1 = 0.0f, de2 = 0.0f, om1 = 0.0f;
   // exchanging pool: transverse X (complex 2x2) and dX/dT2, longitudinal
   // Y (2x2) and dY/dT1, recovery r and dr/dT1
   float2 X[4], dX[4];
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
   const EventCode* __restrict__ stream = p.events;
   const int64_t E = p.event_count;
   for (int64_t e = 0; e < E; ++e) {
-    const EventCode ev = stream[e];
+    const EventCode ev = load_event(stream, e);
     const float dt = ev.dt;
     const uint32_t act = ev.ctrl & 0xFFu;
     const int32_t kind = (int32_t)(ev.ctrl >> CTRL_KIND_SHIFT);
@@ -284,7 +284,10 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
         sincosf(-2.0f * 3.14159265358979323846f * b0 * dt, &s, &c);
         turn = mk(c * dtr, s * dtr);
         if constexpr (!EXCH) {
-          e1 = expf(-r1 * dt);
+          // 1 - e1 from expm1: a rounded e1 would cost it five digits when dt << T1
+          const float e1m = expm1f(-r1 * dt);
+          e1 = 1.0f + e1m;
+          om1 = -e1m;
           e2 = expf(-r2 * dt);
           de1 = e1 * (k1 * dt);
           de2 = e2 * (k2 * dt);
@@ -346,7 +349,7 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
         v.fp[0] = mul(v.fp[0], tp);
         v.fm[0] = mul(v.fm[0], tm);
         v.z[0] = scl(v.z[0], lo);
-        if (sub == 0) v.z[0].x += 1.0f - e1;
+        if (sub == 0) v.z[0].x += om1;
       } else {
         // transverse: F+ <- turn X F+, F- <- conj(turn) conj(X) F-
 #pragma unroll

@@ -90,6 +90,171 @@ int compile_stream(const float* duration, const int32_t* kind, const float* flip
 }
 
 // ---------------------------------------------------------------------------
+// Macro-ops: the event stream regrouped for the real kernel.
+//
+// Each event contributes up to five parts in its own order: a relaxation
+// (dt != 0), a pre-shift, its action (rotation or inversion for an RF event,
+// a record for a recording ADC event), a post-shift, and a spoil or shift
+// after it. A part has a position: relax 0, rotation/inversion/spoil 1
+// (the kernel applies all three as one plane transform), record 2, shift 3. An op is a maximal run of parts with increasing positions
+// that ends at the first position-3 part, so a new op starts wherever a part's
+// position does not exceed its predecessor's, or follows a shift/spoil. This
+// cut rule is local, which makes the grouping a scan: one block, each thread
+// summarising a contiguous chunk of events, a block scan of the summaries,
+// then each thread placing its parts. The parts of one op may come from
+// several threads; each writes its own fields, and the control bits are
+// or-ed in atomically.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr int kPackThreads = 1024;
+
+struct PartSummary {
+  int has;      // any part in the range
+  int first;    // position of its first part
+  int last;     // position of its last part
+  int64_t cuts; // op boundaries strictly inside the range
+};
+
+__device__ __forceinline__ bool cut_between(int prev, int next) { return next <= prev || prev == 3; }
+
+__device__ __forceinline__ PartSummary combine(const PartSummary& x, const PartSummary& y) {
+  if (!x.has) return y;
+  if (!y.has) return x;
+  PartSummary r;
+  r.has = 1;
+  r.first = x.first;
+  r.last = y.last;
+  r.cuts = x.cuts + y.cuts + (cut_between(x.last, y.first) ? 1 : 0);
+  return r;
+}
+
+// Call f(part, position) for each part of event `ev` in order; part is one
+// of OP_RELAX, OP_SHIFT, OP_ROTATE, OP_INVERT, OP_RECORD, OP_SPOIL.
+template <typename F>
+__device__ __forceinline__ void for_each_part(const EventCode& ev, F&& f) {
+  const uint32_t act = ev.ctrl & 0xFFu;
+  const int32_t kind = (int32_t)(ev.ctrl >> CTRL_KIND_SHIFT);
+  if (ev.dt != 0.0f) f(OP_RELAX, 0);
+  if (act & PRE_SHIFT) f(OP_SHIFT, 3);
+  if (kind == KIND_RF) {
+    f((act & INVERSION) ? OP_INVERT : OP_ROTATE, 1);
+  } else if (kind == KIND_ADC && (act & RECORD)) {
+    f(OP_RECORD, 2);
+  }
+  if (act & POST_SHIFT) f(OP_SHIFT, 3);
+  if (act & SPOIL_AFTER) {
+    f(OP_SPOIL, 1);
+  } else if (act & SHIFT_AFTER) {
+    f(OP_SHIFT, 3);
+  }
+}
+
+__global__ void __launch_bounds__(kPackThreads) pack_ops(const EventCode* __restrict__ codes, int64_t events,
+                                                         MacroOp* __restrict__ ops) {
+  __shared__ int s_has[kPackThreads];
+  __shared__ int s_first[kPackThreads];
+  __shared__ int s_last[kPackThreads];
+  __shared__ int64_t s_cuts[kPackThreads];
+  const int t = (int)threadIdx.x;
+  const int64_t chunk = (events + kPackThreads - 1) / kPackThreads;
+  const int64_t lo = chunk * t < events ? chunk * t : events;
+  const int64_t hi = lo + chunk < events ? lo + chunk : events;
+
+  PartSummary mine{0, 0, 0, 0};
+  for (int64_t e = lo; e < hi; ++e) {
+    for_each_part(codes[e], [&](uint32_t, int pos) {
+      if (!mine.has) {
+        mine.has = 1;
+        mine.first = pos;
+      } else if (cut_between(mine.last, pos)) {
+        ++mine.cuts;
+      }
+      mine.last = pos;
+    });
+  }
+  // inclusive Hillis-Steele scan of the summaries
+  s_has[t] = mine.has;
+  s_first[t] = mine.first;
+  s_last[t] = mine.last;
+  s_cuts[t] = mine.cuts;
+  __syncthreads();
+  for (int step = 1; step < kPackThreads; step <<= 1) {
+    PartSummary left{0, 0, 0, 0};
+    if (t >= step) left = PartSummary{s_has[t - step], s_first[t - step], s_last[t - step], s_cuts[t - step]};
+    const PartSummary self{s_has[t], s_first[t], s_last[t], s_cuts[t]};
+    __syncthreads();
+    const PartSummary merged = combine(left, self);
+    s_has[t] = merged.has;
+    s_first[t] = merged.first;
+    s_last[t] = merged.last;
+    s_cuts[t] = merged.cuts;
+    __syncthreads();
+  }
+  PartSummary before{0, 0, 0, 0};
+  if (t > 0) before = PartSummary{s_has[t - 1], s_first[t - 1], s_last[t - 1], s_cuts[t - 1]};
+  const int64_t count = s_has[kPackThreads - 1] ? s_cuts[kPackThreads - 1] + 1 : 0;
+  __syncthreads();
+
+  // clear every op and the end marker's slot, then mark the end
+  for (int64_t i = t; i <= count; i += kPackThreads) {
+    MacroOp z;
+    z.dt = 0.0f;
+    z.flip = 0.0f;
+    z.cosine = 1.0f;
+    z.sine = 0.0f;
+    z.ctrl = i == count ? OP_END : 0u;
+    z.out = -1;
+    z.event = -1;
+    z.pad = 0;
+    ops[i] = z;
+  }
+  __syncthreads();
+
+  int has = before.has;
+  int last = before.last;
+  int64_t index = before.has ? before.cuts : 0;
+  for (int64_t e = lo; e < hi; ++e) {
+    const EventCode ev = codes[e];
+    for_each_part(ev, [&](uint32_t part, int pos) {
+      if (has && cut_between(last, pos)) ++index;
+      has = 1;
+      last = pos;
+      MacroOp* op = ops + index;
+      if (part == OP_RELAX) {
+        op->dt = ev.dt;
+      } else if (part == OP_ROTATE) {
+        op->flip = ev.flip;
+        op->cosine = ev.cosine;
+        op->sine = ev.sine;
+        op->event = ev.event;
+      } else if (part == OP_RECORD) {
+        op->out = ev.out;
+      }
+      atomicOr(&op->ctrl, part);
+    });
+  }
+}
+
+}  // namespace
+
+int compile_ops(int64_t event_count, Compiled& out) {
+  const int64_t capacity = op_capacity(event_count > 0 ? event_count : 0);
+  int st = hip_status(hipMallocAsync((void**)&out.ops, sizeof(MacroOp) * (size_t)capacity, out.stream),
+                      "hipMallocAsync(macro-ops)");
+  if (st != OK) return st;
+  hipLaunchKernelGGL(pack_ops, dim3(1), dim3(kPackThreads), 0, out.stream, out.codes,
+                     event_count > 0 ? event_count : 0, out.ops);
+  return hip_status(hipGetLastError(), "pack_ops launch");
+}
+
+int pack_stream(const EventCode* codes, int64_t events, MacroOp* ops, hipStream_t stream) {
+  hipLaunchKernelGGL(pack_ops, dim3(1), dim3(kPackThreads), 0, stream, codes, events > 0 ? events : 0, ops);
+  return hip_status(hipGetLastError(), "pack_ops launch");
+}
+
+// ---------------------------------------------------------------------------
 // The real-subspace kernel.
 // ---------------------------------------------------------------------------
 
@@ -300,6 +465,7 @@ bool tangent_mask(const int32_t* columns, int64_t column_count, int& tm, int col
 namespace tsim {
 
 Compiled::~Compiled() {
+  if (ops) (void)hipFreeAsync(ops, stream);
   if (codes) (void)hipFreeAsync(codes, stream);
 }
 
@@ -415,7 +581,9 @@ int tsim_simulate(void* const* pointers, int64_t pointer_count, int64_t atom_cou
   Compiled compiled;
   st = compile_packed(pointers, event_count, s, compiled);
   if (st != OK) return st;
-  args.events = compiled.codes;
+  st = compile_ops(event_count, compiled);
+  if (st != OK) return st;
+  args.ops = compiled.ops;
   const bool diff = (features & FEATURE_DIFFUSING) != 0;
   const bool b1map = (features & FEATURE_TRANSMIT) != 0;
   return run_real(args, 0, diff, b1map, -1, s);
@@ -468,7 +636,9 @@ int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t even
   Compiled compiled;
   st = compile_packed(packed, event_count, s, compiled);
   if (st != OK) return st;
-  args.events = compiled.codes;
+  st = compile_ops(event_count, compiled);
+  if (st != OK) return st;
+  args.ops = compiled.ops;
   const bool diff = (features & FEATURE_DIFFUSING) != 0;
   const bool b1map = (features & FEATURE_TRANSMIT) != 0;
   return run_real(args, tm, diff, b1map, tile, s);
@@ -546,7 +716,9 @@ int tsim_gauss_newton_normal(void* const* packed, int64_t atom_count, int64_t ev
   Compiled compiled;
   st = compile_packed(packed, event_count, s, compiled);
   if (st != OK) return st;
-  args.events = compiled.codes;
+  st = compile_ops(event_count, compiled);
+  if (st != OK) return st;
+  args.ops = compiled.ops;
   const int L = kTiles[tile].l;
   const int64_t blocks = (atom_count * L + 255) / 256;
   pick_normal(tile, tm, (features & FEATURE_TRANSMIT) != 0)(args, dim3((unsigned)blocks), dim3(256), s);
@@ -712,17 +884,24 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
       work.push_back(w);
     }
   }
-  // one workspace: compiled events, the request table, the work table
+  // one workspace: compiled events, their macro-ops, the request table, the
+  // work table
+  int64_t total_ops = 0;
+  for (const RealArgs& a : args) total_ops += op_capacity(a.event_count);
   const size_t codes_bytes = sizeof(EventCode) * (size_t)total_events;
+  const size_t ops_bytes = sizeof(MacroOp) * (size_t)total_ops;
   const size_t args_bytes = sizeof(RealArgs) * args.size();
   const size_t work_bytes = sizeof(BatchWork) * work.size();
-  const size_t a_off = (codes_bytes + 255) & ~(size_t)255;
+  const size_t o_off = (codes_bytes + 255) & ~(size_t)255;
+  const size_t a_off = (o_off + ops_bytes + 255) & ~(size_t)255;
   const size_t w_off = (a_off + args_bytes + 255) & ~(size_t)255;
   char* space = nullptr;
   int st = hip_status(hipMallocAsync((void**)&space, w_off + work_bytes, s), "hipMallocAsync(batch workspace)");
   if (st != OK) return st;
   EventCode* codes = (EventCode*)space;
+  MacroOp* ops = (MacroOp*)(space + o_off);
   size_t k = 0;
+  int64_t op_offset = 0;
   for (int64_t r = 0; r < count && st == OK; ++r) {
     const tsim_batch_request& q = requests[r];
     if (q.atom_count == 0 || q.output_count == 0) continue;
@@ -733,7 +912,9 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
                         (const float*)q.packed[TISSUE_COUNT + E_PHASE],
                         (const uint8_t*)q.packed[TISSUE_COUNT + E_ACTION],
                         (const int32_t*)q.packed[TISSUE_COUNT + E_OUTPUT_INDEX], q.event_count, mine, s);
-    args[k].events = mine;
+    if (st == OK) st = pack_stream(mine, q.event_count, ops + op_offset, s);
+    args[k].ops = ops + op_offset;
+    op_offset += op_capacity(q.event_count);
     ++k;
   }
   reap_staged();

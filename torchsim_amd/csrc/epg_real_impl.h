@@ -9,8 +9,54 @@
 
 namespace tsim {
 
+// The real kernel's unit of work: the stream regrouped so that every op is
+// [relax dt] [rotation | inversion | spoil] [record] [shift], each part
+// optional, in that order. A pulse of a gradient-spoiled train (relax TR,
+// pulse, sample, spoiler shift) is one op, whatever number of reference events
+// it was written as. Ops are built on the device by compile_ops from the
+// per-event codes; the last one carries OP_END.
+constexpr uint32_t OP_RELAX = 1;
+constexpr uint32_t OP_ROTATE = 2;
+constexpr uint32_t OP_INVERT = 4;
+constexpr uint32_t OP_RECORD = 8;
+constexpr uint32_t OP_SPOIL = 16;
+constexpr uint32_t OP_SHIFT = 32;
+constexpr uint32_t OP_END = 64;
+
+struct alignas(32) MacroOp {
+  float dt;      // relaxation interval, seconds (OP_RELAX)
+  float flip;    // radians (OP_ROTATE)
+  float cosine;  // cos(flip)
+  float sine;    // sin(flip)
+  uint32_t ctrl;
+  int32_t out;   // output index (OP_RECORD)
+  int32_t event; // packed position of the rotation's event
+  int32_t pad;
+};
+
+// Upper bound on the ops E events compile to (each event contributes at most
+// five parts) plus the end marker.
+inline int64_t op_capacity(int64_t events) { return 5 * events + 1; }
+
+// Ops are read-only for a launch; the constant address space turns the
+// wave-uniform index into scalar loads.
+__device__ __forceinline__ MacroOp load_op(const MacroOp* ops, int64_t i) {
+  using Word = const __attribute__((address_space(4))) uint32_t;
+  Word* w = (Word*)(ops + i);
+  MacroOp m;
+  m.dt = __uint_as_float(w[0]);
+  m.flip = __uint_as_float(w[1]);
+  m.cosine = __uint_as_float(w[2]);
+  m.sine = __uint_as_float(w[3]);
+  m.ctrl = w[4];
+  m.out = (int32_t)w[5];
+  m.event = (int32_t)w[6];
+  m.pad = 0;
+  return m;
+}
+
 struct RealArgs {
-  const EventCode* events;
+  const MacroOp* ops;
   int64_t event_count;
   const float* t1;
   const float* t2;
@@ -178,7 +224,6 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   // after a shift F+ carries a value into it, which the truncation drops.
   const int base = sub * SL;
   const int pad = (p.states - base >= 0 && p.states - base < SL) ? (int)(p.states - base) : SL;
-  const bool padded = SL * L != p.states;
 
   const float t1 = p.t1[atom * p.stride_t1];
   const float t2 = p.t2[atom * p.stride_t2];
@@ -226,33 +271,41 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
     dtr[r] = 1.0f;
   }
 
-  const EventCode* __restrict__ stream = p.events;
-  const int64_t E = p.event_count;
+  const MacroOp* __restrict__ ops = p.ops;
   int64_t e = 0;
-  // Where the event at `e` resumes after the shift that ended a segment:
-  // 0 from its start, 1 after its pre-shift, 2 after its post-shift.
-  int phase = 0;
+
+  // Decay factors over dt, recomputed only when dt changes: e1 = exp(-dt/T1),
+  // e2 = exp(-dt/T2), their T1/T2 derivatives, k2 dt = de2/e2, and the
+  // recovery 1 - e1 on the first lane. The recovery comes from expm1: 1 - e1
+  // formed from a rounded e1 loses about five digits when dt << T1, and that
+  // error recurs identically on every pulse of a steady train.
+  float k2dt = 0.0f, rec = 0.0f;
+  auto decay = [&](float dt) __attribute__((always_inline)) {
+    if (dt == last_dt) return;
+    last_dt = dt;
+    const float e1m = expm1f(-r1 * dt);
+    e1 = 1.0f + e1m;
+    e2 = expf(-r2 * dt);
+    k2dt = k2 * dt;
+    de1 = e1 * (k1 * dt);
+    de2 = e2 * k2dt;
+    rec = first ? -e1m : 0.0f;
+    if constexpr (DIFF) {
+      const float bf = drate * dt;
+#pragma unroll
+      for (int r = 0; r < SL; ++r) {
+        const float l = (float)(base + r);
+        dl[r] = expf(-(l * l) * bf);
+        dtr[r] = expf(-(l * l + l + (1.0f / 3.0f)) * bf);
+      }
+    }
+  };
 
   // Relaxation and recovery over dt (reference relax_real,
   // _epg_cpu.cpp:2930-2975), register map of rotation R.
   auto relax = [&](auto rc, float dt) __attribute__((always_inline)) {
     constexpr int R = decltype(rc)::value;
-    if (dt != last_dt) {
-      last_dt = dt;
-      e1 = expf(-r1 * dt);
-      e2 = expf(-r2 * dt);
-      de1 = e1 * (k1 * dt);
-      de2 = e2 * (k2 * dt);
-      if constexpr (DIFF) {
-        const float bf = drate * dt;
-#pragma unroll
-        for (int r = 0; r < SL; ++r) {
-          const float l = (float)(base + r);
-          dl[r] = expf(-(l * l) * bf);
-          dtr[r] = expf(-(l * l + l + (1.0f / 3.0f)) * bf);
-        }
-      }
-    }
+    decay(dt);
     // Every update below reads a register's old value before it overwrites
     // that register, so the allocator can keep each plane where it lives.
 #pragma unroll
@@ -287,88 +340,119 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
       c[j] *= lo;
     }
     // Recovery toward equilibrium lands on order zero only.
-    c[0] += first ? 1.0f - e1 : 0.0f;
+    c[0] += rec;
     if constexpr ((TM & TAN_T1) != 0) tc[ST1][0] -= first ? de1 : 0.0f;
   };
 
-  // An RF rotation about the x axis (reference rf_real, _epg_cpu.cpp:2980-3005)
-  // or an inversion, register map of rotation R.
-  auto pulse = [&](auto rc, const EventCode& ev, uint32_t act) __attribute__((always_inline)) {
-    constexpr int R = decltype(rc)::value;
-    if (act & INVERSION) {
-      const float eff = -eta;
-#pragma unroll
-      for (int r = 0; r < SL; ++r) {
-#pragma unroll
-        for (int k = 0; k < NT; ++k) {
-          if ((TM & TAN_IE) != 0 && k == SIE) {
-            tc[k][r] = fmaf(tc[k][r], eff, -c[r]);
-          } else {
-            tc[k][r] *= eff;
-          }
-        }
-        c[r] *= eff;
-      }
-      return;
-    }
-    float cs, sn;
+  // cos and sin of the op's flip as this atom plays it.
+  auto angle = [&](const MacroOp& ev, float& cs, float& sn) __attribute__((always_inline)) {
     if constexpr (B1MAP) {
       sincosf(ev.flip * b1, &sn, &cs);
     } else {
       cs = ev.cosine;
       sn = ev.sine;
     }
-    // With s = p + m and d = p - m the rotation reads
-    //   p' = s/2 + x,  m' = s/2 - x,  z' = (sin/2) d + cos z,
-    //   x = (cos/2) d - sin z,
-    // which consumes p and m before it writes either, so every plane is
-    // updated in the registers it lives in.
-    const float hcs = 0.5f * cs;
-    const float hsn = 0.5f * sn;
-    // Along B1 the angle moves by the flip itself per unit B1: the extra
-    // terms d x = dhcs d - dsn z and d z' = dhsn d + dcs z.
-    const float dal = ev.flip;
-    const float dhcs = -0.5f * sn * dal;
-    const float dhsn = 0.5f * cs * dal;
-    const float dsn = cs * dal;
-    const float dcs = -sn * dal;
+  };
+
+  // Everything an op does to the planes before its record: a relaxation over
+  // dt (kernels without diffusion), then a rotation about x (reference
+  // rf_real, _epg_cpu.cpp:2980-3005), an inversion Z -> -eta Z, or a spoil
+  // F+ = F- = 0. Z relaxes first, in place (Z[0] = e1 Z[0] + (1 - e1) with a
+  // single rounding). The transverse decay folds into the rotation, written
+  // with d = p - m as
+  //   u = A d + B z,  p' = E p - u,  m' = E m + u,  z' = C d + D z,
+  // where a rotation by a has A = e2 (1 - cos)/2, B = sin, C = e2 sin/2,
+  // D = cos, E = e2; an inversion A = B = C = 0, D = -eta; a spoil E = A =
+  // B = C = 0, D = 1; a bare relaxation A = B = C = 0, D = 1 (so p' = e2 p,
+  // exactly rounded). The T2 derivative of the transverse decay is applied
+  // before the scale: a k2 dt and b k2 dt on the T2 tangents of F+ and F-
+  // (E multiplies them by e2). Along B1 the rotation angle moves by the flip
+  // per unit B1, which adds bA d + bB z to u and bC d + bD z to z' of the B1
+  // set; along the inversion efficiency z' of that set gains -z. Each plane
+  // register is read before it is written, so every plane is updated where
+  // it lives. With small flips this form keeps p' = p - u close to p, where
+  // a sum/difference form would round p and m together.
+  auto apply = [&](auto rc, const MacroOp& m, uint32_t ctl) __attribute__((always_inline)) {
+    constexpr int R = decltype(rc)::value;
+    const bool relaxing = !DIFF && (ctl & OP_RELAX);
+    if (relaxing) decay(m.dt);
+    const float f1 = relaxing ? e1 : 1.0f;
+    const float f2 = relaxing ? e2 : 1.0f;
+    const float g1 = relaxing ? de1 : 0.0f;
+    const float q2 = relaxing ? k2dt : 0.0f;
+    const float r0 = relaxing ? rec : 0.0f;
+    float cs = 1.0f, sn = 0.0f, dal = 0.0f;
+    if (ctl & OP_ROTATE) {
+      angle(m, cs, sn);
+      dal = m.flip;
+    }
+    const bool spoil = (ctl & OP_SPOIL) != 0;
+    const bool invert = (ctl & OP_INVERT) != 0;
+    const float E = spoil ? 0.0f : f2;
+    const float A = f2 * (0.5f * (1.0f - cs));
+    const float B = sn;
+    const float C = f2 * (0.5f * sn);
+    const float D = invert ? -eta : cs;
+    const float bA = f2 * (0.5f * sn * dal), bB = cs * dal, bC = f2 * (0.5f * cs * dal), bD = -sn * dal;
+
+    // longitudinal relaxation and recovery, and the T1/T2 decay derivatives
+#pragma unroll
+    for (int j = 0; j < SL; ++j) {
+      const int ia = Rot<SL, R>::a(j), ib = Rot<SL, R>::b(j);
+      if constexpr ((TM & TAN_T2) != 0) {
+        ta[ST2][ia] = fmaf(a[ia], q2, ta[ST2][ia]);
+        tb[ST2][ib] = fmaf(b[ib], q2, tb[ST2][ib]);
+      }
+#pragma unroll
+      for (int k = 0; k < NT; ++k) {
+        if ((TM & TAN_T1) != 0 && k == ST1) {
+          tc[k][j] = fmaf(c[j], g1, tc[k][j] * f1);
+        } else {
+          tc[k][j] *= f1;
+        }
+      }
+      c[j] = j == 0 ? fmaf(c[j], f1, r0) : c[j] * f1;
+    }
+    if constexpr ((TM & TAN_T1) != 0) tc[ST1][0] -= first ? g1 : 0.0f;
+
 #pragma unroll
     for (int j = 0; j < SL; ++j) {
       const int ia = Rot<SL, R>::a(j), ib = Rot<SL, R>::b(j);
       const float pv = a[ia], mv = b[ib], zv = c[j];
-      float xb = 0.0f, zb = 0.0f;
+      float ub = 0.0f, zb = 0.0f;
       if constexpr ((TM & TAN_B1) != 0) {
-        const float u = pv - mv;
-        xb = fmaf(dhcs, u, -dsn * zv);
-        zb = fmaf(dhsn, u, dcs * zv);
+        const float dv = pv - mv;
+        ub = fmaf(bA, dv, bB * zv);
+        zb = fmaf(bC, dv, bD * zv);
       }
 #pragma unroll
       for (int k = 0; k < NT; ++k) {
         const float dp = ta[k][ia], dm = tb[k][ib], dz = tc[k][j];
         const float d = dp - dm;
-        const float sum = dp + dm;
-        float x = fmaf(hcs, d, -sn * dz);
-        float z = fmaf(hsn, d, cs * dz);
+        float u = fmaf(A, d, B * dz);
+        float z = fmaf(C, d, D * dz);
         if ((TM & TAN_B1) != 0 && k == SB1) {
-          x += xb;
+          u += ub;
           z += zb;
         }
-        ta[k][ia] = fmaf(0.5f, sum, x);
-        tb[k][ib] = fmaf(0.5f, sum, -x);
+        if ((TM & TAN_IE) != 0 && k == SIE) z -= invert ? zv : 0.0f;
+        // m' before p': u is still live at m', so neither update can take
+        // u's register as its destination
+        tb[k][ib] = fmaf(E, dm, u);
+        ta[k][ia] = fmaf(E, dp, -u);
         tc[k][j] = z;
       }
       const float d = pv - mv;
-      const float sum = pv + mv;
-      const float x = fmaf(hcs, d, -sn * zv);
-      c[j] = fmaf(hsn, d, cs * zv);
-      a[ia] = fmaf(0.5f, sum, x);
-      b[ib] = fmaf(0.5f, sum, -x);
+      const float u = fmaf(A, d, B * zv);
+      c[j] = fmaf(C, d, D * zv);
+      b[ib] = fmaf(E, mv, u);
+      a[ia] = fmaf(E, pv, -u);
     }
   };
 
   // The sample i m0 F+[0] and its tangents (reference record,
   // _epg_cpu.cpp:3090-3120); F+[0] is register R of the first lane.
-  auto record = [&](auto rc, const EventCode& ev) __attribute__((always_inline)) {
+  auto record = [&](auto rc, const MacroOp& ev) __attribute__((always_inline)) {
     constexpr int R = decltype(rc)::value;
     const float a0 = a[R];
     float t0[NTA];
@@ -453,66 +537,43 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
       shift_rotating<SL, L, R>(a, b, first, last);
 #pragma unroll
       for (int k = 0; k < NT; ++k) shift_rotating<SL, L, R>(ta[k], tb[k], first, last);
-      if (padded) {
-        clear_order<SL, RN>(a, pad);
+      clear_order<SL, RN>(a, pad);
 #pragma unroll
-        for (int k = 0; k < NT; ++k) clear_order<SL, RN>(ta[k], pad);
-      }
+      for (int k = 0; k < NT; ++k) clear_order<SL, RN>(ta[k], pad);
     } else {
       shift_pair<SL, L>(a, b, first, last);
 #pragma unroll
       for (int k = 0; k < NT; ++k) shift_pair<SL, L>(ta[k], tb[k], first, last);
-      if (padded) {
-        clear_order<SL, 0>(a, pad);
+      clear_order<SL, 0>(a, pad);
 #pragma unroll
-        for (int k = 0; k < NT; ++k) clear_order<SL, 0>(ta[k], pad);
-      }
+      for (int k = 0; k < NT; ++k) clear_order<SL, 0>(ta[k], pad);
     }
   };
 
-  // Events from (e, phase) up to and including the next shift, at rotation
-  // R; false once the stream has ended.
+  // Ops from e up to and including the next shift, at rotation R; false
+  // once the end marker is reached. The loop has one exit (a flag tells the
+  // end marker from a shift): with two, the compiler re-homes every plane
+  // register at each segment boundary.
   auto segment = [&](auto rc) __attribute__((always_inline)) -> bool {
+    bool ended = false;
     for (;;) {
-      if (e >= E) return false;
-      const EventCode ev = stream[e];
-      const uint32_t act = ev.ctrl & 0xFFu;
-      const int32_t kind = (int32_t)(ev.ctrl >> CTRL_KIND_SHIFT);
-      if (phase == 0) {
-        if (ev.dt != 0.0f) relax(rc, ev.dt);
-        if (act & PRE_SHIFT) {
-          phase = 1;
-          break;
-        }
-      }
-      if (phase <= 1) {
-        if (kind == KIND_RF) {
-          pulse(rc, ev, act);
-        } else if (kind == KIND_ADC && (act & RECORD) && ev.out >= 0 && ev.out < p.outputs) {
-          record(rc, ev);
-        }
-        if (act & POST_SHIFT) {
-          phase = 2;
-          break;
-        }
-      }
-      phase = 0;
+      const MacroOp m = load_op(ops, e);
       ++e;
-      if (act & SPOIL_AFTER) {
-#pragma unroll
-        for (int r = 0; r < SL; ++r) {
-          a[r] = 0.0f;
-          b[r] = 0.0f;
-#pragma unroll
-          for (int k = 0; k < NT; ++k) {
-            ta[k][r] = 0.0f;
-            tb[k][r] = 0.0f;
-          }
-        }
-      } else if (act & SHIFT_AFTER) {
+      const uint32_t ctl = m.ctrl;
+      if (ctl & OP_END) {
+        ended = true;
         break;
       }
+      if constexpr (DIFF) {
+        if (ctl & OP_RELAX) relax(rc, m.dt);
+      }
+      if (ctl & (OP_RELAX | OP_ROTATE | OP_INVERT | OP_SPOIL)) apply(rc, m, ctl);
+      if (ctl & OP_RECORD) {
+        if (m.out >= 0 && m.out < p.outputs) record(rc, m);
+      }
+      if (ctl & OP_SHIFT) break;
     }
+    if (ended) return false;
     shift(rc);
     return true;
   };
