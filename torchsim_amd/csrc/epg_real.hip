@@ -197,14 +197,15 @@ __global__ void __launch_bounds__(kPackThreads) pack_ops(const EventCode* __rest
   const int64_t count = s_has[kPackThreads - 1] ? s_cuts[kPackThreads - 1] + 1 : 0;
   __syncthreads();
 
-  // clear every op and the end marker's slot, then mark the end
-  for (int64_t i = t; i <= count; i += kPackThreads) {
+  // clear every op and the two end-marker slots (the kernel reads one op
+  // past the first), then mark the end
+  for (int64_t i = t; i <= count + 1; i += kPackThreads) {
     MacroOp z;
     z.dt = 0.0f;
     z.flip = 0.0f;
     z.cosine = 1.0f;
     z.sine = 0.0f;
-    z.ctrl = i == count ? OP_END : 0u;
+    z.ctrl = i >= count ? OP_END : 0u;
     z.out = -1;
     z.event = -1;
     z.pad = 0;

@@ -35,8 +35,9 @@ struct alignas(32) MacroOp {
 };
 
 // Upper bound on the ops E events compile to (each event contributes at most
-// five parts) plus the end marker.
-inline int64_t op_capacity(int64_t events) { return 5 * events + 1; }
+// five parts), plus the end marker and one more end marker that the kernel's
+// one-op-ahead load may read.
+inline int64_t op_capacity(int64_t events) { return 5 * events + 2; }
 
 // Ops are read-only for a launch; the constant address space turns the
 // wave-uniform index into scalar loads.
@@ -159,7 +160,7 @@ __device__ __forceinline__ void clear_padding(float (&a)[SL], unsigned keep) {
 // SL-fold code would not fit the instruction cache).
 // ---------------------------------------------------------------------------
 
-constexpr int ROTATE_MAX = 8;
+constexpr int ROTATE_MAX = 12;
 
 template <int SL, int R>
 struct Rot {
@@ -193,6 +194,39 @@ template <int SL, int R>
 __device__ __forceinline__ void clear_order(float (&a)[SL], int pad) {
 #pragma unroll
   for (int j = 0; j < SL; ++j) a[Rot<SL, R>::a(j)] = (pad == j) ? 0.0f : a[Rot<SL, R>::a(j)];
+}
+
+// sin and cos of x for |x| up to a few thousand radians, branch-free (so the
+// scheduler can interleave it with other work). Cody-Waite reduction by
+// pi/2 split in three parts (the first two exact in float for |q| < 2^12),
+// then the Taylor series on [-pi/4, pi/4] to x^11 and x^12, whose truncation
+// error (below 2e-10) is far under float rounding, and the quadrant's swap
+// and signs.
+__device__ __forceinline__ void sincos_reduced(float x, float& sn, float& cs) {
+  constexpr float kTwoOverPi = 0.636619772367581343f;
+  constexpr float kP1 = 1.5703125f;                 // 12 significant bits
+  constexpr float kP2 = 4.83751296997070312e-4f;    // next 12 bits
+  constexpr float kP3 = 7.54978995489188216e-8f;    // remainder of pi/2
+  const float q = rintf(x * kTwoOverPi);
+  float r = fmaf(-q, kP1, x);
+  r = fmaf(-q, kP2, r);
+  r = fmaf(-q, kP3, r);
+  const float r2 = r * r;
+  float ps = fmaf(r2, 2.75573192239858907e-6f, -1.98412698412698413e-4f);
+  ps = fmaf(r2, ps, 8.33333333333333333e-3f);
+  ps = fmaf(r2, ps, -1.66666666666666667e-1f);
+  const float sr = fmaf(r * r2, ps, r);
+  float pc = fmaf(r2, -2.75573192239858907e-7f, 2.48015873015873016e-5f);
+  pc = fmaf(r2, pc, -1.38888888888888889e-3f);
+  pc = fmaf(r2, pc, 4.16666666666666667e-2f);
+  pc = fmaf(r2, pc, -0.5f);
+  const float cr = fmaf(r2, pc, 1.0f);
+  const int iq = (int)q;
+  const bool swap = (iq & 1) != 0;
+  const float s0 = swap ? cr : sr;
+  const float c0 = swap ? sr : cr;
+  sn = (iq & 2) ? -s0 : s0;
+  cs = ((iq + 1) & 2) ? -c0 : c0;
 }
 
 // One turn of the rotation: segments at R = 0, SL-1, ..., 1 (a shift takes R
@@ -261,7 +295,7 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
 #pragma unroll
   for (int i = 0; i < (ACCUM ? NORMAL_COLUMNS * (NORMAL_COLUMNS + 1) / 2 : 1); ++i) acc_h[i] = 0.0;
 
-  float last_dt = -1.0f;
+  uint32_t last_dt = 0xFFFFFFFFu;  // bit pattern of the cached interval (a NaN: never matches)
   float e1 = 1.0f, e2 = 1.0f, de1 = 0.0f, de2 = 0.0f;
   // diffusion damping per logical order of this lane's block
   float dl[DIFF ? SL : 1], dtr[DIFF ? SL : 1];
@@ -272,7 +306,6 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   }
 
   const MacroOp* __restrict__ ops = p.ops;
-  int64_t e = 0;
 
   // Decay factors over dt, recomputed only when dt changes: e1 = exp(-dt/T1),
   // e2 = exp(-dt/T2), their T1/T2 derivatives, k2 dt = de2/e2, and the
@@ -281,8 +314,9 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   // error recurs identically on every pulse of a steady train.
   float k2dt = 0.0f, rec = 0.0f;
   auto decay = [&](float dt) __attribute__((always_inline)) {
-    if (dt == last_dt) return;
-    last_dt = dt;
+    // dt is wave-uniform: compare bit patterns on the scalar unit
+    if (__float_as_uint(dt) == last_dt) return;
+    last_dt = __float_as_uint(dt);
     const float e1m = expm1f(-r1 * dt);
     e1 = 1.0f + e1m;
     e2 = expf(-r2 * dt);
@@ -347,7 +381,7 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   // cos and sin of the op's flip as this atom plays it.
   auto angle = [&](const MacroOp& ev, float& cs, float& sn) __attribute__((always_inline)) {
     if constexpr (B1MAP) {
-      sincosf(ev.flip * b1, &sn, &cs);
+      sincos_reduced(ev.flip * b1, sn, cs);
     } else {
       cs = ev.cosine;
       sn = ev.sine;
@@ -372,7 +406,7 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   // register is read before it is written, so every plane is updated where
   // it lives. With small flips this form keeps p' = p - u close to p, where
   // a sum/difference form would round p and m together.
-  auto apply = [&](auto rc, const MacroOp& m, uint32_t ctl) __attribute__((always_inline)) {
+  auto apply = [&](auto rc, const MacroOp& m, uint32_t ctl, float cs, float sn) __attribute__((always_inline)) {
     constexpr int R = decltype(rc)::value;
     const bool relaxing = !DIFF && (ctl & OP_RELAX);
     if (relaxing) decay(m.dt);
@@ -381,11 +415,7 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
     const float g1 = relaxing ? de1 : 0.0f;
     const float q2 = relaxing ? k2dt : 0.0f;
     const float r0 = relaxing ? rec : 0.0f;
-    float cs = 1.0f, sn = 0.0f, dal = 0.0f;
-    if (ctl & OP_ROTATE) {
-      angle(m, cs, sn);
-      dal = m.flip;
-    }
+    const float dal = (ctl & OP_ROTATE) ? m.flip : 0.0f;
     const bool spoil = (ctl & OP_SPOIL) != 0;
     const bool invert = (ctl & OP_INVERT) != 0;
     const float E = spoil ? 0.0f : f2;
@@ -550,24 +580,36 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
     }
   };
 
-  // Ops from e up to and including the next shift, at rotation R; false
-  // once the end marker is reached. The loop has one exit (a flag tells the
-  // end marker from a shift): with two, the compiler re-homes every plane
-  // register at each segment boundary.
+  // Ops up to and including the next shift, at rotation R; false once the
+  // end marker is reached. The loop has one exit (a flag tells the end
+  // marker from a shift): with two, the compiler re-homes every plane
+  // register at each segment boundary. It runs one op ahead: the next op's
+  // scalar load and its cos/sin are issued before this op's plane work, so
+  // their latency hides behind it.
+  MacroOp next = load_op(ops, 0);
+  int64_t e = 1;
+  float next_cs, next_sn;
+  angle(next, next_cs, next_sn);
   auto segment = [&](auto rc) __attribute__((always_inline)) -> bool {
     bool ended = false;
     for (;;) {
-      const MacroOp m = load_op(ops, e);
-      ++e;
+      const MacroOp m = next;
+      const float cs = next_cs, sn = next_sn;
       const uint32_t ctl = m.ctrl;
       if (ctl & OP_END) {
         ended = true;
         break;
       }
+      next = load_op(ops, e);
+      ++e;
       if constexpr (DIFF) {
         if (ctl & OP_RELAX) relax(rc, m.dt);
       }
-      if (ctl & (OP_RELAX | OP_ROTATE | OP_INVERT | OP_SPOIL)) apply(rc, m, ctl);
+      // every op runs the plane transform (the identity where it has nothing
+      // to apply), and the next op's angle in the same straight-line block;
+      // an op that does not rotate has flip 0, cos 1, sin 0
+      apply(rc, m, ctl, cs, sn);
+      angle(next, next_cs, next_sn);
       if (ctl & OP_RECORD) {
         if (m.out >= 0 && m.out < p.outputs) record(rc, m);
       }
