@@ -274,16 +274,27 @@ constexpr Tile kTiles[] = {TSIM_TILES(TSIM_TILE_ENTRY)};
 #undef TSIM_TILE_ENTRY
 constexpr int kTileCount = sizeof(kTiles) / sizeof(kTiles[0]);
 
-RealLaunch pick_real(int tile, int tm, bool diff, bool b1map) {
+RealLaunch pick_real(int tile, int tm, int variant) {
   switch (tile) {
 #define TSIM_TILE_CASE(ID, SL, L) \
   case ID:                        \
-    return pick_real_tile_##ID(tm, diff, b1map);
+    return pick_real_tile_##ID(tm, variant);
     TSIM_TILES(TSIM_TILE_CASE)
 #undef TSIM_TILE_CASE
     default:
       return nullptr;
   }
+}
+
+// The variant bits of a launch: what the features ask for, and whether the
+// tile carries orders past the state count (which must then be kept zero).
+int variant_of(int features, bool accum, int tile, int64_t states) {
+  int v = 0;
+  if (features & FEATURE_DIFFUSING) v |= V_DIFF;
+  if (features & FEATURE_TRANSMIT) v |= V_B1MAP;
+  if (accum) v |= V_ACCUM;
+  if ((int64_t)kTiles[tile].sl * kTiles[tile].l != states) v |= V_PADDED;
+  return v;
 }
 
 int default_tile(int states, int tangents) {
@@ -308,7 +319,7 @@ int default_tile(int states, int tangents) {
   return best;
 }
 
-int run_real(RealArgs& args, int tm, bool diff, bool b1map, int tile, hipStream_t stream) {
+int run_real(RealArgs& args, int tm, int features, int tile, hipStream_t stream) {
   if (tile < 0) tile = default_tile(args.states, popcount4(tangent_class(tm)));
   if (tile < 0 || tile >= kTileCount) {
     set_error("no state tile covers %d orders (at most %d)", args.states,
@@ -329,30 +340,22 @@ int run_real(RealArgs& args, int tm, bool diff, bool b1map, int tile, hipStream_
     set_error("too many atoms for one launch: %lld", (long long)args.atoms);
     return BAD_ARGUMENT;
   }
-  RealLaunch launch = pick_real(tile, tm, diff, b1map);
+  RealLaunch launch = pick_real(tile, tm, variant_of(features, false, tile, args.states));
+  if (!launch) {
+    set_error("no real kernel for this variant");
+    return UNSUPPORTED;
+  }
   launch(args, dim3((unsigned)blocks), dim3(threads), stream);
   return hip_status(hipGetLastError(), "real_kernel launch");
 }
 
-BatchLaunch pick_batch(int tile, int tm, bool diff, bool b1map, bool accum) {
+BatchLaunch pick_batch(int tile, int tm, int variant) {
   switch (tile) {
 #define TSIM_BATCH_CASE(ID, SL, L) \
   case ID:                         \
-    return pick_batch_tile_##ID(tm, diff, b1map, accum);
+    return pick_batch_tile_##ID(tm, variant);
     TSIM_TILES(TSIM_BATCH_CASE)
 #undef TSIM_BATCH_CASE
-    default:
-      return nullptr;
-  }
-}
-
-RealLaunch pick_normal(int tile, int tm, bool b1map) {
-  switch (tile) {
-#define TSIM_NORMAL_CASE(ID, SL, L) \
-  case ID:                          \
-    return pick_normal_tile_##ID(tm, b1map);
-    TSIM_TILES(TSIM_NORMAL_CASE)
-#undef TSIM_NORMAL_CASE
     default:
       return nullptr;
   }
@@ -585,9 +588,7 @@ int tsim_simulate(void* const* pointers, int64_t pointer_count, int64_t atom_cou
   st = compile_ops(event_count, compiled);
   if (st != OK) return st;
   args.ops = compiled.ops;
-  const bool diff = (features & FEATURE_DIFFUSING) != 0;
-  const bool b1map = (features & FEATURE_TRANSMIT) != 0;
-  return run_real(args, 0, diff, b1map, -1, s);
+  return run_real(args, 0, features, -1, s);
 }
 
 int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
@@ -640,9 +641,7 @@ int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t even
   st = compile_ops(event_count, compiled);
   if (st != OK) return st;
   args.ops = compiled.ops;
-  const bool diff = (features & FEATURE_DIFFUSING) != 0;
-  const bool b1map = (features & FEATURE_TRANSMIT) != 0;
-  return run_real(args, tm, diff, b1map, tile, s);
+  return run_real(args, tm, features, tile, s);
 }
 
 int tsim_tile_count(void) { return kTileCount; }
@@ -722,7 +721,12 @@ int tsim_gauss_newton_normal(void* const* packed, int64_t atom_count, int64_t ev
   args.ops = compiled.ops;
   const int L = kTiles[tile].l;
   const int64_t blocks = (atom_count * L + 255) / 256;
-  pick_normal(tile, tm, (features & FEATURE_TRANSMIT) != 0)(args, dim3((unsigned)blocks), dim3(256), s);
+  RealLaunch launch = pick_real(tile, tm, variant_of(features, true, tile, args.states));
+  if (!launch) {
+    set_error("no normal-equation kernel for this variant");
+    return UNSUPPORTED;
+  }
+  launch(args, dim3((unsigned)blocks), dim3(256), s);
   return hip_status(hipGetLastError(), "normal-equation kernel launch");
 }
 
@@ -945,7 +949,11 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
     const int64_t blocks = (int64_t)work.size();
     int64_t grid = grid_blocks > 0 ? grid_blocks : blocks;
     if (grid > blocks) grid = blocks;
-    BatchLaunch launch = pick_batch(tile, key_tm, key_diff != 0, key_b1 != 0, mode == 1);
+    int variant = (key_diff ? V_DIFF : 0) | (key_b1 ? V_B1MAP : 0) | (mode == 1 ? V_ACCUM : 0);
+    for (const RealArgs& a : args) {
+      if ((int64_t)kTiles[tile].sl * kTiles[tile].l != a.states) variant |= V_PADDED;
+    }
+    BatchLaunch launch = pick_batch(tile, key_tm, variant);
     if (!launch) {
       set_error("no batch kernel for this variant");
       st = UNSUPPORTED;

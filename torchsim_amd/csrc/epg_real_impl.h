@@ -238,8 +238,24 @@ __device__ __forceinline__ bool run_rotations(F& f, std::integer_sequence<int, I
 
 // One atom group's walk of its event stream. `group` is the atom this lane's
 // group stands for (lanes past the last atom run clamped and write nothing).
-template <int SL, int L, int TM, bool DIFF, bool B1MAP, bool ACCUM>
+// Kernel variant bits (the fourth template argument of the real kernels).
+constexpr int V_DIFF = 1;    // diffusion damps each order by its own factor
+constexpr int V_B1MAP = 2;   // a per-atom B1 scales every flip
+constexpr int V_ACCUM = 4;   // accumulate normal equations instead of writing outputs
+constexpr int V_PADDED = 8;  // the tile covers more orders than the state count
+
+// Normal equations carry at most (T1, T2, B1) and no diffusion.
+template <int TM, int V>
+constexpr bool valid_variant() {
+  return !((V & V_ACCUM) && ((V & V_DIFF) || TM == 15));
+}
+
+template <int SL, int L, int TM, int V>
 __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
+  constexpr bool DIFF = (V & V_DIFF) != 0;
+  constexpr bool B1MAP = (V & V_B1MAP) != 0;
+  constexpr bool ACCUM = (V & V_ACCUM) != 0;
+  constexpr bool PADDED = (V & V_PADDED) != 0;
   constexpr int NT = popcount4(TM);
   constexpr int NTA = NT > 0 ? NT : 1;
   constexpr int ST1 = slot_of(TM, TAN_T1);
@@ -567,16 +583,20 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
       shift_rotating<SL, L, R>(a, b, first, last);
 #pragma unroll
       for (int k = 0; k < NT; ++k) shift_rotating<SL, L, R>(ta[k], tb[k], first, last);
-      clear_order<SL, RN>(a, pad);
+      if constexpr (PADDED) {
+        clear_order<SL, RN>(a, pad);
 #pragma unroll
-      for (int k = 0; k < NT; ++k) clear_order<SL, RN>(ta[k], pad);
+        for (int k = 0; k < NT; ++k) clear_order<SL, RN>(ta[k], pad);
+      }
     } else {
       shift_pair<SL, L>(a, b, first, last);
 #pragma unroll
       for (int k = 0; k < NT; ++k) shift_pair<SL, L>(ta[k], tb[k], first, last);
-      clear_order<SL, 0>(a, pad);
+      if constexpr (PADDED) {
+        clear_order<SL, 0>(a, pad);
 #pragma unroll
-      for (int k = 0; k < NT; ++k) clear_order<SL, 0>(ta[k], pad);
+        for (int k = 0; k < NT; ++k) clear_order<SL, 0>(ta[k], pad);
+      }
     }
   };
 
@@ -640,10 +660,10 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   }
 }
 
-template <int SL, int L, int TM, bool DIFF, bool B1MAP, bool ACCUM = false>
+template <int SL, int L, int TM, int V>
 __global__ void __launch_bounds__(256) real_kernel(const RealArgs p) {
   const int64_t group = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L;
-  real_body<SL, L, TM, DIFF, B1MAP, ACCUM>(p, group);
+  real_body<SL, L, TM, V>(p, group);
 }
 
 // The uneven-batch launch: a table of requests, each with its own event
@@ -657,32 +677,28 @@ struct BatchWork {
   int64_t first_atom;
 };
 
-template <int SL, int L, int TM, bool DIFF, bool B1MAP, bool ACCUM = false>
+template <int SL, int L, int TM, int V>
 __global__ void __launch_bounds__(256) real_batch_kernel(const RealArgs* __restrict__ requests,
                                                         const BatchWork* __restrict__ work, int64_t blocks) {
   for (int64_t b = blockIdx.x; b < blocks; b += gridDim.x) {
     const BatchWork w = work[b];
     const RealArgs p = requests[w.request];
     const int64_t group = w.first_atom + (int64_t)threadIdx.x / L;
-    real_body<SL, L, TM, DIFF, B1MAP, ACCUM>(p, group);
+    real_body<SL, L, TM, V>(p, group);
   }
 }
 
+using RealLaunch = void (*)(const RealArgs&, dim3, dim3, hipStream_t);
 using BatchLaunch = void (*)(const RealArgs*, const BatchWork*, int64_t, dim3, dim3, hipStream_t);
 
-template <int SL, int L, int TM, bool DIFF, bool B1MAP, bool ACCUM>
-void launch_batch(const RealArgs* r, const BatchWork* w, int64_t blocks, dim3 grid, dim3 block, hipStream_t s) {
-  hipLaunchKernelGGL((real_batch_kernel<SL, L, TM, DIFF, B1MAP, ACCUM>), grid, block, 0, s, r, w, blocks);
+template <int SL, int L, int TM, int V>
+void launch_real(const RealArgs& args, dim3 grid, dim3 block, hipStream_t stream) {
+  hipLaunchKernelGGL((real_kernel<SL, L, TM, V>), grid, block, 0, stream, args);
 }
 
-template <int SL, int L, int TM, bool ACCUM>
-BatchLaunch pick_batch_flags(bool diff, bool b1map) {
-  if constexpr (ACCUM) {
-    return b1map ? &launch_batch<SL, L, TM, false, true, true> : &launch_batch<SL, L, TM, false, false, true>;
-  } else {
-    if (diff) return b1map ? &launch_batch<SL, L, TM, true, true, false> : &launch_batch<SL, L, TM, true, false, false>;
-    return b1map ? &launch_batch<SL, L, TM, false, true, false> : &launch_batch<SL, L, TM, false, false, false>;
-  }
+template <int SL, int L, int TM, int V>
+void launch_batch(const RealArgs* r, const BatchWork* w, int64_t blocks, dim3 grid, dim3 block, hipStream_t s) {
+  hipLaunchKernelGGL((real_batch_kernel<SL, L, TM, V>), grid, block, 0, s, r, w, blocks);
 }
 
 // Tangent sets come in four kernels: none, (T1, T2), (T1, T2, B1) and all
@@ -692,59 +708,51 @@ __host__ __device__ constexpr int tangent_class(int tm) {
   return tm == 0 ? 0 : (tm & ~3) == 0 ? 3 : (tm & ~7) == 0 ? 7 : 15;
 }
 
-template <int SL, int L>
-BatchLaunch pick_batch_mask(int tm, bool diff, bool b1map, bool accum) {
-  switch (tangent_class(tm)) {
-    case 0: return accum ? pick_batch_flags<SL, L, 0, true>(diff, b1map) : pick_batch_flags<SL, L, 0, false>(diff, b1map);
-    case 3: return accum ? pick_batch_flags<SL, L, 3, true>(diff, b1map) : pick_batch_flags<SL, L, 3, false>(diff, b1map);
-    case 7: return accum ? pick_batch_flags<SL, L, 7, true>(diff, b1map) : pick_batch_flags<SL, L, 7, false>(diff, b1map);
-    default: return accum ? nullptr : pick_batch_flags<SL, L, 15, false>(diff, b1map);
+// The launcher of variant v (a run-time value) among the 16 compiled ones;
+// null for the combinations no kernel carries.
+template <int SL, int L, int TM, int V = 0>
+RealLaunch pick_real_variant(int v) {
+  if constexpr (V < 16) {
+    if (v == V) {
+      if constexpr (valid_variant<TM, V>()) return &launch_real<SL, L, TM, V>;
+      return nullptr;
+    }
+    return pick_real_variant<SL, L, TM, V + 1>(v);
+  } else {
+    return nullptr;
   }
 }
 
-using RealLaunch = void (*)(const RealArgs&, dim3, dim3, hipStream_t);
-
-template <int SL, int L, int TM, bool DIFF, bool B1MAP>
-void launch_real(const RealArgs& args, dim3 grid, dim3 block, hipStream_t stream) {
-  hipLaunchKernelGGL((real_kernel<SL, L, TM, DIFF, B1MAP>), grid, block, 0, stream, args);
-}
-
-template <int SL, int L, int TM, bool B1MAP>
-void launch_normal(const RealArgs& args, dim3 grid, dim3 block, hipStream_t stream) {
-  hipLaunchKernelGGL((real_kernel<SL, L, TM, false, B1MAP, true>), grid, block, 0, stream, args);
-}
-
-template <int SL, int L, int TM>
-RealLaunch pick_normal_b1(bool b1map) {
-  return b1map ? &launch_normal<SL, L, TM, true> : &launch_normal<SL, L, TM, false>;
-}
-
-template <int SL, int L>
-RealLaunch pick_normal_mask(int tm, bool b1map) {
-  switch (tangent_class(tm)) {
-    case 0: return pick_normal_b1<SL, L, 0>(b1map);
-    case 3: return pick_normal_b1<SL, L, 3>(b1map);
-    default: return pick_normal_b1<SL, L, 7>(b1map);
+template <int SL, int L, int TM, int V = 0>
+BatchLaunch pick_batch_variant(int v) {
+  if constexpr (V < 16) {
+    if (v == V) {
+      if constexpr (valid_variant<TM, V>()) return &launch_batch<SL, L, TM, V>;
+      return nullptr;
+    }
+    return pick_batch_variant<SL, L, TM, V + 1>(v);
+  } else {
+    return nullptr;
   }
 }
 
-template <int SL, int L, int TM, bool DIFF>
-RealLaunch pick_b1(bool b1map) {
-  return b1map ? &launch_real<SL, L, TM, DIFF, true> : &launch_real<SL, L, TM, DIFF, false>;
-}
-
-template <int SL, int L, int TM>
-RealLaunch pick_diff(bool diff, bool b1map) {
-  return diff ? pick_b1<SL, L, TM, true>(b1map) : pick_b1<SL, L, TM, false>(b1map);
+template <int SL, int L>
+RealLaunch pick_real_kernel(int tm, int v) {
+  switch (tangent_class(tm)) {
+    case 0: return pick_real_variant<SL, L, 0>(v);
+    case 3: return pick_real_variant<SL, L, 3>(v);
+    case 7: return pick_real_variant<SL, L, 7>(v);
+    default: return pick_real_variant<SL, L, 15>(v);
+  }
 }
 
 template <int SL, int L>
-RealLaunch pick_mask(int tm, bool diff, bool b1map) {
+BatchLaunch pick_batch_kernel(int tm, int v) {
   switch (tangent_class(tm)) {
-    case 0: return pick_diff<SL, L, 0>(diff, b1map);
-    case 3: return pick_diff<SL, L, 3>(diff, b1map);
-    case 7: return pick_diff<SL, L, 7>(diff, b1map);
-    default: return pick_diff<SL, L, 15>(diff, b1map);
+    case 0: return pick_batch_variant<SL, L, 0>(v);
+    case 3: return pick_batch_variant<SL, L, 3>(v);
+    case 7: return pick_batch_variant<SL, L, 7>(v);
+    default: return pick_batch_variant<SL, L, 15>(v);
   }
 }
 
@@ -765,10 +773,9 @@ RealLaunch pick_mask(int tm, bool diff, bool b1map) {
   X(12, 12, 4)        \
   X(13, 16, 4)
 
-#define TSIM_DECLARE_TILE(ID, SL, L)                                   \
-  RealLaunch pick_real_tile_##ID(int tm, bool diff, bool b1map); \
-  RealLaunch pick_normal_tile_##ID(int tm, bool b1map);          \
-  BatchLaunch pick_batch_tile_##ID(int tm, bool diff, bool b1map, bool accum);
+#define TSIM_DECLARE_TILE(ID, SL, L)                \
+  RealLaunch pick_real_tile_##ID(int tm, int variant); \
+  BatchLaunch pick_batch_tile_##ID(int tm, int variant);
 TSIM_TILES(TSIM_DECLARE_TILE)
 #undef TSIM_DECLARE_TILE
 

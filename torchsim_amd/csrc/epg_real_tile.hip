@@ -12,16 +12,12 @@
 
 namespace tsim {
 
-RealLaunch TSIM_CAT(pick_real_tile_, TSIM_TILE_ID)(int tm, bool diff, bool b1map) {
-  return pick_mask<TSIM_TILE_SL, TSIM_TILE_L>(tm, diff, b1map);
+RealLaunch TSIM_CAT(pick_real_tile_, TSIM_TILE_ID)(int tm, int variant) {
+  return pick_real_kernel<TSIM_TILE_SL, TSIM_TILE_L>(tm, variant);
 }
 
-RealLaunch TSIM_CAT(pick_normal_tile_, TSIM_TILE_ID)(int tm, bool b1map) {
-  return pick_normal_mask<TSIM_TILE_SL, TSIM_TILE_L>(tm, b1map);
-}
-
-BatchLaunch TSIM_CAT(pick_batch_tile_, TSIM_TILE_ID)(int tm, bool diff, bool b1map, bool accum) {
-  return pick_batch_mask<TSIM_TILE_SL, TSIM_TILE_L>(tm, diff, b1map, accum);
+BatchLaunch TSIM_CAT(pick_batch_tile_, TSIM_TILE_ID)(int tm, int variant) {
+  return pick_batch_kernel<TSIM_TILE_SL, TSIM_TILE_L>(tm, variant);
 }
 
 }  // namespace tsim
