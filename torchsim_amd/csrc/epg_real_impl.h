@@ -196,6 +196,19 @@ __device__ __forceinline__ void clear_order(float (&a)[SL], int pad) {
   for (int j = 0; j < SL; ++j) a[Rot<SL, R>::a(j)] = (pad == j) ? 0.0f : a[Rot<SL, R>::a(j)];
 }
 
+// e * m + u and e * p - u, computed in the register of m (of p). Written
+// with fmaf the compiler picks the two-address v_fmac with u as its
+// accumulator wherever u dies, and the result then has to be copied back
+// into the plane's register before the next op.
+__device__ __forceinline__ float scale_add_in_place(float m, float e, float u) {
+  asm("v_fma_f32 %0, %0, %1, %2" : "+v"(m) : "v"(e), "v"(u));
+  return m;
+}
+__device__ __forceinline__ float scale_sub_in_place(float p, float e, float u) {
+  asm("v_fma_f32 %0, %0, %1, -%2" : "+v"(p) : "v"(e), "v"(u));
+  return p;
+}
+
 // sin and cos of x for |x| up to a few thousand radians, branch-free (so the
 // scheduler can interleave it with other work). Cody-Waite reduction by
 // pi/2 split in three parts (the first two exact in float for |q| < 2^12),
@@ -484,15 +497,15 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
         if ((TM & TAN_IE) != 0 && k == SIE) z -= invert ? zv : 0.0f;
         // m' before p': u is still live at m', so neither update can take
         // u's register as its destination
-        tb[k][ib] = fmaf(E, dm, u);
-        ta[k][ia] = fmaf(E, dp, -u);
+        tb[k][ib] = scale_add_in_place(dm, E, u);
+        ta[k][ia] = scale_sub_in_place(dp, E, u);
         tc[k][j] = z;
       }
       const float d = pv - mv;
       const float u = fmaf(A, d, B * zv);
       c[j] = fmaf(C, d, D * zv);
-      b[ib] = fmaf(E, mv, u);
-      a[ia] = fmaf(E, pv, -u);
+      b[ib] = scale_add_in_place(mv, E, u);
+      a[ia] = scale_sub_in_place(pv, E, u);
     }
   };
 
