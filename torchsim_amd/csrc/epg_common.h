@@ -106,7 +106,7 @@ struct Compiled {
 int compile_packed(void* const* packed, int64_t event_count, hipStream_t stream, Compiled& out);
 
 // Group compiled events into the real kernel's macro-ops (after compile_packed).
-int compile_ops(int64_t event_count, Compiled& out);
+int compile_ops(int64_t event_count, int64_t outputs, Compiled& out);
 
 void set_error(const char* fmt, ...);
 const char* last_error();

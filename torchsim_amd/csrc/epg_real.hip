@@ -152,19 +152,23 @@ __device__ __forceinline__ void for_each_part(const EventCode& ev, F&& f) {
 }
 
 __global__ void __launch_bounds__(kPackThreads) pack_ops(const EventCode* __restrict__ codes, int64_t events,
-                                                         MacroOp* __restrict__ ops) {
+                                                         int64_t outputs, MacroOp* __restrict__ ops) {
   __shared__ int s_has[kPackThreads];
   __shared__ int s_first[kPackThreads];
   __shared__ int s_last[kPackThreads];
   __shared__ int64_t s_cuts[kPackThreads];
+  __shared__ int64_t s_recs[kPackThreads];
+  __shared__ int s_sequential;
   const int t = (int)threadIdx.x;
   const int64_t chunk = (events + kPackThreads - 1) / kPackThreads;
   const int64_t lo = chunk * t < events ? chunk * t : events;
   const int64_t hi = lo + chunk < events ? lo + chunk : events;
+  if (t == 0) s_sequential = 1;
 
   PartSummary mine{0, 0, 0, 0};
+  int64_t my_recs = 0;
   for (int64_t e = lo; e < hi; ++e) {
-    for_each_part(codes[e], [&](uint32_t, int pos) {
+    for_each_part(codes[e], [&](uint32_t part, int pos) {
       if (!mine.has) {
         mine.has = 1;
         mine.first = pos;
@@ -172,8 +176,19 @@ __global__ void __launch_bounds__(kPackThreads) pack_ops(const EventCode* __rest
         ++mine.cuts;
       }
       mine.last = pos;
+      if (part == OP_RECORD) ++my_recs;
     });
   }
+  // exclusive scan of the record counts (the ordinal of each thread's first record)
+  s_recs[t] = my_recs;
+  __syncthreads();
+  for (int step = 1; step < kPackThreads; step <<= 1) {
+    const int64_t left = t >= step ? s_recs[t - step] : 0;
+    __syncthreads();
+    s_recs[t] += left;
+    __syncthreads();
+  }
+  int64_t record = s_recs[t] - my_recs;
   // inclusive Hillis-Steele scan of the summaries
   s_has[t] = mine.has;
   s_first[t] = mine.first;
@@ -232,27 +247,57 @@ __global__ void __launch_bounds__(kPackThreads) pack_ops(const EventCode* __rest
         op->event = ev.event;
       } else if (part == OP_RECORD) {
         op->out = ev.out;
+        // staged writes need the k-th record to fill output k
+        if ((int64_t)ev.out != record || ev.out < 0 || (int64_t)ev.out >= outputs) s_sequential = 0;
+        ++record;
       }
       atomicOr(&op->ctrl, part);
     });
   }
+  __syncthreads();
+  if (t == 0 && count > 0) ops[0].pad = s_sequential ? (int32_t)OPS_SEQUENTIAL : 0;
 }
 
 }  // namespace
 
-int compile_ops(int64_t event_count, Compiled& out) {
+int compile_ops(int64_t event_count, int64_t outputs, Compiled& out) {
   const int64_t capacity = op_capacity(event_count > 0 ? event_count : 0);
   int st = hip_status(hipMallocAsync((void**)&out.ops, sizeof(MacroOp) * (size_t)capacity, out.stream),
                       "hipMallocAsync(macro-ops)");
   if (st != OK) return st;
   hipLaunchKernelGGL(pack_ops, dim3(1), dim3(kPackThreads), 0, out.stream, out.codes,
-                     event_count > 0 ? event_count : 0, out.ops);
+                     event_count > 0 ? event_count : 0, outputs, out.ops);
   return hip_status(hipGetLastError(), "pack_ops launch");
 }
 
-int pack_stream(const EventCode* codes, int64_t events, MacroOp* ops, hipStream_t stream) {
-  hipLaunchKernelGGL(pack_ops, dim3(1), dim3(kPackThreads), 0, stream, codes, events > 0 ? events : 0, ops);
+int pack_stream(const EventCode* codes, int64_t events, int64_t outputs, MacroOp* ops, hipStream_t stream) {
+  hipLaunchKernelGGL(pack_ops, dim3(1), dim3(kPackThreads), 0, stream, codes, events > 0 ? events : 0, outputs,
+                     ops);
   return hip_status(hipGetLastError(), "pack_ops launch");
+}
+
+// Output staging for a launch writing complex signal / Jacobian rows: the
+// log2 of the samples per flush (K) and of the padded stream count, sized
+// so one wave's stage fits STAGE_BYTES_PER_WAVE; K < 2 writes directly.
+void set_staging(RealArgs& args, int lanes_per_atom) {
+  args.stage_k_log2 = -1;
+  args.stage_s_log2 = 0;
+  if (args.out_real != nullptr || (args.signal == nullptr && args.jacobian == nullptr)) return;
+  const int streams = 1 + args.columns;
+  int s_log2 = 0;
+  while ((1 << s_log2) < streams) ++s_log2;
+  const int64_t slot_bytes = (int64_t)(64 / lanes_per_atom) * (1 << s_log2) * (int64_t)sizeof(float2);
+  int k_log2 = 0;
+  while (k_log2 < 4 && slot_bytes * (2 << k_log2) <= STAGE_BYTES_PER_WAVE) ++k_log2;
+  if (k_log2 < 1) return;
+  args.stage_k_log2 = k_log2;
+  args.stage_s_log2 = s_log2;
+}
+
+// Dynamic LDS bytes of a 256-thread block for the staging above.
+size_t stage_bytes(const RealArgs& args, int lanes_per_atom) {
+  if (args.stage_k_log2 < 1) return 0;
+  return (size_t)4 * (64 / lanes_per_atom) * ((size_t)1 << (args.stage_s_log2 + args.stage_k_log2)) * sizeof(float2);
 }
 
 // ---------------------------------------------------------------------------
@@ -345,7 +390,8 @@ int run_real(RealArgs& args, int tm, int features, int tile, hipStream_t stream)
     set_error("no real kernel for this variant");
     return UNSUPPORTED;
   }
-  launch(args, dim3((unsigned)blocks), dim3(threads), stream);
+  set_staging(args, L);
+  launch(args, dim3((unsigned)blocks), dim3(threads), stage_bytes(args, L), stream);
   return hip_status(hipGetLastError(), "real_kernel launch");
 }
 
@@ -517,6 +563,7 @@ int fill_real_args(RealArgs& args, void* const* packed, int64_t atom_count, int6
     return BAD_ARGUMENT;
   }
   memset(&args, 0, sizeof(args));
+  args.stage_k_log2 = -1;
   args.event_count = event_count;
   args.t1 = (const float*)packed[T_T1];
   args.t2 = (const float*)packed[T_T2];
@@ -585,7 +632,7 @@ int tsim_simulate(void* const* pointers, int64_t pointer_count, int64_t atom_cou
   Compiled compiled;
   st = compile_packed(pointers, event_count, s, compiled);
   if (st != OK) return st;
-  st = compile_ops(event_count, compiled);
+  st = compile_ops(event_count, output_count, compiled);
   if (st != OK) return st;
   args.ops = compiled.ops;
   return run_real(args, 0, features, -1, s);
@@ -638,7 +685,7 @@ int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t even
   Compiled compiled;
   st = compile_packed(packed, event_count, s, compiled);
   if (st != OK) return st;
-  st = compile_ops(event_count, compiled);
+  st = compile_ops(event_count, output_count, compiled);
   if (st != OK) return st;
   args.ops = compiled.ops;
   return run_real(args, tm, features, tile, s);
@@ -716,7 +763,7 @@ int tsim_gauss_newton_normal(void* const* packed, int64_t atom_count, int64_t ev
   Compiled compiled;
   st = compile_packed(packed, event_count, s, compiled);
   if (st != OK) return st;
-  st = compile_ops(event_count, compiled);
+  st = compile_ops(event_count, output_count, compiled);
   if (st != OK) return st;
   args.ops = compiled.ops;
   const int L = kTiles[tile].l;
@@ -726,7 +773,8 @@ int tsim_gauss_newton_normal(void* const* packed, int64_t atom_count, int64_t ev
     set_error("no normal-equation kernel for this variant");
     return UNSUPPORTED;
   }
-  launch(args, dim3((unsigned)blocks), dim3(256), s);
+  args.stage_k_log2 = -1;
+  launch(args, dim3((unsigned)blocks), dim3(256), 0, s);
   return hip_status(hipGetLastError(), "normal-equation kernel launch");
 }
 
@@ -879,6 +927,16 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
   }
   const int L = kTiles[tile].l;
   const int64_t per_block = 256 / L;
+  size_t shared = 0;
+  for (RealArgs& a : args) {
+    if (mode == 1) {
+      a.stage_k_log2 = -1;
+    } else {
+      set_staging(a, L);
+    }
+    const size_t bytes = stage_bytes(a, L);
+    shared = bytes > shared ? bytes : shared;
+  }
   std::vector<BatchWork> work;
   for (size_t r = 0; r < args.size(); ++r) {
     for (int64_t first = 0; first < args[r].atoms; first += per_block) {
@@ -917,7 +975,7 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
                         (const float*)q.packed[TISSUE_COUNT + E_PHASE],
                         (const uint8_t*)q.packed[TISSUE_COUNT + E_ACTION],
                         (const int32_t*)q.packed[TISSUE_COUNT + E_OUTPUT_INDEX], q.event_count, mine, s);
-    if (st == OK) st = pack_stream(mine, q.event_count, ops + op_offset, s);
+    if (st == OK) st = pack_stream(mine, q.event_count, q.output_count, ops + op_offset, s);
     args[k].ops = ops + op_offset;
     op_offset += op_capacity(q.event_count);
     ++k;
@@ -959,7 +1017,7 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
       st = UNSUPPORTED;
     } else {
       launch((const RealArgs*)(space + a_off), (const BatchWork*)(space + w_off), blocks, dim3((unsigned)grid),
-             dim3(256), s);
+             dim3(256), shared, s);
       st = hip_status(hipGetLastError(), "real_batch_kernel launch");
     }
   }

@@ -31,7 +31,7 @@ struct alignas(32) MacroOp {
   uint32_t ctrl;
   int32_t out;   // output index (OP_RECORD)
   int32_t event; // packed position of the rotation's event
-  int32_t pad;
+  int32_t pad;   // op 0: stream flags (OPS_SEQUENTIAL)
 };
 
 // Upper bound on the ops E events compile to (each event contributes at most
@@ -52,7 +52,7 @@ __device__ __forceinline__ MacroOp load_op(const MacroOp* ops, int64_t i) {
   m.ctrl = w[4];
   m.out = (int32_t)w[5];
   m.event = (int32_t)w[6];
-  m.pad = 0;
+  m.pad = (int32_t)w[7];
   return m;
 }
 
@@ -88,7 +88,22 @@ struct RealArgs {
   // (cost, gradient[4], upper triangle of J^T J [10]) in FP64.
   const float2* data;  // (atoms, outputs) complex64
   double* normal;      // (atoms, NORMAL_STRIDE)
+  // Staged output writes: log2 of the pulses buffered per flush and of the
+  // padded stream count (signal + columns); stage_k_log2 < 0 writes each
+  // sample directly.
+  int stage_k_log2;
+  int stage_s_log2;
 };
+
+// Staged output writes. Storing one complex sample per atom per pulse puts
+// every atom of a wave on its own cache line (the rows of the (atom, output)
+// layout are a whole row apart), so lines leave L2 part-written and HBM sees
+// several times the bytes. Instead each wave keeps its atoms' last K samples
+// of every stream in LDS and writes them every K pulses, K consecutive
+// samples of one row together. This needs the records to arrive in output
+// order 0, 1, 2, ... (the op compiler checks it and marks the stream).
+constexpr int STAGE_BYTES_PER_WAVE = 8192;
+constexpr uint32_t OPS_SEQUENTIAL = 1;  // MacroOp[0].pad: records come in output order
 
 // Accumulators one atom hands back in normal-equation mode: the cost
 // sum |model - data|^2, J^T r (4), and the upper triangle of J^T J (10),
@@ -509,6 +524,54 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
     }
   };
 
+  // Output staging (see STAGE_BYTES_PER_WAVE). `nrec` counts the records so
+  // far; every stage_k of them the wave writes its stage out.
+  extern __shared__ float2 stage_memory[];
+  const bool staged = !ACCUM && p.stage_k_log2 > 0 && (load_op(ops, 0).pad & OPS_SEQUENTIAL) != 0;
+  const int stage_k = 1 << (p.stage_k_log2 > 0 ? p.stage_k_log2 : 0);
+  const int lane = (int)(threadIdx.x & 63);
+  const int wave_group = lane / L;
+  float2* const stage =
+      stage_memory + (((threadIdx.x >> 6) * (64 / L)) << ((p.stage_s_log2 > 0 ? p.stage_s_log2 : 0) +
+                                                         (p.stage_k_log2 > 0 ? p.stage_k_log2 : 0)));
+  int64_t nrec = 0;
+
+  // Write the first `count` slots of every row of this wave's stage: row r
+  // holds stream (r mod S) of the wave's group (r / S), samples
+  // nrec - count ... nrec - 1. Two samples per lane per store, K samples of
+  // a row contiguous.
+  auto flush = [&](int count) __attribute__((always_inline)) {
+    const int64_t g0 = group - wave_group;
+    const int64_t o0 = nrec - count;
+    const int rows_log2 = p.stage_s_log2 + p.stage_k_log2;
+    const int total = (64 / L) << rows_log2;
+    for (int i = 2 * lane; i < total; i += 128) {
+      const int kk = i & (stage_k - 1);
+      if (kk >= count) continue;
+      const int row = i >> p.stage_k_log2;
+      const int64_t at = g0 + (row >> p.stage_s_log2);
+      if (at >= p.atoms) continue;
+      const int stream = row & ((1 << p.stage_s_log2) - 1);
+      const int64_t o = o0 + kk;
+      float2* dst;
+      if (stream == 0) {
+        if (!p.signal) continue;
+        dst = p.signal + at * p.outputs + o;
+      } else {
+        if (stream > p.columns) continue;
+        dst = p.jacobian + (at * p.columns + (stream - 1)) * p.outputs + o;
+      }
+      const float2 lo = stage[i];
+      if (kk + 1 < count) {
+        const float2 hi = stage[i + 1];
+        dst[0] = lo;
+        dst[1] = hi;
+      } else {
+        dst[0] = lo;
+      }
+    }
+  };
+
   // The sample i m0 F+[0] and its tangents (reference record,
   // _epg_cpu.cpp:3090-3120); F+[0] is register R of the first lane.
   auto record = [&](auto rc, const MacroOp& ev) __attribute__((always_inline)) {
@@ -547,6 +610,38 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
 #pragma unroll
         for (int kk = cc; kk < NORMAL_COLUMNS; ++kk) acc_h[h++] += ff * d[cc] * d[kk];
       }
+    } else if (staged) {
+      // this group's row of the wave's stage: stream s at slot s * K
+      const float v = m0 * a0;
+      const float sign = p.sample_sign ? p.sample_sign[o] : 1.0f;
+      const float fr = p.factor_re * sign, fi = p.factor_im * sign;
+      float2* row = stage + ((wave_group << p.stage_s_log2) << p.stage_k_log2) + (nrec & (stage_k - 1));
+      row[0] = make_float2(-fi * v, fr * v);
+      if constexpr ((TM & TAN_T1) != 0) {
+        if (p.col_t1 >= 0) {
+          const float d = m0 * t0[ST1];
+          row[(1 + p.col_t1) << p.stage_k_log2] = make_float2(-fi * d, fr * d);
+        }
+      }
+      if constexpr ((TM & TAN_T2) != 0) {
+        if (p.col_t2 >= 0) {
+          const float d = m0 * t0[ST2];
+          row[(1 + p.col_t2) << p.stage_k_log2] = make_float2(-fi * d, fr * d);
+        }
+      }
+      if constexpr ((TM & TAN_B1) != 0) {
+        if (p.col_b1 >= 0) {
+          const float d = m0 * t0[SB1];
+          row[(1 + p.col_b1) << p.stage_k_log2] = make_float2(-fi * d, fr * d);
+        }
+      }
+      if constexpr ((TM & TAN_IE) != 0) {
+        if (p.col_ie >= 0) {
+          const float d = m0 * t0[SIE];
+          row[(1 + p.col_ie) << p.stage_k_log2] = make_float2(-fi * d, fr * d);
+        }
+      }
+      if (p.col_m0 >= 0) row[(1 + p.col_m0) << p.stage_k_log2] = make_float2(-fi * a0, fr * a0);
     } else {
       const float v = m0 * a0;
       if (p.out_real) {
@@ -644,7 +739,13 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
       apply(rc, m, ctl, cs, sn);
       angle(next, next_cs, next_sn);
       if (ctl & OP_RECORD) {
-        if (m.out >= 0 && m.out < p.outputs) record(rc, m);
+        if (m.out >= 0 && m.out < p.outputs) {
+          record(rc, m);
+          if (staged) {
+            ++nrec;
+            if ((nrec & (stage_k - 1)) == 0) flush(stage_k);
+          }
+        }
       }
       if (ctl & OP_SHIFT) break;
     }
@@ -660,6 +761,7 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
     while (segment(std::integral_constant<int, 0>{})) {
     }
   }
+  if (staged && (nrec & (stage_k - 1)) != 0) flush((int)(nrec & (stage_k - 1)));
 
   if constexpr (ACCUM) {
     if (first && live) {
@@ -701,17 +803,19 @@ __global__ void __launch_bounds__(256) real_batch_kernel(const RealArgs* __restr
   }
 }
 
-using RealLaunch = void (*)(const RealArgs&, dim3, dim3, hipStream_t);
-using BatchLaunch = void (*)(const RealArgs*, const BatchWork*, int64_t, dim3, dim3, hipStream_t);
+// Launchers take the dynamic LDS bytes of the output stage.
+using RealLaunch = void (*)(const RealArgs&, dim3, dim3, size_t, hipStream_t);
+using BatchLaunch = void (*)(const RealArgs*, const BatchWork*, int64_t, dim3, dim3, size_t, hipStream_t);
 
 template <int SL, int L, int TM, int V>
-void launch_real(const RealArgs& args, dim3 grid, dim3 block, hipStream_t stream) {
-  hipLaunchKernelGGL((real_kernel<SL, L, TM, V>), grid, block, 0, stream, args);
+void launch_real(const RealArgs& args, dim3 grid, dim3 block, size_t shared, hipStream_t stream) {
+  hipLaunchKernelGGL((real_kernel<SL, L, TM, V>), grid, block, shared, stream, args);
 }
 
 template <int SL, int L, int TM, int V>
-void launch_batch(const RealArgs* r, const BatchWork* w, int64_t blocks, dim3 grid, dim3 block, hipStream_t s) {
-  hipLaunchKernelGGL((real_batch_kernel<SL, L, TM, V>), grid, block, 0, s, r, w, blocks);
+void launch_batch(const RealArgs* r, const BatchWork* w, int64_t blocks, dim3 grid, dim3 block, size_t shared,
+                  hipStream_t s) {
+  hipLaunchKernelGGL((real_batch_kernel<SL, L, TM, V>), grid, block, shared, s, r, w, blocks);
 }
 
 // Tangent sets come in four kernels: none, (T1, T2), (T1, T2, B1) and all
