@@ -371,7 +371,8 @@ def main() -> int:
     parser.add_argument("--steps", type=int, default=10)
     parser.add_argument("--warmup", type=int, default=3)
     parser.add_argument("--tiles", type=int, default=TILES, help="copies of the 106,182-atom grid")
-    parser.add_argument("--tile", type=int, default=-1, help="state tile index (-1: library choice)")
+    parser.add_argument("--tile", default="auto",
+                        help="state tile index, -1 for the library's cost model, or 'auto' (measured choice)")
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-torch-baseline", action="store_true")
     parser.add_argument("--workload", choices=("dictionary", "design", "fit", "serve"), default="dictionary")
@@ -408,11 +409,15 @@ def main() -> int:
 
     device = torch.device("cuda", 0)
     if args.calibrate_copy:
+        # a kernel of known traffic for the HBM counters: reads and writes
+        # calibrate_copy bytes once each (an elementwise multiply, so it runs
+        # on the shader engines, where FETCH_SIZE / WRITE_SIZE count)
         source = torch.ones(args.calibrate_copy // 4, dtype=torch.float32, device=device)
         target = torch.empty_like(source)
-        target.copy_(source)
+        torch.mul(source, 2.0, out=target)
         torch.cuda.synchronize()
         del source, target
+    tile = args.tile if args.tile == "auto" else int(args.tile)
     T1, T2, B1 = dictionary(args.tiles)
     atoms = T1.size
     t1 = torch.tensor(T1, device=device)
@@ -421,7 +426,7 @@ def main() -> int:
     seq = MRFSimulator(flip=torch.tensor(flips(), dtype=torch.float32), TR=10.0, states=STATES)
 
     def step():
-        return seq.jacobian(NAMES, T1=t1, T2=t2, B1=b1, tile=args.tile)
+        return seq.jacobian(NAMES, T1=t1, T2=t2, B1=b1, tile=tile)
 
     out = None
     for _ in range(args.warmup):
@@ -468,11 +473,12 @@ def main() -> int:
         },
         "roofline": {
             "bound": "valu",
+            "bound_note": "FP32 vector ALU (an EPG recursion has no contraction for MFMA); peak = MI355X published FP32",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(),
             "kernel_ms": kernel_ms,
             "flops_per_atom_pulse": FLOPS_PER_ATOM_PULSE,
             "write_GBs": BYTES_PER_ATOM_PULSE * work / (kernel_ms * 1e-3) / 1e9,
@@ -489,6 +495,19 @@ def main() -> int:
         record["speedup_vs_torch_per_op"] = value / per_op["value"]
     print(json.dumps(record), flush=True)
     return 0
+
+
+def measured_traffic():
+    """HBM bytes per launch of the dictionary kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (each calibrated against a kernel
+    of known traffic), or None when that summary is absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "real_kernel_traffic.json")
+    try:
+        with open(path, encoding="utf-8") as handle:
+            summary = json.load(handle)
+        return float(summary["fetch_bytes"]) + float(summary["write_bytes"])
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 if __name__ == "__main__":

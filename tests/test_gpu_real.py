@@ -250,3 +250,56 @@ def test_staged_and_direct_output_writes_agree_bitwise():
     assert torch.equal(jac, torch.flip(jac_r, dims=(-1,)))
     ref, _ = _oracle_mrf(flip, 10.0, t1, t2, states=20, b1=b1)
     assert _cases.rel_err(sig.cpu().numpy(), ref) < SIGNAL_TOL
+
+
+def test_every_covering_tile_gives_the_same_bits():
+    """The tuner may pick any covering tile: each runs the same per-order
+    arithmetic (padded tiles only keep their extra orders at zero), so the
+    outputs agree bit for bit whatever the choice."""
+    from torchsim_amd import _native
+    from torchsim_amd.simulators import MRFSimulator
+
+    flip = _cases.benchmark_flips(120)
+    t1, t2 = _cases.random_tissue(300, seed=21)
+    b1 = np.linspace(0.7, 1.3, 300).astype(np.float32)
+    seq = MRFSimulator(flip=torch.tensor(flip, dtype=torch.float32), TR=10.0, states=20)
+    names = ("T1", "T2", "B1")
+    args = dict(T1=torch.tensor(t1).cuda(), T2=torch.tensor(t2).cuda(), B1=torch.tensor(b1).cuda())
+    shapes = _native.tiles()
+    covering = [index for index, (sl, lanes) in enumerate(shapes) if sl * lanes >= 20]
+    first = None
+    for index in covering:
+        sig, jac = seq.jacobian(names, tile=index, **args)
+        if first is None:
+            first = (sig, jac)
+        else:
+            assert torch.equal(sig, first[0]), shapes[index]
+            assert torch.equal(jac, first[1]), shapes[index]
+
+
+def test_tile_tuner_measures_once_per_configuration():
+    from torchsim_amd.sequence import _tuning
+
+    _tuning.forget()
+    calls = []
+
+    def probe(tile):
+        calls.append(tile)
+        torch.cuda._sleep(1000)
+
+    device = torch.device("cuda", 0)
+    candidates = _tuning.candidates(20)
+    assert candidates and all(sl * lanes >= 20 for sl, lanes in (_native_tiles()[i] for i in candidates))
+    chosen = _tuning.tile_for(device, 20, ("t1_ms",), 0, probe)
+    assert chosen in candidates
+    measured = len(calls)
+    assert measured >= len(candidates)
+    assert _tuning.tile_for(device, 20, ("t1_ms",), 0, probe) == chosen
+    assert len(calls) == measured  # cached: no second measurement
+    _tuning.forget()
+
+
+def _native_tiles():
+    from torchsim_amd import _native
+
+    return _native.tiles()

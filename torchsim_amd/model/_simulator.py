@@ -290,7 +290,7 @@ class Simulator:
             factor=factor,
             flip=flip,
             samples=samples,
-            tile=int(sequence.get("tile", -1)),
+            tile=sequence.get("tile", "auto"),
         )
 
     def evaluate(self, properties: Mapping[str, Any], **sequence: Any) -> torch.Tensor:

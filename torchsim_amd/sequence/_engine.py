@@ -11,6 +11,7 @@ exchanging pool, instantaneous pulses.
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 from dataclasses import dataclass
 from typing import Any
 
@@ -191,14 +192,23 @@ def simulate_real(
     factor: complex = 1.0,
     flip: torch.Tensor | None = None,
     samples: torch.Tensor | None = None,
-    tile: int = -1,
+    tile: int | str = "auto",
     duration: torch.Tensor | None = None,
 ) -> tuple[torch.Tensor, torch.Tensor | None]:
     """Signal and (optionally) Jacobian columns in one real-subspace launch.
 
     Returns complex64 ``(atoms, outputs)`` and ``(atoms, len(columns), outputs)``
-    already multiplied by ``factor`` (and by the sample signs).
+    already multiplied by ``factor`` (and by the sample signs). ``tile`` is a
+    state-tile index, -1 for the library's cost model, or ``"auto"`` for the
+    measured choice (``_tuning.tile_for``, probed once per configuration).
     """
+    if tile == "auto":
+        # measuring synchronizes, which a stream under graph capture cannot do
+        if torch.cuda.is_current_stream_capturing():
+            tile = -1
+        else:
+            tile = _tuned_tile(packed, tissue, states=states, columns=columns, factor=factor, flip=flip,
+                               samples=samples, duration=duration)
     device = tissue.device
     _native.require_device(device)
     lib = _native.load()
@@ -240,6 +250,32 @@ def simulate_real(
         )
     _native.check(status)
     return signal, jacobian
+
+
+def narrow_tissue(tissue: PreparedTissue, atoms: int) -> PreparedTissue:
+    """The first ``atoms`` atoms of a prepared tissue (views, no copy)."""
+    atoms = min(atoms, tissue.atoms)
+    buffers = []
+    for index, buffer in enumerate(tissue.buffers):
+        per_atom = buffer is not None and (index in (0, 1) or tissue.stride == 1)
+        buffers.append(buffer[:atoms] if per_atom else buffer)
+    return dataclasses.replace(tissue, buffers=buffers, shape=torch.Size([atoms]), atoms=atoms)
+
+
+def _tuned_tile(packed: PackedEvents, tissue: PreparedTissue, *, states: int, columns: tuple[str, ...],
+                factor: complex, flip: torch.Tensor | None, samples: torch.Tensor | None,
+                duration: torch.Tensor | None) -> int:
+    from . import _tuning
+
+    probe_tissue = narrow_tissue(tissue, _tuning.PROBE_ATOMS)
+    if probe_tissue.atoms == 0 or packed.output_count == 0:
+        return -1
+
+    def probe(candidate: int) -> None:
+        simulate_real(packed, probe_tissue, states=states, columns=columns, factor=factor, flip=flip,
+                      samples=samples, tile=candidate, duration=duration)
+
+    return _tuning.tile_for(tissue.device, states, tuple(columns), tissue.features, probe)
 
 
 def adjoint_real(
