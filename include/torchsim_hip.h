@@ -154,6 +154,18 @@ int tsim_gauss_newton_normal(void* const* packed, int64_t atom_count, int64_t ev
                              const void* data, double* normal, int tile, void* stream);
 
 /*
+ * The same normal equations from a Jacobian that passed through memory (any
+ * sequence the fused kernel does not carry: complex states, exchange,
+ * diffusion): signal, data DEVICE complex64 (voxels, outputs), jacobian
+ * DEVICE complex64 (voxels, columns, outputs), columns 1..4; normal as above.
+ * One wavefront per voxel reads its rows once, accumulating in FP64.
+ * Replaces the stacked least-squares set-up of the reference's solve
+ * (recon/_gauss_newton.py:247-307).
+ */
+int tsim_normal_from_jacobian(const void* signal, const void* jacobian, const void* data, int64_t voxels,
+                              int64_t outputs, int columns, double* normal, void* stream);
+
+/*
  * The damped Gauss-Newton step of every voxel in FP64, one thread per voxel:
  * H = D H D, g = D g with D the per-voxel chain-rule scale of the bound
  * transform (DEVICE FP64 (voxels, columns), NULL for none); then

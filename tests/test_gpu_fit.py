@@ -170,3 +170,40 @@ def test_fit_agrees_with_fp64_levenberg_marquardt_on_noisy_voxels():
                      bounds, 30)
     assert np.abs(ours - ref).max(0).max() / 1.0 < 5.0  # ms-scale absolute agreement
     assert (np.abs(ours - ref) / np.abs(ref)).max() < 1e-3
+
+
+@pytest.mark.parametrize("columns, outputs", [(1, 37), (3, 500), (4, 64), (2, 1)])
+def test_normal_equations_from_a_stored_jacobian(columns, outputs):
+    """tsim_normal_from_jacobian (the path of sequences the fused kernel does
+    not carry) against the same sums in complex128 torch on the CPU."""
+    from torchsim_amd import _native
+
+    lib = _native.load()
+    stride = lib.tsim_normal_stride()
+    gen = torch.Generator().manual_seed(columns * 1000 + outputs)
+    voxels = 333
+    sig = torch.randn(voxels, outputs, dtype=torch.complex64, generator=gen)
+    jac = torch.randn(voxels, columns, outputs, dtype=torch.complex64, generator=gen)
+    dat = torch.randn(voxels, outputs, dtype=torch.complex64, generator=gen)
+    normal = torch.full((voxels, stride), float("nan"), dtype=torch.float64, device="cuda")
+    s_d, j_d, d_d = sig.cuda(), jac.cuda(), dat.cuda()
+    status = lib.tsim_normal_from_jacobian(s_d.data_ptr(), j_d.data_ptr(), d_d.data_ptr(), voxels, outputs, columns,
+                                           normal.data_ptr(), _native.stream_of(torch.device("cuda", 0)))
+    _native.check(status)
+    got = normal.cpu()
+    r = sig.to(torch.complex128) - dat.to(torch.complex128)
+    j = jac.to(torch.complex128)
+    cost = (r.abs() ** 2).sum(-1)
+    grad = (j.conj() * r[:, None, :]).real.sum(-1)
+    hess = torch.einsum("vps,vqs->vpq", j.conj(), j.contiguous()).real
+    assert torch.allclose(got[:, 0], cost, rtol=1e-12, atol=0.0)
+    assert torch.allclose(got[:, 1:1 + columns], grad, rtol=1e-10, atol=1e-12)
+    assert torch.all(got[:, 1 + columns:5] == 0.0)
+    k = 0
+    for c in range(4):
+        for q in range(c, 4):
+            if c < columns and q < columns:
+                assert torch.allclose(got[:, 5 + k], hess[:, c, q], rtol=1e-10, atol=1e-12)
+            else:
+                assert torch.all(got[:, 5 + k] == 0.0)
+            k += 1

@@ -46,3 +46,27 @@ def test_plain_protocols_still_take_the_generic_packing():
     seq = MRFSimulator(states=10)
     flips = torch.linspace(5.0, 60.0, 20)
     assert torch.equal(seq.packed(flip=flips, TR=10.0).duration, Simulator._packed(seq, flip=flips, TR=10.0).duration)
+
+
+@pytest.mark.parametrize("simulator, protocol", [
+    ("MRFSimulator", dict(flip=np.r_[np.linspace(5, 60, 300), np.linspace(60, 2, 300), np.full(280, 2.0)], TR=10.0)),
+    ("MRFSimulator", dict(flip=np.linspace(3, 70, 200), TR=np.linspace(8, 15, 200), TI=20.0)),
+    ("FSESimulator", dict(flip=np.linspace(180, 120, 96), ESP=5.0, TR=3000.0)),
+    ("TrueFISPSimulator", dict(flip=35.0, TR=4.5, nshots=400)),
+    ("TrueFISPSimulator", dict(flip=50.0, TR=4.5, nshots=300, inversion="ideal", spoiler=5.0)),
+])
+def test_host_scalar_layouts_pack_the_same_bits(simulator, protocol, monkeypatch):
+    """Layouts walk NumPy scalars when no graph runs through the protocol;
+    the buffers are those that 0-dim tensors give, bit for bit."""
+    import torchsim_amd.simulators as sims
+    from torchsim_amd.simulators import fse, mrf, truefisp
+
+    sim = getattr(sims, simulator)()
+    args = {k: (torch.tensor(v, dtype=torch.float32) if isinstance(v, np.ndarray) else v) for k, v in protocol.items()}
+    fast = Simulator._packed(sim, **args)
+    for module in (mrf, fse, truefisp):
+        monkeypatch.setattr(module, "host_values", lambda *arrays: arrays)
+    tensors = Simulator._packed(sim, **args)
+    for field in FIELDS:
+        assert torch.equal(getattr(fast, field), getattr(tensors, field)), field
+    assert fast.output_count == tensors.output_count

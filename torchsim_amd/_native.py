@@ -134,6 +134,7 @@ SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
         [ctypes.POINTER(_P), _I64, _I64, _I64, _I64, _I32, _I64, ctypes.POINTER(ctypes.c_int32), _I64,
          _F32, _F32, _P, _P, _P, _I32, _P],
     ),
+    "tsim_normal_from_jacobian": (_I32, [_P, _P, _P, _I64, _I64, _I32, _P, _P]),
     "tsim_solve_damped": (_I32, [_P, _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P]),
     "tsim_normal_stride": (_I32, []),
     "tsim_tile_count": (_I32, []),

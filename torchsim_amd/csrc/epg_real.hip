@@ -282,8 +282,9 @@ int pack_stream(const EventCode* codes, int64_t events, int64_t outputs, MacroOp
 void set_staging(RealArgs& args, int lanes_per_atom) {
   args.stage_k_log2 = -1;
   args.stage_s_log2 = 0;
-  if (args.out_real != nullptr || (args.signal == nullptr && args.jacobian == nullptr)) return;
-  const int streams = 1 + args.columns;
+  const bool reads = args.normal != nullptr;  // normal equations stage the measured samples
+  if (!reads && (args.out_real != nullptr || (args.signal == nullptr && args.jacobian == nullptr))) return;
+  const int streams = reads ? 1 : 1 + args.columns;
   int s_log2 = 0;
   while ((1 << s_log2) < streams) ++s_log2;
   const int64_t slot_bytes = (int64_t)(64 / lanes_per_atom) * (1 << s_log2) * (int64_t)sizeof(float2);
@@ -404,6 +405,58 @@ BatchLaunch pick_batch(int tile, int tm, int variant) {
 #undef TSIM_BATCH_CASE
     default:
       return nullptr;
+  }
+}
+
+// Normal equations from a stored Jacobian: one wavefront per voxel, each lane
+// a stride-64 slice of the samples (so a wave reads 64 consecutive complex
+// samples of a row at a time), FP64 products and sums, then a butterfly over
+// the wave.
+__global__ void __launch_bounds__(256) normal_from_jacobian(const float2* __restrict__ signal,
+                                                            const float2* __restrict__ jacobian,
+                                                            const float2* __restrict__ data, int64_t voxels,
+                                                            int64_t outputs, int cols,
+                                                            double* __restrict__ normal) {
+  const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = (int)(threadIdx.x & 63);
+  if (v >= voxels) return;  // whole waves leave together
+  double acc[NORMAL_STRIDE];
+#pragma unroll
+  for (int i = 0; i < NORMAL_STRIDE; ++i) acc[i] = 0.0;
+  const float2* srow = signal + v * outputs;
+  const float2* drow = data + v * outputs;
+  const float2* jrow = jacobian + v * (int64_t)cols * outputs;
+  for (int64_t n = lane; n < outputs; n += 64) {
+    const float2 s = srow[n];
+    const float2 y = drow[n];
+    const double rx = (double)s.x - (double)y.x;
+    const double ry = (double)s.y - (double)y.y;
+    acc[0] += rx * rx + ry * ry;
+    double jx[NORMAL_COLUMNS], jy[NORMAL_COLUMNS];
+#pragma unroll
+    for (int c = 0; c < NORMAL_COLUMNS; ++c) {
+      const float2 j = c < cols ? jrow[(int64_t)c * outputs + n] : make_float2(0.0f, 0.0f);
+      jx[c] = j.x;
+      jy[c] = j.y;
+      acc[1 + c] += jx[c] * rx + jy[c] * ry;  // Re(conj(J_c) r)
+    }
+    int h = 0;
+#pragma unroll
+    for (int c = 0; c < NORMAL_COLUMNS; ++c) {
+#pragma unroll
+      for (int q = c; q < NORMAL_COLUMNS; ++q) acc[1 + NORMAL_COLUMNS + h++] += jx[c] * jx[q] + jy[c] * jy[q];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NORMAL_STRIDE; ++i) {
+    double x = acc[i];
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1) x += __shfl_xor(x, offset, 64);
+    acc[i] = x;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NORMAL_STRIDE; ++i) normal[v * NORMAL_STRIDE + i] = acc[i];
   }
 }
 
@@ -773,9 +826,31 @@ int tsim_gauss_newton_normal(void* const* packed, int64_t atom_count, int64_t ev
     set_error("no normal-equation kernel for this variant");
     return UNSUPPORTED;
   }
-  args.stage_k_log2 = -1;
-  launch(args, dim3((unsigned)blocks), dim3(256), 0, s);
+  set_staging(args, L);
+  launch(args, dim3((unsigned)blocks), dim3(256), stage_bytes(args, L), s);
   return hip_status(hipGetLastError(), "normal-equation kernel launch");
+}
+
+int tsim_normal_from_jacobian(const void* signal, const void* jacobian, const void* data, int64_t voxels,
+                              int64_t outputs, int columns, double* normal, void* stream) {
+  if (columns < 1 || columns > NORMAL_COLUMNS || voxels < 0 || outputs < 0) {
+    set_error("between 1 and %d columns and non-negative sizes", NORMAL_COLUMNS);
+    return BAD_ARGUMENT;
+  }
+  if (voxels == 0) return OK;
+  if (signal == nullptr || jacobian == nullptr || data == nullptr || normal == nullptr) {
+    set_error("signal, jacobian, data and normal buffers are required");
+    return BAD_ARGUMENT;
+  }
+  const int64_t blocks = (voxels + 3) / 4;
+  if (blocks > 0x7FFFFFFF) {
+    set_error("too many voxels for one launch: %lld", (long long)voxels);
+    return BAD_ARGUMENT;
+  }
+  hipLaunchKernelGGL(normal_from_jacobian, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     (const float2*)signal, (const float2*)jacobian, (const float2*)data, voxels, outputs, columns,
+                     normal);
+  return hip_status(hipGetLastError(), "normal_from_jacobian launch");
 }
 
 int tsim_solve_damped(const double* normal, const double* scale, const double* damping, const double* offset,
@@ -929,11 +1004,7 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
   const int64_t per_block = 256 / L;
   size_t shared = 0;
   for (RealArgs& a : args) {
-    if (mode == 1) {
-      a.stage_k_log2 = -1;
-    } else {
-      set_staging(a, L);
-    }
+    set_staging(a, L);
     const size_t bytes = stage_bytes(a, L);
     shared = bytes > shared ? bytes : shared;
   }

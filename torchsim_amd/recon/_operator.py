@@ -96,7 +96,8 @@ class FusedModel:
     def _normal_from_jacobian(self, values: torch.Tensor, data: torch.Tensor, live: torch.Tensor) -> torch.Tensor:
         """Any sequence: the simulator's fused signal + Jacobian (complex
         kernel, post-processing included), reduced to the normal equations in
-        FP64 -- the Jacobian passes through memory on this path."""
+        FP64 by tsim_normal_from_jacobian -- the Jacobian passes through memory
+        once on this path."""
         device = data.device
         held = {}
         for name, value in self.held.items():
@@ -105,18 +106,19 @@ class FusedModel:
         for index, name in enumerate(self.names):
             held[name] = values[:, index].to(torch.float32)
         signal, jacobian = self.simulator.linearize(held, self.names, **self.sequence)
-        stride = _native.load().tsim_normal_stride()
-        normal = torch.zeros((signal.shape[0], stride), dtype=torch.float64, device=device)
-        residual = (signal - data.to(signal.dtype)).to(torch.complex128)
-        jac = jacobian.to(torch.complex128)
-        normal[:, 0] = (residual.real**2 + residual.imag**2).sum(-1)
-        g = (jac.conj() * residual[:, None, :]).real.sum(-1)
-        h = torch.einsum("vps,vqs->vpq", jac.conj(), jac).real
-        normal[:, 1:1 + self.columns] = g
-        k = 0
-        for c in range(4):
-            for q in range(c, 4):
-                if c < self.columns and q < self.columns:
-                    normal[:, 5 + k] = h[:, c, q]
-                k += 1
+        lib = _native.load()
+        stride = lib.tsim_normal_stride()
+        voxels, outputs = signal.shape
+        normal = torch.zeros((voxels, stride), dtype=torch.float64, device=device)
+        if voxels == 0:
+            return normal
+        signal = signal.to(torch.complex64).contiguous()
+        jacobian = jacobian.to(torch.complex64).contiguous()
+        measured = data.to(device=device, dtype=torch.complex64).contiguous()
+        with torch.cuda.device(device):
+            status = lib.tsim_normal_from_jacobian(
+                signal.data_ptr(), jacobian.data_ptr(), measured.data_ptr(), voxels, outputs, self.columns,
+                normal.data_ptr(), _native.stream_of(device),
+            )
+        _native.check(status)
         return normal

@@ -19,6 +19,7 @@ from dataclasses import dataclass
 from enum import IntEnum, IntFlag
 from typing import Any
 
+import numpy as np
 import torch
 
 # Bits of the packed action word beyond EventAction's own (reference
@@ -252,8 +253,27 @@ class PackedEvents:
         return cache[key]
 
 
+def host_values(*arrays: Any) -> tuple[Any, ...]:
+    """A layout's per-pulse arrays as host NumPy arrays of the same dtype when
+    no autograd graph runs through them. Indexing them then yields NumPy
+    scalars, whose float32 / float64 arithmetic is the same IEEE arithmetic as
+    that of 0-dim tensors of the dtype (Python numbers stay weak in both), so
+    the packed buffers are bit for bit those the tensors would give, at a
+    small fraction of the per-operation cost of a layout of a thousand pulses.
+    """
+    if any(isinstance(a, torch.Tensor) and a.requires_grad for a in arrays):
+        return arrays
+    return tuple(a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else a for a in arrays)
+
+
+def _graphed(values: Sequence[Any]) -> bool:
+    return any(isinstance(value, torch.Tensor) and value.requires_grad for value in values)
+
+
 def _stack(values: Sequence[Any]) -> torch.Tensor:
     """Stack per-event values at float32, keeping any autograd graph."""
+    if not _graphed(values):
+        return torch.from_numpy(np.array([float(value) for value in values], dtype=np.float64).astype(np.float32))
     if any(isinstance(value, torch.Tensor) for value in values):
         return torch.stack(
             [
@@ -267,10 +287,14 @@ def _stack(values: Sequence[Any]) -> torch.Tensor:
 def _clock(values: Sequence[Any]) -> torch.Tensor:
     """The stream's timestamps at the width the description timed them in.
 
-    Timestamps accumulated from tensors are float32 (reference
-    sequence/_accelerators.py:1222-1247); plain Python numbers stay float64.
+    Timestamps accumulated from tensors (or the NumPy scalars of
+    ``host_values``) are float32 (reference sequence/_accelerators.py:1222-1247);
+    plain Python numbers stay float64.
     """
-    if any(isinstance(value, torch.Tensor) for value in values):
+    timed = any(isinstance(value, (torch.Tensor, np.floating)) for value in values)
+    if timed and not _graphed(values):
+        return torch.from_numpy(np.array([float(value) for value in values], dtype=np.float64).astype(np.float32))
+    if timed:
         return torch.stack(
             [
                 value.to(torch.float32).reshape(()) if isinstance(value, torch.Tensor) else torch.tensor(float(value), dtype=torch.float32)

@@ -8,7 +8,7 @@ from typing import Any
 import torch
 
 from ..model._simulator import Simulator, SpinPhysics
-from ..sequence._events import REFOCUSED
+from ..sequence._events import REFOCUSED, host_values
 from .mrf import _matched
 
 
@@ -30,6 +30,7 @@ class FSESimulator(Simulator):
         angles = torch.deg2rad(torch.atleast_1d(torch.as_tensor(flip)))
         turns = torch.deg2rad(_matched(phases, angles))
         spacing_s = ESP * 1e-3
+        angles, turns = host_values(angles, turns)
         ops = self.model.operators
         parts = [(0.0, ops.excitation(torch.pi / 180.0 * exc_flip, torch.pi / 180.0 * exc_phase))]
         for index in range(angles.shape[-1]):

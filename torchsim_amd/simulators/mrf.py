@@ -9,7 +9,7 @@ import numpy as np
 import torch
 
 from ..model._simulator import Simulator, SpinPhysics
-from ..sequence._events import INVERSION, UNBALANCED, Inversion, PackedEvents
+from ..sequence._events import INVERSION, UNBALANCED, Inversion, PackedEvents, host_values
 
 
 def _matched(value: Any, reference: torch.Tensor) -> torch.Tensor:
@@ -49,9 +49,10 @@ class MRFSimulator(Simulator):
         angles = torch.deg2rad(torch.atleast_1d(torch.as_tensor(flip)))
         turns = torch.deg2rad(_matched(phases, angles))
         spacing_s = _matched(TR, angles) * 1e-3
+        angles, turns, spacing_s = host_values(angles, turns, spacing_s)
         ops = self.model.operators
         parts = [Inversion(duration_s=TI * 1e-3)]
-        for index in range(angles.numel()):
+        for index in range(len(angles)):
             parts.append(ops.excitation(angles[index], turns[index]))
             parts.append(ops.readout(turns[index], duration_s=spacing_s[index]))
         return parts

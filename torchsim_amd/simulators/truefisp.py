@@ -7,7 +7,7 @@ from typing import Any
 import torch
 
 from ..model._simulator import Simulator, SpinPhysics
-from ..sequence._events import Delay, Excitation, Inversion, Readout, Spoil, module
+from ..sequence._events import Delay, Excitation, Inversion, Readout, Spoil, host_values, module
 
 
 class TrueFISPSimulator(Simulator):
@@ -37,6 +37,7 @@ class TrueFISPSimulator(Simulator):
             angles = angles.reshape(1).expand(nshots)
         if angles.numel() != nshots:
             raise ValueError("flip is a scalar or one value per shot")
+        (angles,) = host_values(angles)
         parts: list = []
         if inversion == "ideal":
             parts.append(Inversion())
