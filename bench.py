@@ -279,7 +279,9 @@ def serve_workload(duration_s: float = 60.0, rate: float = 50.0, seed: int = 0) 
     """Poisson arrivals (rate/s for duration_s): half simulate, half fit;
     voxels log-uniform 1k-200k; bSSFP 200-500 pulses, FSE 64-256 echoes or
     MRF 500-1000 pulses; a third with a two-pool exchange model, a sixth
-    diffusion-weighted. Reports p50/p99 latency and throughput."""
+    diffusion-weighted. A fit request's data is what a scanner would hand
+    over: each voxel one of 256 tissue classes of its protocol plus noise.
+    Reports p50/p99 latency (submission to result) and throughput."""
     import torch
 
     from torchsim_amd.serving import FitRequest, Server, SimulateRequest
@@ -310,6 +312,8 @@ def serve_workload(duration_s: float = 60.0, rate: float = 50.0, seed: int = 0) 
         plans.append((arrivals[k], voxels, family, pooled, diffusing, sim, rng.uniform() < 0.5,
                       int(rng.integers(1 << 30))))
 
+    classes = 256  # tissue classes the measured data of a fit request is drawn from
+
     def tissue(voxels, pooled, diffusing, draw_seed):
         g = np.random.default_rng(draw_seed)
         props = {"T1": torch.tensor(g.uniform(300, 2500, voxels), dtype=torch.float32).cuda(),
@@ -321,6 +325,20 @@ def serve_workload(duration_s: float = 60.0, rate: float = 50.0, seed: int = 0) 
         if diffusing:
             props["D"] = torch.tensor(g.uniform(0.5, 3.0, voxels), dtype=torch.float32).cuda()
         return props
+
+    def measured(sim, voxels, pooled, diffusing, draw_seed):
+        """What a scanner hands a fit request: every voxel one of 256 tissue
+        classes of the protocol (the classes simulated, the voxels gathered)
+        plus complex noise at SNR 50, and the known per-voxel properties."""
+        table = tissue(classes, pooled, diffusing, draw_seed)
+        signals = sim.simulate(**table)
+        g = torch.Generator(device="cuda").manual_seed(draw_seed)
+        pick = torch.randint(0, classes, (voxels,), device="cuda", generator=g)
+        data = signals[pick]
+        sigma = signals.abs().max() / 50.0
+        data = data + sigma * torch.randn(data.shape, dtype=data.dtype, device="cuda", generator=g)
+        known = {k: (v[pick] if isinstance(v, torch.Tensor) else v) for k, v in table.items() if k not in ("T1", "T2")}
+        return data, known
 
     bounds = {"T1": (100.0, 5000.0), "T2": (10.0, 2000.0), "M0": (0.01, 2.0)}
     client = torch.cuda.Stream()
@@ -341,16 +359,15 @@ def serve_workload(duration_s: float = 60.0, rate: float = 50.0, seed: int = 0) 
                 time.sleep(delay)
             # the client builds each request as it arrives (its own stream)
             with torch.cuda.stream(client):
-                props = tissue(voxels, pooled, diffusing, draw_seed)
                 if is_fit:
-                    measured = sim.simulate(**props)
-                    known = {k: v for k, v in props.items() if k not in ("T1", "T2")}
-                    request = FitRequest(sim, measured, bounds=bounds, known=known,
+                    data, known = measured(sim, voxels, pooled, diffusing, draw_seed)
+                    request = FitRequest(sim, data, bounds=bounds, known=known,
                                          initial={"T1": 1000.0, "T2": 100.0, "M0": 0.8}, iterations=10)
                 else:
-                    request = SimulateRequest(sim, props, diff=("T1", "T2"))
+                    request = SimulateRequest(sim, tissue(voxels, pooled, diffusing, draw_seed), diff=("T1", "T2"))
             client.synchronize()
             server.submit(request).add_done_callback(finished)
+        submitted = time.perf_counter() - start
         while True:
             with lock:
                 if outcome["done"] >= count:
@@ -361,7 +378,8 @@ def serve_workload(duration_s: float = 60.0, rate: float = 50.0, seed: int = 0) 
     return {"metric": "served requests/s with p50/p99 latency", "value": count / elapsed, "unit": "req/s",
             "p50_s": stats["p50_s"], "p99_s": stats["p99_s"], "completed": stats["completed"],
             "errors": outcome["errors"], "launches": stats["launches"], "batched_jobs": stats["batched_jobs"],
-            "seconds": elapsed,
+            "seconds": elapsed, "submission_s": submitted, "server_host_s": stats.get("host_s"),
+            "data": "fit data: 256 tissue classes per protocol, simulated, gathered to the voxels, SNR 50 noise",
             "config": {"workload": "Poisson mix", "rate_per_s": rate, "duration_s": duration_s, "requests": count}}
 
 

@@ -94,7 +94,24 @@ int run_complex(ComplexArgs& a, int tm, bool exch, hipStream_t stream) {
     set_error("too many atoms for one launch: %lld", (long long)a.atoms);
     return BAD_ARGUMENT;
   }
-  launch(a, dim3((unsigned)blocks), dim3(threads), stream);
+  // output staging: K samples per flush with the padded stream count, one
+  // wave's stage within COMPLEX_STAGE_BYTES_PER_WAVE; K < 2 stores directly
+  a.stage_k_log2 = -1;
+  a.stage_s_log2 = 0;
+  size_t shared = 0;
+  if (a.out_real == nullptr && (a.signal != nullptr || a.jacobian != nullptr)) {
+    int s_log2 = 0;
+    while ((1 << s_log2) < 1 + a.columns) ++s_log2;
+    const int64_t slot = (int64_t)(64 / lanes) * (1 << s_log2) * (int64_t)sizeof(float2);
+    int k_log2 = 0;
+    while (k_log2 < 4 && slot * (2 << k_log2) <= COMPLEX_STAGE_BYTES_PER_WAVE) ++k_log2;
+    if (k_log2 >= 1) {
+      a.stage_k_log2 = k_log2;
+      a.stage_s_log2 = s_log2;
+      shared = (size_t)(threads / 64) * (size_t)slot * ((size_t)1 << k_log2);
+    }
+  }
+  launch(a, dim3((unsigned)blocks), dim3(threads), shared, stream);
   return hip_status(hipGetLastError(), "complex_kernel launch");
 }
 

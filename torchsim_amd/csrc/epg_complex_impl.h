@@ -53,7 +53,14 @@ struct ComplexArgs {
   float factor_im;
   int columns;
   int col_t1, col_t2, col_b1, col_m0;
+  // Staged output writes (as the real kernel's): log2 of the samples per
+  // flush and of the padded stream count; stage_k_log2 < 1 writes directly.
+  int stage_k_log2;
+  int stage_s_log2;
 };
+
+// Bytes of one wave's output stage.
+constexpr int COMPLEX_STAGE_BYTES_PER_WAVE = 8192;
 
 namespace cx {
 
@@ -266,6 +273,44 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
     rec[0] = rec[1] = drec[0] = drec[1] = 0.0f;
   }
 
+  // Output staging. Each wave keeps its atoms' last K samples of every
+  // stream (signal, then the Jacobian columns) in LDS and writes them K at a
+  // time, K consecutive samples of a row together, instead of one 8-byte
+  // store per atom per sample on its own cache line. It holds while records
+  // arrive in output order 0, 1, 2, ...; the first that does not flushes what
+  // is staged and the rest are stored directly.
+  extern __shared__ float2 complex_stage[];
+  bool staging = p.stage_k_log2 > 0 && p.out_real == nullptr;
+  const int k_log2 = p.stage_k_log2 > 0 ? p.stage_k_log2 : 0;
+  const int s_log2 = p.stage_s_log2 > 0 ? p.stage_s_log2 : 0;
+  const int stage_k = 1 << k_log2;
+  const int wave_group = lane / L;
+  float2* const stage = complex_stage + (((threadIdx.x >> 6) * (64 / L)) << (s_log2 + k_log2));
+  int64_t nrec = 0;
+  auto flush = [&](int count) __attribute__((always_inline)) {
+    const int64_t g0 = group - wave_group;
+    const int64_t o0 = nrec - count;
+    const int total = (64 / L) << (s_log2 + k_log2);
+    for (int i = 2 * lane; i < total; i += 128) {
+      const int kk = i & (stage_k - 1);
+      if (kk >= count) continue;
+      const int row = i >> k_log2;
+      const int64_t at = g0 + (row >> s_log2);
+      if (at >= p.atoms) continue;
+      const int stream = row & ((1 << s_log2) - 1);
+      float2* dst;
+      if (stream == 0) {
+        if (!p.signal) continue;
+        dst = p.signal + at * p.outputs + o0 + kk;
+      } else {
+        if (stream > p.columns || !p.jacobian) continue;
+        dst = p.jacobian + (at * p.columns + (stream - 1)) * p.outputs + o0 + kk;
+      }
+      dst[0] = stage[i];
+      if (kk + 1 < count) dst[1] = stage[i + 1];
+    }
+  };
+
   const EventCode* __restrict__ stream = p.events;
   const int64_t E = p.event_count;
   for (int64_t e = 0; e < E; ++e) {
@@ -475,7 +520,37 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
         }
       }
     } else if (kind == KIND_ADC && (act & RECORD) && ev.out >= 0 && ev.out < p.outputs) {
-      if (sub == 0 && live) {
+      if (staging && ev.out != nrec) {
+        if ((nrec & (stage_k - 1)) != 0) flush((int)(nrec & (stage_k - 1)));
+        staging = false;
+      }
+      if (staging) {
+        if (sub == 0 && live) {
+          float sp, cp;
+          sincosf(-ev.phase, &sp, &cp);
+          const float2 demod = mk(cp, sp);
+          float2 total = v.fp[0];
+          if constexpr (EXCH) total = add(total, v.fp[1]);
+          const float2 sample = mul(scl(total, m0), demod);
+          const float2 factor = mk(p.factor_re, p.factor_im);
+          float2* row = stage + ((wave_group << s_log2) << k_log2) + (nrec & (stage_k - 1));
+          row[0] = mul(factor, sample);
+          const float2 fm0 = mul(factor, scl(demod, m0));
+#pragma unroll
+          for (int k = 0; k < NT; ++k) {
+            float2 d = t[k].fp[0];
+            if constexpr (EXCH) d = add(d, t[k].fp[1]);
+            int col = -1;
+            if ((TM & CT1) != 0 && k == S1) col = p.col_t1;
+            if ((TM & CT2) != 0 && k == S2) col = p.col_t2;
+            if ((TM & CB1) != 0 && k == SB) col = p.col_b1;
+            if (col >= 0) row[(1 + col) << k_log2] = mul(fm0, d);
+          }
+          if (p.col_m0 >= 0) row[(1 + p.col_m0) << k_log2] = mul(factor, mul(total, demod));
+        }
+        ++nrec;
+        if ((nrec & (stage_k - 1)) == 0) flush(stage_k);
+      } else if (sub == 0 && live) {
         float sp, cp;
         sincosf(-ev.phase, &sp, &cp);
         const float2 demod = mk(cp, sp);
@@ -531,13 +606,15 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
       }
     }
   }
+  if (staging && (nrec & (stage_k - 1)) != 0) flush((int)(nrec & (stage_k - 1)));
 }
 
-using ComplexLaunch = void (*)(const ComplexArgs&, dim3, dim3, hipStream_t);
+// Launchers take the dynamic LDS bytes of the output stage.
+using ComplexLaunch = void (*)(const ComplexArgs&, dim3, dim3, size_t, hipStream_t);
 
 template <int L, int TM, bool EXCH>
-void launch_complex(const ComplexArgs& args, dim3 grid, dim3 block, hipStream_t stream) {
-  hipLaunchKernelGGL((complex_kernel<L, TM, EXCH>), grid, block, 0, stream, args);
+void launch_complex(const ComplexArgs& args, dim3 grid, dim3 block, size_t shared, hipStream_t stream) {
+  hipLaunchKernelGGL((complex_kernel<L, TM, EXCH>), grid, block, shared, stream, args);
 }
 
 template <int L, int TM>

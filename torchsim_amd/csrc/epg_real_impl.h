@@ -527,7 +527,9 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   // Output staging (see STAGE_BYTES_PER_WAVE). `nrec` counts the records so
   // far; every stage_k of them the wave writes its stage out.
   extern __shared__ float2 stage_memory[];
-  const bool staged = !ACCUM && p.stage_k_log2 > 0 && (load_op(ops, 0).pad & OPS_SEQUENTIAL) != 0;
+  // Staged writes (outputs) or, in normal-equation mode, staged reads of the
+  // measured samples: the same per-atom-row traffic, the other direction.
+  const bool staged = p.stage_k_log2 > 0 && (load_op(ops, 0).pad & OPS_SEQUENTIAL) != 0;
   const int stage_k = 1 << (p.stage_k_log2 > 0 ? p.stage_k_log2 : 0);
   const int lane = (int)(threadIdx.x & 63);
   const int wave_group = lane / L;
@@ -572,6 +574,20 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
     }
   };
 
+  // Normal-equation mode: load samples nrec ... nrec + K - 1 of every atom
+  // of the wave into its stage (one row per atom, K consecutive samples read
+  // together).
+  auto fill = [&]() __attribute__((always_inline)) {
+    const int64_t g0 = group - wave_group;
+    const int total = (64 / L) << p.stage_k_log2;
+    for (int i = lane; i < total; i += 64) {
+      const int kk = i & (stage_k - 1);
+      const int64_t at = g0 + (i >> p.stage_k_log2);
+      const int64_t o = nrec + kk;
+      if (at < p.atoms && o < p.outputs) stage[i] = p.data[at * p.outputs + o];
+    }
+  };
+
   // The sample i m0 F+[0] and its tangents (reference record,
   // _epg_cpu.cpp:3090-3120); F+[0] is register R of the first lane.
   auto record = [&](auto rc, const MacroOp& ev) __attribute__((always_inline)) {
@@ -586,7 +602,8 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
       const float sign = p.sample_sign ? p.sample_sign[o] : 1.0f;
       const float fr = p.factor_re * sign, fi = p.factor_im * sign;
       const float v = m0 * a0;
-      const float2 y = p.data[atom * p.outputs + o];
+      const float2 y = staged ? stage[(wave_group << p.stage_k_log2) + (nrec & (stage_k - 1))]
+                              : p.data[atom * p.outputs + o];
       const double rx = (double)(-fi * v) - (double)y.x;
       const double ry = (double)(fr * v) - (double)y.y;
       acc_cost += rx * rx + ry * ry;
@@ -740,10 +757,15 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
       angle(next, next_cs, next_sn);
       if (ctl & OP_RECORD) {
         if (m.out >= 0 && m.out < p.outputs) {
+          if constexpr (ACCUM) {
+            if (staged && (nrec & (stage_k - 1)) == 0) fill();
+          }
           record(rc, m);
           if (staged) {
             ++nrec;
-            if ((nrec & (stage_k - 1)) == 0) flush(stage_k);
+            if constexpr (!ACCUM) {
+              if ((nrec & (stage_k - 1)) == 0) flush(stage_k);
+            }
           }
         }
       }
@@ -761,7 +783,9 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
     while (segment(std::integral_constant<int, 0>{})) {
     }
   }
-  if (staged && (nrec & (stage_k - 1)) != 0) flush((int)(nrec & (stage_k - 1)));
+  if constexpr (!ACCUM) {
+    if (staged && (nrec & (stage_k - 1)) != 0) flush((int)(nrec & (stage_k - 1)));
+  }
 
   if constexpr (ACCUM) {
     if (first && live) {

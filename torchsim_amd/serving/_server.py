@@ -320,6 +320,7 @@ class Server:
         self._stop = threading.Event()
         self._thread: threading.Thread | None = None
         self.latencies: list[float] = []
+        self.host_seconds = {"admit": 0.0, "step": 0.0, "batched": 0.0, "unbatched": 0.0, "complete": 0.0}
         self.launches = 0
         self.batched_jobs = 0
         self.completed = 0
@@ -378,6 +379,8 @@ class Server:
             "launches": self.launches,
             "batched_jobs": self.batched_jobs,
             "hbm_budget": self.hbm_budget,
+            # host seconds the scheduler spent building, launching and retiring work
+            "host_s": dict(self.host_seconds),
         }
 
     # -- the loop ------------------------------------------------------------
@@ -386,9 +389,15 @@ class Server:
         torch.cuda.set_device(self.device)
         while not self._stop.is_set() or self._waiting or self._active or self._finishing or not self._incoming.empty():
             busy = False
+            t0 = time.perf_counter()
             busy |= self._admit()
+            t1 = time.perf_counter()
             busy |= self._step()
+            t2 = time.perf_counter()
             busy |= self._complete()
+            self.host_seconds["admit"] += t1 - t0
+            self.host_seconds["step"] += t2 - t1
+            self.host_seconds["complete"] += time.perf_counter() - t2
             if not busy:
                 time.sleep(self.tick)
 
@@ -455,6 +464,7 @@ class Server:
             key = (mode, *item.variant(), _tile_key(item))
             groups.setdefault(key, []).append((ticket, item))
         self._active = [t for t in self._active if t.program is not None]
+        batched_from = time.perf_counter()
         for key, members in groups.items():
             stream = self.streams[next(self._rotation)]
             for ticket, _ in members:
@@ -478,6 +488,8 @@ class Server:
                 continue
             self.launches += 1
             self.batched_jobs += len(members)
+        self.host_seconds["batched"] += time.perf_counter() - batched_from
+        unbatched_from = time.perf_counter()
         for ticket in unbatched:
             try:
                 with torch.cuda.stream(ticket.stream):
@@ -487,6 +499,7 @@ class Server:
             except Exception as error:
                 self._fail(ticket, error)
                 ticket.program = None
+        self.host_seconds["unbatched"] += time.perf_counter() - unbatched_from
         self._active = [t for t in self._active if t.program is not None]
         done = [t for t in self._active if t.program.finished]
         for ticket in done:
