@@ -226,6 +226,33 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
 /* sizeof(tsim_batch_request) as compiled into the library. */
 int tsim_batch_request_size(void);
 
+/*
+ * Dictionary matching (replaces DictionaryMatcher._match_here with its
+ * correlate / topk chunks, _density and _scaled: reference
+ * estimators/_dictionary.py:308-431, estimators/_grouped.py:219-245).
+ * A dictionary is laid out once as two fp16 operand planes of
+ * atoms_pad x k_pad halves each (tsim_match_operand_size), with its FP64 row
+ * norms; tsim_match then scores every voxel against every atom on the MFMA
+ * (normalized |<y, d>|, never stored), keeps the best top_k + 1 per voxel,
+ * recomputes those exactly in FP64 and writes, per voxel and rank, the atom
+ * index (int64), score (float32), complex least-squares scale (complex64)
+ * and density (float32). Ties rank the lower atom index first.
+ * mode: 0 real dictionary (float32); complex64 dictionaries 1 (imaginary
+ * parts all zero), 2 (real parts all zero) or 3 (general), chosen from the
+ * flags tsim_match_planes ORs into a device int (bit 0: some real part is
+ * nonzero, bit 1: some imaginary part is). Signals are float32 or complex64
+ * rows of `contrasts` values. top_k 1..3; splits <= 0 picks the split of the
+ * dictionary across workgroups.
+ */
+int tsim_match_operand_size(int64_t atoms, int64_t contrasts, int mode, int64_t* atoms_pad, int64_t* k_pad);
+int tsim_match_planes(const void* dictionary, int64_t atoms, int64_t contrasts, int* flags, void* stream);
+int tsim_match_layout_atoms(const void* dictionary, int dictionary_complex, int64_t atoms, int64_t contrasts,
+                            int mode, double* atom_norm, void* hi, void* lo, void* stream);
+int tsim_match(const void* atoms_hi, const void* atoms_lo, int mode, int64_t atoms, const void* dictionary,
+               int dictionary_complex, const double* atom_norm, const void* signals, int signals_complex,
+               int64_t voxels, int64_t contrasts, int top_k, int64_t splits, int64_t* indices, float* scores,
+               void* scales, float* densities, void* stream);
+
 /* Number of state tiles compiled into the library, and the shape of one:
  * orders per lane and lanes per atom. No reference counterpart (tuning). */
 int tsim_tile_count(void);

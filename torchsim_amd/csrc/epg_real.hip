@@ -881,28 +881,63 @@ int tsim_normal_stride(void) { return NORMAL_STRIDE; }
 
 namespace {
 
-// Pinned staging for the request and work tables: freed once the copy that
-// reads it has completed (checked at the next batch call), so a batch launch
-// never waits on the host.
+// Pinned staging for the request and work tables. Buffers are allocated once
+// and reused: a buffer returns to the free list when the copy that read it
+// has completed (checked at the next batch call). Allocating and freeing
+// pinned memory per launch would stall the host (hipHostFree waits for the
+// device), so nothing is freed while the library is loaded.
 struct Staged {
-  void* host;
-  hipEvent_t copied;
+  char* host = nullptr;
+  size_t bytes = 0;
+  hipEvent_t copied = nullptr;
 };
 std::mutex g_staged_mutex;
-std::vector<Staged> g_staged;
+std::vector<Staged> g_in_flight;
+std::vector<Staged> g_landed;
 
-void reap_staged() {
+int take_staging(size_t bytes, Staged& out) {
   std::lock_guard<std::mutex> lock(g_staged_mutex);
-  for (size_t i = 0; i < g_staged.size();) {
-    if (hipEventQuery(g_staged[i].copied) == hipSuccess) {
-      (void)hipHostFree(g_staged[i].host);
-      (void)hipEventDestroy(g_staged[i].copied);
-      g_staged[i] = g_staged.back();
-      g_staged.pop_back();
+  for (size_t i = 0; i < g_in_flight.size();) {
+    if (hipEventQuery(g_in_flight[i].copied) == hipSuccess) {
+      g_landed.push_back(g_in_flight[i]);
+      g_in_flight[i] = g_in_flight.back();
+      g_in_flight.pop_back();
     } else {
       ++i;
     }
   }
+  size_t best = g_landed.size();
+  for (size_t i = 0; i < g_landed.size(); ++i)
+    if (g_landed[i].bytes >= bytes && (best == g_landed.size() || g_landed[i].bytes < g_landed[best].bytes)) best = i;
+  if (best < g_landed.size()) {
+    out = g_landed[best];
+    g_landed[best] = g_landed.back();
+    g_landed.pop_back();
+    return OK;
+  }
+  size_t size = 65536;
+  while (size < bytes) size <<= 1;
+  Staged fresh;
+  int st = hip_status(hipHostMalloc((void**)&fresh.host, size, hipHostMallocDefault), "hipHostMalloc(staging)");
+  if (st != OK) return st;
+  fresh.bytes = size;
+  st = hip_status(hipEventCreateWithFlags(&fresh.copied, hipEventDisableTiming), "hipEventCreate");
+  if (st != OK) {
+    g_landed.push_back(Staged{fresh.host, fresh.bytes, nullptr});
+    return st;
+  }
+  out = fresh;
+  return OK;
+}
+
+// Hand a buffer back after its copy was enqueued (its event recorded), or,
+// with `landed`, when nothing was enqueued from it.
+void give_staging(const Staged& used, bool landed) {
+  std::lock_guard<std::mutex> lock(g_staged_mutex);
+  if (landed || used.copied == nullptr)
+    g_landed.push_back(used);
+  else
+    g_in_flight.push_back(used);
 }
 
 }  // namespace
@@ -1051,28 +1086,27 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
     op_offset += op_capacity(q.event_count);
     ++k;
   }
-  reap_staged();
-  char* staging = nullptr;
-  hipEvent_t copied = nullptr;
-  if (st == OK)
-    st = hip_status(hipHostMalloc((void**)&staging, w_off + work_bytes, hipHostMallocDefault), "hipHostMalloc(staging)");
+  Staged staging;
+  bool have_staging = false;
   if (st == OK) {
-    memcpy(staging + a_off, args.data(), args_bytes);
-    memcpy(staging + w_off, work.data(), work_bytes);
-    st = hip_status(hipMemcpyAsync(space + a_off, staging + a_off, w_off + work_bytes - a_off, hipMemcpyHostToDevice, s),
-                    "hipMemcpyAsync(batch tables)");
+    st = take_staging(w_off + work_bytes, staging);
+    have_staging = st == OK;
   }
-  if (st == OK) st = hip_status(hipEventCreateWithFlags(&copied, hipEventDisableTiming), "hipEventCreate");
-  if (st == OK) st = hip_status(hipEventRecord(copied, s), "hipEventRecord");
-  if (staging) {
-    if (st == OK) {
-      std::lock_guard<std::mutex> lock(g_staged_mutex);
-      g_staged.push_back(Staged{staging, copied});
-    } else {
-      (void)hipStreamSynchronize(s);
-      (void)hipHostFree(staging);
-      if (copied) (void)hipEventDestroy(copied);
-    }
+  bool enqueued = false;
+  if (st == OK) {
+    memcpy(staging.host + a_off, args.data(), args_bytes);
+    memcpy(staging.host + w_off, work.data(), work_bytes);
+    st = hip_status(hipMemcpyAsync(space + a_off, staging.host + a_off, w_off + work_bytes - a_off,
+                                   hipMemcpyHostToDevice, s),
+                    "hipMemcpyAsync(batch tables)");
+    enqueued = st == OK;
+  }
+  if (st == OK) st = hip_status(hipEventRecord(staging.copied, s), "hipEventRecord");
+  if (have_staging) {
+    // a copy that was enqueued but whose event could not be recorded is waited
+    // for here before the buffer can be reused
+    if (enqueued && st != OK) (void)hipStreamSynchronize(s);
+    give_staging(staging, st != OK);
   }
   if (st == OK) {
     const int64_t blocks = (int64_t)work.size();

@@ -1,5 +1,6 @@
 """Estimators over the fused kernels."""
 
+from ._dictionary import DictionaryMatcher, MatchResult
 from ._nlls import NonlinearLeastSquares
 
-__all__ = ["NonlinearLeastSquares"]
+__all__ = ["DictionaryMatcher", "MatchResult", "NonlinearLeastSquares"]

@@ -16,6 +16,7 @@ per property.
 
 from __future__ import annotations
 
+import threading
 from collections import OrderedDict
 from collections.abc import Mapping, Sequence
 from copy import copy as shallow_copy
@@ -74,7 +75,9 @@ def damping_scale(sequence: Mapping[str, Any]) -> float:
 
 
 _PACKED_CACHE: OrderedDict = OrderedDict()
-_PACKED_CACHE_SIZE = 64
+_PACKED_CACHE_SIZE = 256
+_PACKED_MEMO_SIZE = 8
+_PACKED_LOCK = threading.Lock()
 
 
 def _protocol_key(kind: type, sequence: Mapping[str, Any]) -> tuple | None:
@@ -205,17 +208,28 @@ class Simulator:
         model/_binding.py:94-139); later calls with equal arguments reuse it.
         """
         key = _protocol_key(type(self), sequence)
-        if key is not None:
+        if key is None:
+            return self._packed(**sequence)
+        # the protocols this simulator has played stay with it (a request in
+        # flight never loses its stream to the shared cache's eviction)
+        memo = self.__dict__.setdefault("_packed_memo", {})
+        cached = memo.get(key)
+        if cached is not None:
+            return cached
+        with _PACKED_LOCK:
             cached = _PACKED_CACHE.get(key)
             if cached is not None:
                 _PACKED_CACHE.move_to_end(key)
-                return cached
-        built = self._packed(**sequence)
-        if key is not None:
-            _PACKED_CACHE[key] = built
-            while len(_PACKED_CACHE) > _PACKED_CACHE_SIZE:
-                _PACKED_CACHE.popitem(last=False)
-        return built
+        if cached is None:
+            cached = self._packed(**sequence)
+            with _PACKED_LOCK:
+                _PACKED_CACHE[key] = cached
+                while len(_PACKED_CACHE) > _PACKED_CACHE_SIZE:
+                    _PACKED_CACHE.popitem(last=False)
+        if len(memo) >= _PACKED_MEMO_SIZE:
+            memo.clear()
+        memo[key] = cached
+        return cached
 
     def _packed(self, **sequence: Any) -> PackedEvents:
         played = self.played(**sequence)
@@ -264,7 +278,7 @@ class Simulator:
         differentiated = tuple(self._tissue_fields({name: held[name] for name in diff}).keys())
         tissue = _engine.prepare_tissue(fields, device, differentiated)
         states = int(sequence.get("states", self.states) or 10)
-        real, flip, samples = _engine.real_subspace(packed, tissue)
+        real, flip, samples = _engine.real_subspace(packed, tissue, device)
         factor = self.signal_factor if self.signal_factor is not None else 1.0
         if not real:
             from ..sequence import _complex

@@ -42,6 +42,17 @@ class FusedModel:
         self.packed = simulator.packed(**sequence)
         self.states = int(sequence.get("states", simulator.states) or 10)
 
+    def held_on(self, device: torch.device) -> dict[str, Any]:
+        """The fixed properties as the kernels take them: per-voxel tensors
+        moved to ``device`` once, numbers and host scalars left as they are
+        (they become device fills, with no wait on the stream)."""
+        cache = self.__dict__.setdefault("_held_device", {})
+        key = str(device)
+        if key not in cache:
+            cache[key] = {name: value.to(device) if isinstance(value, torch.Tensor) and value.numel() > 1 else value
+                          for name, value in self.held.items()}
+        return cache[key]
+
     @property
     def columns(self) -> int:
         return len(self.names)
@@ -60,28 +71,27 @@ class FusedModel:
         device = data.device
         values = self.natural(x)
         fields: dict[str, Any] = {}
-        for name, value in self.held.items():
-            tensor = torch.as_tensor(value, device=device)
-            if tensor.numel() > 1:
-                tensor = tensor.reshape(-1)[live]
-            fields.update(self.simulator._tissue_fields({name: tensor}))
+        for name, value in self.held_on(device).items():
+            if isinstance(value, torch.Tensor) and value.numel() > 1:
+                value = value.reshape(-1)[live]
+            fields.update(self.simulator._tissue_fields({name: value}))
         for index, field in enumerate(self.fields):
             fields[field] = values[:, index].to(torch.float32)
         tissue = _engine.prepare_tissue(fields, device, self.fields)
-        real, flip, samples = _engine.real_subspace(self.packed, tissue)
+        real, flip, samples = _engine.real_subspace(self.packed, tissue, device)
         if not real or tissue.features & _native.FEATURE_DIFFUSING or self.simulator.signal_factor is None:
             return self._normal_from_jacobian(values, data, live)
         lib = _native.load()
         events = list(self.packed.on(device))
         if flip is not None:
-            events[2] = flip.to(device=device, dtype=torch.float32).contiguous()
+            events[2] = flip
         stride = lib.tsim_normal_stride()
         normal = torch.empty((tissue.atoms, stride), dtype=torch.float64, device=device)
         if tissue.atoms == 0:
             return normal
         measured = data.to(device=device, dtype=torch.complex64).contiguous()
         cols = (ctypes.c_int32 * self.columns)(*[_COLUMN_INDEX[f] for f in self.fields])
-        sign = None if samples is None else samples.to(device=device, dtype=torch.float32).contiguous()
+        sign = samples
         factor = complex(self.simulator.signal_factor)
         with torch.cuda.device(device):
             status = lib.tsim_gauss_newton_normal(
@@ -100,9 +110,8 @@ class FusedModel:
         once on this path."""
         device = data.device
         held = {}
-        for name, value in self.held.items():
-            tensor = torch.as_tensor(value, device=device)
-            held[name] = tensor.reshape(-1)[live] if tensor.numel() > 1 else tensor
+        for name, value in self.held_on(device).items():
+            held[name] = value.reshape(-1)[live] if isinstance(value, torch.Tensor) and value.numel() > 1 else value
         for index, name in enumerate(self.names):
             held[name] = values[:, index].to(torch.float32)
         signal, jacobian = self.simulator.linearize(held, self.names, **self.sequence)

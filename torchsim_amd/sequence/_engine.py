@@ -15,10 +15,11 @@ import dataclasses
 from dataclasses import dataclass
 from typing import Any
 
+import numpy as np
 import torch
 
 from .. import _native
-from ._events import INVERSION, RECORD, PackedEvents
+from ._events import INVERSION, RECORD, PackedEvents, resident
 
 #: Relaxation times are clamped here before they are used as denominators
 #: (reference sequence/_simulation.py:59, :734-736).
@@ -78,6 +79,18 @@ def _at_identity(name: str, value: Any) -> bool:
     return float(value) == _IDENTITY[name]
 
 
+def _on_device(value: Any, device: torch.device) -> torch.Tensor:
+    """A property as a float32 tensor on ``device``; numbers and host scalars
+    become a device fill (no host-to-device copy, so no wait on the stream)."""
+    if isinstance(value, torch.Tensor):
+        if value.device.type == "cpu" and value.numel() == 1 and not value.requires_grad:
+            return torch.full(value.shape, float(value.reshape(())), dtype=torch.float32, device=device)
+        return value.to(device=device, dtype=torch.float32)
+    if isinstance(value, (int, float, np.floating, np.integer)):
+        return torch.full((), float(value), dtype=torch.float32, device=device)
+    return torch.as_tensor(value, device=device).to(torch.float32)
+
+
 def prepare_tissue(
     properties: dict[str, Any],
     device: torch.device,
@@ -99,7 +112,7 @@ def prepare_tissue(
     for name, value in given.items():
         if name not in _native.TISSUE_INDEX:
             raise ValueError(f"unknown tissue property {name!r}")
-        tensor = torch.as_tensor(value, device=device).to(torch.float32)
+        tensor = _on_device(value, device)
         if name in ("t1_ms", "t2_ms") or name in differentiated or not _at_identity(name, value):
             live[name] = tensor
             features |= _FEATURE_OF.get(name, 0)
@@ -157,21 +170,32 @@ def half_turns(packed: PackedEvents) -> tuple[torch.Tensor, torch.Tensor, torch.
     return out_by, flips, samples
 
 
-def real_subspace(packed: PackedEvents, tissue: PreparedTissue) -> tuple[bool, torch.Tensor | None, torch.Tensor | None]:
-    """Whether the real-subspace kernel applies; the flip and sample signs if so."""
+def real_subspace(packed: PackedEvents, tissue: PreparedTissue,
+                  device: torch.device | None = None) -> tuple[bool, torch.Tensor | None, torch.Tensor | None]:
+    """Whether the real-subspace kernel applies; the signed flips and the
+    sample signs if so (None where no sign flips). The verdict is worked out
+    once per packed stream; with ``device``, the flips and signs come back
+    resident there (uploaded once, asynchronously)."""
     if tissue.exchanging or tissue.features & (_native.FEATURE_OFF_AXIS | _native.FEATURE_MOVING):
         return False, None, None
-    turns = half_turns(packed)
-    if turns is None:
-        return False, None, None
-    _, flips, samples = turns
-    if bool((flips != 1).any()):
-        flip = (packed.flip.detach().cpu() * flips.to(torch.float32)).to(packed.flip.device)
-    else:
-        flip = None
-    if samples is not None and bool((samples == 1).all()):
-        samples = None
-    return True, flip, samples
+    verdict = packed.__dict__.get("_real_subspace")
+    if verdict is None:
+        turns = half_turns(packed)
+        if turns is None:
+            verdict = (False, None, None)
+        else:
+            _, flips, samples = turns
+            flip = packed.flip.detach().cpu() * flips.to(torch.float32) if bool((flips != 1).any()) else None
+            if samples is not None and bool((samples == 1).all()):
+                samples = None
+            if samples is not None:
+                samples = samples.to(torch.float32)
+            verdict = (True, flip, samples)
+        packed.__dict__["_real_subspace"] = verdict
+    real, flip, samples = verdict
+    if real and device is not None and (flip is not None or samples is not None):
+        flip, samples = resident(packed, "signs", [flip, samples], device)
+    return real, flip, samples
 
 
 _COLUMN_INDEX = {

@@ -243,14 +243,71 @@ class PackedEvents:
         return PackedEvents(*moved, output_count=self.output_count)
 
     def on(self, device: torch.device) -> list[torch.Tensor]:
-        """The nine buffers on ``device``, contiguous, copied there once."""
+        """The nine buffers on ``device``, contiguous, copied there once.
+
+        The copy is asynchronous (one pinned staging buffer, one transfer on
+        the current stream); every call makes the current stream wait for it,
+        so callers use the buffers on their own stream without a host wait.
+        """
         if any(value.requires_grad for value in self.buffers()):
             return [value.to(device).contiguous() for value in self.buffers()]
-        cache = self.__dict__.setdefault("_on_device", {})
-        key = str(device)
-        if key not in cache:
-            cache[key] = [value.detach().to(device).contiguous() for value in self.buffers()]
-        return cache[key]
+        return resident(self, "events", self.buffers(), device)
+
+
+class _Upload:
+    """Device copies of some host tensors and the event that says they landed."""
+
+    def __init__(self, tensors: list[torch.Tensor | None], device: torch.device) -> None:
+        host = [None if t is None else t.detach().contiguous() for t in tensors]
+        self.values: list[torch.Tensor | None] = []
+        self.ready: torch.cuda.Event | None = None
+        if any(t is not None and t.device.type != "cpu" for t in host):
+            self.values = [None if t is None else t.to(device) for t in host]
+            return
+        offsets, total = [], 0
+        for t in host:
+            offsets.append(total)
+            if t is not None:
+                total += -(-t.numel() * t.element_size() // 256) * 256
+        if total == 0:
+            self.values = [None if t is None else t.to(device) for t in host]
+            return
+        staging = torch.empty(total, dtype=torch.uint8, pin_memory=True)
+        for t, at in zip(host, offsets):
+            if t is not None and t.numel():
+                staging[at:at + t.numel() * t.element_size()].copy_(t.reshape(-1).view(torch.uint8))
+        stream = torch.cuda.current_stream(device)
+        with torch.cuda.device(device):
+            landed = staging.to(device, non_blocking=True)
+            self.ready = torch.cuda.Event()
+            self.ready.record(stream)
+        for t, at in zip(host, offsets):
+            if t is None:
+                self.values.append(None)
+            else:
+                nbytes = t.numel() * t.element_size()
+                self.values.append(landed[at:at + nbytes].view(t.dtype).reshape(t.shape))
+
+    def wait(self, device: torch.device) -> list[torch.Tensor | None]:
+        ready = self.ready
+        if ready is not None:
+            if ready.query():
+                self.ready = None
+            else:
+                torch.cuda.current_stream(device).wait_event(ready)
+        return self.values
+
+
+def resident(owner: Any, name: str, tensors: list[torch.Tensor | None], device: torch.device) -> list:
+    """``tensors`` (host) on ``device``, uploaded once per ``owner`` and
+    ``name`` (asynchronously; the current stream waits for the copy)."""
+    cache = owner.__dict__.setdefault("_resident", {})
+    key = (name, str(device))
+    upload = cache.get(key)
+    if upload is None:
+        upload = _Upload(list(tensors), device)
+        cache[key] = upload
+    return upload.wait(device)
 
 
 def host_values(*arrays: Any) -> tuple[Any, ...]:

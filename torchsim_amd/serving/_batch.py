@@ -55,7 +55,9 @@ def launch_batch(items: list[BatchItem], *, mode: int, stream: torch.cuda.Stream
     keep = []
     device = stream.device
     for entry, item in zip(table, items):
-        events = list(item.packed.on(device))
+        with torch.cuda.stream(stream):
+            # (the launch stream waits for the event buffers' upload, if any)
+            events = list(item.packed.on(device))
         if item.flip is not None:
             events[2] = item.flip
         buffers = [*item.tissue.buffers, *events]

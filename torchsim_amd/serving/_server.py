@@ -107,17 +107,14 @@ class _SimulateProgram:
         batchable = sim.signal_factor is not None and "diffusion_um2_per_ms" not in fields
         if batchable:
             tissue = _engine.prepare_tissue(fields, device, differentiated)
-            real, flip, samples = _engine.real_subspace(packed, tissue)
+            real, flip, samples = _engine.real_subspace(packed, tissue, device)
             if real:
                 states = int(sequence.get("states", sim.states) or 10)
                 atoms, n = tissue.atoms, packed.output_count
                 signal = torch.empty((atoms, n), dtype=torch.complex64, device=device)
                 jacobian = torch.empty((atoms, len(differentiated), n), dtype=torch.complex64, device=device)
-                self.batch_item = BatchItem(
-                    packed, tissue, states, differentiated, sim.signal_factor,
-                    flip=None if flip is None else flip.to(device=device, dtype=torch.float32).contiguous(),
-                    samples=None if samples is None else samples.to(device=device, dtype=torch.float32).contiguous(),
-                    signal=signal, jacobian=jacobian)
+                self.batch_item = BatchItem(packed, tissue, states, differentiated, sim.signal_factor,
+                                            flip=flip, samples=samples, signal=signal, jacobian=jacobian)
                 self.batch = sim._batch(held)
 
     def job(self) -> tuple[int, BatchItem] | None:
@@ -165,8 +162,9 @@ class _FitProgram:
             -1, request.measured.shape[-1]).contiguous()
         voxels = self.data.shape[0]
         self.live = torch.arange(voxels, device=device)
-        start = torch.tensor([float(request.initial.get(n, 1.0)) for n in request.unknown], dtype=torch.float64,
-                             device=device)
+        # the start as device fills (no host-to-device copy, no wait)
+        start = torch.stack([torch.full((), float(request.initial.get(n, 1.0)), dtype=torch.float64, device=device)
+                             for n in request.unknown])
         self.x = self.op.free(start[None, :].expand(voxels, -1)).contiguous()
         self.iteration = -1  # -1: the start point still needs its normal equations
         self.normal = None
@@ -181,11 +179,11 @@ class _FitProgram:
     def _fusable(self) -> bool:
         try:
             fields = {}
-            for name, value in self.op.held.items():
+            for name, value in self.op.held_on(self.data.device).items():
                 fields.update(self.op.simulator._tissue_fields({name: value}))
             if "diffusion_um2_per_ms" in fields or self.op.simulator.signal_factor is None:
                 return False
-            probe = {f: torch.ones(1) for f in self.op.fields}
+            probe = {f: 1.0 for f in self.op.fields}
             tissue = _engine.prepare_tissue({**fields, **probe}, self.data.device, self.op.fields)
             return _engine.real_subspace(self.op.packed, tissue)[0]
         except (NotImplementedError, ValueError):
@@ -195,18 +193,16 @@ class _FitProgram:
         device = self.data.device
         values = self.op.natural(x)
         fields: dict[str, Any] = {}
-        for name, value in self.op.held.items():
-            fields.update(self.op.simulator._tissue_fields({name: torch.as_tensor(value, device=device)}))
+        for name, value in self.op.held_on(device).items():
+            fields.update(self.op.simulator._tissue_fields({name: value}))
         for index, f in enumerate(self.op.fields):
             fields[f] = values[:, index].to(torch.float32)
         tissue = _engine.prepare_tissue(fields, device, self.op.fields)
-        real, flip, samples = _engine.real_subspace(self.op.packed, tissue)
+        real, flip, samples = _engine.real_subspace(self.op.packed, tissue, device)
         stride = _native.load().tsim_normal_stride()
         normal = torch.empty((tissue.atoms, stride), dtype=torch.float64, device=device)
         return BatchItem(self.op.packed, tissue, self.op.states, self.op.fields, self.op.simulator.signal_factor,
-                         flip=None if flip is None else flip.to(device=device, dtype=torch.float32).contiguous(),
-                         samples=None if samples is None else samples.to(device=device, dtype=torch.float32).contiguous(),
-                         data=self.data, normal=normal)
+                         flip=flip, samples=samples, data=self.data, normal=normal)
 
     def job(self) -> tuple[int, BatchItem] | None:
         if self.finished or not self.fused:
@@ -250,16 +246,19 @@ class _FitProgram:
         self.iteration += 1
 
     def run_unbatched(self) -> None:
-        """Any sequence: the same algorithm with the normal equations formed
-        from the simulator's fused signal + Jacobian."""
-        while not self.finished:
-            x = self.x if self.iteration < 0 else None
-            if x is None:
-                self._prepare_step()
-                x = self.candidate
-            normal = self.op.normal(x, self.data, self.live)
-            self._pending = None
-            self.after_job(normal)
+        """Any sequence: one iteration of the same algorithm, with the normal
+        equations formed from the simulator's fused signal + Jacobian (the
+        scheduler calls this once per tick, so other requests interleave)."""
+        if self.finished:
+            return
+        if self.iteration < 0:
+            x = self.x
+        else:
+            self._prepare_step()
+            x = self.candidate
+        normal = self.op.normal(x, self.data, self.live)
+        self._pending = None
+        self.after_job(normal)
 
     @property
     def finished(self) -> bool:

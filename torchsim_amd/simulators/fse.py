@@ -8,6 +8,7 @@ from typing import Any
 import torch
 
 from ..model._simulator import Simulator, SpinPhysics
+from ..sequence import _engine
 from ..sequence._events import REFOCUSED, host_values
 from .mrf import _matched
 
@@ -44,11 +45,12 @@ class FSESimulator(Simulator):
         echoes = torch.atleast_1d(torch.as_tensor(played["flip"])).shape[-1]
         left_ms = torch.as_tensor(played.get("TR", 1e6), dtype=torch.float32) - torch.as_tensor(
             played["ESP"], dtype=torch.float32) * echoes
-        t1 = torch.as_tensor(properties["T1"], dtype=torch.float32, device=device).reshape(-1, 1)
-        left_ms = left_ms.to(device)
+        t1 = _engine._on_device(properties["T1"], device).reshape(-1, 1)
+        if left_ms.dim() != 0 or left_ms.requires_grad:
+            left_ms = left_ms.to(device)
+        # (a 0-dim host tensor enters device arithmetic as a scalar operand)
         recovered = torch.exp(-1e3 / t1 * left_ms * 1e-3)
-        density = properties.get("M0", 1.0)
-        density = torch.as_tensor(density, dtype=torch.float32, device=device).reshape(-1, 1)
+        density = _engine._on_device(properties.get("M0", 1.0), device).reshape(-1, 1)
         return recovered, density, left_ms, t1
 
     def evaluate(self, properties: Mapping[str, Any], **sequence: Any) -> torch.Tensor:
