@@ -78,14 +78,24 @@ def _compile(hipcc: str, source: Path, defines: list[str], target: Path) -> None
         )
 
 
-def build(jobs: int | None = None, verbose: bool = False) -> Path:
-    """Compile every HIP source for gfx950 and link the shared library."""
+def _stale(source: Path, target: Path) -> bool:
+    """Whether `target` is older than its source or any header it may include."""
+    if not target.exists():
+        return True
+    newest = max([source.stat().st_mtime] + [h.stat().st_mtime for h in CSRC.glob("*.h")])
+    return target.stat().st_mtime < newest
+
+
+def build(jobs: int | None = None, verbose: bool = False, incremental: bool = False) -> Path:
+    """Compile every HIP source for gfx950 and link the shared library
+    (with `incremental`, only the units older than their sources)."""
     hipcc = _hipcc()
     BUILD.mkdir(parents=True, exist_ok=True)
     units = _units()
+    todo = [unit for unit in units if not incremental or _stale(unit[0], unit[2])]
     jobs = jobs or min(16, max(1, len(os.sched_getaffinity(0))))
     with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
-        futures = [pool.submit(_compile, hipcc, *unit) for unit in units]
+        futures = [pool.submit(_compile, hipcc, *unit) for unit in todo]
         for future in futures:
             future.result()
     objects = [str(target) for _, _, target in units]
@@ -101,4 +111,4 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)

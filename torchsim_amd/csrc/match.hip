@@ -41,14 +41,18 @@ constexpr int KSTEP = 16;              // k per MFMA (v_mfma_f32_32x32x16_f16)
 constexpr int TILE = 32;               // MFMA rows (16 voxels x 2 projections) and columns (atoms)
 constexpr int VOXELS_PER_TILE = 16;
 constexpr int ATOM_TILES = 4;          // per step: 128 atoms
+// The 4 waves of a workgroup share its voxel rows and take consecutive
+// 128-atom tiles of each 512-atom step: the voxel operand is fetched once per
+// workgroup step (and reused from cache by the other three waves) instead of
+// once per wave.
 constexpr int WAVES = 4;
-constexpr int ATOMS_PER_STEP = ATOM_TILES * TILE;
+constexpr int ATOMS_PER_STEP = WAVES * ATOM_TILES * TILE;
 // Row tiles per wave: 2 (32 voxels) when two candidates are kept per voxel,
 // 1 (16 voxels) with four, so lists and accumulators fit without spilling.
 template <int KC>
 constexpr int row_tiles() { return KC <= 2 ? 2 : 1; }
 template <int KC>
-constexpr int voxels_per_block() { return WAVES * row_tiles<KC>() * VOXELS_PER_TILE; }
+constexpr int voxels_per_block() { return row_tiles<KC>() * VOXELS_PER_TILE; }
 constexpr float SPLIT_SCALE = 256.0f;
 constexpr int MAX_CANDIDATES_PER_LANE = 8;  // refine: splits * KC <= 512
 constexpr float NORM_EPS = 1.1920928955078125e-07f;  // float32 epsilon, the reference's norm floor
@@ -241,7 +245,7 @@ __global__ __launch_bounds__(256) void match_candidates(const half8* __restrict_
   constexpr int ROW_TILES = row_tiles<KC>();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int h = lane >> 5, col = lane & 31;
-  const int64_t row_tile0 = ((int64_t)blockIdx.x * WAVES + wave) * ROW_TILES;
+  const int64_t row_tile0 = (int64_t)blockIdx.x * ROW_TILES;
   const int64_t split = blockIdx.y;
   const int64_t begin = split * steps_per_split;
   const int64_t end = begin + steps_per_split < atom_steps ? begin + steps_per_split : atom_steps;
@@ -267,7 +271,7 @@ __global__ __launch_bounds__(256) void match_candidates(const half8* __restrict_
   }
 
   for (int64_t step = begin; step < end; ++step) {
-    const int64_t tile0 = step * ATOM_TILES;
+    const int64_t tile0 = (step * WAVES + wave) * ATOM_TILES;
     const half8* dh = d_hi + tile0 * ks * 64 + lane;
     const half8* dl = d_lo + tile0 * ks * 64 + lane;
     const int64_t tile_stride = ks * 64;
@@ -277,56 +281,58 @@ __global__ __launch_bounds__(256) void match_candidates(const half8* __restrict_
 #pragma unroll
       for (int c = 0; c < ATOM_TILES; ++c) acc[r][c] = floatx16{};
 
-    half8 ah[ROW_TILES], al[ROW_TILES], bh[ATOM_TILES], bl[ATOM_TILES];
-#pragma unroll
-    for (int r = 0; r < ROW_TILES; ++r) {
-      ah[r] = qh[r][0];
-      al[r] = ql[r][0];
-    }
-#pragma unroll
-    for (int c = 0; c < ATOM_TILES; ++c) {
-      bh[c] = dh[c * tile_stride];
-      bl[c] = dl[c * tile_stride];
-    }
-    for (int64_t s = 0; s < ks; ++s) {
-      // next k-step's operands in flight while this one multiplies
-      const int64_t n = s + 1 < ks ? s + 1 : s;
-      half8 nah[ROW_TILES], nal[ROW_TILES], nbh[ATOM_TILES], nbl[ATOM_TILES];
+    // two operand buffers: the loads of k-step s + 2 are issued as soon as
+    // step s has been multiplied, so a load has two steps of MFMA work
+    // (~1,500 cycles) to land in; the depth is padded to an even step count
+    struct Operands {
+      half8 ah[ROW_TILES], al[ROW_TILES], bh[ATOM_TILES], bl[ATOM_TILES];
+    };
+    auto load = [&](Operands& o, int64_t step_k) {
+      const int64_t k = (step_k < ks ? step_k : ks - 1) * 64;  // clamped: no branch at the tail
 #pragma unroll
       for (int r = 0; r < ROW_TILES; ++r) {
-        nah[r] = qh[r][n * 64];
-        nal[r] = ql[r][n * 64];
+        o.ah[r] = qh[r][k];
+        o.al[r] = ql[r][k];
       }
 #pragma unroll
       for (int c = 0; c < ATOM_TILES; ++c) {
-        nbh[c] = dh[c * tile_stride + n * 64];
-        nbl[c] = dl[c * tile_stride + n * 64];
+        o.bh[c] = dh[c * tile_stride + k];
+        o.bl[c] = dl[c * tile_stride + k];
       }
+    };
+    auto multiply = [&](const Operands& o) {
 #pragma unroll
       for (int r = 0; r < ROW_TILES; ++r)
 #pragma unroll
         for (int c = 0; c < ATOM_TILES; ++c)
-          acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[r], bh[c], acc[r][c], 0, 0, 0);
+          acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(o.ah[r], o.bh[c], acc[r][c], 0, 0, 0);
 #pragma unroll
       for (int r = 0; r < ROW_TILES; ++r)
 #pragma unroll
         for (int c = 0; c < ATOM_TILES; ++c)
-          acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[r], bl[c], acc[r][c], 0, 0, 0);
+          acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(o.ah[r], o.bl[c], acc[r][c], 0, 0, 0);
 #pragma unroll
       for (int r = 0; r < ROW_TILES; ++r)
 #pragma unroll
         for (int c = 0; c < ATOM_TILES; ++c)
-          acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[r], bh[c], acc[r][c], 0, 0, 0);
-#pragma unroll
-      for (int r = 0; r < ROW_TILES; ++r) {
-        ah[r] = nah[r];
-        al[r] = nal[r];
-      }
-#pragma unroll
-      for (int c = 0; c < ATOM_TILES; ++c) {
-        bh[c] = nbh[c];
-        bl[c] = nbl[c];
-      }
+          acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(o.al[r], o.bh[c], acc[r][c], 0, 0, 0);
+    };
+    Operands even, odd;
+    load(even, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    load(odd, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    // the scheduling barriers keep the compiler from sinking each load next
+    // to its use, which would put the wait for it inside the same step
+    for (int64_t s = 0; s < ks; s += 2) {
+      multiply(even);
+      __builtin_amdgcn_sched_barrier(0);
+      load(even, s + 2);
+      __builtin_amdgcn_sched_barrier(0);
+      multiply(odd);
+      __builtin_amdgcn_sched_barrier(0);
+      load(odd, s + 3);
+      __builtin_amdgcn_sched_barrier(0);
     }
     // epilogue: lane holds column `col` of each tile; registers j and j + 4
     // (j = 0..3, 8..11) are the two projections of voxel 4 h + j of the row tile
@@ -371,7 +377,8 @@ __global__ __launch_bounds__(256) void match_candidates(const half8* __restrict_
       if (col == r * 8 + q) {
         const int j = q < 4 ? q : q + 4;
         const int64_t voxel = (row_tile0 + r) * VOXELS_PER_TILE + 4 * h + j;
-        const int64_t at = (split * voxels_pad + voxel) * KC;
+        // each wave's lists are a split of their own for the refine pass
+        const int64_t at = ((split * WAVES + wave) * voxels_pad + voxel) * KC;
 #pragma unroll
         for (int i = 0; i < KC; ++i) {
           part_score[at + i] = best[r][q][i];
@@ -530,7 +537,7 @@ int run_match(const half8* d_hi, const half8* d_lo, int mode, int64_t atoms, int
                             VOXELS_PER_BLOCK;
   const int64_t blocks_x = chunk_pad / VOXELS_PER_BLOCK;
   int64_t splits = splits_hint > 0 ? splits_hint : (2048 + blocks_x - 1) / blocks_x;
-  const int64_t max_splits = (int64_t)MAX_CANDIDATES_PER_LANE * 64 / KC;
+  const int64_t max_splits = (int64_t)MAX_CANDIDATES_PER_LANE * 64 / (KC * WAVES);
   if (splits > max_splits) splits = max_splits;
   if (splits > atom_steps) splits = atom_steps;
   if (splits < 1) splits = 1;
@@ -543,7 +550,7 @@ int run_match(const half8* d_hi, const half8* d_lo, int mode, int64_t atoms, int
   double* y_norm = nullptr;
   float* p_score = nullptr;
   int* p_index = nullptr;
-  const size_t part = (size_t)splits * chunk_pad * KC;
+  const size_t part = (size_t)splits * WAVES * chunk_pad * KC;
   hipError_t err = hipMallocAsync((void**)&q_hi, q_elems * sizeof(half8), stream);
   if (err == hipSuccess) err = hipMallocAsync((void**)&q_lo, q_elems * sizeof(half8), stream);
   if (err == hipSuccess) err = hipMallocAsync((void**)&y_norm, (size_t)chunk_pad * sizeof(double), stream);
@@ -565,7 +572,7 @@ int run_match(const half8* d_hi, const half8* d_lo, int mode, int64_t atoms, int
                        dim3(256), 0, stream, q_hi, q_lo, d_hi, d_lo, ks, atoms, atom_steps, steps_per_split,
                        count_pad, p_score, p_index);
     hipLaunchKernelGGL(match_refine<KC>, dim3((unsigned)((count + 3) / 4)), dim3(256), 0, stream, p_score,
-                       p_index, splits, count_pad, count, (const void*)y, signals_complex, y_norm, dictionary,
+                       p_index, splits * WAVES, count_pad, count, (const void*)y, signals_complex, y_norm, dictionary,
                        dictionary_complex, atom_norm, contrasts, top_k, out_index + first * top_k,
                        out_score + first * top_k, out_scale + first * top_k, out_density + first * top_k);
     status = hip_status(hipGetLastError(), "match launch");
@@ -592,7 +599,8 @@ int tsim_match_operand_size(int64_t atoms, int64_t contrasts, int mode, int64_t*
   }
   const int64_t columns = mode == COMPLEX ? 2 * contrasts : contrasts;
   *atoms_pad = (atoms + ATOMS_PER_STEP - 1) / ATOMS_PER_STEP * ATOMS_PER_STEP;
-  *k_pad = (columns + KSTEP - 1) / KSTEP * KSTEP;
+  // an even number of k-steps (the candidate kernel double-buffers them)
+  *k_pad = (columns + 2 * KSTEP - 1) / (2 * KSTEP) * (2 * KSTEP);
   if (*atoms_pad > INT_MAX) {
     set_error("at most %d atoms", INT_MAX - ATOMS_PER_STEP);
     return BAD_ARGUMENT;
