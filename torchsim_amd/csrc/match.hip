@@ -287,18 +287,19 @@ __global__ __launch_bounds__(256) void match_candidates(const half8* __restrict_
     struct Operands {
       half8 ah[ROW_TILES], al[ROW_TILES], bh[ATOM_TILES], bl[ATOM_TILES];
     };
+    // loads issued in the order the MFMAs first read them, so each wait
+    // leaves the later loads of the buffer in flight
     auto load = [&](Operands& o, int64_t step_k) {
       const int64_t k = (step_k < ks ? step_k : ks - 1) * 64;  // clamped: no branch at the tail
+      o.ah[0] = qh[0][k];
 #pragma unroll
-      for (int r = 0; r < ROW_TILES; ++r) {
-        o.ah[r] = qh[r][k];
-        o.al[r] = ql[r][k];
-      }
+      for (int c = 0; c < ATOM_TILES; ++c) o.bh[c] = dh[c * tile_stride + k];
 #pragma unroll
-      for (int c = 0; c < ATOM_TILES; ++c) {
-        o.bh[c] = dh[c * tile_stride + k];
-        o.bl[c] = dl[c * tile_stride + k];
-      }
+      for (int r = 1; r < ROW_TILES; ++r) o.ah[r] = qh[r][k];
+#pragma unroll
+      for (int c = 0; c < ATOM_TILES; ++c) o.bl[c] = dl[c * tile_stride + k];
+#pragma unroll
+      for (int r = 0; r < ROW_TILES; ++r) o.al[r] = ql[r][k];
     };
     auto multiply = [&](const Operands& o) {
 #pragma unroll
