@@ -50,6 +50,68 @@ def test_batched_launch_matches_single_launches_bitwise():
         assert torch.equal(item.jacobian, jac)
 
 
+def test_batched_launch_replays_from_a_hip_graph():
+    """The uneven batch captured once into a HIP graph: every replay equals an
+    eager launch bitwise, including after the tissue changed in place."""
+    from torchsim_amd.serving import BatchItem, launch_batch
+    from torchsim_amd.sequence import _engine
+
+    items = []
+    for k, (atoms, pulses, states) in enumerate([(300, 150, 10), (1025, 220, 8), (7, 90, 10)]):
+        seq = _mrf(pulses, states)
+        t1, t2 = _cases.random_tissue(atoms, seed=60 + k)
+        tissue = _engine.prepare_tissue({"t1_ms": torch.tensor(t1), "t2_ms": torch.tensor(t2)},
+                                        torch.device("cuda", 0), ("t1_ms", "t2_ms"))
+        sig = torch.empty((atoms, pulses), dtype=torch.complex64, device="cuda")
+        jac = torch.empty((atoms, 2, pulses), dtype=torch.complex64, device="cuda")
+        items.append(BatchItem(seq.packed(**seq.bound), tissue, states, ("t1_ms", "t2_ms"), 1j, signal=sig,
+                               jacobian=jac))
+
+    def eager():
+        launch_batch(items, mode=0, stream=torch.cuda.current_stream(), tile=5)
+        torch.cuda.synchronize()
+        return [(i.signal.clone(), i.jacobian.clone()) for i in items]
+
+    def cleared():
+        for i in items:
+            i.signal.fill_(float("nan"))
+            i.jacobian.fill_(float("nan"))
+
+    print("case: eager warm-up", flush=True)
+    eager()
+    reference = eager()
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    print("case: capture", flush=True)
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph, stream=side):
+            launch_batch(items, mode=0, stream=side, tile=5)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    for replay in range(3):
+        print(f"case: replay {replay}", flush=True)
+        cleared()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        for item, (sig, jac) in zip(items, reference):
+            assert torch.equal(item.signal, sig)
+            assert torch.equal(item.jacobian, jac)
+    # new tissue values, same buffers: the graph recomputes from them
+    for item in items:
+        item.tissue.buffers[0].mul_(1.1)
+    changed = eager()
+    assert not torch.equal(changed[0][0], reference[0][0])
+    cleared()
+    torch.cuda.synchronize()
+    graph.replay()
+    torch.cuda.synchronize()
+    for item, (sig, jac) in zip(items, changed):
+        assert torch.equal(item.signal, sig)
+        assert torch.equal(item.jacobian, jac)
+
+
 def test_same_alone_and_in_the_mix():
     from torchsim_amd.serving import FitRequest, Server
 

@@ -895,6 +895,26 @@ std::mutex g_staged_mutex;
 std::vector<Staged> g_in_flight;
 std::vector<Staged> g_landed;
 
+int allocate_staging(size_t bytes, Staged& fresh) {
+  size_t size = 65536;
+  while (size < bytes) size <<= 1;
+  int st = hip_status(hipHostMalloc((void**)&fresh.host, size, hipHostMallocDefault), "hipHostMalloc(staging)");
+  if (st != OK) return st;
+  fresh.bytes = size;
+  st = hip_status(hipEventCreateWithFlags(&fresh.copied, hipEventDisableTiming), "hipEventCreate");
+  if (st != OK) fresh.copied = nullptr;
+  return st;
+}
+
+// Keep one idle buffer of at least `bytes` in reserve (called with the lock
+// held): a capture of the same batch then finds it without allocating.
+void keep_spare(size_t bytes) {
+  for (const Staged& idle : g_landed)
+    if (idle.bytes >= bytes) return;
+  Staged spare;
+  if (allocate_staging(bytes, spare) == OK) g_landed.push_back(spare);
+}
+
 int take_staging(size_t bytes, Staged& out) {
   std::lock_guard<std::mutex> lock(g_staged_mutex);
   for (size_t i = 0; i < g_in_flight.size();) {
@@ -913,20 +933,38 @@ int take_staging(size_t bytes, Staged& out) {
     out = g_landed[best];
     g_landed[best] = g_landed.back();
     g_landed.pop_back();
+    keep_spare(bytes);
     return OK;
   }
-  size_t size = 65536;
-  while (size < bytes) size <<= 1;
   Staged fresh;
-  int st = hip_status(hipHostMalloc((void**)&fresh.host, size, hipHostMallocDefault), "hipHostMalloc(staging)");
-  if (st != OK) return st;
-  fresh.bytes = size;
-  st = hip_status(hipEventCreateWithFlags(&fresh.copied, hipEventDisableTiming), "hipEventCreate");
+  const int st = allocate_staging(bytes, fresh);
   if (st != OK) {
-    g_landed.push_back(Staged{fresh.host, fresh.bytes, nullptr});
+    if (fresh.host != nullptr) g_landed.push_back(Staged{fresh.host, fresh.bytes, nullptr});
     return st;
   }
   out = fresh;
+  keep_spare(bytes);
+  return OK;
+}
+
+// A buffer a captured graph reads at every replay: kept until the library
+// is unloaded. Captures take an idle buffer (no allocation happens while a
+// stream captures), so a batch is launched once eagerly before it is captured.
+std::vector<Staged> g_retained;
+
+int take_staging_for_capture(size_t bytes, Staged& out) {
+  std::lock_guard<std::mutex> lock(g_staged_mutex);
+  size_t best = g_landed.size();
+  for (size_t i = 0; i < g_landed.size(); ++i)
+    if (g_landed[i].bytes >= bytes && (best == g_landed.size() || g_landed[i].bytes < g_landed[best].bytes)) best = i;
+  if (best == g_landed.size()) {
+    set_error("capturing a batch needs an idle staging buffer: launch the batch once before capturing it");
+    return UNSUPPORTED;
+  }
+  out = g_landed[best];
+  g_landed[best] = g_landed.back();
+  g_landed.pop_back();
+  g_retained.push_back(out);
   return OK;
 }
 
@@ -1086,9 +1124,15 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
     op_offset += op_capacity(q.event_count);
     ++k;
   }
+  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+  if (st == OK) st = hip_status(hipStreamIsCapturing(s, &capture), "hipStreamIsCapturing");
+  const bool capturing = capture != hipStreamCaptureStatusNone;
   Staged staging;
   bool have_staging = false;
-  if (st == OK) {
+  if (st == OK && capturing) {
+    // the graph replays the table copy from this buffer: it is never reused
+    st = take_staging_for_capture(w_off + work_bytes, staging);
+  } else if (st == OK) {
     st = take_staging(w_off + work_bytes, staging);
     have_staging = st == OK;
   }
@@ -1101,7 +1145,7 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
                     "hipMemcpyAsync(batch tables)");
     enqueued = st == OK;
   }
-  if (st == OK) st = hip_status(hipEventRecord(staging.copied, s), "hipEventRecord");
+  if (st == OK && have_staging) st = hip_status(hipEventRecord(staging.copied, s), "hipEventRecord");
   if (have_staging) {
     // a copy that was enqueued but whose event could not be recorded is waited
     // for here before the buffer can be reused

@@ -291,6 +291,10 @@ class _Upload:
     def wait(self, device: torch.device) -> list[torch.Tensor | None]:
         ready = self.ready
         if ready is not None:
+            if torch.cuda.is_current_stream_capturing():
+                # a graph cannot wait on a copy made outside it
+                raise RuntimeError("device buffers are still being uploaded: run the launch once, synchronize "
+                                   "and run it again before capturing it")
             if ready.query():
                 self.ready = None
             else:

@@ -47,6 +47,13 @@ def launch_batch(items: list[BatchItem], *, mode: int, stream: torch.cuda.Stream
 
     Items must share one kernel variant (see :meth:`BatchItem.variant`). The
     caller keeps every tensor alive until the stream has passed the launch.
+
+    The launch can be captured into a HIP graph (``torch.cuda.graph``) and
+    replayed: run it eagerly, synchronize and run it once more before the
+    capture (event buffers and the pinned table staging then exist, and
+    nothing is allocated on the host while the stream captures); the graph
+    keeps its copy of the request and work tables, re-reads the items'
+    tissue and writes their outputs at every replay.
     """
     if not items:
         return
