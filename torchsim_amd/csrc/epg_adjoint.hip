@@ -494,7 +494,7 @@ AdjointLaunch pick_adjoint_mask(int tm) {
 struct AdjointTile {
   int sl, l;
 };
-constexpr AdjointTile kAdjointTiles[] = {{2, 2}, {4, 2}, {5, 2}, {8, 2}, {5, 4}, {8, 4}, {16, 4}};
+constexpr AdjointTile kAdjointTiles[] = {{2, 2}, {4, 2}, {5, 2}, {8, 2}, {5, 4}, {8, 4}, {16, 4}, {3, 4}, {2, 8}};
 
 AdjointLaunch pick_adjoint(int tile, int tm) {
   switch (tile) {
@@ -504,19 +504,38 @@ AdjointLaunch pick_adjoint(int tile, int tm) {
     case 3: return pick_adjoint_mask<8, 2>(tm);
     case 4: return pick_adjoint_mask<5, 4>(tm);
     case 5: return pick_adjoint_mask<8, 4>(tm);
-    default: return pick_adjoint_mask<16, 4>(tm);
+    case 6: return pick_adjoint_mask<16, 4>(tm);
+    case 7: return pick_adjoint_mask<3, 4>(tm);
+    default: return pick_adjoint_mask<2, 8>(tm);
   }
 }
 
-int adjoint_tile(int states) {
-  // smallest covering tile, preferring exact cover
-  int best = -1, best_cover = 1 << 30;
-  for (int t = 0; t < (int)(sizeof(kAdjointTiles) / sizeof(kAdjointTiles[0])); ++t) {
+int best_cover_of(int t) { return kAdjointTiles[t].sl * kAdjointTiles[t].l; }
+
+// With enough atoms to fill the chip the smallest covering tile wins (least
+// padding). With few atoms the adjoint is a latency chain per atom -- every
+// event's recompute and reverse step walk the lane's SL orders in sequence --
+// so the tile with the fewest orders per lane among those padding at most 2x
+// wins instead (more lanes per atom, same work, a shorter chain).
+int adjoint_tile(int states, int64_t atoms) {
+  constexpr int kTiles = (int)(sizeof(kAdjointTiles) / sizeof(kAdjointTiles[0]));
+  constexpr int64_t kFillLanes = 1024 * 64 * 2;  // two waves on every SIMD
+  int smallest = -1, smallest_cover = 1 << 30;
+  for (int t = 0; t < kTiles; ++t) {
     const int cover = kAdjointTiles[t].sl * kAdjointTiles[t].l;
-    if (cover >= states && cover < best_cover) {
-      best = t;
-      best_cover = cover;
+    if (cover >= states && cover < smallest_cover) {
+      smallest = t;
+      smallest_cover = cover;
     }
+  }
+  if (smallest < 0 || atoms * kAdjointTiles[smallest].l >= kFillLanes) return smallest;
+  int best = smallest;
+  for (int t = 0; t < kTiles; ++t) {
+    const int cover = kAdjointTiles[t].sl * kAdjointTiles[t].l;
+    if (cover < states || cover > 2 * smallest_cover) continue;
+    const bool shorter = kAdjointTiles[t].sl < kAdjointTiles[best].sl;
+    const bool tighter = kAdjointTiles[t].sl == kAdjointTiles[best].sl && cover < best_cover_of(best);
+    if (shorter || tighter) best = t;
   }
   return best;
 }
@@ -566,7 +585,7 @@ extern "C" int tsim_simulate_vjp_jacobian(void* const* packed, int64_t atom_coun
         return BAD_ARGUMENT;
     }
   }
-  const int tile = adjoint_tile((int)state_count);
+  const int tile = adjoint_tile((int)state_count, atom_count);
   if (tile < 0) {
     set_error("no adjoint tile covers %lld orders", (long long)state_count);
     return UNSUPPORTED;
