@@ -58,7 +58,7 @@ def _phases(n, seed):
     return np.random.default_rng(seed).uniform(-180, 180, n)
 
 
-@pytest.mark.parametrize("states", [3, 8, 13, 20, 40, 64])
+@pytest.mark.parametrize("states", [3, 8, 10, 11, 13, 17, 20, 40, 64])
 def test_rf_phase_train_all_lane_counts(states):
     flips = _cases.benchmark_flips(60)
     packed = _packed(flips, 10.0, phases=_phases(60, states))

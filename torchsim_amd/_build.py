@@ -62,7 +62,7 @@ def _units() -> list[tuple[Path, list[str], Path]]:
                 ]
                 units.append((source, defines, BUILD / f"epg_real_tile_{ident}.o"))
         elif source.name == "epg_complex_tile.hip":
-            for lanes in (4, 8, 16, 32, 64):
+            for lanes in (4, 8, 10, 12, 16, 20, 32, 64):
                 units.append((source, [f"-DTSIM_COMPLEX_L={lanes}"], BUILD / f"epg_complex_tile_{lanes}.o"))
         else:
             units.append((source, [], BUILD / f"{source.stem}.o"))

@@ -19,8 +19,10 @@ ComplexLaunch pick_complex(int lanes, int tm, bool exch) {
   }
 }
 
+// Fewest lanes covering the orders; 10, 12 and 20 fit 6, 5 and 3 atoms in a
+// wave (60 of 64 lanes busy) where 16 and 32 would fit 4 and 2.
 int lanes_for(int64_t states) {
-  for (int l : {4, 8, 16, 32, 64})
+  for (int l : {4, 8, 10, 12, 16, 20, 32, 64})
     if (states <= l) return l;
   return -1;
 }
@@ -89,7 +91,8 @@ int run_complex(ComplexArgs& a, int tm, bool exch, hipStream_t stream) {
     return UNSUPPORTED;
   }
   const int threads = 256;
-  const int64_t blocks = (a.atoms * lanes + threads - 1) / threads;
+  const int64_t atoms_per_block = (int64_t)(threads / 64) * (64 / lanes);
+  const int64_t blocks = (a.atoms + atoms_per_block - 1) / atoms_per_block;
   if (blocks > 0x7FFFFFFF) {
     set_error("too many atoms for one launch: %lld", (long long)a.atoms);
     return BAD_ARGUMENT;

@@ -156,9 +156,13 @@ __device__ void expm2(const float2 (&m)[4], const float2 (&e)[4], float2 (&x)[4]
 }  // namespace cx
 
 // Cross-lane moves of one value inside an atom's group of L lanes.
+// DPP row moves need every atom inside one 16-lane row: L a divisor of 16.
+template <int L>
+constexpr bool dpp_lanes() { return L <= 16 && 16 % L == 0; }
+
 template <int L>
 __device__ __forceinline__ float from_lower(float v, int lane) {
-  if constexpr (L <= 16) {
+  if constexpr (dpp_lanes<L>()) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, true));
   } else {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane + 63) & 63) << 2, __float_as_int(v)));
@@ -166,7 +170,7 @@ __device__ __forceinline__ float from_lower(float v, int lane) {
 }
 template <int L>
 __device__ __forceinline__ float from_upper(float v, int lane) {
-  if constexpr (L <= 16) {
+  if constexpr (dpp_lanes<L>()) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xF, 0xF, true));
   } else {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane + 1) & 63) << 2, __float_as_int(v)));
@@ -211,10 +215,14 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
   constexpr int SB = cslot(TM, CB1);
   using namespace cx;
 
+  // A wave holds 64 / L atoms on adjacent lanes; with L not a divisor of 64
+  // the last 64 mod L lanes are idle (they run, but record nothing).
+  constexpr int APW = 64 / L;
   const int lane = (int)(threadIdx.x & 63);
-  const int sub = (int)(threadIdx.x % L);
-  const int64_t group = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L;
-  const bool live = group < p.atoms;
+  const int wave_group = lane / L;
+  const int sub = lane - wave_group * L;
+  const int64_t group = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * APW + wave_group;
+  const bool live = wave_group < APW && group < p.atoms;
   const int64_t atom = live ? group : p.atoms - 1;
   const bool beyond = sub >= p.states;  // padding order: held at zero
   const int64_t ia = atom * p.stride_opt;
@@ -284,13 +292,12 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
   const int k_log2 = p.stage_k_log2 > 0 ? p.stage_k_log2 : 0;
   const int s_log2 = p.stage_s_log2 > 0 ? p.stage_s_log2 : 0;
   const int stage_k = 1 << k_log2;
-  const int wave_group = lane / L;
-  float2* const stage = complex_stage + (((threadIdx.x >> 6) * (64 / L)) << (s_log2 + k_log2));
+  float2* const stage = complex_stage + (((threadIdx.x >> 6) * APW) << (s_log2 + k_log2));
   int64_t nrec = 0;
   auto flush = [&](int count) __attribute__((always_inline)) {
     const int64_t g0 = group - wave_group;
     const int64_t o0 = nrec - count;
-    const int total = (64 / L) << (s_log2 + k_log2);
+    const int total = APW << (s_log2 + k_log2);
     for (int i = 2 * lane; i < total; i += 128) {
       const int kk = i & (stage_k - 1);
       if (kk >= count) continue;
@@ -637,7 +644,7 @@ ComplexLaunch pick_complex_mask(int tm, bool exch) {
 }
 
 // Lanes per atom the complex kernel is built for, one translation unit each.
-#define TSIM_COMPLEX_LANES(X) X(4) X(8) X(16) X(32) X(64)
+#define TSIM_COMPLEX_LANES(X) X(4) X(8) X(10) X(12) X(16) X(20) X(32) X(64)
 
 #define TSIM_DECLARE_COMPLEX(LANES) ComplexLaunch pick_complex_##LANES(int tm, bool exch);
 TSIM_COMPLEX_LANES(TSIM_DECLARE_COMPLEX)

@@ -77,9 +77,13 @@ class FusedModel:
             fields.update(self.simulator._tissue_fields({name: value}))
         for index, field in enumerate(self.fields):
             fields[field] = values[:, index].to(torch.float32)
+        if self.__dict__.get("_unfused"):
+            return self._normal_from_jacobian(values, data, live)
         tissue = _engine.prepare_tissue(fields, device, self.fields)
         real, flip, samples = _engine.real_subspace(self.packed, tissue, device)
         if not real or tissue.features & _native.FEATURE_DIFFUSING or self.simulator.signal_factor is None:
+            # the verdict depends on the protocol and the fixed properties only
+            self._unfused = True
             return self._normal_from_jacobian(values, data, live)
         lib = _native.load()
         events = list(self.packed.on(device))
