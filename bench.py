@@ -508,6 +508,10 @@ def main() -> int:
             file=sys.stderr,
         )
         return 2
+    if args.workload == "serve":
+        # many concurrent requests: more hardware queues than HIP's default 4
+        # (read when the HIP runtime starts, so before torch or the library load)
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
     import torch
 

@@ -143,6 +143,46 @@ def test_same_alone_and_in_the_mix():
     assert stats["batched_jobs"] > stats["launches"]  # some launches carried several requests
 
 
+@pytest.mark.skip(reason="graph replay of fit iterations is switched off: it faulted the GPU (illegal address) "
+                         "under the 60 s serving mix, twice; the cause is not found")
+def test_unbatched_fits_replayed_from_a_graph_equal_eager_iterations():
+    """Fits off the fused path (an exchanging pool; FSE's recovery) run their
+    iterations from a captured HIP graph: the maps equal those of eager
+    iterations bitwise, alone and with other requests in flight."""
+    from torchsim_amd.serving import FitRequest, Server
+    from torchsim_amd.simulators import FSESimulator
+
+    rng = np.random.default_rng(21)
+    voxels = 96
+    known = {"pool_b_fraction": torch.tensor(rng.uniform(0.05, 0.3, voxels), dtype=torch.float32).cuda(),
+             "pool_b_exchange": torch.tensor(rng.uniform(5, 20, voxels), dtype=torch.float32).cuda(),
+             "T1_pool_b": 400.0, "T2_pool_b": 15.0}
+    t1 = torch.tensor(rng.uniform(400, 2000, voxels), dtype=torch.float32).cuda()
+    t2 = torch.tensor(rng.uniform(40, 200, voxels), dtype=torch.float32).cuda()
+    mrf = _mrf(160)
+    pooled = mrf.simulate(T1=t1, T2=t2, **known)
+    fse = FSESimulator(flip=torch.full((48,), 150.0), ESP=5.0, TR=3000.0)
+    echoes = fse.simulate(T1=t1, T2=t2)
+    bounds = {"T1": (100.0, 5000.0), "T2": (10.0, 2000.0), "M0": (0.01, 2.0)}
+    start = {"T1": 1000.0, "T2": 100.0, "M0": 0.8}
+    requests = [FitRequest(mrf, pooled, bounds=bounds, initial=start, known=known, iterations=7),
+                FitRequest(fse, echoes, bounds=bounds, initial=start, iterations=7)]
+    results = {}
+    for graphs in (False, True):
+        with Server(graphs=graphs) as server:
+            results[graphs] = [server.submit(r).result(timeout=120) for r in requests]
+    with Server(graphs=True) as mixed:
+        noise = [mixed.submit(_simulate_request(n, p, seed=s)) for n, p, s in [(3000, 300, 31), (50, 120, 32)]]
+        futures = [mixed.submit(r) for r in requests]
+        mixed_maps = [f.result(timeout=120) for f in futures]
+        for f in noise:
+            f.result(timeout=120)
+    for eager, replayed, in_mix in zip(results[False], results[True], mixed_maps):
+        for name in ("T1", "T2", "M0"):
+            assert torch.equal(eager[name], replayed[name]), name
+            assert torch.equal(eager[name], in_mix[name]), name
+
+
 def test_served_results_equal_the_api_and_fits_recover():
     from torchsim_amd.serving import FitRequest, Server
 

@@ -631,7 +631,7 @@ extern "C" int tsim_simulate_vjp_jacobian(void* const* packed, int64_t atom_coun
   const size_t sc = sizeof(float) * (size_t)(segment_events * nfloat * total);
   const size_t pa = sizeof(float) * (size_t)(waves * event_count * 2);
   char* work = nullptr;
-  int st = hip_status(hipMallocAsync((void**)&work, ck + sc + pa, s), "hipMallocAsync(adjoint workspace)");
+  int st = scratch_alloc((void**)&work, ck + sc + pa, s, "adjoint workspace");
   if (st != OK) return st;
   a.checkpoints = (float*)work;
   a.scratch = (float*)(work + ck);

@@ -112,6 +112,11 @@ void set_error(const char* fmt, ...);
 const char* last_error();
 int hip_status(hipError_t err, const char* where);
 
+// Stream-ordered scratch (hipMallocAsync). The first call on a device raises
+// its default pool's release threshold, so freed scratch stays mapped for the
+// next launch instead of going back to the driver at every synchronization.
+int scratch_alloc(void** ptr, size_t bytes, hipStream_t stream, const char* what);
+
 }  // namespace tsim
 
 extern "C" int tsim_complex_simulate(void* const* pointers, int64_t pointer_count,

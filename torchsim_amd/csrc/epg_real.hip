@@ -21,6 +21,8 @@
 // F- = e^{-i phi} i b, Z = c, all real; the recorded sample is i * m0 * a0.
 #include "epg_real_impl.h"
 
+#include <atomic>
+
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -262,8 +264,7 @@ __global__ void __launch_bounds__(kPackThreads) pack_ops(const EventCode* __rest
 
 int compile_ops(int64_t event_count, int64_t outputs, Compiled& out) {
   const int64_t capacity = op_capacity(event_count > 0 ? event_count : 0);
-  int st = hip_status(hipMallocAsync((void**)&out.ops, sizeof(MacroOp) * (size_t)capacity, out.stream),
-                      "hipMallocAsync(macro-ops)");
+  int st = scratch_alloc((void**)&out.ops, sizeof(MacroOp) * (size_t)capacity, out.stream, "macro-ops");
   if (st != OK) return st;
   hipLaunchKernelGGL(pack_ops, dim3(1), dim3(kPackThreads), 0, out.stream, out.codes,
                      event_count > 0 ? event_count : 0, outputs, out.ops);
@@ -572,6 +573,21 @@ Compiled::~Compiled() {
   if (codes) (void)hipFreeAsync(codes, stream);
 }
 
+int scratch_alloc(void** ptr, size_t bytes, hipStream_t stream, const char* what) {
+  static std::atomic<uint64_t> pools_ready{0};
+  int device = -1;
+  if (hipGetDevice(&device) == hipSuccess && device >= 0 && device < 64 &&
+      !(pools_ready.load(std::memory_order_acquire) & (1ull << device))) {
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+      uint64_t keep = 2ull << 30;  // bytes the pool keeps mapped when idle
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    pools_ready.fetch_or(1ull << device, std::memory_order_acq_rel);
+  }
+  return hip_status(hipMallocAsync(ptr, bytes, stream), what);
+}
+
 int compile_packed(void* const* packed, int64_t event_count, hipStream_t stream, Compiled& out) {
   out.stream = stream;
   if (event_count <= 0) return OK;
@@ -581,8 +597,7 @@ int compile_packed(void* const* packed, int64_t event_count, hipStream_t stream,
       return BAD_ARGUMENT;
     }
   }
-  int st = hip_status(hipMallocAsync((void**)&out.codes, sizeof(EventCode) * (size_t)event_count, stream),
-                      "hipMallocAsync(event codes)");
+  int st = scratch_alloc((void**)&out.codes, sizeof(EventCode) * (size_t)event_count, stream, "event codes");
   if (st != OK) return st;
   return compile_stream((const float*)packed[TISSUE_COUNT + E_DURATION],
                         (const int32_t*)packed[TISSUE_COUNT + E_KIND],
@@ -1103,7 +1118,7 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
   const size_t a_off = (o_off + ops_bytes + 255) & ~(size_t)255;
   const size_t w_off = (a_off + args_bytes + 255) & ~(size_t)255;
   char* space = nullptr;
-  int st = hip_status(hipMallocAsync((void**)&space, w_off + work_bytes, s), "hipMallocAsync(batch workspace)");
+  int st = scratch_alloc((void**)&space, w_off + work_bytes, s, "batch workspace");
   if (st != OK) return st;
   EventCode* codes = (EventCode*)space;
   MacroOp* ops = (MacroOp*)(space + o_off);

@@ -552,12 +552,11 @@ int run_match(const half8* d_hi, const half8* d_lo, int mode, int64_t atoms, int
   float* p_score = nullptr;
   int* p_index = nullptr;
   const size_t part = (size_t)splits * WAVES * chunk_pad * KC;
-  hipError_t err = hipMallocAsync((void**)&q_hi, q_elems * sizeof(half8), stream);
-  if (err == hipSuccess) err = hipMallocAsync((void**)&q_lo, q_elems * sizeof(half8), stream);
-  if (err == hipSuccess) err = hipMallocAsync((void**)&y_norm, (size_t)chunk_pad * sizeof(double), stream);
-  if (err == hipSuccess) err = hipMallocAsync((void**)&p_score, part * sizeof(float), stream);
-  if (err == hipSuccess) err = hipMallocAsync((void**)&p_index, part * sizeof(int), stream);
-  int status = err == hipSuccess ? (int)OK : hip_status(err, "match scratch");
+  int status = scratch_alloc((void**)&q_hi, q_elems * sizeof(half8), stream, "match scratch");
+  if (status == OK) status = scratch_alloc((void**)&q_lo, q_elems * sizeof(half8), stream, "match scratch");
+  if (status == OK) status = scratch_alloc((void**)&y_norm, (size_t)chunk_pad * sizeof(double), stream, "match scratch");
+  if (status == OK) status = scratch_alloc((void**)&p_score, part * sizeof(float), stream, "match scratch");
+  if (status == OK) status = scratch_alloc((void**)&p_index, part * sizeof(int), stream, "match scratch");
   const size_t width = signals_complex ? 2 : 1;
   for (int64_t first = 0; status == OK && first < voxels; first += chunk) {
     const int64_t count = voxels - first < chunk ? voxels - first : chunk;

@@ -66,14 +66,15 @@ class FusedModel:
     def scale(self, x: torch.Tensor) -> torch.Tensor:
         return widen(self.bounds, self.names, x).to(torch.float64)
 
-    def normal(self, x: torch.Tensor, data: torch.Tensor, live: torch.Tensor) -> torch.Tensor:
-        """(voxels, stride) FP64: cost, J^T r (natural columns), J^T J."""
+    def normal(self, x: torch.Tensor, data: torch.Tensor, live: torch.Tensor | None) -> torch.Tensor:
+        """(voxels, stride) FP64: cost, J^T r (natural columns), J^T J, for the
+        voxels ``live`` indexes into per-voxel fixed properties (None: all)."""
         device = data.device
         values = self.natural(x)
         fields: dict[str, Any] = {}
         for name, value in self.held_on(device).items():
             if isinstance(value, torch.Tensor) and value.numel() > 1:
-                value = value.reshape(-1)[live]
+                value = value.reshape(-1) if live is None else value.reshape(-1)[live]
             fields.update(self.simulator._tissue_fields({name: value}))
         for index, field in enumerate(self.fields):
             fields[field] = values[:, index].to(torch.float32)
@@ -107,7 +108,8 @@ class FusedModel:
         _native.check(status)
         return normal
 
-    def _normal_from_jacobian(self, values: torch.Tensor, data: torch.Tensor, live: torch.Tensor) -> torch.Tensor:
+    def _normal_from_jacobian(self, values: torch.Tensor, data: torch.Tensor,
+                              live: torch.Tensor | None) -> torch.Tensor:
         """Any sequence: the simulator's fused signal + Jacobian (complex
         kernel, post-processing included), reduced to the normal equations in
         FP64 by tsim_normal_from_jacobian -- the Jacobian passes through memory
@@ -115,7 +117,9 @@ class FusedModel:
         device = data.device
         held = {}
         for name, value in self.held_on(device).items():
-            held[name] = value.reshape(-1)[live] if isinstance(value, torch.Tensor) and value.numel() > 1 else value
+            if isinstance(value, torch.Tensor) and value.numel() > 1:
+                value = value.reshape(-1) if live is None else value.reshape(-1)[live]
+            held[name] = value
         for index, name in enumerate(self.names):
             held[name] = values[:, index].to(torch.float32)
         signal, jacobian = self.simulator.linearize(held, self.names, **self.sequence)

@@ -258,6 +258,10 @@ class _Upload:
     """Device copies of some host tensors and the event that says they landed."""
 
     def __init__(self, tensors: list[torch.Tensor | None], device: torch.device) -> None:
+        if torch.cuda.is_current_stream_capturing():
+            # a captured copy would re-read a staging buffer that is reused,
+            # and the cached device copy would live in the graph's memory
+            raise RuntimeError("device buffers are uploaded outside graph capture: run the launch eagerly first")
         host = [None if t is None else t.detach().contiguous() for t in tensors]
         self.values: list[torch.Tensor | None] = []
         self.ready: torch.cuda.Event | None = None
