@@ -227,6 +227,30 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
 int tsim_batch_request_size(void);
 
 /*
+ * One fixed-iteration Levenberg-Marquardt step of every voxel, in two
+ * launches around the normal equations at the candidate (no reference
+ * counterpart as entry points: they fuse the per-voxel bookkeeping of
+ * recon/_gauss_newton.py:365-415 and 474-631 -- bounds chain rule, damped
+ * solve, gain ratio and Nielsen's damping update -- that the serving loop
+ * runs per request and iteration).
+ * tsim_lm_propose: from x (voxels x columns, free variables), its normal
+ * equations (voxels x tsim_normal_stride(), FP64) and damping, writes the
+ * candidate (free), its natural values as float32 planes [column][voxel],
+ * the predicted reduction and a failed-solve flag (1 byte per voxel).
+ * bound_kind per column: 0 free, 1 two-sided (low + (high - low) sigmoid),
+ * 2 lower (low + softplus), 3 upper (high - softplus(-x)).
+ * tsim_lm_accept: from the normal equations at the candidate, accepts or
+ * rejects each voxel's step and updates x, normal, cost, damping and its
+ * growth factor in place.
+ */
+int tsim_lm_propose(const double* normal, const double* x, const double* damping, const int32_t* bound_kind,
+                    const double* low, const double* high, int64_t voxels, int columns, double* candidate,
+                    float* natural, double* predicted, uint8_t* singular, void* stream);
+int tsim_lm_accept(const double* trial, const double* predicted, const uint8_t* singular, const double* candidate,
+                   int64_t voxels, int columns, double* x, double* normal, double* cost, double* damping,
+                   double* rising, void* stream);
+
+/*
  * Dictionary matching (replaces DictionaryMatcher._match_here with its
  * correlate / topk chunks, _density and _scaled: reference
  * estimators/_dictionary.py:308-431, estimators/_grouped.py:219-245).

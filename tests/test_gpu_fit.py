@@ -207,3 +207,72 @@ def test_normal_equations_from_a_stored_jacobian(columns, outputs):
             else:
                 assert torch.all(got[:, 5 + k] == 0.0)
             k += 1
+
+
+def test_fused_lm_step_matches_the_operator_formulas():
+    """tsim_lm_propose / tsim_lm_accept against the same step written with
+    the host-side transforms and solve_damped (FP64 throughout)."""
+    import ctypes
+
+    from torchsim_amd import _native
+    from torchsim_amd.recon._bounds import to_natural, widen
+    from torchsim_amd.recon._gauss_newton import solve_damped
+
+    g = torch.Generator(device="cuda").manual_seed(5)
+    voxels, cols = 1000, 4
+    names = ("a", "b", "c", "d")
+    bounds = {"a": (100.0, 5000.0), "b": (10.0, None), "c": (None, 2.0)}  # d free
+    lib = _native.load()
+    stride = lib.tsim_normal_stride()
+    jac = torch.randn(voxels, 12, cols, dtype=torch.float64, device="cuda", generator=g)
+    res = torch.randn(voxels, 12, dtype=torch.float64, device="cuda", generator=g)
+    normal = torch.zeros(voxels, stride, dtype=torch.float64, device="cuda")
+    normal[:, 0] = (res * res).sum(-1)
+    normal[:, 1:1 + cols] = -(jac * res[..., None]).sum(1)
+    h = 1 + cols
+    for i in range(cols):
+        for k in range(i, cols):
+            normal[:, h] = (jac[..., i] * jac[..., k]).sum(1)
+            h += 1
+    x = torch.randn(voxels, cols, dtype=torch.float64, device="cuda", generator=g)
+    damping = torch.rand(voxels, dtype=torch.float64, device="cuda", generator=g) + 0.01
+    damping[7] = -1e6  # an indefinite system: the step fails, the candidate stays at x
+    kinds = (ctypes.c_int32 * cols)(1, 2, 3, 0)
+    lows = (ctypes.c_double * cols)(100.0, 10.0, 0.0, 0.0)
+    highs = (ctypes.c_double * cols)(5000.0, 0.0, 2.0, 0.0)
+    candidate = torch.empty_like(x)
+    natural = torch.empty(cols, voxels, dtype=torch.float32, device="cuda")
+    predicted = torch.empty(voxels, dtype=torch.float64, device="cuda")
+    singular = torch.empty(voxels, dtype=torch.uint8, device="cuda")
+    stream = _native.stream_of(torch.device("cuda", 0))
+    _native.check(lib.tsim_lm_propose(normal.data_ptr(), x.data_ptr(), damping.data_ptr(), kinds, lows, highs,
+                                      voxels, cols, candidate.data_ptr(), natural.data_ptr(), predicted.data_ptr(),
+                                      singular.data_ptr(), stream))
+    step, pred_ref, _, _ = solve_damped(normal, widen(bounds, names, x), damping, cols)
+    sing_ref = ~torch.isfinite(step).all(-1)
+    cand_ref = torch.where(sing_ref[:, None], x, x + torch.nan_to_num(step))
+    assert bool(sing_ref[7]) and int(singular[7]) == 1
+    assert torch.equal(singular.bool(), sing_ref)
+    torch.testing.assert_close(candidate, cand_ref, rtol=1e-12, atol=1e-12)
+    ok = ~sing_ref
+    torch.testing.assert_close(predicted[ok], pred_ref[ok], rtol=1e-10, atol=1e-12)
+    torch.testing.assert_close(natural.T, to_natural(cand_ref, bounds, names).to(torch.float32), rtol=2e-7, atol=0)
+
+    trial = normal.clone()
+    trial[:, 0] = normal[:, 0] * (0.5 + torch.rand(voxels, dtype=torch.float64, device="cuda", generator=g))
+    cost = normal[:, 0].clone()
+    rising = torch.full_like(damping, 2.0)
+    rising[::3] = 8.0
+    state = [x.clone(), normal.clone(), cost.clone(), damping.clone(), rising.clone()]
+    _native.check(lib.tsim_lm_accept(trial.data_ptr(), predicted.data_ptr(), singular.data_ptr(),
+                                     candidate.data_ptr(), voxels, cols, *(t.data_ptr() for t in state), stream))
+    tiny = torch.finfo(torch.float32).tiny
+    gain = torch.where(predicted > 0, (cost - trial[:, 0]) / predicted.clamp_min(tiny), torch.zeros_like(predicted))
+    better = (gain > 0) & ~singular.bool()
+    lowered = damping * (1.0 - (2.0 * gain - 1.0).pow(3)).clamp_min(1.0 / 3.0)
+    expect = [torch.where(better[:, None], candidate, x), torch.where(better[:, None], trial, normal),
+              torch.where(better, trial[:, 0], cost), torch.where(better, lowered, damping * rising),
+              torch.where(better, torch.full_like(rising, 2.0), rising * 2.0)]
+    assert 0 < int(better.sum()) < voxels
+    for got, want in zip(state, expect):
+        torch.testing.assert_close(got, want, rtol=1e-12, atol=0)

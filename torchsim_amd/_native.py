@@ -135,6 +135,9 @@ SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
          _F32, _F32, _P, _P, _P, _I32, _P],
     ),
     "tsim_normal_from_jacobian": (_I32, [_P, _P, _P, _I64, _I64, _I32, _P, _P]),
+    "tsim_lm_propose": (_I32, [_P, _P, _P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double), _I64, _I32, _P, _P, _P, _P, _P]),
+    "tsim_lm_accept": (_I32, [_P, _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P]),
     "tsim_match_operand_size": (_I32, [_I64, _I64, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "tsim_match_planes": (_I32, [_P, _I64, _I64, _P, _P]),
     "tsim_match_layout_atoms": (_I32, [_P, _I32, _I64, _I64, _I32, _P, _P, _P, _P]),

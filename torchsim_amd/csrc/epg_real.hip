@@ -526,6 +526,133 @@ __global__ void solve_damped(const double* __restrict__ normal, const double* __
   if (predicted) predicted[v] = pred;
 }
 
+// Change of variables of a bounded unknown (two-sided: scaled sigmoid;
+// one-sided: softplus), as the host-side transforms define it.
+struct LMBounds {
+  int kind[NORMAL_COLUMNS];  // 0 free, 1 two-sided, 2 lower only, 3 upper only
+  double low[NORMAL_COLUMNS];
+  double high[NORMAL_COLUMNS];
+};
+
+__device__ inline double lm_sigmoid(double x) { return 1.0 / (1.0 + exp(-x)); }
+__device__ inline double lm_softplus(double x) { return x > 20.0 ? x : log1p(exp(x)); }
+
+__device__ inline double lm_natural(const LMBounds& b, int c, double x) {
+  switch (b.kind[c]) {
+    case 1: return b.low[c] + (b.high[c] - b.low[c]) * lm_sigmoid(x);
+    case 2: return b.low[c] + lm_softplus(x);
+    case 3: return b.high[c] - lm_softplus(-x);
+    default: return x;
+  }
+}
+
+__device__ inline double lm_widen(const LMBounds& b, int c, double x) {
+  switch (b.kind[c]) {
+    case 1: {
+      const double s = lm_sigmoid(x);
+      return (b.high[c] - b.low[c]) * s * (1.0 - s);
+    }
+    case 2: return lm_sigmoid(x);
+    case 3: return lm_sigmoid(-x);
+    default: return 1.0;
+  }
+}
+
+// One Levenberg-Marquardt proposal per voxel: the chain-rule factors of the
+// bounded unknowns at x, the damped step of solve_damped (same arithmetic),
+// its predicted reduction, whether it failed, the candidate x + step (x where
+// it failed) and the candidate's natural values as float32 planes
+// [column][voxel], which are the tissue fields of the next normal equations.
+__global__ void lm_propose(const double* __restrict__ normal, const double* __restrict__ x,
+                           const double* __restrict__ damping, LMBounds bounds, int64_t voxels, int cols,
+                           double* __restrict__ candidate, float* __restrict__ natural,
+                           double* __restrict__ predicted, uint8_t* __restrict__ singular) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= voxels) return;
+  const double* n = normal + v * NORMAL_STRIDE;
+  double xs[NORMAL_COLUMNS], D[NORMAL_COLUMNS], g[NORMAL_COLUMNS], H[NORMAL_COLUMNS][NORMAL_COLUMNS];
+  for (int c = 0; c < NORMAL_COLUMNS; ++c) {
+    xs[c] = c < cols ? x[v * cols + c] : 0.0;
+    D[c] = c < cols ? lm_widen(bounds, c, xs[c]) : 0.0;
+  }
+  int h = 0;
+  for (int c = 0; c < NORMAL_COLUMNS; ++c) {
+    g[c] = D[c] * n[1 + c];
+    for (int k = c; k < NORMAL_COLUMNS; ++k) {
+      const double e = D[c] * n[1 + NORMAL_COLUMNS + h++] * D[k];
+      H[c][k] = e;
+      H[k][c] = e;
+    }
+  }
+  const double lam = damping[v];
+  double Lm[NORMAL_COLUMNS][NORMAL_COLUMNS] = {};
+  bool ok = true;
+  for (int i = 0; i < cols; ++i) {
+    for (int j = 0; j <= i; ++j) {
+      double sum = H[i][j] + (i == j ? lam : 0.0);
+      for (int k = 0; k < j; ++k) sum -= Lm[i][k] * Lm[j][k];
+      if (i == j) {
+        if (!(sum > 0.0)) ok = false;
+        Lm[i][i] = sqrt(fmax(sum, 0.0));
+      } else {
+        Lm[i][j] = sum / Lm[j][j];
+      }
+    }
+  }
+  double y[NORMAL_COLUMNS], d[NORMAL_COLUMNS];
+  for (int i = 0; i < cols; ++i) {
+    double sum = -g[i];
+    for (int k = 0; k < i; ++k) sum -= Lm[i][k] * y[k];
+    y[i] = sum / Lm[i][i];
+  }
+  for (int i = cols - 1; i >= 0; --i) {
+    double sum = y[i];
+    for (int k = i + 1; k < cols; ++k) sum -= Lm[k][i] * d[k];
+    d[i] = sum / Lm[i][i];
+  }
+  double pred = 0.0;
+  bool failed = false;
+  for (int c = 0; c < cols; ++c) {
+    const double dc = ok ? d[c] : __builtin_nan("");
+    d[c] = dc;
+    pred += dc * (lam * dc - g[c]);
+    failed = failed || !isfinite(dc);
+  }
+  predicted[v] = pred;
+  singular[v] = failed ? 1 : 0;
+  for (int c = 0; c < cols; ++c) {
+    const double next = failed ? xs[c] : xs[c] + d[c];
+    candidate[v * cols + c] = next;
+    natural[(int64_t)c * voxels + v] = (float)lm_natural(bounds, c, next);
+  }
+}
+
+// Accept or reject each voxel's candidate from the normal equations at it
+// (Nielsen's damping update on the gain ratio), in place.
+__global__ void lm_accept(const double* __restrict__ trial, const double* __restrict__ predicted,
+                          const uint8_t* __restrict__ singular, const double* __restrict__ candidate,
+                          int64_t voxels, int cols, double* __restrict__ x, double* __restrict__ normal,
+                          double* __restrict__ cost, double* __restrict__ damping, double* __restrict__ rising) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= voxels) return;
+  const double tiny = 1.1754943508222875e-38;  // float32's smallest normal, the host-side floor
+  const double trial_cost = trial[v * NORMAL_STRIDE];
+  const double pred = predicted[v];
+  const double gain = pred > 0.0 ? (cost[v] - trial_cost) / fmax(pred, tiny) : 0.0;
+  const bool better = gain > 0.0 && singular[v] == 0;
+  const double lam = damping[v];
+  const double grow = rising[v];
+  const double t = 2.0 * gain - 1.0;
+  const double lowered = lam * fmax(1.0 - t * t * t, 1.0 / 3.0);
+  damping[v] = better ? lowered : lam * grow;
+  rising[v] = better ? 2.0 : grow * 2.0;
+  if (better) {
+    for (int c = 0; c < cols; ++c) x[v * cols + c] = candidate[v * cols + c];
+    for (int k = 0; k < NORMAL_STRIDE; ++k) normal[v * NORMAL_STRIDE + k] = trial[v * NORMAL_STRIDE + k];
+    cost[v] = trial_cost;
+  }
+}
+
 }  // namespace tsim
 
 // ---------------------------------------------------------------------------
@@ -887,6 +1014,53 @@ int tsim_solve_damped(const double* normal, const double* scale, const double* d
 }
 
 int tsim_normal_stride(void) { return NORMAL_STRIDE; }
+
+int tsim_lm_propose(const double* normal, const double* x, const double* damping, const int32_t* bound_kind,
+                    const double* low, const double* high, int64_t voxels, int columns, double* candidate,
+                    float* natural, double* predicted, uint8_t* singular, void* stream) {
+  if (columns < 1 || columns > NORMAL_COLUMNS || voxels < 0) {
+    set_error("between 1 and %d columns and a non-negative voxel count", NORMAL_COLUMNS);
+    return BAD_ARGUMENT;
+  }
+  if (voxels == 0) return OK;
+  if (!normal || !x || !damping || !bound_kind || !low || !high || !candidate || !natural || !predicted ||
+      !singular) {
+    set_error("every lm_propose buffer is required");
+    return BAD_ARGUMENT;
+  }
+  LMBounds b = {};
+  for (int c = 0; c < columns; ++c) {
+    if (bound_kind[c] < 0 || bound_kind[c] > 3) {
+      set_error("column %d: bound kind %d (0 free, 1 two-sided, 2 lower, 3 upper)", c, bound_kind[c]);
+      return BAD_ARGUMENT;
+    }
+    b.kind[c] = bound_kind[c];
+    b.low[c] = low[c];
+    b.high[c] = high[c];
+  }
+  const int64_t blocks = (voxels + 255) / 256;
+  hipLaunchKernelGGL(lm_propose, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, normal, x, damping, b,
+                     voxels, columns, candidate, natural, predicted, singular);
+  return hip_status(hipGetLastError(), "lm_propose launch");
+}
+
+int tsim_lm_accept(const double* trial, const double* predicted, const uint8_t* singular, const double* candidate,
+                   int64_t voxels, int columns, double* x, double* normal, double* cost, double* damping,
+                   double* rising, void* stream) {
+  if (columns < 1 || columns > NORMAL_COLUMNS || voxels < 0) {
+    set_error("between 1 and %d columns and a non-negative voxel count", NORMAL_COLUMNS);
+    return BAD_ARGUMENT;
+  }
+  if (voxels == 0) return OK;
+  if (!trial || !predicted || !singular || !candidate || !x || !normal || !cost || !damping || !rising) {
+    set_error("every lm_accept buffer is required");
+    return BAD_ARGUMENT;
+  }
+  const int64_t blocks = (voxels + 255) / 256;
+  hipLaunchKernelGGL(lm_accept, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, trial, predicted,
+                     singular, candidate, voxels, columns, x, normal, cost, damping, rising);
+  return hip_status(hipGetLastError(), "lm_accept launch");
+}
 
 }  // extern "C"
 

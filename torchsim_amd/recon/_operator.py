@@ -69,23 +69,29 @@ class FusedModel:
     def normal(self, x: torch.Tensor, data: torch.Tensor, live: torch.Tensor | None) -> torch.Tensor:
         """(voxels, stride) FP64: cost, J^T r (natural columns), J^T J, for the
         voxels ``live`` indexes into per-voxel fixed properties (None: all)."""
-        device = data.device
         values = self.natural(x)
+        return self.normal_at([values[:, i].to(torch.float32) for i in range(self.columns)], data, live)
+
+    def normal_at(self, columns: Sequence[torch.Tensor], data: torch.Tensor,
+                  live: torch.Tensor | None) -> torch.Tensor:
+        """As :meth:`normal`, at the unknowns' natural values given as one
+        float32 tensor per unknown."""
+        device = data.device
         fields: dict[str, Any] = {}
         for name, value in self.held_on(device).items():
             if isinstance(value, torch.Tensor) and value.numel() > 1:
                 value = value.reshape(-1) if live is None else value.reshape(-1)[live]
             fields.update(self.simulator._tissue_fields({name: value}))
         for index, field in enumerate(self.fields):
-            fields[field] = values[:, index].to(torch.float32)
+            fields[field] = columns[index]
         if self.__dict__.get("_unfused"):
-            return self._normal_from_jacobian(values, data, live)
+            return self._normal_from_jacobian(columns, data, live)
         tissue = _engine.prepare_tissue(fields, device, self.fields)
         real, flip, samples = _engine.real_subspace(self.packed, tissue, device)
         if not real or tissue.features & _native.FEATURE_DIFFUSING or self.simulator.signal_factor is None:
             # the verdict depends on the protocol and the fixed properties only
             self._unfused = True
-            return self._normal_from_jacobian(values, data, live)
+            return self._normal_from_jacobian(columns, data, live)
         lib = _native.load()
         events = list(self.packed.on(device))
         if flip is not None:
@@ -108,7 +114,7 @@ class FusedModel:
         _native.check(status)
         return normal
 
-    def _normal_from_jacobian(self, values: torch.Tensor, data: torch.Tensor,
+    def _normal_from_jacobian(self, columns: Sequence[torch.Tensor], data: torch.Tensor,
                               live: torch.Tensor | None) -> torch.Tensor:
         """Any sequence: the simulator's fused signal + Jacobian (complex
         kernel, post-processing included), reduced to the normal equations in
@@ -121,7 +127,7 @@ class FusedModel:
                 value = value.reshape(-1) if live is None else value.reshape(-1)[live]
             held[name] = value
         for index, name in enumerate(self.names):
-            held[name] = values[:, index].to(torch.float32)
+            held[name] = columns[index]
         signal, jacobian = self.simulator.linearize(held, self.names, **self.sequence)
         lib = _native.load()
         stride = lib.tsim_normal_stride()

@@ -32,6 +32,7 @@ fit) run on their own stream through the ordinary simulator path.
 
 from __future__ import annotations
 
+import ctypes
 import itertools
 import queue
 import threading
@@ -43,11 +44,11 @@ from typing import Any
 import torch
 
 from .. import _native
+from ..recon._bounds import bound_of
 from ..recon._gauss_newton import solve_damped
 from ..sequence import _engine
 from ._batch import BatchItem, launch_batch
 
-_TINY = torch.finfo(torch.float32).tiny
 
 
 @dataclass
@@ -153,13 +154,10 @@ class _FitProgram:
     few asynchronous elementwise updates.
     """
 
-    def __init__(self, request: FitRequest, device: torch.device, graphs: bool = False) -> None:
+    def __init__(self, request: FitRequest, device: torch.device) -> None:
         from ..recon import FusedModel
 
         self.request = request
-        self.graphs = graphs
-        self._graph: torch.cuda.CUDAGraph | None = None
-        self._graph_failed = False
         self.op = FusedModel(request.simulator, request.unknown, bounds=request.bounds, known=request.known)
         self.data = request.measured.to(device=device, dtype=torch.complex64).reshape(
             -1, request.measured.shape[-1]).contiguous()
@@ -192,14 +190,15 @@ class _FitProgram:
         except (NotImplementedError, ValueError):
             return False
 
-    def _item(self, x: torch.Tensor) -> BatchItem:
+    def _item(self, columns: list[torch.Tensor]) -> BatchItem:
+        """The batched normal-equation job at the unknowns' natural values
+        (float32, one contiguous tensor per unknown)."""
         device = self.data.device
-        values = self.op.natural(x)
         fields: dict[str, Any] = {}
         for name, value in self.op.held_on(device).items():
             fields.update(self.op.simulator._tissue_fields({name: value}))
         for index, f in enumerate(self.op.fields):
-            fields[f] = values[:, index].to(torch.float32)
+            fields[f] = columns[index]
         tissue = _engine.prepare_tissue(fields, device, self.op.fields)
         real, flip, samples = _engine.real_subspace(self.op.packed, tissue, device)
         stride = _native.load().tsim_normal_stride()
@@ -207,128 +206,85 @@ class _FitProgram:
         return BatchItem(self.op.packed, tissue, self.op.states, self.op.fields, self.op.simulator.signal_factor,
                          flip=flip, samples=samples, data=self.data, normal=normal)
 
+    def _start_columns(self) -> list[torch.Tensor]:
+        values = self.op.natural(self.x)
+        return [values[:, i].to(torch.float32) for i in range(self.op.columns)]
+
     def job(self) -> tuple[int, BatchItem] | None:
         if self.finished or not self.fused:
             return None
         if self.iteration < 0:
-            self._pending = self._item(self.x)
+            self._pending = self._item(self._start_columns())
         else:
-            self._prepare_step()
-            self._pending = self._item(self.candidate)
+            self._pending = self._item(self._propose())
         return 1, self._pending
 
-    def _prepare_step(self) -> None:
-        columns = self.op.columns
-        step, predicted, _, _ = solve_damped(self.normal, self.op.scale(self.x), self.damping, columns)
-        singular = ~torch.isfinite(step).all(-1)
-        self.singular = singular
-        self.step_predicted = predicted
-        self.candidate = torch.where(singular[:, None], self.x, self.x + torch.nan_to_num(step))
+    def _bound_arrays(self) -> tuple:
+        cached = self.__dict__.get("_bounds_c")
+        if cached is None:
+            n = self.op.columns
+            kinds, lows, highs = (ctypes.c_int32 * n)(), (ctypes.c_double * n)(), (ctypes.c_double * n)()
+            for i, name in enumerate(self.request.unknown):
+                low, high = bound_of(self.request.bounds, name)
+                kinds[i] = 1 if low is not None and high is not None else 2 if low is not None else \
+                    3 if high is not None else 0
+                lows[i] = 0.0 if low is None else float(low)
+                highs[i] = 0.0 if high is None else float(high)
+            cached = self._bounds_c = (kinds, lows, highs)
+        return cached
+
+    def _propose(self) -> list[torch.Tensor]:
+        """One fused launch: the damped step at x, the candidate and its
+        natural values (the fields of the next normal equations)."""
+        voxels, columns = self.x.shape
+        device = self.x.device
+        self.candidate = torch.empty_like(self.x)
+        natural = torch.empty((columns, voxels), dtype=torch.float32, device=device)
+        self.step_predicted = torch.empty(voxels, dtype=torch.float64, device=device)
+        self.singular = torch.empty(voxels, dtype=torch.uint8, device=device)
+        kinds, lows, highs = self._bound_arrays()
+        with torch.cuda.device(device):
+            _native.check(_native.load().tsim_lm_propose(
+                self.normal.data_ptr(), self.x.data_ptr(), self.damping.data_ptr(), kinds, lows, highs, voxels,
+                columns, self.candidate.data_ptr(), natural.data_ptr(), self.step_predicted.data_ptr(),
+                self.singular.data_ptr(), _native.stream_of(device)))
+        return [natural[i] for i in range(columns)]
+
+    def _accept(self, trial: torch.Tensor) -> None:
+        """One fused launch: gain ratio, Nielsen damping, and the state
+        (x, normal equations, cost, damping, its growth) updated in place."""
+        voxels, columns = self.x.shape
+        device = self.x.device
+        with torch.cuda.device(device):
+            _native.check(_native.load().tsim_lm_accept(
+                trial.data_ptr(), self.step_predicted.data_ptr(), self.singular.data_ptr(),
+                self.candidate.data_ptr(), voxels, columns, self.x.data_ptr(), self.normal.data_ptr(),
+                self.cost.data_ptr(), self.damping.data_ptr(), self.rising.data_ptr(), _native.stream_of(device)))
 
     def after_job(self, normal: torch.Tensor | None = None) -> None:
         normal = self._pending.normal if normal is None else normal
         if self.iteration < 0:
-            self.normal = normal.clone()
+            self.normal = normal.contiguous().clone()
             self.cost = normal[:, 0].clone()
             _, _, _, curvature = solve_damped(normal, self.op.scale(self.x), None, self.op.columns, step=False)
-            self.damping = self.request.tau * curvature
+            self.damping = (self.request.tau * curvature).contiguous()
             self.rising = torch.full_like(self.damping, 2.0)
         else:
-            trial_cost = normal[:, 0]
-            predicted = self.step_predicted
-            gain = torch.where(predicted > 0, (self.cost - trial_cost) / predicted.clamp_min(_TINY),
-                               torch.zeros_like(predicted))
-            better = (gain > 0) & ~self.singular
-            lowered = self.damping * (1.0 - (2.0 * gain - 1.0).pow(3)).clamp_min(1.0 / 3.0)
-            rising = self.rising
-            self.rising = torch.where(better, torch.full_like(rising, 2.0), rising * 2.0)
-            self.damping = torch.where(better, lowered, self.damping * rising)
-            self.x = torch.where(better[:, None], self.candidate, self.x)
-            self.normal = torch.where(better[:, None], normal, self.normal)
-            self.cost = torch.where(better, trial_cost, self.cost)
+            self._accept(normal.contiguous())
         self.iteration += 1
 
     def run_unbatched(self) -> None:
         """Any sequence: one iteration of the same algorithm, with the normal
         equations formed from the simulator's fused signal + Jacobian (the
-        scheduler calls this once per tick, so other requests interleave).
-
-        The start point and the first iteration run eagerly; the iteration is
-        then captured once into a HIP graph on the request's stream and
-        replayed for the rest, so an iteration costs the host one replay
-        instead of ~100 operator dispatches. Replays run exactly the kernels
-        the eager iteration runs, on the same state tensors, updated in place.
-        """
+        scheduler calls this once per tick, so other requests interleave)."""
         if self.finished:
             return
         if self.iteration < 0:
             normal = self.op.normal(self.x, self.data, self.live)
-            self._pending = None
-            self.after_job(normal)
-            return
-        if self._graph is None and self.graphs and not self._graph_failed and self.iteration >= 1 \
-                and self._uploads_settled():
-            self._capture()
-        if self._graph is not None:
-            self._graph.replay()
         else:
-            self._iterate_in_place()
-        self.iteration += 1
-
-    def release(self) -> None:
-        """Drop the captured iteration and its memory. Only once the GPU has
-        passed the request's last work: the graph's buffers (its private
-        pool, its allocation nodes) must outlive every replay in flight."""
-        self._graph = None
-
-    def _iterate_in_place(self) -> None:
-        """One Levenberg-Marquardt iteration on the state tensors (x, normal
-        equations, cost, damping, its growth), written back in place."""
-        step, predicted, _, _ = solve_damped(self.normal, self.op.scale(self.x), self.damping, self.op.columns)
-        singular = ~torch.isfinite(step).all(-1)
-        candidate = torch.where(singular[:, None], self.x, self.x + torch.nan_to_num(step))
-        normal = self.op.normal(candidate, self.data, self.live)
-        trial_cost = normal[:, 0]
-        gain = torch.where(predicted > 0, (self.cost - trial_cost) / predicted.clamp_min(_TINY),
-                           torch.zeros_like(predicted))
-        better = (gain > 0) & ~singular
-        lowered = self.damping * (1.0 - (2.0 * gain - 1.0).pow(3)).clamp_min(1.0 / 3.0)
-        rising = torch.where(better, torch.full_like(self.rising, 2.0), self.rising * 2.0)
-        damping = torch.where(better, lowered, self.damping * self.rising)
-        x = torch.where(better[:, None], candidate, self.x)
-        normal = torch.where(better[:, None], normal, self.normal)
-        cost = torch.where(better, trial_cost, self.cost)
-        self.x.copy_(x)
-        self.normal.copy_(normal)
-        self.cost.copy_(cost)
-        self.damping.copy_(damping)
-        self.rising.copy_(rising)
-
-    def _uploads_settled(self) -> bool:
-        """Whether the protocol's device buffers have landed (a graph cannot
-        wait for an upload made outside it)."""
-        uploads = self.op.packed.__dict__.get("_resident", {})
-        for upload in uploads.values():
-            if upload.ready is not None:
-                if not upload.ready.query():
-                    return False
-                upload.ready = None
-        return True
-
-    def _capture(self) -> None:
-        graph = torch.cuda.CUDAGraph()
-        try:
-            # thread-local: other threads (callers building requests) keep
-            # allocating and synchronizing while this stream captures
-            graph.capture_begin(capture_error_mode="thread_local")
-            try:
-                self._iterate_in_place()
-            finally:
-                graph.capture_end()
-        except Exception:  # anything not capturable: stay eager for this request
-            self._graph_failed = True
-            return
-        self._graph = graph
+            normal = self.op.normal_at(self._propose(), self.data, self.live)
+        self._pending = None
+        self.after_job(normal)
 
     @property
     def finished(self) -> bool:
@@ -507,7 +463,7 @@ class Server:
             try:
                 with torch.cuda.stream(ticket.stream):
                     if isinstance(ticket.request, FitRequest):
-                        ticket.program = _FitProgram(ticket.request, self.device, graphs=self.graphs)
+                        ticket.program = _FitProgram(ticket.request, self.device)
                     else:
                         ticket.program = _SimulateProgram(ticket.request, self.device)
             except Exception as error:  # a request that cannot be built fails alone
@@ -614,10 +570,6 @@ class Server:
                     self._resident_bytes[request.resident] = ticket.bytes
                 else:
                     self._used -= ticket.bytes
-            # the GPU has passed the request's last work: its graph may go
-            release = getattr(ticket.program, "release", None)
-            if release is not None:
-                release()
             self.latencies.append(now - ticket.arrival)
             self.completed += 1
             ticket.future.set_result(result)
