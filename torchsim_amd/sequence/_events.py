@@ -14,6 +14,7 @@ train packs to the same buffers.
 
 from __future__ import annotations
 
+import threading
 from collections.abc import Callable, Sequence
 from dataclasses import dataclass
 from enum import IntEnum, IntFlag
@@ -306,15 +307,23 @@ class _Upload:
         return self.values
 
 
+_RESIDENT_LOCK = threading.Lock()
+
+
 def resident(owner: Any, name: str, tensors: list[torch.Tensor | None], device: torch.device) -> list:
     """``tensors`` (host) on ``device``, uploaded once per ``owner`` and
-    ``name`` (asynchronously; the current stream waits for the copy)."""
+    ``name`` (asynchronously; the current stream waits for the copy). Callers
+    on several threads (a client building requests, the serving loop) share
+    one upload per key."""
     cache = owner.__dict__.setdefault("_resident", {})
     key = (name, str(device))
     upload = cache.get(key)
     if upload is None:
-        upload = _Upload(list(tensors), device)
-        cache[key] = upload
+        with _RESIDENT_LOCK:
+            upload = cache.get(key)
+            if upload is None:
+                upload = _Upload(list(tensors), device)
+                cache[key] = upload
     return upload.wait(device)
 
 
