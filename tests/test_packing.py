@@ -70,3 +70,33 @@ def test_host_scalar_layouts_pack_the_same_bits(simulator, protocol, monkeypatch
     for field in FIELDS:
         assert torch.equal(getattr(fast, field), getattr(tensors, field)), field
     assert fast.output_count == tensors.output_count
+
+
+@pytest.mark.parametrize("pulses, tr, ti, phases", [
+    (1, 10.0, 0.0, 0.0), (2, 10.0, 0.0, 0.0), (3, 12.5, 5.0, 0.0), (17, "ramp", 0.0, 0.3),
+    (880, 10.0, 0.0, 0.0), (750, "ramp", 20.0, 0.0),
+])
+def test_tiled_packing_of_plain_protocols_matches_generic(pulses, tr, ti, phases):
+    """Protocols without a graph pack from the cached two-pulse template,
+    tiled: every buffer equals the event-by-event packing bit for bit."""
+    rng = np.random.default_rng(pulses)
+    flips = torch.tensor(rng.uniform(2.0, 70.0, pulses), dtype=torch.float32)
+    trs = torch.tensor(np.linspace(8.0, 15.0, pulses), dtype=torch.float32) if tr == "ramp" else tr
+    seq = MRFSimulator(states=10)
+    fast = seq._packed(flip=flips, TR=trs, TI=ti, phases=phases)
+    generic = Simulator._packed(seq, flip=flips, TR=trs, TI=ti, phases=phases)
+    for field in FIELDS + ("shim_index", "saturation", "rf_frequency_hz"):
+        assert torch.equal(getattr(fast, field), getattr(generic, field)), field
+    assert fast.output_count == generic.output_count == pulses
+
+
+def test_tiled_packing_is_fast():
+    import time
+
+    seq = MRFSimulator(states=10)
+    flips = torch.linspace(5.0, 60.0, 1000)
+    seq._packed(flip=flips, TR=10.0)  # the template, once
+    start = time.perf_counter()
+    for k in range(20):
+        seq._packed(flip=flips + 0.01 * k, TR=10.0)
+    assert (time.perf_counter() - start) / 20 < 0.01

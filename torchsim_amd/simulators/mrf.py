@@ -2,14 +2,18 @@
 
 from __future__ import annotations
 
-import dataclasses
+import threading
 from typing import Any
 
 import numpy as np
 import torch
 
-from ..model._simulator import Simulator, SpinPhysics
+from ..model._simulator import Simulator, SpinPhysics, _protocol_key
 from ..sequence._events import INVERSION, UNBALANCED, Inversion, PackedEvents, host_values
+
+# one two-pulse template per protocol (everything but the flips and TRs)
+_TEMPLATES: dict = {}
+_TEMPLATE_LOCK = threading.Lock()
 
 
 def _matched(value: Any, reference: torch.Tensor) -> torch.Tensor:
@@ -58,45 +62,93 @@ class MRFSimulator(Simulator):
         return parts
 
     def _packed(self, **sequence: Any) -> PackedEvents:
-        """The packed stream; when only the flips and/or TRs carry a graph
-        (sequence design), built in a handful of vectorised ops instead of one
-        autograd node per event.
+        """The packed stream, built in a handful of vectorised ops instead of
+        one Python object per event (the generic packing costs ~25 ms for a
+        thousand pulses; this ~1 ms), bit for bit the generic buffers.
 
-        The structure (kinds, actions, phases, output slots) comes from the
-        generic packing of the same protocol with placeholder values, which is
-        cached; the flips are scattered into it; the intervals are the float32
-        clock differences the generic packing computes, evaluated with the same
-        float32 operations in the same order, plus a term that is zero in value
-        and carries d(interval)/d(TR) = 1e-3 s/ms into the graph.
+        The structure (kinds, actions, phases, output slots) is that of the
+        generic packing of the same protocol with two placeholder pulses,
+        cached and tiled to N pulses; the flips are scattered into it; the
+        intervals are the float32 clock differences the generic packing
+        computes, evaluated with the same float32 operations in the same
+        order. When the flips and/or TRs carry a graph (sequence design) the
+        intervals also carry a term that is zero in value and has
+        d(interval)/d(TR) = 1e-3 s/ms. Anything else (repetitions, a TI or
+        phases that are tensors or per pulse, non-float32 flips) takes the
+        generic packing.
         """
         flip, tr = sequence.get("flip"), sequence.get("TR")
-        graphed = [v for v in (flip, tr) if isinstance(v, torch.Tensor) and v.requires_grad]
         other = [k for k, v in sequence.items()
                  if k not in ("flip", "TR") and isinstance(v, torch.Tensor) and v.requires_grad]
         ti = sequence.get("TI", 0.0)
-        if (not graphed or other or isinstance(ti, torch.Tensor)
+        phases = sequence.get("phases", 0.0)
+        if (other or flip is None or tr is None or not isinstance(ti, (int, float))
+                or not isinstance(phases, (int, float))
                 or int(sequence.get("repetitions", self.repetitions) or 1) > 1):
             return super()._packed(**sequence)
         angles = torch.deg2rad(torch.atleast_1d(torch.as_tensor(flip)))
+        if angles.dtype != torch.float32 or angles.ndim != 1 or angles.numel() == 0:
+            return super()._packed(**sequence)
         n = angles.numel()
         spacing_s = _matched(tr, angles) * 1e-3
-        placeholder = {**sequence, "flip": torch.ones(n, dtype=angles.dtype), "TR": torch.ones(n, dtype=angles.dtype)}
-        template = self.packed(**placeholder)
-        turning = ((template.kind == 1) & ((template.action & INVERSION) == 0)).nonzero().reshape(-1)
-        if turning.numel() != n or template.kind.numel() != 1 + 4 * n:
+        block = self._pulse_block(sequence)
+        if block is None:
             return super()._packed(**sequence)
-        flips = template.flip.detach().clone().index_put((turning,), angles.to(template.flip.dtype))
-        # the clock as compose() accumulates it: Python float until the first
-        # TR is added, float32 tensors after
+        head, body, turn_at, record_at = block
+
+        def tiled(name: str) -> torch.Tensor:
+            return torch.cat([getattr(head, name), getattr(body, name).repeat(n)])
+
+        pulse = torch.arange(n) * 4 + 1
+        output_index = tiled("output_index")
+        output_index[pulse + record_at] = torch.arange(n, dtype=output_index.dtype)
+        flips = tiled("flip").index_put((pulse + turn_at,), angles.to(body.flip.dtype))
+        # the clock as compose() accumulates it: a Python float until the first
+        # TR is added, float32 after (add.accumulate is sequential float32)
         steps = spacing_s.detach().to(torch.float32).numpy()
-        clock = np.zeros(1 + 4 * n, dtype=np.float32)
-        cursor: Any = 0.0 + float(ti) * 1e-3
-        for k in range(n):
-            stamp = np.float32(1e6) * cursor if isinstance(cursor, np.float32) else np.float32(1e6 * cursor)
-            clock[1 + 4 * k: 5 + 4 * k] = stamp
-            cursor = np.float32(np.float32(cursor) + steps[k])
+        cursor0 = 0.0 + float(ti) * 1e-3
+        cursors = np.add.accumulate(np.concatenate([np.array([cursor0], dtype=np.float32), steps[:-1]]),
+                                    dtype=np.float32)
+        stamps = np.float32(1e6) * cursors
+        stamps[0] = np.float32(1e6 * cursor0)
+        clock = np.concatenate([np.zeros(1, dtype=np.float32), np.repeat(stamps, 4)])
         intervals = np.diff(clock, prepend=np.float32(0.0)) * np.float32(1e-6)
-        live = torch.zeros(1 + 4 * n, dtype=torch.float32).index_put(
-            (turning[1:],), (spacing_s[:-1] - spacing_s[:-1].detach()).to(torch.float32))
-        duration = torch.from_numpy(intervals.astype(np.float32)) + live
-        return dataclasses.replace(template, duration=duration, flip=flips)
+        duration = torch.from_numpy(intervals.astype(np.float32))
+        if isinstance(tr, torch.Tensor) and tr.requires_grad:
+            live = torch.zeros(1 + 4 * n, dtype=torch.float32).index_put(
+                (pulse[1:] + turn_at,), (spacing_s[:-1] - spacing_s[:-1].detach()).to(torch.float32))
+            duration = duration + live
+        return PackedEvents(duration=duration, kind=tiled("kind"), flip=flips, phase=tiled("phase"),
+                            action=tiled("action"), output_index=output_index, shim_index=tiled("shim_index"),
+                            saturation=tiled("saturation"), rf_frequency_hz=tiled("rf_frequency_hz"),
+                            output_count=n)
+
+    def _pulse_block(self, sequence: dict) -> tuple | None:
+        """(head, one pulse's four events, the excitation's and the sample's
+        place in them) from the generic packing of the protocol with two
+        placeholder pulses, or None where that packing is not one inversion
+        then identical four-event pulses."""
+        key = _protocol_key(type(self), {k: v for k, v in sequence.items() if k not in ("flip", "TR")})
+        if key is None:
+            return None
+        with _TEMPLATE_LOCK:
+            if key in _TEMPLATES:
+                return _TEMPLATES[key]
+        placeholder = {**sequence, "flip": torch.ones(2, dtype=torch.float32),
+                       "TR": torch.ones(2, dtype=torch.float32)}
+        t = Simulator._packed(self, **placeholder)
+        block = None
+        if t.kind.numel() == 9 and t.output_count == 2:
+            fields = ("kind", "action", "shim_index", "saturation", "rf_frequency_hz", "phase", "flip")
+            periodic = all(torch.equal(getattr(t, f)[1:5], getattr(t, f)[5:9]) for f in fields)
+            turning = ((t.kind[1:5] == 1) & ((t.action[1:5] & INVERSION) == 0)).nonzero().reshape(-1)
+            recorded = (t.output_index[1:5] >= 0).nonzero().reshape(-1)
+            if (periodic and turning.numel() == 1 and recorded.numel() == 1
+                    and int(t.output_index[1 + int(recorded)]) == 0 and int(t.output_index[5 + int(recorded)]) == 1
+                    and int(t.kind[0]) == 1 and bool((t.action[0] & INVERSION) != 0)):
+                head = PackedEvents(*(b[:1].detach() for b in t.buffers()), output_count=0)
+                body = PackedEvents(*(b[1:5].detach() for b in t.buffers()), output_count=0)
+                block = (head, body, int(turning), int(recorded))
+        with _TEMPLATE_LOCK:
+            _TEMPLATES[key] = block
+        return block
