@@ -109,6 +109,32 @@ int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t even
                            void* signal, void* jacobian, int tile, void* stream);
 
 /*
+ * tsim_simulate_jacobian with the samples projected onto a temporal basis
+ * inside the kernel: what the reference computes as
+ * `Subspace.project(signal)` and `Subspace.project(derivative)` after a full
+ * simulate / jacobian (recon/_operator.py:446-473, estimators/_nlls.py:268-273,
+ * _subspace.py:85-95: coefficients = samples @ basis). The samples never
+ * reach HBM; each atom's L lanes accumulate 4 basis directions per lane in
+ * FP32 over the recorded samples.
+ *
+ * basis:    DEVICE complex64 (output_count, rank), row-major.
+ * rank:     1 .. 16 directions (a tile of L lanes per atom carries up to 4 L;
+ *           the library's tile choice takes that into account).
+ * columns:  HOST array of column_count (<= 4) indices among {0 t1, 1 t2, 2 m0,
+ *           3 b1}; no diffusion feature.
+ * signal:   complex64 (atom_count, rank), or NULL.
+ * jacobian: complex64 (atom_count, column_count, rank), or NULL when
+ *           column_count is 0.
+ * Other arguments as tsim_simulate_jacobian; output_count must be >= 1.
+ */
+int tsim_simulate_projected(void* const* packed, int64_t atom_count, int64_t event_count,
+                            int64_t state_count, int64_t output_count, int features,
+                            int64_t tissue_stride, const int32_t* columns, int64_t column_count,
+                            float factor_re, float factor_im, const float* sample_sign,
+                            const void* basis, int64_t rank, void* signal, void* jacobian, int tile,
+                            void* stream);
+
+/*
  * Adjoint of the fused real-subspace system: gradients of a scalar loss with
  * respect to every event's flip angle and duration, given the loss's real
  * cotangents on the recorded m0*a0 and on the Jacobian columns. Replaces the

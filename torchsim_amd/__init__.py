@@ -6,7 +6,8 @@ The public API mirrors torchsim's (``mrf_sim``, ``simulators.MRFSimulator``,
 
 from . import estimators, model, optim, recon, sequence, simulators
 from ._functional import bssfp_sim, fse_sim, mrf_sim
+from ._subspace import Subspace
 from .optim import crlb
 
-__all__ = ["bssfp_sim", "crlb", "estimators", "fse_sim", "model", "mrf_sim", "optim", "recon", "sequence",
-           "simulators"]
+__all__ = ["Subspace", "bssfp_sim", "crlb", "estimators", "fse_sim", "model", "mrf_sim", "optim", "recon",
+           "sequence", "simulators"]

@@ -344,17 +344,18 @@ int variant_of(int features, bool accum, int tile, int64_t states) {
   return v;
 }
 
-int default_tile(int states, int tangents) {
+int default_tile(int states, int tangents, int min_lanes = 1) {
   // Among the tiles covering S orders, weigh the padded orders (work spent on
   // zeros), the in-lane register moves and DPP crossings of a wider atom, and
   // the occupancy the register file leaves: 3 planes * SL orders *
-  // (1 + tangents) floats of state per lane.
+  // (1 + tangents) floats of state per lane. A projection needs enough lanes
+  // per atom for its basis directions (min_lanes).
   int best = -1;
   float best_cost = 1e30f;
   for (int t = 0; t < kTileCount; ++t) {
     const Tile tile = kTiles[t];
     const int cover = tile.sl * tile.l;
-    if (cover < states) continue;
+    if (cover < states || tile.l < min_lanes) continue;
     const int regs = 3 * tile.sl * (1 + tangents);
     const float occupancy = regs <= 96 ? 1.0f : (regs <= 150 ? 1.12f : (regs <= 200 ? 1.3f : 1.8f));
     const float cost = (float)cover / (float)states * (1.0f + 0.06f * (float)(tile.l - 1)) * occupancy;
@@ -884,6 +885,106 @@ int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t even
   if (st != OK) return st;
   args.ops = compiled.ops;
   return run_real(args, tm, features, tile, s);
+}
+
+int tsim_simulate_projected(void* const* packed, int64_t atom_count, int64_t event_count,
+                            int64_t state_count, int64_t output_count, int features,
+                            int64_t tissue_stride, const int32_t* columns, int64_t column_count,
+                            float factor_re, float factor_im, const float* sample_sign,
+                            const void* basis, int64_t rank, void* signal, void* jacobian, int tile,
+                            void* stream) {
+  if (packed == nullptr) {
+    set_error("packed buffers are null");
+    return BAD_ARGUMENT;
+  }
+  const int64_t max_rank = (int64_t)PROJECT_PER_LANE * kTiles[kTileCount - 1].l;
+  if (rank < 1 || rank > max_rank) {
+    set_error("a projection keeps between 1 and %lld basis directions, got %lld", (long long)max_rank,
+              (long long)rank);
+    return UNSUPPORTED;
+  }
+  if (basis == nullptr && atom_count > 0 && output_count > 0) {
+    set_error("the basis buffer is null");
+    return BAD_ARGUMENT;
+  }
+  if (column_count < 0 || column_count > 4 || (column_count > 0 && columns == nullptr)) {
+    set_error("between 0 and 4 Jacobian columns, got %lld", (long long)column_count);
+    return BAD_ARGUMENT;
+  }
+  if (column_count > 0 && jacobian == nullptr && atom_count > 0 && output_count > 0) {
+    set_error("Jacobian columns asked for but the Jacobian buffer is null");
+    return BAD_ARGUMENT;
+  }
+  if (features & FEATURE_DIFFUSING) {
+    set_error("a projected launch carries no diffusion");
+    return UNSUPPORTED;
+  }
+  RealArgs args;
+  int st = fill_real_args(args, packed, atom_count, event_count, state_count, output_count, features,
+                          tissue_stride);
+  if (st != OK) return st;
+  int tm = 0;
+  int cols[5];
+  if (!tangent_mask(columns, column_count, tm, cols)) return BAD_ARGUMENT;
+  if (cols[3] >= 0) {
+    set_error("a projected launch carries no inversion-efficiency column");
+    return UNSUPPORTED;
+  }
+  if (cols[2] >= 0 && !(features & FEATURE_TRANSMIT)) {
+    set_error("a B1 column needs the transmit feature and a b1 buffer");
+    return BAD_ARGUMENT;
+  }
+  const int min_lanes = (int)((rank + PROJECT_PER_LANE - 1) / PROJECT_PER_LANE);
+  if (tile < 0) tile = default_tile(args.states, popcount4(tangent_class(tm)), min_lanes);
+  if (tile < 0 || tile >= kTileCount || kTiles[tile].sl * kTiles[tile].l < args.states) {
+    set_error("no state tile covers %d orders", args.states);
+    return UNSUPPORTED;
+  }
+  if (kTiles[tile].l < min_lanes) {
+    set_error("tile %d has %d lanes per atom, fewer than the %d that %lld directions need", tile,
+              kTiles[tile].l, min_lanes, (long long)rank);
+    return BAD_ARGUMENT;
+  }
+  args.col_t1 = cols[0];
+  args.col_t2 = cols[1];
+  args.col_b1 = cols[2];
+  args.col_ie = -1;
+  args.col_m0 = cols[4];
+  args.columns = (int)column_count;
+  args.signal = (float2*)signal;
+  args.jacobian = column_count > 0 ? (float2*)jacobian : nullptr;
+  args.sample_sign = sample_sign;
+  args.factor_re = factor_re;
+  args.factor_im = factor_im;
+  args.basis = (const float2*)basis;
+  args.rank = (int)rank;
+  args.stage_k_log2 = -1;  // no sample leaves the registers
+  args.stage_s_log2 = 0;
+  if (args.atoms == 0) return OK;
+  if (args.outputs == 0) {
+    set_error("a projection needs at least one recorded sample");
+    return BAD_ARGUMENT;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Compiled compiled;
+  st = compile_packed(packed, event_count, s, compiled);
+  if (st != OK) return st;
+  st = compile_ops(event_count, output_count, compiled);
+  if (st != OK) return st;
+  args.ops = compiled.ops;
+  const int L = kTiles[tile].l;
+  const int64_t blocks = (atom_count * L + 255) / 256;
+  if (blocks > 0x7FFFFFFF) {
+    set_error("too many atoms for one launch: %lld", (long long)atom_count);
+    return BAD_ARGUMENT;
+  }
+  RealLaunch launch = pick_real(tile, tm, variant_of(features, false, tile, args.states) | V_PROJECT);
+  if (!launch) {
+    set_error("no projecting kernel for this variant");
+    return UNSUPPORTED;
+  }
+  launch(args, dim3((unsigned)blocks), dim3(256), 0, s);
+  return hip_status(hipGetLastError(), "real_kernel launch");
 }
 
 int tsim_tile_count(void) { return kTileCount; }

@@ -93,6 +93,11 @@ struct RealArgs {
   // sample directly.
   int stage_k_log2;
   int stage_s_log2;
+  // Projection mode: the recorded samples are contracted on the fly with a
+  // temporal basis (outputs, rank) complex64, and signal / jacobian receive
+  // (atoms, rank) and (atoms, columns, rank) coefficients instead of samples.
+  const float2* basis;
+  int rank;
 };
 
 // Staged output writes. Storing one complex sample per atom per pulse puts
@@ -117,9 +122,30 @@ constexpr int TAN_T2 = 2;
 constexpr int TAN_B1 = 4;
 constexpr int TAN_IE = 8;
 
+// Projection mode: each of an atom's L lanes keeps PROJECT_PER_LANE basis
+// directions (r = lane-in-atom + L j), so a tile projects onto at most
+// PROJECT_PER_LANE * L directions.
+constexpr int PROJECT_PER_LANE = 4;
+
 constexpr int popcount4(int m) { return (m & 1) + ((m >> 1) & 1) + ((m >> 2) & 1) + ((m >> 3) & 1); }
 // Slot of direction `bit` among those present in `mask`.
 constexpr int slot_of(int mask, int bit) { return popcount4(mask & (bit - 1)); }
+
+// The value of the first lane of this lane's atom (L adjacent lanes, L <= 4,
+// so an atom never straddles a DPP quad).
+template <int L>
+__device__ __forceinline__ float atom_broadcast(float x) {
+  static_assert(L == 1 || L == 2 || L == 4, "atoms of 1, 2 or 4 lanes");
+  if constexpr (L == 1) {
+    return x;
+  } else if constexpr (L == 2) {
+    // quad_perm [0, 0, 2, 2]
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xA0, 0xF, 0xF, false));
+  } else {
+    // quad_perm [0, 0, 0, 0]
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x00, 0xF, 0xF, false));
+  }
+}
 
 __device__ __forceinline__ float dpp_from_lower(float x) {
   // row_shr:1 -- lane i receives lane i-1 within its 16-lane row.
@@ -271,11 +297,15 @@ constexpr int V_DIFF = 1;    // diffusion damps each order by its own factor
 constexpr int V_B1MAP = 2;   // a per-atom B1 scales every flip
 constexpr int V_ACCUM = 4;   // accumulate normal equations instead of writing outputs
 constexpr int V_PADDED = 8;  // the tile covers more orders than the state count
+constexpr int V_PROJECT = 16;  // contract the samples with a temporal basis instead of writing them
 
-// Normal equations carry at most (T1, T2, B1) and no diffusion.
+// Normal equations carry at most (T1, T2, B1) and no diffusion; so does a
+// projection, which never accumulates normal equations.
 template <int TM, int V>
 constexpr bool valid_variant() {
-  return !((V & V_ACCUM) && ((V & V_DIFF) || TM == 15));
+  if ((V & V_ACCUM) && ((V & V_DIFF) || TM == 15)) return false;
+  if ((V & V_PROJECT) && ((V & V_ACCUM) || (V & V_DIFF) || TM == 15)) return false;
+  return true;
 }
 
 template <int SL, int L, int TM, int V>
@@ -284,6 +314,7 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   constexpr bool B1MAP = (V & V_B1MAP) != 0;
   constexpr bool ACCUM = (V & V_ACCUM) != 0;
   constexpr bool PADDED = (V & V_PADDED) != 0;
+  constexpr bool PROJECT = (V & V_PROJECT) != 0;
   constexpr int NT = popcount4(TM);
   constexpr int NTA = NT > 0 ? NT : 1;
   constexpr int ST1 = slot_of(TM, TAN_T1);
@@ -338,6 +369,20 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   for (int i = 0; i < (ACCUM ? NORMAL_COLUMNS : 1); ++i) acc_g[i] = 0.0;
 #pragma unroll
   for (int i = 0; i < (ACCUM ? NORMAL_COLUMNS * (NORMAL_COLUMNS + 1) / 2 : 1); ++i) acc_h[i] = 0.0;
+
+  // Projection accumulators: stream s (0 the unscaled sample a0, 1 + k the
+  // carried tangent k) against direction sub + L j, summed over the samples
+  // as sum_n sign_n x_n basis[n][r]; and this op's basis values.
+  constexpr int PJ = PROJECT ? PROJECT_PER_LANE : 1;
+  constexpr int NPS = PROJECT ? 1 + NT : 1;
+  float2 proj[NPS][PJ];
+  float2 pbasis[PJ];
+#pragma unroll
+  for (int j = 0; j < PJ; ++j) {
+    pbasis[j] = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int q = 0; q < NPS; ++q) proj[q][j] = make_float2(0.0f, 0.0f);
+  }
 
   uint32_t last_dt = 0xFFFFFFFFu;  // bit pattern of the cached interval (a NaN: never matches)
   float e1 = 1.0f, e2 = 1.0f, de1 = 0.0f, de2 = 0.0f;
@@ -596,6 +641,24 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
     float t0[NTA];
 #pragma unroll
     for (int k = 0; k < NTA; ++k) t0[k] = NT > 0 ? ta[k][R] : 0.0f;
+    if constexpr (PROJECT) {
+      // every lane of the atom takes the first lane's sample and tangents
+      // and adds them along its own basis directions (loaded at op start)
+      const float sign = p.sample_sign ? p.sample_sign[ev.out] : 1.0f;
+      float x[NPS];
+      x[0] = atom_broadcast<L>(a0) * sign;
+#pragma unroll
+      for (int k = 0; k < NPS - 1; ++k) x[1 + k] = atom_broadcast<L>(t0[k]) * sign;
+#pragma unroll
+      for (int q = 0; q < NPS; ++q) {
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+          proj[q][j].x = fmaf(x[q], pbasis[j].x, proj[q][j].x);
+          proj[q][j].y = fmaf(x[q], pbasis[j].y, proj[q][j].y);
+        }
+      }
+      return;
+    }
     if (!(first && live)) return;
     const int64_t o = ev.out;
     if constexpr (ACCUM) {
@@ -747,6 +810,17 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
       }
       next = load_op(ops, e);
       ++e;
+      if constexpr (PROJECT) {
+        // this op's basis row, issued before the plane work hides its latency
+        if ((ctl & OP_RECORD) && m.out >= 0 && m.out < p.outputs) {
+          const float2* row = p.basis + (int64_t)m.out * p.rank;
+#pragma unroll
+          for (int j = 0; j < PJ; ++j) {
+            const int r = sub + L * j;
+            pbasis[j] = r < p.rank ? row[r] : make_float2(0.0f, 0.0f);
+          }
+        }
+      }
       if constexpr (DIFF) {
         if (ctl & OP_RELAX) relax(rc, m.dt);
       }
@@ -761,7 +835,7 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
             if (staged && (nrec & (stage_k - 1)) == 0) fill();
           }
           record(rc, m);
-          if (staged) {
+          if (!PROJECT && staged) {
             ++nrec;
             if constexpr (!ACCUM) {
               if ((nrec & (stage_k - 1)) == 0) flush(stage_k);
@@ -783,8 +857,38 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
     while (segment(std::integral_constant<int, 0>{})) {
     }
   }
-  if constexpr (!ACCUM) {
+  if constexpr (!ACCUM && !PROJECT) {
     if (staged && (nrec & (stage_k - 1)) != 0) flush((int)(nrec & (stage_k - 1)));
+  }
+
+  if constexpr (PROJECT) {
+    // coefficient = factor * i * (m0 or 1) * accumulator: the projection of
+    // the samples factor * sign * (0 + i m0 a0) and of their tangents
+    if (live) {
+      const float gr = -p.factor_im, gi = p.factor_re;
+      auto scaled = [&](float2 v, float k) __attribute__((always_inline)) {
+        return make_float2(k * (gr * v.x - gi * v.y), k * (gr * v.y + gi * v.x));
+      };
+#pragma unroll
+      for (int j = 0; j < PJ; ++j) {
+        const int r = sub + L * j;
+        if (r >= p.rank) continue;
+        if (p.signal) p.signal[atom * p.rank + r] = scaled(proj[0][j], m0);
+        if (p.jacobian) {
+          float2* row = p.jacobian + atom * (int64_t)p.columns * p.rank + r;
+          if constexpr ((TM & TAN_T1) != 0) {
+            if (p.col_t1 >= 0) row[(int64_t)p.col_t1 * p.rank] = scaled(proj[1 + ST1][j], m0);
+          }
+          if constexpr ((TM & TAN_T2) != 0) {
+            if (p.col_t2 >= 0) row[(int64_t)p.col_t2 * p.rank] = scaled(proj[1 + ST2][j], m0);
+          }
+          if constexpr ((TM & TAN_B1) != 0) {
+            if (p.col_b1 >= 0) row[(int64_t)p.col_b1 * p.rank] = scaled(proj[1 + SB1][j], m0);
+          }
+          if (p.col_m0 >= 0) row[(int64_t)p.col_m0 * p.rank] = scaled(proj[0][j], 1.0f);
+        }
+      }
+    }
   }
 
   if constexpr (ACCUM) {
@@ -853,7 +957,7 @@ __host__ __device__ constexpr int tangent_class(int tm) {
 // null for the combinations no kernel carries.
 template <int SL, int L, int TM, int V = 0>
 RealLaunch pick_real_variant(int v) {
-  if constexpr (V < 16) {
+  if constexpr (V < 32) {
     if (v == V) {
       if constexpr (valid_variant<TM, V>()) return &launch_real<SL, L, TM, V>;
       return nullptr;

@@ -266,8 +266,10 @@ class Simulator:
         held: Mapping[str, Any],
         sequence: Mapping[str, Any],
         diff: tuple[str, ...] = (),
+        subspace: Any = None,
     ) -> tuple[torch.Tensor, torch.Tensor | None]:
-        """Signal (atoms, N) and Jacobian (atoms, P, N) of the recorded samples."""
+        """Signal (atoms, N) and Jacobian (atoms, P, N) of the recorded samples;
+        with a ``subspace``, their projections (atoms, R) and (atoms, P, R)."""
         device = _target_device({**held, **sequence}, sequence.get("device"))
         packed = self.packed(**sequence)
         fields = self._tissue_fields(held)
@@ -280,6 +282,16 @@ class Simulator:
         states = int(sequence.get("states", self.states) or 10)
         real, flip, samples = _engine.real_subspace(packed, tissue, device)
         factor = self.signal_factor if self.signal_factor is not None else 1.0
+        if subspace is not None:
+            if subspace.contrasts != packed.output_count:
+                raise ValueError(f"the subspace spans {subspace.contrasts} contrasts, the sequence records "
+                                 f"{packed.output_count}")
+            grad = packed.flip.requires_grad or packed.duration.requires_grad
+            if real and not grad and _engine.projects_fused(tissue, differentiated, subspace.rank):
+                return _engine.simulate_projected(packed, tissue, subspace.basis, states=states,
+                                                  columns=differentiated, factor=factor, flip=flip, samples=samples)
+            signal, jacobian = self._run(held, sequence, diff)
+            return subspace.project(signal), (None if jacobian is None else subspace.project(jacobian))
         if not real:
             from ..sequence import _complex
 
@@ -345,6 +357,44 @@ class Simulator:
         batch = self._batch(held)
         signal, jacobian = self.linearize(held, names, **sequence)
         signal = signal.reshape(*batch, signal.shape[-1])
+        jacobian = jacobian.reshape(*batch, len(names), jacobian.shape[-1])
+        if isinstance(diff, str):
+            jacobian = jacobian[..., 0, :]
+        return _like(signal, backend), _like(jacobian, backend)
+
+    def project(self, subspace: Any, diff: str | Sequence[str] | None = None, **values: Any) -> Any:
+        """The recorded signal projected onto ``subspace`` (a
+        :class:`~torchsim_amd.Subspace` over this sequence's samples), and
+        with ``diff`` its Jacobian projected too: what
+        ``subspace.project(simulate(...))`` and ``subspace.project(jacobian(...))``
+        return (reference recon/_operator.py:446-473), shaped like the
+        caller's maps with the sample axis replaced by the rank.
+
+        A real-subspace sequence without diffusion contracts the samples with
+        the basis inside the kernel (ranks up to 16), so the full-length
+        signal and Jacobian are never stored; any other runs the full
+        simulation and projects it on the device.
+        """
+        names = () if diff is None else ((diff,) if isinstance(diff, str) else tuple(diff))
+        backend = _backend(values)
+        held, sequence = self._split({**self.bound, **values})
+        for name in names:
+            if name not in held:
+                raise ValueError(f"{name!r} is not among the properties this call gave "
+                                 f"{type(self).__name__}: {sorted(held)}")
+            if not torch.is_floating_point(torch.as_tensor(held[name])):
+                raise ValueError(f"{name!r} does not carry a derivative")
+        if self.signal_factor is None:
+            signal, jacobian = (self.evaluate(held, **sequence), None) if not names else \
+                self.linearize(held, names, **sequence)
+            signal = subspace.project(signal)
+            jacobian = None if jacobian is None else subspace.project(jacobian)
+        else:
+            signal, jacobian = self._run(held, sequence, names, subspace=subspace)
+        batch = self._batch(held)
+        signal = signal.reshape(*batch, signal.shape[-1])
+        if not names:
+            return _like(signal, backend)
         jacobian = jacobian.reshape(*batch, len(names), jacobian.shape[-1])
         if isinstance(diff, str):
             jacobian = jacobian[..., 0, :]

@@ -276,6 +276,67 @@ def simulate_real(
     return signal, jacobian
 
 
+#: Basis directions one projected launch carries (4 per lane, 4 lanes per atom).
+PROJECTED_RANK_MAX = 16
+
+
+def projects_fused(tissue: PreparedTissue, columns: tuple[str, ...], rank: int) -> bool:
+    """Whether ``simulate_projected`` takes this launch: no diffusion, no
+    inversion-efficiency column, at most PROJECTED_RANK_MAX directions."""
+    return (rank <= PROJECTED_RANK_MAX and not tissue.features & _native.FEATURE_DIFFUSING
+            and "inversion_efficiency" not in columns)
+
+
+def simulate_projected(
+    packed: PackedEvents,
+    tissue: PreparedTissue,
+    basis: torch.Tensor,
+    *,
+    states: int,
+    columns: tuple[str, ...] = (),
+    factor: complex = 1.0,
+    flip: torch.Tensor | None = None,
+    samples: torch.Tensor | None = None,
+    tile: int = -1,
+) -> tuple[torch.Tensor, torch.Tensor | None]:
+    """``simulate_real`` with the samples contracted with ``basis``
+    ``(outputs, rank)`` in the kernel: complex64 ``(atoms, rank)`` and
+    ``(atoms, len(columns), rank)``, i.e. ``signal @ basis`` and
+    ``jacobian @ basis`` without either reaching memory."""
+    device = tissue.device
+    _native.require_device(device)
+    lib = _native.load()
+    outputs = packed.output_count
+    if basis.ndim != 2 or basis.shape[0] != outputs:
+        raise ValueError(f"the basis must be (samples={outputs}, rank), got {tuple(basis.shape)}")
+    rank = int(basis.shape[1])
+    atoms = tissue.atoms
+    if outputs == 0:
+        signal = torch.zeros((atoms, rank), dtype=torch.complex64, device=device)
+        jacobian = torch.zeros((atoms, len(columns), rank), dtype=torch.complex64, device=device) if columns else None
+        return signal, jacobian
+    events = list(packed.on(device))
+    if flip is not None:
+        events[2] = flip.detach().to(device=device, dtype=torch.float32).contiguous()
+    held = basis.detach().to(device=device, dtype=torch.complex64).contiguous()
+    signal = torch.empty((atoms, rank), dtype=torch.complex64, device=device)
+    jacobian = torch.empty((atoms, len(columns), rank), dtype=torch.complex64, device=device) if columns else None
+    if atoms == 0:
+        return signal, jacobian
+    ptrs = _native.pointers([*tissue.buffers, *events])
+    cols = (ctypes.c_int32 * max(1, len(columns)))(*[_COLUMN_INDEX[name] for name in columns])
+    sign_buffer = None if samples is None else samples.to(device=device, dtype=torch.float32).contiguous()
+    with torch.cuda.device(device):
+        status = lib.tsim_simulate_projected(
+            ptrs, atoms, packed.event_count, states, outputs, tissue.features, tissue.stride, cols, len(columns),
+            float(complex(factor).real), float(complex(factor).imag),
+            None if sign_buffer is None else sign_buffer.data_ptr(), held.data_ptr(), rank, signal.data_ptr(),
+            None if jacobian is None else jacobian.data_ptr(), tile, _native.stream_of(device),
+        )
+    _native.check(status)
+    return signal, jacobian
+
+
 def narrow_tissue(tissue: PreparedTissue, atoms: int) -> PreparedTissue:
     """The first ``atoms`` atoms of a prepared tissue (views, no copy)."""
     atoms = min(atoms, tissue.atoms)
