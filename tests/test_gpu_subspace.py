@@ -145,3 +145,69 @@ def test_no_atoms_and_shapes():
     assert maps[0].shape == (3, 4, 3) and maps[1].shape == (3, 4, 3)
     empty = mrf.project(subspace, T1=grid[:0], T2=grid[:0])
     assert empty.shape == (0, 3)
+
+
+def test_subspace_normal_equations_match_the_oracle():
+    """Cost, J^T r and J^T J of coefficients: the kernel-projected model and
+    Jacobian against the FP64 oracle's, projected in FP64."""
+    import oracle
+    from tests.test_gpu_fit import _mrf
+    from torchsim_amd import Subspace
+    from torchsim_amd.recon import FusedModel
+
+    seq = _mrf()
+    t1, t2 = _cases.random_tissue(200, seed=23)
+    m0 = np.linspace(0.5, 1.0, 200)
+    events = oracle.events_of(seq.packed(**seq.bound))
+    sig, tan = oracle.real(events, t1=t1, t2=t2, m0=m0, states=10, outputs=880,
+                           tangents=[{"t1": 1.0}, {"t2": 1.0}, {"m0": 1.0}])
+    model, jac = -sig, -tan  # MRF: 1j * (i m0 a0)
+    basis = _basis(model.astype(np.complex128), 8).astype(np.complex64)
+    b64 = basis.astype(np.complex128)
+    pm, pj = model @ b64, jac @ b64
+    rng = np.random.default_rng(4)
+    data = (pm + 0.01 * (rng.normal(size=pm.shape) + 1j * rng.normal(size=pm.shape))).astype(np.complex64)
+    r = pm - data.astype(np.complex128)
+    cost = (np.abs(r) ** 2).sum(-1)
+    g = np.einsum("vps,vs->vp", pj.conj(), r).real
+    h = np.einsum("vps,vqs->vpq", pj.conj(), pj).real
+    op = FusedModel(seq, ("T1", "T2", "M0"), subspace=Subspace(basis=torch.from_numpy(basis),
+                                                               singular_values=torch.ones(8)))
+    x = torch.tensor(np.stack([t1, t2, m0], -1), dtype=torch.float64, device="cuda")
+    normal = op.normal(x, torch.tensor(data).cuda(), None).cpu().numpy()
+    assert _cases.rel_err(normal[:, 0], cost) < 1e-3  # cost of a small residual: cancellation-limited
+    for p in range(3):
+        assert _cases.rel_err(normal[:, 1 + p], g[:, p]) < 1e-3
+    index = {(0, 0): 5, (0, 1): 6, (0, 2): 7, (1, 1): 9, (1, 2): 10, (2, 2): 12}
+    for (p, q), k in index.items():
+        assert _cases.rel_err(normal[:, k], h[:, p, q]) < 1e-4, (p, q)
+
+
+def test_fits_in_a_fitted_subspace_recover_noiseless_maps():
+    """NonlinearLeastSquares with a rank: a basis fitted to simulated
+    signals, the measured volume projected onto it, Levenberg-Marquardt on
+    the coefficients; from_coefficients on the same coefficients agrees."""
+    from tests.test_gpu_fit import _mrf
+    from torchsim_amd.estimators import NonlinearLeastSquares
+    from torchsim_amd.recon import GaussNewton, TrustRegion
+
+    seq = _mrf()
+    t1, t2 = _cases.random_tissue(300, seed=5)
+    t1 = np.clip(t1, 300, 2500).astype(np.float32)
+    t2 = np.clip(t2, 30, 250).astype(np.float32)
+    m0 = np.linspace(0.5, 1.0, 300).astype(np.float32)
+    data = seq.simulate(T1=torch.tensor(t1).cuda(), T2=torch.tensor(t2).cuda(), M0=torch.tensor(m0).cuda())
+    fit = NonlinearLeastSquares(seq, bounds={"T1": (100.0, 5000.0), "T2": (10.0, 2000.0), "M0": (0.01, 2.0)},
+                                loop=GaussNewton(TrustRegion(), max_iterations=40))
+    fit.fit(rank=8, samples=4000, T1=(300.0, 2500.0), T2=(30.0, 250.0), M0=(0.5, 1.0))
+    assert fit.subspace.rank == 8 and fit.subspace.retained > 0.999
+    maps = fit.map(data)
+    for name, truth in (("T1", t1), ("T2", t2), ("M0", m0)):
+        err = np.abs(maps[name].cpu().numpy() - truth) / truth
+        assert np.median(err) < 1e-3, name
+        assert np.percentile(err, 99) < 2e-2, name
+    again = fit.from_coefficients(fit.subspace.project(data))
+    for name in maps:
+        torch.testing.assert_close(again[name], maps[name])
+    with pytest.raises(ValueError, match="rank 8"):
+        fit.from_coefficients(data)

@@ -3,6 +3,7 @@
 
 from __future__ import annotations
 
+import dataclasses
 from collections.abc import Mapping
 from typing import Any
 
@@ -22,8 +23,10 @@ class NonlinearLeastSquares:
     """
 
     def __init__(self, acquisition: Any = None, *, bounds: Mapping[str, Any] | None = None,
-                 initial: Mapping[str, float] | None = None, loop: GaussNewton | None = None) -> None:
+                 initial: Mapping[str, float] | None = None, loop: GaussNewton | None = None,
+                 subspace: Any = None) -> None:
         self.acquisition = acquisition
+        self._subspace = subspace
         self.bounds = dict(bounds or {})
         self.initial = dict(initial or {})
         self.loop = loop if loop is not None else GaussNewton(TrustRegion(), max_iterations=20)
@@ -37,10 +40,23 @@ class NonlinearLeastSquares:
     def fitted(self) -> bool:
         return self.acquisition is not None and self._start is not None
 
-    def fit(self, *, seed: int | None = None, samples: int = 10_000, **ranges: Any) -> NonlinearLeastSquares:
-        """Name the unknowns with their ranges ``name=(low, high)`` and choose the start."""
+    @property
+    def subspace(self) -> Any:
+        """The temporal basis fits run in, if any (reference
+        estimators/_mapping.py:101-104)."""
+        return self._subspace
+
+    def fit(self, *, seed: int | None = None, samples: int = 10_000, rank: int | None = None,
+            subspace: Any = None, **ranges: Any) -> NonlinearLeastSquares:
+        """Name the unknowns with their ranges ``name=(low, high)`` and choose
+        the start. ``rank`` fits a temporal basis of that rank to ``samples``
+        signals simulated uniformly over the ranges; ``subspace`` hands one in
+        (reference estimators/_mapping.py:118-255). With either, every fit
+        runs on subspace coefficients."""
         if not ranges:
             raise ValueError("name at least one unknown property with its range")
+        if rank is not None and subspace is not None:
+            raise ValueError("give a rank to fit a basis here, or a subspace to work in one, not both")
         generator = torch.Generator().manual_seed(0 if seed is None else seed)
         start = []
         for name, value in ranges.items():
@@ -52,10 +68,44 @@ class NonlinearLeastSquares:
             start.append(float(drawn.median()))
         self.unknown = tuple(ranges)
         self._start = torch.tensor(start, dtype=torch.float64)
+        if subspace is not None:
+            self._subspace = subspace
+        elif rank is not None:
+            self._subspace = self._fit_basis(ranges, samples, rank, generator)
         return self
+
+    def _fit_basis(self, ranges: Mapping[str, Any], samples: int, rank: int, generator: torch.Generator) -> Any:
+        """A rank-``rank`` basis of the acquisition's signals over the ranges
+        (uniform draws; the reference's training-set draw is its own)."""
+        from .._subspace import Subspace
+
+        if self.acquisition is None:
+            raise RuntimeError("a basis is fitted to simulated signals: give this estimator its acquisition")
+        values = {}
+        for name, (low, high) in ranges.items():
+            drawn = low + (high - low) * torch.rand(samples, generator=generator, dtype=torch.float64)
+            values[name] = drawn.to(device="cuda", dtype=torch.float32)
+        return Subspace.fit(torch.as_tensor(self.acquisition.simulate(**values)).cpu(), rank)
 
     def map(self, volume: Any, known: Mapping[str, Any] | None = None) -> dict[str, torch.Tensor]:
         """``{name: map}`` shaped like ``volume``'s leading axes."""
+        if not self.fitted:
+            raise RuntimeError("the estimator has no model and starting point to fit from")
+        return self._solve(volume, known, project=True)
+
+    def from_coefficients(self, coefficients: Any, known: Mapping[str, Any] | None = None) -> dict[str, torch.Tensor]:
+        """Maps from coefficients already in :attr:`subspace` (what a subspace
+        reconstruction returns), without projecting them again (reference
+        estimators/_mapping.py:453-498)."""
+        if self._subspace is None:
+            raise RuntimeError("this estimator has no subspace, so there are no coefficients to read; "
+                               "state a rank when it is fitted")
+        values = torch.as_tensor(coefficients)
+        if values.shape[-1] != self._subspace.rank:
+            raise ValueError(f"the basis has rank {self._subspace.rank}, got {values.shape[-1]} coefficients")
+        return self._solve(values, known, project=False)
+
+    def _solve(self, volume: Any, known: Mapping[str, Any] | None, *, project: bool) -> dict[str, torch.Tensor]:
         if not self.fitted:
             raise RuntimeError("the estimator has no model and starting point to fit from")
         signals = torch.as_tensor(volume)
@@ -63,7 +113,13 @@ class NonlinearLeastSquares:
             signals = signals.cuda()
         shape = signals.shape[:-1]
         measured = signals.reshape(-1, signals.shape[-1])
-        operator = FusedModel(self.acquisition, self.unknown, bounds=self.bounds, known=known)
+        subspace = self._subspace
+        if subspace is not None:
+            # the basis is read by every iteration's launch: hold it where they run
+            subspace = dataclasses.replace(subspace, basis=subspace.basis.to(measured.device))
+            if project:
+                measured = subspace.project(measured)
+        operator = FusedModel(self.acquisition, self.unknown, bounds=self.bounds, known=known, subspace=subspace)
         start = operator.free(self._start.to(measured.device)[None, :].expand(measured.shape[0], -1))
         found = self.loop.minimize(operator, measured, start)
         self.iterations = found.iterations

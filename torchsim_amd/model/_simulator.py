@@ -384,13 +384,7 @@ class Simulator:
                                  f"{type(self).__name__}: {sorted(held)}")
             if not torch.is_floating_point(torch.as_tensor(held[name])):
                 raise ValueError(f"{name!r} does not carry a derivative")
-        if self.signal_factor is None:
-            signal, jacobian = (self.evaluate(held, **sequence), None) if not names else \
-                self.linearize(held, names, **sequence)
-            signal = subspace.project(signal)
-            jacobian = None if jacobian is None else subspace.project(jacobian)
-        else:
-            signal, jacobian = self._run(held, sequence, names, subspace=subspace)
+        signal, jacobian = self.linearize_projected(held, names, subspace, **sequence)
         batch = self._batch(held)
         signal = signal.reshape(*batch, signal.shape[-1])
         if not names:
@@ -399,6 +393,20 @@ class Simulator:
         if isinstance(diff, str):
             jacobian = jacobian[..., 0, :]
         return _like(signal, backend), _like(jacobian, backend)
+
+    def linearize_projected(self, held: Mapping[str, Any], names: tuple[str, ...], subspace: Any,
+                            **sequence: Any) -> tuple[torch.Tensor, torch.Tensor | None]:
+        """Signal (voxels, R) and Jacobian (voxels, P, R) projected onto
+        ``subspace`` (fused into the kernel where the model allows it)."""
+        if self.signal_factor is not None:
+            return self._run(held, sequence, tuple(names), subspace=subspace)
+        if subspace.contrasts != self.packed(**sequence).output_count:
+            raise ValueError(f"the subspace spans {subspace.contrasts} contrasts, the sequence records "
+                             f"{self.packed(**sequence).output_count}")
+        if not names:
+            return subspace.project(self.evaluate(held, **sequence)), None
+        signal, jacobian = self.linearize(held, tuple(names), **sequence)
+        return subspace.project(signal), subspace.project(jacobian)
 
     def linearize(self, held: Mapping[str, Any], names: tuple[str, ...], **sequence: Any) -> tuple[torch.Tensor, torch.Tensor]:
         """Signal (voxels, N) and Jacobian (voxels, P, N) along ``names``."""

@@ -27,8 +27,11 @@ class FusedModel:
     names, e.g. ``("T1", "T2", "M0")``), bounded by ``bounds``."""
 
     def __init__(self, simulator: Any, names: Sequence[str], *, bounds: Mapping[str, Any] | None = None,
-                 known: Mapping[str, Any] | None = None) -> None:
+                 known: Mapping[str, Any] | None = None, subspace: Any = None) -> None:
         self.simulator = simulator
+        # with a subspace, data and model are (voxels, rank) coefficients
+        # (reference recon/_operator.py:451-473 projects both)
+        self.subspace = subspace
         self.names = tuple(names)
         self.bounds = dict(bounds or {})
         self.known = dict(known or {})
@@ -84,7 +87,7 @@ class FusedModel:
             fields.update(self.simulator._tissue_fields({name: value}))
         for index, field in enumerate(self.fields):
             fields[field] = columns[index]
-        if self.__dict__.get("_unfused"):
+        if self.subspace is not None or self.__dict__.get("_unfused"):
             return self._normal_from_jacobian(columns, data, live)
         tissue = _engine.prepare_tissue(fields, device, self.fields)
         real, flip, samples = _engine.real_subspace(self.packed, tissue, device)
@@ -119,7 +122,8 @@ class FusedModel:
         """Any sequence: the simulator's fused signal + Jacobian (complex
         kernel, post-processing included), reduced to the normal equations in
         FP64 by tsim_normal_from_jacobian -- the Jacobian passes through memory
-        once on this path."""
+        once on this path. With a subspace, signal and Jacobian arrive as
+        coefficients, projected inside the kernel where the model allows."""
         device = data.device
         held = {}
         for name, value in self.held_on(device).items():
@@ -128,7 +132,10 @@ class FusedModel:
             held[name] = value
         for index, name in enumerate(self.names):
             held[name] = columns[index]
-        signal, jacobian = self.simulator.linearize(held, self.names, **self.sequence)
+        if self.subspace is not None:
+            signal, jacobian = self.simulator.linearize_projected(held, self.names, self.subspace, **self.sequence)
+        else:
+            signal, jacobian = self.simulator.linearize(held, self.names, **self.sequence)
         lib = _native.load()
         stride = lib.tsim_normal_stride()
         voxels, outputs = signal.shape
