@@ -77,6 +77,7 @@ class DictionaryMatcher:
         self._operand: tuple[torch.Tensor, torch.Tensor] | None = None
         self._norm: torch.Tensor | None = None
         self._mode = 0
+        self._subspace: Any = None
         if dictionary is not None:
             self._adopt(dictionary, parameters)
 
@@ -88,10 +89,41 @@ class DictionaryMatcher:
     def grouping(self) -> None:
         return None
 
-    def fit(self, signals: Any, parameters: Any = None) -> DictionaryMatcher:
-        """Adopt ``signals`` ``(atoms, contrasts)`` as the dictionary."""
+    @property
+    def subspace(self) -> Any:
+        """The temporal basis the dictionary is held in, if any."""
+        return self._subspace
+
+    def fit(self, signals: Any, parameters: Any = None, *, rank: int | None = None,
+            subspace: Any = None) -> DictionaryMatcher:
+        """Adopt ``signals`` ``(atoms, contrasts)`` as the dictionary. With
+        ``rank`` a basis of that rank is fitted to them first, with
+        ``subspace`` one is given; either way the atoms are held as their
+        coefficients and :meth:`match` projects the voxels onto the same
+        basis (reference estimators/_mapping.py:118-255, 508-520)."""
+        if rank is not None and subspace is not None:
+            raise ValueError("give a rank to fit a basis here, or a subspace to work in one, not both")
+        if rank is not None:
+            from .._subspace import Subspace
+
+            subspace = Subspace.fit(torch.as_tensor(signals), rank)
+        if subspace is not None:
+            signals = subspace.project(torch.as_tensor(signals))
         self._adopt(signals, parameters)
+        self._subspace = subspace
         return self
+
+    @torch.no_grad()
+    def from_coefficients(self, coefficients: Any) -> MatchResult:
+        """Match coefficients already in :attr:`subspace` without projecting
+        them again (reference estimators/_mapping.py:453-498)."""
+        if self._subspace is None:
+            raise RuntimeError("this matcher has no subspace, so there are no coefficients to read; "
+                               "give fit a rank or a subspace")
+        values = torch.as_tensor(coefficients)
+        if values.shape[-1] != self._subspace.rank:
+            raise ValueError(f"the basis has rank {self._subspace.rank}, got {values.shape[-1]} coefficients")
+        return self._match(values)
 
     def _adopt(self, dictionary: Any, parameters: Any) -> None:
         """Keep these atoms and lay them out for the kernel
@@ -145,6 +177,13 @@ class DictionaryMatcher:
         if not self.fitted:
             raise RuntimeError("the matcher has no dictionary to match against")
         signals = torch.as_tensor(signals)
+        if self._subspace is not None:
+            if signals.shape[-1] != self._subspace.contrasts:
+                raise ValueError("signal and dictionary contrast counts differ")
+            signals = self._subspace.project(signals)
+        return self._match(signals)
+
+    def _match(self, signals: torch.Tensor) -> MatchResult:
         if signals.shape[-1] != self.dictionary.shape[-1]:
             raise ValueError("signal and dictionary contrast counts differ")
         sample_shape = signals.shape[:-1]
