@@ -39,6 +39,13 @@ PUBLIC_PROPERTIES = {
     "inv_efficiency": "inversion_efficiency",
     "D": "diffusion_um2_per_ms",
     "B0": "b0_hz",
+    "B1phase": "b1_phase_rad",
+    "poolB_fraction": "pool_b_fraction",
+    "poolB_exchange": "pool_b_exchange_rate",
+    "poolB_T1": "t1_pool_b_ms",
+    "poolB_T2": "t2_pool_b_ms",
+    "poolB_shift": "pool_b_shift_hz",
+    # the same pool, also accepted under these names
     "pool_b_fraction": "pool_b_fraction",
     "pool_b_exchange": "pool_b_exchange_rate",
     "T1_pool_b": "t1_pool_b_ms",
