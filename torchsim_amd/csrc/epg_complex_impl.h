@@ -269,7 +269,8 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
   // single pool
   float e1 = 1.0f, e2 = 1.0f, de1 = 0.0f, de2 = 0.0f, om1 = 0.0f;
   // exchanging pool: transverse X (complex 2x2) and dX/dT2, longitudinal
-  // Y (2x2) and dY/dT1, recovery r and dr/dT1
+  // Y (2x2) and dY/dT1, recovery r and dr/dT1; once formed for an interval,
+  // X and dX carry this order's turn and Y and dY its longitudinal damping
   float2 X[4], dX[4];
   float Y[4], dY[4], rec[2], drec[2];
   if constexpr (EXCH) {
@@ -377,6 +378,15 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
             drec[0] = dY[0] * u0 + dY[1] * u1 + (Y[0] - 1.0f) * du0 + Y[1] * du1;
             drec[1] = dY[2] * u0 + dY[3] * u1 + Y[2] * du0 + (Y[3] - 1.0f) * du1;
           }
+          // every event of this interval applies turn X (F+), its conjugate
+          // (F-) and dlong Y (Z): fold the per-order factors in once here
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            X[i] = mul(turn, X[i]);
+            dX[i] = mul(turn, dX[i]);
+            Y[i] *= dlong;
+            dY[i] *= dlong;
+          }
         }
       }
       const float2 turn_c = cj(turn);
@@ -403,7 +413,8 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
         v.z[0] = scl(v.z[0], lo);
         if (sub == 0) v.z[0].x += om1;
       } else {
-        // transverse: F+ <- turn X F+, F- <- conj(turn) conj(X) F-
+        // transverse: F+ <- (turn X) F+, F- <- conj(turn X) F-; longitudinal
+        // Z <- (dlong Y) Z + recovery
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
           float2 a0 = add(mul(X[0], t[k].fp[0]), mul(X[1], t[k].fp[1]));
@@ -427,12 +438,12 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
               z1 = drec[1];
             }
           }
-          t[k].fp[0] = mul(turn, a0);
-          t[k].fp[1] = mul(turn, a1);
-          t[k].fm[0] = mul(turn_c, b0v);
-          t[k].fm[1] = mul(turn_c, b1v);
-          t[k].z[0] = scl(zz0, dlong);
-          t[k].z[1] = scl(zz1, dlong);
+          t[k].fp[0] = a0;
+          t[k].fp[1] = a1;
+          t[k].fm[0] = b0v;
+          t[k].fm[1] = b1v;
+          t[k].z[0] = zz0;
+          t[k].z[1] = zz1;
           t[k].z[0].x += z0;
           t[k].z[1].x += z1;
         }
@@ -442,12 +453,12 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
         const float2 b1v = add(mul(cj(X[2]), v.fm[0]), mul(cj(X[3]), v.fm[1]));
         const float2 zz0 = add(scl(v.z[0], Y[0]), scl(v.z[1], Y[1]));
         const float2 zz1 = add(scl(v.z[0], Y[2]), scl(v.z[1], Y[3]));
-        v.fp[0] = mul(turn, a0);
-        v.fp[1] = mul(turn, a1);
-        v.fm[0] = mul(turn_c, b0v);
-        v.fm[1] = mul(turn_c, b1v);
-        v.z[0] = scl(zz0, dlong);
-        v.z[1] = scl(zz1, dlong);
+        v.fp[0] = a0;
+        v.fp[1] = a1;
+        v.fm[0] = b0v;
+        v.fm[1] = b1v;
+        v.z[0] = zz0;
+        v.z[1] = zz1;
         if (sub == 0) {
           v.z[0].x += rec[0];
           v.z[1].x += rec[1];
