@@ -150,24 +150,26 @@ def half_turns(packed: PackedEvents) -> tuple[torch.Tensor, torch.Tensor, torch.
     flip (pulses) or on the recorded sample (samples); ``None`` if some pulse
     or sample sits off that axis (reference sequence/_accelerators.py:500-546).
     """
-    kind = packed.kind.cpu()
-    action = packed.action.cpu().to(torch.int32)
-    phase = packed.phase.detach().cpu().to(torch.float64)
+    # a few thousand events: NumPy on the host, no torch dispatch per op
+    kind = packed.kind.cpu().numpy()
+    action = packed.action.cpu().numpy().astype(np.int32)
+    phase = packed.phase.detach().cpu().numpy().astype(np.float64)
     turning = (kind == 1) & ((action & INVERSION) == 0)
     recording = (kind == 2) & ((action & RECORD) != 0)
-    if not bool(turning.any()):
+    if not turning.any():
         return None
-    anchor = recording if bool(recording.any()) else turning
+    anchor = recording if recording.any() else turning
     delta = phase - phase[anchor][0]
     on_axis = turning | recording
-    out_by = torch.where(on_axis, torch.round(delta / torch.pi), torch.zeros_like(delta))
-    residue = torch.where(on_axis, delta - out_by * torch.pi, torch.zeros_like(delta))
-    if float(residue.abs().max()) > PHASE_TOLERANCE:
+    out_by = np.where(on_axis, np.round(delta / np.pi), 0.0)
+    residue = np.where(on_axis, delta - out_by * np.pi, 0.0)
+    if float(np.abs(residue).max()) > PHASE_TOLERANCE:
         return None
-    sign = torch.cos(out_by * torch.pi)
-    flips = torch.where(turning, sign, torch.ones_like(sign))
-    samples = sign[recording] if bool(recording.any()) else None
-    return out_by, flips, samples
+    # cos(k pi) for a whole number of half turns k: +1 for even k, -1 for odd
+    sign = np.where(np.fmod(out_by, 2.0) == 0.0, 1.0, -1.0)
+    flips = np.where(turning, sign, 1.0)
+    samples = torch.from_numpy(sign[recording]) if recording.any() else None
+    return torch.from_numpy(out_by), torch.from_numpy(flips), samples
 
 
 def real_subspace(packed: PackedEvents, tissue: PreparedTissue,
