@@ -604,7 +604,11 @@ extern "C" int tsim_simulate_vjp_jacobian(void* const* packed, int64_t atom_coun
   const int SL = kAdjointTiles[tile].sl;
   const int NT = popcount4(tm);
   const int64_t nfloat = 3LL * SL * (1 + NT);
-  const int threads = 256;
+  // Few atoms leave most compute units idle with 4-wave blocks (1,024 atoms x
+  // 16 lanes fill 64 of 256): one-wave blocks spread the same waves over
+  // every unit, each with its own caches and memory path. Every index below
+  // derives from the global thread id and the launch total, not the block.
+  const int threads = atom_count * L < 256LL * 256 ? 64 : 256;
   const int64_t blocks = (atom_count * L + threads - 1) / threads;
   const int64_t total = blocks * threads;
   const int64_t waves = total / 64;

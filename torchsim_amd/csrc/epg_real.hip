@@ -296,11 +296,19 @@ void set_staging(RealArgs& args, int lanes_per_atom) {
   args.stage_s_log2 = s_log2;
 }
 
-// Dynamic LDS bytes of a 256-thread block for the staging above.
-size_t stage_bytes(const RealArgs& args, int lanes_per_atom) {
+// Dynamic LDS bytes of a block of `threads` threads for the staging above.
+size_t stage_bytes(const RealArgs& args, int lanes_per_atom, int threads = 256) {
   if (args.stage_k_log2 < 1) return 0;
-  return (size_t)4 * (64 / lanes_per_atom) * ((size_t)1 << (args.stage_s_log2 + args.stage_k_log2)) * sizeof(float2);
+  return (size_t)(threads / 64) * (64 / lanes_per_atom) * ((size_t)1 << (args.stage_s_log2 + args.stage_k_log2)) *
+         sizeof(float2);
 }
+
+// Threads per block of a real-kernel launch over `lanes` lanes: one-wave
+// blocks when four-wave blocks would occupy fewer than all 256 compute units,
+// so the waves spread over every unit (each with its own caches and memory
+// path); four-wave blocks otherwise. The kernel indexes by global thread id
+// and by wave within the block only.
+int real_block_threads(int64_t lanes) { return lanes < 256LL * 256 ? 64 : 256; }
 
 // ---------------------------------------------------------------------------
 // The real-subspace kernel.
@@ -381,8 +389,8 @@ int run_real(RealArgs& args, int tm, int features, int tile, hipStream_t stream)
   }
   if (args.atoms == 0 || args.outputs == 0) return OK;
   const int L = kTiles[tile].l;
-  const int threads = 256;
   const int64_t lanes = args.atoms * L;
+  const int threads = real_block_threads(lanes);
   const int64_t blocks = (lanes + threads - 1) / threads;
   if (blocks > 0x7FFFFFFF) {
     set_error("too many atoms for one launch: %lld", (long long)args.atoms);
@@ -394,7 +402,7 @@ int run_real(RealArgs& args, int tm, int features, int tile, hipStream_t stream)
     return UNSUPPORTED;
   }
   set_staging(args, L);
-  launch(args, dim3((unsigned)blocks), dim3(threads), stage_bytes(args, L), stream);
+  launch(args, dim3((unsigned)blocks), dim3(threads), stage_bytes(args, L, threads), stream);
   return hip_status(hipGetLastError(), "real_kernel launch");
 }
 
@@ -973,7 +981,8 @@ int tsim_simulate_projected(void* const* packed, int64_t atom_count, int64_t eve
   if (st != OK) return st;
   args.ops = compiled.ops;
   const int L = kTiles[tile].l;
-  const int64_t blocks = (atom_count * L + 255) / 256;
+  const int threads = real_block_threads(atom_count * L);
+  const int64_t blocks = (atom_count * L + threads - 1) / threads;
   if (blocks > 0x7FFFFFFF) {
     set_error("too many atoms for one launch: %lld", (long long)atom_count);
     return BAD_ARGUMENT;
@@ -983,7 +992,7 @@ int tsim_simulate_projected(void* const* packed, int64_t atom_count, int64_t eve
     set_error("no projecting kernel for this variant");
     return UNSUPPORTED;
   }
-  launch(args, dim3((unsigned)blocks), dim3(256), 0, s);
+  launch(args, dim3((unsigned)blocks), dim3(threads), 0, s);
   return hip_status(hipGetLastError(), "real_kernel launch");
 }
 
@@ -1063,14 +1072,15 @@ int tsim_gauss_newton_normal(void* const* packed, int64_t atom_count, int64_t ev
   if (st != OK) return st;
   args.ops = compiled.ops;
   const int L = kTiles[tile].l;
-  const int64_t blocks = (atom_count * L + 255) / 256;
+  const int threads = real_block_threads(atom_count * L);
+  const int64_t blocks = (atom_count * L + threads - 1) / threads;
   RealLaunch launch = pick_real(tile, tm, variant_of(features, true, tile, args.states));
   if (!launch) {
     set_error("no normal-equation kernel for this variant");
     return UNSUPPORTED;
   }
   set_staging(args, L);
-  launch(args, dim3((unsigned)blocks), dim3(256), stage_bytes(args, L), s);
+  launch(args, dim3((unsigned)blocks), dim3(threads), stage_bytes(args, L, threads), s);
   return hip_status(hipGetLastError(), "normal-equation kernel launch");
 }
 
