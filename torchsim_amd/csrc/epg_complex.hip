@@ -90,7 +90,11 @@ int run_complex(ComplexArgs& a, int tm, bool exch, hipStream_t stream) {
     set_error("no complex kernel for %d orders", a.states);
     return UNSUPPORTED;
   }
-  const int threads = 256;
+  // one-wave blocks when four-wave blocks would occupy fewer than all 256
+  // compute units, so a small launch's waves spread over every unit (the
+  // kernel indexes by global wave and wave-within-block only)
+  const int64_t waves = (a.atoms + 64 / lanes - 1) / (64 / lanes);
+  const int threads = waves < 4LL * 256 ? 64 : 256;
   const int64_t atoms_per_block = (int64_t)(threads / 64) * (64 / lanes);
   const int64_t blocks = (a.atoms + atoms_per_block - 1) / atoms_per_block;
   if (blocks > 0x7FFFFFFF) {
