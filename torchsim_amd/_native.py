@@ -57,6 +57,9 @@ _I32 = ctypes.c_int
 _F32 = ctypes.c_float
 
 #: Every symbol include/torchsim_hip.h declares, with its ctypes signature.
+#: ABI revision these bindings are written against (``tsim_version``).
+ABI_VERSION = 2
+
 SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "tsim_last_error": (ctypes.c_char_p, []),
     "tsim_version": (_I32, []),
@@ -137,7 +140,7 @@ SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "tsim_gauss_newton_normal": (
         _I32,
         [ctypes.POINTER(_P), _I64, _I64, _I64, _I64, _I32, _I64, ctypes.POINTER(ctypes.c_int32), _I64,
-         _F32, _F32, _P, _P, _P, _I32, _P],
+         _F32, _F32, _P, _P, _P, _P, _I32, _P],
     ),
     "tsim_normal_from_jacobian": (_I32, [_P, _P, _P, _I64, _I64, _I32, _P, _P]),
     "tsim_lm_propose": (_I32, [_P, _P, _P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double),
@@ -204,6 +207,9 @@ def load(path: str | os.PathLike | None = None) -> ctypes.CDLL:
         function = getattr(lib, name)
         function.restype = restype
         function.argtypes = argtypes
+    if lib.tsim_version() != ABI_VERSION:
+        raise RuntimeError(f"{where} has ABI revision {lib.tsim_version()}, these bindings expect {ABI_VERSION}: "
+                           "rebuild it with `python -m torchsim_amd._build`")
     if lib.tsim_batch_request_size() != ctypes.sizeof(BatchRequest):
         raise RuntimeError("tsim_batch_request layout differs between the library and the bindings")
     if path is None:

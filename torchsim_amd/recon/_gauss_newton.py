@@ -121,8 +121,10 @@ class GaussNewton:
     """Solve a per-voxel nonlinear least-squares problem by repeated linearization.
 
     ``operator`` provides ``normal(x, data, live) -> (voxels, stride) FP64`` (cost,
-    J^T r, J^T J in the free variables' natural columns), ``scale(x)`` (the
-    chain-rule factor d natural / d free) and ``columns``.
+    J^T r, J^T J in the free variables' natural columns; ``data`` holds every
+    voxel's measured row and ``live`` the voxels still iterating, so retiring
+    voxels never copies the data), ``scale(x)`` (the chain-rule factor
+    d natural / d free) and ``columns``.
     """
 
     def __init__(self, damping: Any = None, *, max_iterations: int = 8, gradient_tolerance: float = 1e-8,
@@ -213,4 +215,4 @@ class GaussNewton:
             return x, live, data, normal, cost, damping, gradient
         if hasattr(self.damping, "retire"):
             self.damping.retire(keep)
-        return x[keep], live[keep], data[keep], normal[keep], cost[keep], damping[keep], gradient[keep]
+        return x[keep], live[keep], data, normal[keep], cost[keep], damping[keep], gradient[keep]
