@@ -88,8 +88,9 @@ class TrustRegion:
         self._rising = torch.where(better, torch.full_like(rising, 2.0), rising * 2.0)
         return better, torch.where(better, lowered, damping * rising)
 
-    def retire(self, keep: torch.Tensor) -> None:
-        self._rising = self._rising[keep]
+    def retire(self, kept: torch.Tensor) -> None:
+        """Keep the damping growth of the voxels ``kept`` (indices) only."""
+        self._rising = self._rising.index_select(0, kept)
 
 
 def solve_damped(normal: torch.Tensor, scale: torch.Tensor | None, damping: torch.Tensor | None, columns: int,
@@ -208,11 +209,17 @@ class GaussNewton:
         )
 
     def _retire(self, keep, answer, live, x, data, normal, cost, damping, gradient):
-        done = ~keep
-        if bool(done.any()):
-            answer[live[done]] = x[done]
-        if bool(keep.all()):
+        """Freeze the voxels that leave (their answer is their x) and compact
+        the rest. One wait on the device, for how many stay; everything else
+        is index arithmetic queued behind it."""
+        kept = keep.nonzero().squeeze(1)
+        if kept.numel() == keep.numel():
             return x, live, data, normal, cost, damping, gradient
+        answer.index_copy_(0, live, torch.where(keep[:, None], answer.index_select(0, live), x))
         if hasattr(self.damping, "retire"):
-            self.damping.retire(keep)
-        return x[keep], live[keep], data, normal[keep], cost[keep], damping[keep], gradient[keep]
+            self.damping.retire(kept)
+
+        def pick(t):
+            return t.index_select(0, kept)
+
+        return pick(x), pick(live), data, pick(normal), pick(cost), pick(damping), pick(gradient)
