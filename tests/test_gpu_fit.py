@@ -276,3 +276,35 @@ def test_fused_lm_step_matches_the_operator_formulas():
     assert 0 < int(better.sum()) < voxels
     for got, want in zip(state, expect):
         torch.testing.assert_close(got, want, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("every,iterations", [(2, 15), (3, 15), (4, 15), (7, 15), (4, 3), (2, 5)])
+def test_deferred_compaction_equals_compacting_at_every_retirement(every, iterations):
+    """Voxels that leave are masked and compacted every few iterations; the
+    answers, the steps taken, the voxels left moving and the history equal a
+    loop that compacts at every retirement (COMPACT_EVERY = 1)."""
+    from torchsim_amd.recon import FusedModel, GaussNewton, TrustRegion
+
+    seq = _mrf()
+    rng = np.random.default_rng(9)
+    t1 = rng.uniform(400, 2200, 300).astype(np.float32)
+    t2 = rng.uniform(40, 200, 300).astype(np.float32)
+    m0 = rng.uniform(0.5, 1.0, 300).astype(np.float32)
+    data = seq.simulate(T1=torch.tensor(t1).cuda(), T2=torch.tensor(t2).cuda(), M0=torch.tensor(m0).cuda())
+    bounds = {"T1": (100.0, 5000.0), "T2": (10.0, 2000.0), "M0": (0.01, 2.0)}
+    op = FusedModel(seq, ("T1", "T2", "M0"), bounds=bounds)
+    natural = torch.tensor(np.stack([t1, t2, m0], -1), dtype=torch.float64, device="cuda")
+    # a third start at the answer (flat at once), the rest at varied distances
+    natural[100:] *= torch.linspace(0.7, 1.3, 200, dtype=torch.float64, device="cuda")[:, None]
+    start = op.free(natural)
+    runs = []
+    for compact in (1, every):
+        loop = GaussNewton(TrustRegion(), max_iterations=iterations, gradient_tolerance=1e-3, step_tolerance=1e-7)
+        loop.COMPACT_EVERY = compact
+        runs.append(loop.minimize(op, data, start))
+    a, b = runs
+    assert a.unconverged < 300  # the voxels started at the answer leave at once
+    torch.testing.assert_close(b.x, a.x, rtol=0, atol=0)
+    assert (b.iterations, b.unconverged) == (a.iterations, a.unconverged)
+    torch.testing.assert_close(b.cost, a.cost, rtol=1e-12, atol=0)
+    torch.testing.assert_close(b.damping, a.damping, rtol=1e-12, atol=0)

@@ -142,84 +142,131 @@ class GaussNewton:
         shape = initial.shape[:-1]
         start = initial.reshape(-1, initial.shape[-1]).to(torch.float64)
         data = measured.reshape(start.shape[0], -1)
+        if getattr(self.damping, "per_voxel", False):
+            x, cost, weights, iterations, unconverged = self._retiring(operator, data, start)
+        else:
+            x, cost, weights, iterations, unconverged = self._together(operator, data, start)
+        return Solution(
+            x=x.reshape(*shape, start.shape[-1]),
+            cost=cost,
+            damping=weights,
+            iterations=iterations,
+            unconverged=unconverged,
+        )
+
+    def _step(self, operator, data, live, start, x, normal, cost, damping, columns):
+        """One damped step of every voxel in ``x``: the candidate, its normal
+        equations, the accepted state and the voxels whose step was short."""
+        scale = operator.scale(x)
+        offset = None
+        if getattr(self.damping, "anchored", False):
+            # damping pulls the step toward the start: (H + a I) d = -g + a (x0 - x)
+            offset = damping.reshape(-1, 1) * (start[live] - x)
+        step, predicted, gradient, _ = solve_damped(normal, scale, damping, columns, offset=offset)
+        walked = step.norm(dim=-1)
+        short = walked < self.step_tolerance * (x.norm(dim=-1) + self.step_tolerance)
+        singular = ~torch.isfinite(step).all(-1)
+        candidate = torch.where(singular[:, None], x, x + step)
+        trial = operator.normal(candidate, data, live)
+        trial_cost = trial[:, 0]
+        gain = torch.where(predicted > 0, (cost - trial_cost) / predicted.clamp_min(_TINY),
+                           torch.zeros_like(predicted))
+        better, damping = self.damping.judge(damping, gain, singular)
+        return candidate, trial, trial_cost, better, damping, short & ~singular
+
+    def _together(self, operator, data, start):
+        """Every voxel steps until all are flat or all steps are short."""
         columns = start.shape[-1]
-        retiring = getattr(self.damping, "per_voxel", False)
-        answer = start.clone()
         live = torch.arange(start.shape[0], device=start.device)
         x = start.clone()
         normal = operator.normal(x, data, live)
         cost = normal[:, 0].clone()
         _, _, gradient, curvature = solve_damped(normal, operator.scale(x), None, columns, step=False)
         damping = self.damping.begin(curvature)
-        history = [cost.sum()]
-        weights = []
-        iterations = 0
+        history, weights, iterations = [cost.sum()], [], 0
         for _ in range(self.max_iterations):
-            flat = gradient.abs().amax(-1) < self.gradient_tolerance
-            if retiring:
-                keep = ~flat
-                x, live, data, normal, cost, damping, gradient = self._retire(keep, answer, live, x, data, normal,
-                                                                              cost, damping, gradient)
-                if x.shape[0] == 0:
-                    break
-            elif bool(flat.all()):
+            if bool((gradient.abs().amax(-1) < self.gradient_tolerance).all()):
                 break
             iterations += 1
             weights.append(damping.detach().reshape(-1).mean())
-            scale = operator.scale(x)
-            offset = None
-            if getattr(self.damping, "anchored", False):
-                # damping pulls the step toward the start: (H + a I) d = -g + a (x0 - x)
-                offset = damping.reshape(-1, 1) * (start[live] - x)
-            step, predicted, gradient, _ = solve_damped(normal, scale, damping, columns, offset=offset)
-            walked = step.norm(dim=-1)
-            short = walked < self.step_tolerance * (x.norm(dim=-1) + self.step_tolerance)
-            singular = ~torch.isfinite(step).all(-1)
-            candidate = torch.where(singular[:, None], x, x + step)
-            trial = operator.normal(candidate, data, live)
-            trial_cost = trial[:, 0]
-            gain = torch.where(predicted > 0, (cost - trial_cost) / predicted.clamp_min(_TINY),
-                               torch.zeros_like(predicted))
-            better, damping = self.damping.judge(damping, gain, singular)
+            candidate, trial, trial_cost, better, damping, short = self._step(
+                operator, data, live, start, x, normal, cost, damping, columns)
             x = torch.where(better[:, None], candidate, x)
             normal = torch.where(better[:, None], trial, normal)
             cost = torch.where(better, trial_cost, cost)
             _, _, gradient, _ = solve_damped(normal, operator.scale(x), None, columns, step=False)
             history.append(cost.sum())
-            if retiring:
-                staying = ~(short & ~singular)
-                x, live, data, normal, cost, damping, gradient = self._retire(staying, answer, live, x, data, normal,
-                                                                              cost, damping, gradient)
+            if bool(short.all()):
+                break
+        stacked = torch.stack(weights) if weights else torch.zeros(0, device=start.device)
+        return x, torch.stack(history), stacked, iterations, 0
+
+    def _retiring(self, operator, data, start):
+        """Each voxel leaves once its gradient is flat or its step short
+        (reference recon/_gauss_newton.py:759-793), its answer frozen there.
+
+        Leaving is a mask: a voxel that left keeps its x and is computed with
+        the rest until the arrays are compacted, every COMPACT_EVERY
+        iterations, so the loop waits on the device once per compaction rather
+        than at every retirement. The history and the mean damping count the
+        voxels still moving only, and the steps taken are those that began
+        with a voxel still moving, as in a loop that compacts every time."""
+        columns = start.shape[-1]
+        device = start.device
+        answer = start.clone()
+        live = torch.arange(start.shape[0], device=device)
+        x = start.clone()
+        normal = operator.normal(x, data, live)
+        cost = normal[:, 0].clone()
+        _, _, gradient, curvature = solve_damped(normal, operator.scale(x), None, columns, step=False)
+        damping = self.damping.begin(curvature)
+        moving = torch.ones(start.shape[0], dtype=torch.bool, device=device)
+        history, weights, began = [cost.sum()], [], []
+        for iteration in range(self.max_iterations):
+            moving = moving & ~(gradient.abs().amax(-1) < self.gradient_tolerance)
+            if iteration % self.COMPACT_EVERY == 0:
+                x, live, normal, cost, damping, gradient, moving = self._compact(
+                    answer, live, x, normal, cost, damping, gradient, moving)
                 if x.shape[0] == 0:
                     break
-            elif bool((short & ~singular).all()):
-                break
-        unconverged = int(x.shape[0]) if retiring else 0
-        if retiring:
-            if unconverged:
-                answer[live] = x
-        else:
-            answer = x
-        return Solution(
-            x=answer.reshape(*shape, columns),
-            cost=torch.stack(history),
-            damping=torch.stack(weights) if weights else torch.zeros(0, device=start.device),
-            iterations=iterations,
-            unconverged=unconverged,
-        )
+            count = moving.sum()
+            began.append(count > 0)
+            weights.append(torch.where(moving, damping.reshape(-1), 0.0).sum() / count.clamp_min(1))
+            candidate, trial, trial_cost, better, damping, short = self._step(
+                operator, data, live, start, x, normal, cost, damping, columns)
+            accept = better & moving
+            x = torch.where(accept[:, None], candidate, x)
+            normal = torch.where(accept[:, None], trial, normal)
+            cost = torch.where(accept, trial_cost, cost)
+            _, _, gradient, _ = solve_damped(normal, operator.scale(x), None, columns, step=False)
+            history.append(torch.where(moving, cost, 0.0).sum())
+            moving = moving & ~short
+        x, live, normal, cost, damping, gradient, moving = self._compact(
+            answer, live, x, normal, cost, damping, gradient, moving)
+        unconverged = int(x.shape[0])
+        if unconverged:
+            answer[live] = x
+        ran = torch.stack(began).cpu() if began else torch.zeros(0, dtype=torch.bool)
+        # a step that began with nobody moving was not taken; those come last
+        taken = int(ran.sum())
+        history = torch.stack(history[: 1 + taken])
+        stacked = torch.stack(weights[:taken]) if taken else torch.zeros(0, device=device)
+        return answer, history, stacked, taken, unconverged
 
-    def _retire(self, keep, answer, live, x, data, normal, cost, damping, gradient):
-        """Freeze the voxels that leave (their answer is their x) and compact
-        the rest. One wait on the device, for how many stay; everything else
-        is index arithmetic queued behind it."""
-        kept = keep.nonzero().squeeze(1)
-        if kept.numel() == keep.numel():
-            return x, live, data, normal, cost, damping, gradient
-        answer.index_copy_(0, live, torch.where(keep[:, None], answer.index_select(0, live), x))
+    #: Iterations between compactions of the voxels still moving.
+    COMPACT_EVERY = 4
+
+    def _compact(self, answer, live, x, normal, cost, damping, gradient, moving):
+        """Freeze the voxels that left (their answer is their x) and keep the
+        rest: one wait on the device, for how many stay."""
+        kept = moving.nonzero().squeeze(1)
+        if kept.numel() == moving.numel():
+            return x, live, normal, cost, damping, gradient, moving
+        answer.index_copy_(0, live, torch.where(moving[:, None], answer.index_select(0, live), x))
         if hasattr(self.damping, "retire"):
             self.damping.retire(kept)
 
         def pick(t):
             return t.index_select(0, kept)
 
-        return pick(x), pick(live), data, pick(normal), pick(cost), pick(damping), pick(gradient)
+        return pick(x), pick(live), pick(normal), pick(cost), pick(damping), pick(gradient), pick(moving)
