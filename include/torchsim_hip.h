@@ -167,10 +167,7 @@ int tsim_simulate_vjp_jacobian(void* const* packed, int64_t atom_count, int64_t 
  * (recon/_gauss_newton.py:247-307, 633-651; recon/_operator.py:342-361).
  *
  * columns: HOST array of 1..4 indices among {0 t1, 1 t2, 2 m0, 3 b1}.
- * data:    DEVICE complex64 (rows, output_count), the measured samples.
- * data_rows: DEVICE int64 (atom_count), the data row of each atom (the
- *          voxels a loop still iterates, without gathering their samples),
- *          or NULL: row = atom, rows = atom_count.
+ * data:    DEVICE complex64 (atom_count, output_count), the measured samples.
  * normal:  DEVICE FP64 (atom_count, tsim_normal_stride()): [cost, g0..g3,
  *          H00 H01 H02 H03 H11 H12 H13 H22 H23 H33] (unused columns zero).
  * The model sample is factor * i * m0 * a0 (times sample_sign), as in
@@ -180,8 +177,7 @@ int tsim_gauss_newton_normal(void* const* packed, int64_t atom_count, int64_t ev
                              int64_t state_count, int64_t output_count, int features,
                              int64_t tissue_stride, const int32_t* columns, int64_t column_count,
                              float factor_re, float factor_im, const float* sample_sign,
-                             const void* data, const int64_t* data_rows, double* normal, int tile,
-                             void* stream);
+                             const void* data, double* normal, int tile, void* stream);
 
 /*
  * The same normal equations from a Jacobian that passed through memory (any

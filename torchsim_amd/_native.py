@@ -58,7 +58,7 @@ _F32 = ctypes.c_float
 
 #: Every symbol include/torchsim_hip.h declares, with its ctypes signature.
 #: ABI revision these bindings are written against (``tsim_version``).
-ABI_VERSION = 2
+ABI_VERSION = 1
 
 SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "tsim_last_error": (ctypes.c_char_p, []),
@@ -140,7 +140,7 @@ SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "tsim_gauss_newton_normal": (
         _I32,
         [ctypes.POINTER(_P), _I64, _I64, _I64, _I64, _I32, _I64, ctypes.POINTER(ctypes.c_int32), _I64,
-         _F32, _F32, _P, _P, _P, _P, _I32, _P],
+         _F32, _F32, _P, _P, _P, _I32, _P],
     ),
     "tsim_normal_from_jacobian": (_I32, [_P, _P, _P, _I64, _I64, _I32, _P, _P]),
     "tsim_lm_propose": (_I32, [_P, _P, _P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double),

@@ -803,7 +803,7 @@ extern "C" {
 
 const char* tsim_last_error(void) { return tsim::last_error(); }
 
-int tsim_version(void) { return 2; }
+int tsim_version(void) { return 1; }
 
 int tsim_simulate(void* const* pointers, int64_t pointer_count, int64_t atom_count,
                   int64_t train_count, int64_t event_count, int64_t state_count,
@@ -1016,8 +1016,7 @@ int tsim_gauss_newton_normal(void* const* packed, int64_t atom_count, int64_t ev
                              int64_t state_count, int64_t output_count, int features,
                              int64_t tissue_stride, const int32_t* columns, int64_t column_count,
                              float factor_re, float factor_im, const float* sample_sign,
-                             const void* data, const int64_t* data_rows, double* normal, int tile,
-                             void* stream) {
+                             const void* data, double* normal, int tile, void* stream) {
   if (packed == nullptr) {
     set_error("packed buffers are null");
     return BAD_ARGUMENT;
@@ -1058,7 +1057,6 @@ int tsim_gauss_newton_normal(void* const* packed, int64_t atom_count, int64_t ev
   args.factor_re = factor_re;
   args.factor_im = factor_im;
   args.data = (const float2*)data;
-  args.data_rows = data_rows;
   args.normal = normal;
   if (atom_count == 0 || output_count == 0) return OK;
   if (tile < 0) tile = default_tile(args.states, popcount4(tangent_class(tm)));

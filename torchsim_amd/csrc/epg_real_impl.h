@@ -86,8 +86,7 @@ struct RealArgs {
   int col_t1, col_t2, col_b1, col_ie, col_m0;
   // Normal-equation mode: measured samples and the per-atom accumulators
   // (cost, gradient[4], upper triangle of J^T J [10]) in FP64.
-  const float2* data;  // (rows, outputs) complex64
-  const int64_t* data_rows;  // (atoms): the data row of each atom, or null (row = atom)
+  const float2* data;  // (atoms, outputs) complex64
   double* normal;      // (atoms, NORMAL_STRIDE)
   // Staged output writes: log2 of the pulses buffered per flush and of the
   // padded stream count (signal + columns); stage_k_log2 < 0 writes each
@@ -630,7 +629,7 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
       const int kk = i & (stage_k - 1);
       const int64_t at = g0 + (i >> p.stage_k_log2);
       const int64_t o = nrec + kk;
-      if (at < p.atoms && o < p.outputs) stage[i] = p.data[(p.data_rows ? p.data_rows[at] : at) * p.outputs + o];
+      if (at < p.atoms && o < p.outputs) stage[i] = p.data[at * p.outputs + o];
     }
   };
 
@@ -667,7 +666,7 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
       const float fr = p.factor_re * sign, fi = p.factor_im * sign;
       const float v = m0 * a0;
       const float2 y = staged ? stage[(wave_group << p.stage_k_log2) + (nrec & (stage_k - 1))]
-                              : p.data[(p.data_rows ? p.data_rows[atom] : atom) * p.outputs + o];
+                              : p.data[atom * p.outputs + o];
       const double rx = (double)(-fi * v) - (double)y.x;
       const double ry = (double)(fr * v) - (double)y.y;
       acc_cost += rx * rx + ry * ry;

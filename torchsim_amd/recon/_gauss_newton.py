@@ -122,10 +122,9 @@ class GaussNewton:
     """Solve a per-voxel nonlinear least-squares problem by repeated linearization.
 
     ``operator`` provides ``normal(x, data, live) -> (voxels, stride) FP64`` (cost,
-    J^T r, J^T J in the free variables' natural columns; ``data`` holds every
-    voxel's measured row and ``live`` the voxels still iterating, so retiring
-    voxels never copies the data), ``scale(x)`` (the chain-rule factor
-    d natural / d free) and ``columns``.
+    J^T r, J^T J in the free variables' natural columns; ``data`` holds the
+    measured rows of the voxels ``live`` lists), ``scale(x)`` (the chain-rule
+    factor d natural / d free) and ``columns``.
     """
 
     def __init__(self, damping: Any = None, *, max_iterations: int = 8, gradient_tolerance: float = 1e-8,
@@ -225,8 +224,8 @@ class GaussNewton:
         for iteration in range(self.max_iterations):
             moving = moving & ~(gradient.abs().amax(-1) < self.gradient_tolerance)
             if iteration % self.COMPACT_EVERY == 0:
-                x, live, normal, cost, damping, gradient, moving = self._compact(
-                    answer, live, x, normal, cost, damping, gradient, moving)
+                x, live, data, normal, cost, damping, gradient, moving = self._compact(
+                    answer, live, x, data, normal, cost, damping, gradient, moving)
                 if x.shape[0] == 0:
                     break
             count = moving.sum()
@@ -241,8 +240,8 @@ class GaussNewton:
             _, _, gradient, _ = solve_damped(normal, operator.scale(x), None, columns, step=False)
             history.append(torch.where(moving, cost, 0.0).sum())
             moving = moving & ~short
-        x, live, normal, cost, damping, gradient, moving = self._compact(
-            answer, live, x, normal, cost, damping, gradient, moving)
+        x, live, data, normal, cost, damping, gradient, moving = self._compact(
+            answer, live, x, data, normal, cost, damping, gradient, moving)
         unconverged = int(x.shape[0])
         if unconverged:
             answer[live] = x
@@ -256,12 +255,13 @@ class GaussNewton:
     #: Iterations between compactions of the voxels still moving.
     COMPACT_EVERY = 4
 
-    def _compact(self, answer, live, x, normal, cost, damping, gradient, moving):
+    def _compact(self, answer, live, x, data, normal, cost, damping, gradient, moving):
         """Freeze the voxels that left (their answer is their x) and keep the
-        rest: one wait on the device, for how many stay."""
+        rest, measured rows included: one wait on the device, for how many
+        stay."""
         kept = moving.nonzero().squeeze(1)
         if kept.numel() == moving.numel():
-            return x, live, normal, cost, damping, gradient, moving
+            return x, live, data, normal, cost, damping, gradient, moving
         answer.index_copy_(0, live, torch.where(moving[:, None], answer.index_select(0, live), x))
         if hasattr(self.damping, "retire"):
             self.damping.retire(kept)
@@ -269,4 +269,4 @@ class GaussNewton:
         def pick(t):
             return t.index_select(0, kept)
 
-        return pick(x), pick(live), pick(normal), pick(cost), pick(damping), pick(gradient), pick(moving)
+        return pick(x), pick(live), pick(data), pick(normal), pick(cost), pick(damping), pick(gradient), pick(moving)

@@ -70,10 +70,9 @@ class FusedModel:
         return widen(self.bounds, self.names, x).to(torch.float64)
 
     def normal(self, x: torch.Tensor, data: torch.Tensor, live: torch.Tensor | None) -> torch.Tensor:
-        """(voxels, stride) FP64: cost, J^T r (natural columns), J^T J. ``data``
-        holds the measured row of every voxel of the problem; ``live`` (None:
-        all) says which voxels ``x`` stands for, indexing both the data rows
-        and the per-voxel fixed properties, so nothing is gathered."""
+        """(voxels, stride) FP64: cost, J^T r (natural columns), J^T J, for the
+        voxels ``live`` indexes into per-voxel fixed properties (None: all);
+        ``data`` holds those voxels' measured rows, in the same order."""
         values = self.natural(x)
         return self.normal_at([values[:, i].to(torch.float32) for i in range(self.columns)], data, live)
 
@@ -106,12 +105,7 @@ class FusedModel:
         if tissue.atoms == 0:
             return normal
         measured = data.to(device=device, dtype=torch.complex64).contiguous()
-        rows = None
-        if live is not None:
-            rows = live.to(device=device, dtype=torch.int64).contiguous()
-            if rows.shape != (tissue.atoms,):
-                raise ValueError(f"live indexes {tuple(rows.shape)} voxels, the unknowns {tissue.atoms}")
-        elif measured.shape[0] != tissue.atoms:
+        if measured.shape[0] != tissue.atoms:
             raise ValueError(f"data holds {measured.shape[0]} rows for {tissue.atoms} voxels")
         cols = (ctypes.c_int32 * self.columns)(*[_COLUMN_INDEX[f] for f in self.fields])
         sign = samples
@@ -120,8 +114,8 @@ class FusedModel:
             status = lib.tsim_gauss_newton_normal(
                 _native.pointers([*tissue.buffers, *events]), tissue.atoms, self.packed.event_count, self.states,
                 self.packed.output_count, tissue.features, tissue.stride, cols, self.columns, factor.real, factor.imag,
-                None if sign is None else sign.data_ptr(), measured.data_ptr(),
-                None if rows is None else rows.data_ptr(), normal.data_ptr(), -1, _native.stream_of(device),
+                None if sign is None else sign.data_ptr(), measured.data_ptr(), normal.data_ptr(), -1,
+                _native.stream_of(device),
             )
         _native.check(status)
         return normal
@@ -153,7 +147,7 @@ class FusedModel:
             return normal
         signal = signal.to(torch.complex64).contiguous()
         jacobian = jacobian.to(torch.complex64).contiguous()
-        measured = (data if live is None else data[live]).to(device=device, dtype=torch.complex64).contiguous()
+        measured = data.to(device=device, dtype=torch.complex64).contiguous()
         with torch.cuda.device(device):
             status = lib.tsim_normal_from_jacobian(
                 signal.data_ptr(), jacobian.data_ptr(), measured.data_ptr(), voxels, outputs, self.columns,
