@@ -605,8 +605,8 @@ extern "C" int tsim_simulate_vjp_jacobian(void* const* packed, int64_t atom_coun
   const int NT = popcount4(tm);
   const int64_t nfloat = 3LL * SL * (1 + NT);
   // Few atoms leave most compute units idle with 4-wave blocks (1,024 atoms x
-  // 16 lanes fill 64 of 256): one-wave blocks spread the same waves over
-  // every unit, each with its own caches and memory path. Every index below
+  // 8 lanes fill 32 of 256): one-wave blocks spread the same waves over four
+  // times as many units, each with its own caches and memory path. Every index below
   // derives from the global thread id and the launch total, not the block.
   const int threads = atom_count * L < 256LL * 256 ? 64 : 256;
   const int64_t blocks = (atom_count * L + threads - 1) / threads;
