@@ -55,3 +55,28 @@ def test_bad_arguments_are_statuses_not_crashes():
     assert lib.tsim_tile_count() >= 10
     shapes = _native.tiles()
     assert all(sl * lanes <= 64 for sl, lanes in shapes)
+
+
+def test_projection_arguments_are_checked_on_the_host():
+    """tsim_simulate_projected refuses a rank outside 1..16, a missing basis
+    and diffusion before it touches the device."""
+    from torchsim_amd import _native
+
+    if not _native.LIBRARY.exists():
+        pytest.skip("library not built")
+    lib = _native.load()
+    packed = (ctypes.c_void_p * 26)()
+    cols = (ctypes.c_int32 * 1)(0)
+    for rank in (0, 17):
+        status = lib.tsim_simulate_projected(packed, 4, 1, 10, 1, 0, 1, cols, 1, 0.0, 1.0, None, 1, rank, None,
+                                             None, -1, None)
+        assert status == 2 and b"basis directions" in lib.tsim_last_error()
+    status = lib.tsim_simulate_projected(packed, 4, 1, 10, 1, 0, 1, cols, 1, 0.0, 1.0, None, None, 8, None, None,
+                                         -1, None)
+    assert status == 1 and b"basis buffer is null" in lib.tsim_last_error()
+    status = lib.tsim_simulate_projected(packed, 4, 1, 10, 1, 0, 1, cols, 1, 0.0, 1.0, None, 1, 8, None, None, -1,
+                                         None)
+    assert status == 1 and b"Jacobian buffer is null" in lib.tsim_last_error()
+    status = lib.tsim_simulate_projected(packed, 4, 1, 10, 1, _native.FEATURE_DIFFUSING, 1, cols, 0, 0.0, 1.0, None,
+                                         1, 8, None, None, -1, None)
+    assert status == 2 and b"diffusion" in lib.tsim_last_error()
