@@ -449,6 +449,7 @@ def subspace_workload(steps: int = 5, warmup: int = 2, rank: int = 8) -> dict:
         return held.project(signal), held.project(jacobian)
 
     def timed(step):
+        out = None
         for _ in range(warmup):
             out = step()
         del out
