@@ -111,3 +111,18 @@ def test_subspace_modes_flattening_and_dtypes():
         Subspace.fit(signals, 0)
     with pytest.raises(ValueError, match="exceeds"):
         Subspace.fit(signals[:4, :8], 6)
+
+
+def test_a_tall_dictionary_takes_the_gram_route_and_agrees():
+    """2,000 atoms over 64 contrasts: the basis from the contrasts' Gram
+    matrix (FP64 eigh) against the FP64 SVD oracle: the same singular values
+    and the same projector onto the leading directions."""
+    signals = _echo_train(count=2000)
+    subspace = Subspace.fit(torch.from_numpy(signals.astype(np.complex64)), 6)
+    basis, values = oracle.fit(signals, 6)
+    assert subspace.singular_values.shape == (64,)
+    np.testing.assert_allclose(subspace.singular_values.numpy()[:6], values[:6], rtol=1e-5)
+    assert subspace.retained == pytest.approx(oracle.retained(values, 6), abs=1e-6)
+    projector = subspace.basis.numpy().astype(np.complex128) @ subspace.basis.numpy().conj().T
+    np.testing.assert_allclose(projector, basis @ basis.conj().T, atol=1e-4)
+    np.testing.assert_allclose(subspace.basis.numpy().conj().T @ subspace.basis.numpy(), np.eye(6), atol=1e-5)

@@ -85,7 +85,10 @@ class Subspace:
     def fit(cls, signals: torch.Tensor, rank: int) -> Subspace:
         """The leading ``rank`` left singular vectors of the signals'
         transpose; every leading axis is flattened (reference
-        _subspace.py:97-129).
+        _subspace.py:97-129). With at least 8 atoms per contrast they come
+        from the contrasts' Gram matrix in double precision (the same
+        directions and singular values, up to each direction's phase)
+        rather than from an SVD of the whole dictionary.
 
         Raises
         ------
@@ -98,5 +101,28 @@ class Subspace:
         flat = signals.reshape(-1, signals.shape[-1])
         if rank > min(flat.shape):
             raise ValueError(f"rank={rank} exceeds the signals' dimensions {tuple(flat.shape)}")
-        basis, singular_values, _ = torch.linalg.svd(flat.mT, full_matrices=False)
+        if flat.shape[0] >= _GRAM_RATIO * flat.shape[1]:
+            basis, singular_values = _tall(flat)
+        else:
+            basis, singular_values, _ = torch.linalg.svd(flat.mT, full_matrices=False)
         return cls(basis=basis[:, :rank], singular_values=singular_values)
+
+
+#: Atoms per contrast beyond which the basis comes from the contrasts' Gram
+#: matrix: a dictionary of a million atoms has a few hundred contrasts.
+_GRAM_RATIO = 8
+
+
+def _tall(flat: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    """Left singular vectors and singular values of ``flat.mT`` for many more
+    rows (atoms) than columns (contrasts), from the eigen-decomposition of the
+    (contrasts x contrasts) matrix ``flat.mT @ conj(flat)`` = U S^2 U^H, formed
+    in double precision on the signals' device (one GEMM over the atoms
+    instead of an SVD of the whole dictionary). Directions come out in
+    decreasing order; each direction's phase is the eigensolver's."""
+    wide = torch.complex128 if flat.is_complex() else torch.float64
+    work = flat.to(wide)
+    gram = work.mT @ work.conj()
+    energy, vectors = torch.linalg.eigh(gram)
+    energy, vectors = energy.flip(0), vectors.flip(1)
+    return vectors.to(flat.dtype), energy.clamp_min(0.0).sqrt().to(flat.real.dtype)
