@@ -127,6 +127,8 @@ def test_routes_without_the_fused_kernel_project_the_full_simulation():
     torch.testing.assert_close(coef, want)
     torch.testing.assert_close(jcoef, jwant)
 
+    torch.testing.assert_close(fse.project(subspace, **values), subspace.project(fse.simulate(**values)))
+
     mrf = MRFSimulator(flip=torch.tensor(_cases.mrf_flips()[:120], dtype=torch.float32), TR=10.0, states=10)
     wide = Subspace.fit(mrf.simulate(**values).cpu(), 20)
     torch.testing.assert_close(mrf.project(wide, **values), wide.project(mrf.simulate(**values)))
