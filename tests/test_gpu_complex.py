@@ -187,3 +187,18 @@ def test_staged_and_direct_output_writes_agree_bitwise():
     assert np.array_equal(jac, jac_r[:, :, ::-1])
     ref, _ = _oracle(packed, fields, 20, names)
     assert _cases.rel_err(sig, ref) < SIGNAL_TOL
+
+
+def test_complex_launch_shape_switch_leaves_every_atom_bitwise():
+    """The complex kernel runs one-wave blocks below 1,024 waves: 6,000 and
+    6,145 atoms of 10 lanes (6 atoms per wave: 1,000 and 1,025 waves) sit on
+    either side, and the shared atoms give the same bits."""
+    from torchsim_amd.simulators import TrueFISPSimulator
+
+    t1, t2 = _cases.random_tissue(6145, seed=78)
+    b0 = np.linspace(-20.0, 20.0, 6145).astype(np.float32)
+    sim = TrueFISPSimulator(flip=40.0, TR=5.0, nshots=64, states=10)
+    values = {"T1": torch.tensor(t1).cuda(), "T2": torch.tensor(t2).cuda(), "B0": torch.tensor(b0).cuda()}
+    small = sim.simulate(**{k: v[:6000] for k, v in values.items()})
+    large = sim.simulate(**values)
+    torch.testing.assert_close(small, large[:6000], rtol=0, atol=0)

@@ -303,3 +303,23 @@ def _native_tiles():
     from torchsim_amd import _native
 
     return _native.tiles()
+
+
+@pytest.mark.parametrize("atoms", [16383, 1000])
+def test_launch_shape_switch_leaves_every_atom_bitwise(atoms):
+    """Launches of fewer than 65,536 lanes run one-wave blocks, larger ones
+    four-wave blocks: the same atoms give the same bits either way (tile
+    (5, 4): 16,384 atoms is the switch)."""
+    from torchsim_amd import _native
+    from torchsim_amd.simulators import MRFSimulator
+
+    print(f"case atoms={atoms}", flush=True)
+    flip = torch.tensor(_cases.mrf_flips()[:200], dtype=torch.float32)
+    seq = MRFSimulator(flip=flip, TR=10.0, states=20)
+    t1, t2 = _cases.random_tissue(16385, seed=77)
+    T1, T2 = torch.tensor(t1).cuda(), torch.tensor(t2).cuda()
+    tile = _native.tiles().index((5, 4))
+    small, jsmall = seq.jacobian(("T1", "T2"), T1=T1[:atoms], T2=T2[:atoms], tile=tile)
+    large, jlarge = seq.jacobian(("T1", "T2"), T1=T1, T2=T2, tile=tile)
+    torch.testing.assert_close(small, large[:atoms], rtol=0, atol=0)
+    torch.testing.assert_close(jsmall, jlarge[:atoms], rtol=0, atol=0)
