@@ -563,7 +563,7 @@ def main() -> int:
     if args.gpus != 1:
         print(
             f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
-            "(atoms shard across GPUs with no exchange; see DESIGN.md)",
+            "(atoms shard across GPUs with no exchange: run one process per GPU on its share)",
             file=sys.stderr,
         )
         return 2

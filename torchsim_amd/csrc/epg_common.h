@@ -114,7 +114,8 @@ int hip_status(hipError_t err, const char* where);
 
 // Stream-ordered scratch (hipMallocAsync). The first call on a device raises
 // its default pool's release threshold, so freed scratch stays mapped for the
-// next launch instead of going back to the driver at every synchronization.
+// next launch instead of going back to the HIP runtime's allocator at every
+// synchronization.
 int scratch_alloc(void** ptr, size_t bytes, hipStream_t stream, const char* what);
 
 }  // namespace tsim
