@@ -213,3 +213,21 @@ def test_fits_in_a_fitted_subspace_recover_noiseless_maps():
         torch.testing.assert_close(again[name], maps[name])
     with pytest.raises(ValueError, match="rank 8"):
         fit.from_coefficients(data)
+
+
+def test_a_tall_dictionary_fits_on_the_device():
+    """Subspace.fit of 20,000 MRF atoms (x 300 contrasts) held on the GPU: the
+    Gram route runs there (FP64 GEMM + eigh) and agrees with the FP64 SVD
+    oracle of the same atoms."""
+    from torchsim_amd import Subspace
+    from torchsim_amd.simulators import MRFSimulator
+
+    t1, t2 = _cases.random_tissue(20000, seed=12)
+    sim = MRFSimulator(flip=torch.tensor(_cases.mrf_flips()[:300], dtype=torch.float32), TR=10.0, states=10)
+    atoms = sim.simulate(T1=torch.tensor(t1).cuda(), T2=torch.tensor(t2).cuda())
+    subspace = Subspace.fit(atoms, 6)
+    assert subspace.basis.is_cuda and subspace.basis.dtype == torch.complex64
+    basis, values = osub.fit(atoms.cpu().numpy().astype(np.complex128), 6)
+    np.testing.assert_allclose(subspace.singular_values.cpu().numpy()[:6], values[:6], rtol=1e-5)
+    got = subspace.basis.cpu().numpy().astype(np.complex128)
+    np.testing.assert_allclose(got @ got.conj().T, basis @ basis.conj().T, atol=1e-4)

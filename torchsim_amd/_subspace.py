@@ -118,11 +118,15 @@ def _tall(flat: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
     rows (atoms) than columns (contrasts), from the eigen-decomposition of the
     (contrasts x contrasts) matrix ``flat.mT @ conj(flat)`` = U S^2 U^H, formed
     in double precision on the signals' device (one GEMM over the atoms
-    instead of an SVD of the whole dictionary). Directions come out in
-    decreasing order; each direction's phase is the eigensolver's."""
+    instead of an SVD of the whole dictionary). The small Hermitian
+    eigenproblem is solved on the host: the device eigensolver returned NaN
+    eigenvectors for such a matrix (complex128, 300 x 300) with correct
+    eigenvalues. Directions come out in decreasing order; each direction's
+    phase is the eigensolver's."""
     wide = torch.complex128 if flat.is_complex() else torch.float64
     work = flat.to(wide)
-    gram = work.mT @ work.conj()
+    gram = (work.mT @ work.conj()).cpu()
     energy, vectors = torch.linalg.eigh(gram)
     energy, vectors = energy.flip(0), vectors.flip(1)
-    return vectors.to(flat.dtype), energy.clamp_min(0.0).sqrt().to(flat.real.dtype)
+    return (vectors.to(device=flat.device, dtype=flat.dtype),
+            energy.clamp_min(0.0).sqrt().to(device=flat.device, dtype=flat.real.dtype))
