@@ -363,12 +363,16 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   c[0] = first ? 1.0f : 0.0f;
 
   double acc_cost = 0.0;
-  double acc_g[ACCUM ? NORMAL_COLUMNS : 1];
-  double acc_h[ACCUM ? NORMAL_COLUMNS * (NORMAL_COLUMNS + 1) / 2 : 1];
+  // A fit names at most one column per carried tangent plus M0, so a kernel
+  // with NT tangents accumulates 1 + NT columns (the output keeps the
+  // NORMAL_COLUMNS layout, zero beyond).
+  constexpr int NCOL = ACCUM ? (1 + NT < NORMAL_COLUMNS ? 1 + NT : NORMAL_COLUMNS) : 1;
+  double acc_g[NCOL];
+  double acc_h[NCOL * (NCOL + 1) / 2];
 #pragma unroll
-  for (int i = 0; i < (ACCUM ? NORMAL_COLUMNS : 1); ++i) acc_g[i] = 0.0;
+  for (int i = 0; i < NCOL; ++i) acc_g[i] = 0.0;
 #pragma unroll
-  for (int i = 0; i < (ACCUM ? NORMAL_COLUMNS * (NORMAL_COLUMNS + 1) / 2 : 1); ++i) acc_h[i] = 0.0;
+  for (int i = 0; i < NCOL * (NCOL + 1) / 2; ++i) acc_h[i] = 0.0;
 
   // Projection accumulators: stream s (0 the unscaled sample a0, 1 + k the
   // carried tangent k) against direction sub + L j, summed over the samples
@@ -673,9 +677,9 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
       // J_c = (-fi d_c, fr d_c): J_c . r = d_c q, J_c . J_k = |factor|^2 d_c d_k
       const double q = -(double)fi * rx + (double)fr * ry;
       const double ff = (double)fr * fr + (double)fi * fi;
-      double d[NORMAL_COLUMNS];
+      double d[NCOL];
 #pragma unroll
-      for (int cc = 0; cc < NORMAL_COLUMNS; ++cc) {
+      for (int cc = 0; cc < NCOL; ++cc) {
         float x = 0.0f;
         if constexpr ((TM & TAN_T1) != 0) x = (p.col_t1 == cc) ? m0 * t0[ST1] : x;
         if constexpr ((TM & TAN_T2) != 0) x = (p.col_t2 == cc) ? m0 * t0[ST2] : x;
@@ -685,10 +689,10 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
       }
       int h = 0;
 #pragma unroll
-      for (int cc = 0; cc < NORMAL_COLUMNS; ++cc) {
+      for (int cc = 0; cc < NCOL; ++cc) {
         acc_g[cc] += d[cc] * q;
 #pragma unroll
-        for (int kk = cc; kk < NORMAL_COLUMNS; ++kk) acc_h[h++] += ff * d[cc] * d[kk];
+        for (int kk = cc; kk < NCOL; ++kk) acc_h[h++] += ff * d[cc] * d[kk];
       }
     } else if (staged) {
       // this group's row of the wave's stage: stream s at slot s * K
@@ -896,9 +900,17 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
       double* out = p.normal + atom * NORMAL_STRIDE;
       out[0] = acc_cost;
 #pragma unroll
-      for (int cc = 0; cc < NORMAL_COLUMNS; ++cc) out[1 + cc] = acc_g[cc];
+      for (int cc = 0; cc < NORMAL_COLUMNS; ++cc) out[1 + cc] = cc < NCOL ? acc_g[cc] : 0.0;
+      int hn = 0, h4 = 0;
 #pragma unroll
-      for (int h = 0; h < NORMAL_COLUMNS * (NORMAL_COLUMNS + 1) / 2; ++h) out[1 + NORMAL_COLUMNS + h] = acc_h[h];
+      for (int cc = 0; cc < NORMAL_COLUMNS; ++cc) {
+#pragma unroll
+        for (int kk = cc; kk < NORMAL_COLUMNS; ++kk, ++h4) {
+          const bool held = cc < NCOL && kk < NCOL;
+          out[1 + NORMAL_COLUMNS + h4] = held ? acc_h[held ? hn : 0] : 0.0;
+          hn += held ? 1 : 0;
+        }
+      }
     }
   }
 }
