@@ -176,23 +176,18 @@ _TISSUE_ORDER = (
 _TISSUE_DEFAULT = {"m0": 1.0, "b1": 1.0, "ie": 1.0, "t1_bound": 1.0, "t1_pool_b": 1.0, "t2_pool_b": 1.0}
 
 
-def reference_real(
+def _reference_call(
     events: dict[str, Any],
     tissue: dict[str, Any],
     *,
     states: int,
     outputs: int,
     features: int,
-    tangent: dict[str, Any] | None = None,
-    threads: int = 1,
-    real_axis: int = 1,
+    real_axis: int,
+    pool_kind: int,
+    tangent: dict[str, Any] | None,
+    threads: int,
 ) -> np.ndarray:
-    """Run the reference's own FP32 CPU kernel (built from source) on packed buffers.
-
-    Returns the imaginary output plane (atoms, outputs): the recorded
-    ``m0 * a0`` for the forward, or its tangent along ``tangent`` (per-atom
-    rows keyed like ``tissue``) through ``simulate_jvp``.
-    """
     module = reference_module()
     if module is None:
         raise RuntimeError("oracle/_ref is not built")
@@ -226,13 +221,69 @@ def reference_real(
     real = np.zeros((atoms, outputs), dtype=np.float32)
     imag = np.zeros((atoms, outputs), dtype=np.float32)
     tail = [0] * 13
-    dims = (atoms, 1, count, states, outputs, threads, real_axis, 0.0, 0.0, 1, 1, 0, 1.0, 0, 1.0, 0, features, 0, 0, 0)
+    dims = (atoms, 1, count, states, outputs, threads, real_axis, 0.0, 0.0, 1, 1, 0, 1.0, 0, 1.0, pool_kind,
+            features, 0, 0, 0)
     if tangent is None:
         module.simulate((*tissue_ptrs, *event_ptrs, real.ctypes.data, imag.ctypes.data, *tail), *dims)
     else:
         tangent_ptrs = [f32(tangent.get(name, 0.0), atoms) for name in _TISSUE_ORDER]
-        tangent_ptrs += [f32(0.0, count), f32(0.0, count), f32(0.0, count)]
+        # event tangents in the reference's forward order: duration, flip, phase
+        tangent_ptrs += [f32(tangent.get(name, 0.0), count) for name in ("duration", "flip", "phase")]
         module.simulate_jvp(
             (*tissue_ptrs, *event_ptrs, *tangent_ptrs, real.ctypes.data, imag.ctypes.data, *tail), *dims
         )
-    return imag.astype(np.float64)
+    return real.astype(np.float64) + 1j * imag.astype(np.float64)
+
+
+#: Feature bits of the reference's packed ABI (src/torchsim/_epg_cpu.cpp:90-104).
+FEATURE_ALL = 63
+
+
+def reference_real(
+    events: dict[str, Any],
+    tissue: dict[str, Any],
+    *,
+    states: int,
+    outputs: int,
+    features: int,
+    tangent: dict[str, Any] | None = None,
+    threads: int = 1,
+    real_axis: int = 1,
+) -> np.ndarray:
+    """Run the reference's own FP32 CPU kernel (built from source) on packed buffers.
+
+    Returns the imaginary output plane (atoms, outputs): the recorded
+    ``m0 * a0`` for the forward, or its tangent along ``tangent`` (per-atom
+    rows keyed like ``tissue``) through ``simulate_jvp``.
+    """
+    out = _reference_call(events, tissue, states=states, outputs=outputs, features=features,
+                          real_axis=real_axis, pool_kind=0, tangent=tangent, threads=threads)
+    return out.imag.copy()
+
+
+def reference_complex(
+    events: dict[str, Any],
+    tissue: dict[str, Any],
+    *,
+    states: int,
+    outputs: int,
+    features: int = FEATURE_ALL,
+    exchanging: bool = False,
+    tangent: dict[str, Any] | None = None,
+    threads: int = 1,
+) -> np.ndarray:
+    """The reference's own FP32 complex kernel: recorded samples (atoms, outputs), complex.
+
+    Runs ``simulate`` (or ``simulate_jvp`` along ``tangent``) with
+    ``real_axis = -1``, i.e. the general complex state machine
+    (src/torchsim/_epg_cpu.cpp:4216-4566, dispatch :10789-10900), with pool
+    kind 2 -- the chemically exchanging second pool
+    (:885-932, :1714-1764) -- when ``exchanging``. ``tissue`` is keyed by the
+    reference's 17 ABI names (``t1, t2, m0, b1, b1_phase, b0, ie, diffusion,
+    velocity, bound_fraction, bound_exchange, t1_bound, pool_b_fraction,
+    pool_b_exchange, t1_pool_b, t2_pool_b, pool_b_shift``); ``diffusion`` is
+    the rate already scaled to 1/s per unit order squared. ``tangent`` takes
+    the same names plus per-event ``duration``, ``flip`` and ``phase``.
+    """
+    return _reference_call(events, tissue, states=states, outputs=outputs, features=features,
+                           real_axis=-1, pool_kind=2 if exchanging else 0, tangent=tangent, threads=threads)

@@ -10,10 +10,16 @@ solve -- a different algorithm from the closed form the kernels evaluate),
 per-order diffusion damping, off-resonance, and RF phase. Shifts truncate
 exactly as the kernels do (top order dropped, F-[S-1] = 0).
 
-Parity: pinned -- against the C oracle on real-subspace inputs (which the
-benchmark checksums pin), against the isochromat brute force with phases and
-off-resonance, against scipy.linalg.expm for the exchange operators, and by
-central differences (tests/test_oracle_packed.py).
+Parity: pinned (tests/test_oracle_packed.py) -- against the C oracle on
+real-subspace inputs (which the benchmark checksums pin); against the
+isochromat brute force with phases and off-resonance; against the reference's
+own FP32 complex kernel built from source (``oracle.reference_complex``) for
+the exchanging pool with relaxation, chemical shift, B0, B1 and phases, and
+for diffusion on the real and complex paths, signal and every tissue column;
+by the exchange closed forms (equilibrium populations are stationary; fast
+exchange coalesces at the population-weighted frequency); and by central
+differences. A planted swap of the exchange rates, or the transverse
+diffusion weight planted on Z, fails those tests.
 """
 
 from __future__ import annotations
@@ -45,6 +51,7 @@ def simulate(
     *,
     states: int,
     outputs: int,
+    _plant: str | None = None,
 ) -> torch.Tensor:
     """Recorded complex samples (atoms, outputs), complex128.
 
@@ -54,6 +61,11 @@ def simulate(
     already scaled to 1/s per unit order^2), and for an exchanging pool
     fraction, exchange (1/s), t1b, t2b (ms), shift (Hz). Tissue entries may be
     tensors requiring grad; flip and duration may be too.
+
+    ``_plant`` exists only so the tests can show that their checks fail on a
+    plausible mistake: ``"swap_exchange"`` exchanges the two rate constants
+    (k_ab = k (1 - f), k_ba = k f), ``"z_transverse_weight"`` damps the
+    longitudinal states with the transverse weight l^2 + l + 1/3.
     """
     def tt(name, default=None):
         value = tissue.get(name, default)
@@ -80,8 +92,8 @@ def simulate(
     out_index = [int(o) for o in torch.as_tensor(events["output_index"]).tolist()]
 
     order = torch.arange(states, dtype=_R)
-    wl = order**2
     wt = order**2 + order + 1.0 / 3.0
+    wl = wt if _plant == "z_transverse_weight" else order**2
     shape = (atoms, states)
     pools = 2 if exchanging else 1
     fp = [torch.zeros(shape, dtype=_C) for _ in range(pools)]
@@ -94,7 +106,7 @@ def simulate(
         r2b = 1000.0 / (tt("t2b") * ones)
         dfb = tt("shift", 0.0) * ones
         free = 1.0 - f
-        kab, kba = k * f, k * free
+        kab, kba = (k * free, k * f) if _plant == "swap_exchange" else (k * f, k * free)
         r1a, r2a = 1000.0 / t1, 1000.0 / t2
         gl = torch.stack((torch.stack((-kab - r1a, kba), -1), torch.stack((kab, -kba - r1b), -1)), -2)
         gt = torch.stack(
