@@ -91,10 +91,11 @@ def _reference_fp32_error(sim, t1, t2, ref):
 
 @pytest.mark.parametrize("inversion", [None, "ideal"])
 def test_truefisp_balanced_train(inversion):
-    """A balanced train carries no shift, so FP32 rounding accumulates over
-    every pulse; the reference's own FP32 kernel sits near 1e-5 here too.
-    The bound is 1e-5 or 1.5x the reference kernel's error on the same input,
-    whichever is larger (2e-5 where the reference kernel is not built)."""
+    """A balanced train carries no shift, so a systematic error in the
+    per-pulse coefficients is summed over the train's whole memory (T1/TR
+    pulses). The kernels relax Z and apply the flip's versine as increments,
+    which keeps the signal within the north-star 1e-5 of FP64; the
+    reference's own FP32 kernel, where built, is reported beside it."""
     from torchsim_amd.simulators import TrueFISPSimulator
 
     t1, t2 = _cases.random_tissue(30, seed=32)
@@ -102,9 +103,10 @@ def test_truefisp_balanced_train(inversion):
     sig, jac = sim.jacobian(("T1", "T2"), T1=torch.tensor(t1).cuda(), T2=torch.tensor(t2).cuda())
     events = _exact_half_turns(oracle.events_of(sim.packed(**sim.bound)))
     ref, rjac = P.jacobian(events, {"t1": t1, "t2": t2}, ("t1", "t2"), states=1, outputs=300)
+    ours = _cases.rel_err(sig.cpu().numpy(), ref.numpy())
     theirs = _reference_fp32_error(sim, t1, t2, ref.numpy())
-    bound = 2e-5 if theirs is None else max(1e-5, 1.5 * theirs)
-    assert _cases.rel_err(sig.cpu().numpy(), ref.numpy()) < bound
+    print(f"balanced train ({inversion}): ours {ours:.2e}, reference FP32 kernel {theirs}")
+    assert ours < 1e-5
     for p in range(2):
         assert _cases.rel_err(jac.cpu().numpy()[:, p], rjac[:, p].numpy()) < 1e-4
 

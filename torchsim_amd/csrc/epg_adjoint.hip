@@ -190,11 +190,11 @@ __device__ __forceinline__ void shift_adjoint(Dual<SL, TM>& l, bool first, bool 
 
 struct Rotation {
   float ch, sh, sn, cs, hs;
-  __device__ __forceinline__ void set(float cosine, float sine) {
-    cs = cosine;
+  __device__ __forceinline__ void set(float versine, float sine) {
+    cs = 1.0f - versine;
     sn = sine;
-    ch = 0.5f * (1.0f + cs);
-    sh = 0.5f * (1.0f - cs);
+    sh = 0.5f * versine;
+    ch = 1.0f - sh;
     hs = 0.5f * sn;
   }
 };
@@ -252,11 +252,10 @@ __device__ __forceinline__ void event_forward(Dual<SL, TM>& x, const EventCode& 
     } else {
       Rotation t;
       if (transmit) {
-        float s, c;
-        sincosf(ev.flip * b1, &s, &c);
-        t.set(c, s);
+        const float s = sinf(ev.flip * b1), h = sinf(0.5f * (ev.flip * b1));
+        t.set(2.0f * (h * h), s);
       } else {
-        t.set(ev.cosine, ev.sine);
+        t.set(ev.versine, ev.sine);
       }
       rotate3<SL>(x.a, x.b, x.c, t);
 #pragma unroll
@@ -382,11 +381,10 @@ __global__ void __launch_bounds__(256) adjoint_kernel(const AdjointArgs p) {
         } else {
           Rotation t;
           if (transmit) {
-            float s, c;
-            sincosf(ev.flip * b1, &s, &c);
-            t.set(c, s);
+            const float s = sinf(ev.flip * b1), h = sinf(0.5f * (ev.flip * b1));
+            t.set(2.0f * (h * h), s);
           } else {
-            t.set(ev.cosine, ev.sine);
+            t.set(ev.versine, ev.sine);
           }
           float g = rotate3_adjoint<SL>(lam.a, lam.b, lam.c, x2.a, x2.b, x2.c, t);
 #pragma unroll

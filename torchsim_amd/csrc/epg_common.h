@@ -56,7 +56,7 @@ constexpr uint32_t CTRL_KIND_SHIFT = 8;
 struct alignas(32) EventCode {
   float dt;        // interval before the event, seconds
   float flip;      // radians
-  float cosine;    // cos(flip), for a launch whose flip carries no B1 map
+  float versine;   // 1 - cos(flip) = 2 sin^2(flip/2), for a launch whose flip carries no B1 map
   float sine;      // sin(flip)
   float phase;     // radians
   uint32_t ctrl;   // action | kind << 8
@@ -73,7 +73,7 @@ __device__ __forceinline__ EventCode load_event(const EventCode* codes, int64_t 
   EventCode ev;
   ev.dt = __uint_as_float(w[0]);
   ev.flip = __uint_as_float(w[1]);
-  ev.cosine = __uint_as_float(w[2]);
+  ev.versine = __uint_as_float(w[2]);
   ev.sine = __uint_as_float(w[3]);
   ev.phase = __uint_as_float(w[4]);
   ev.ctrl = w[5];

@@ -485,18 +485,21 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
         }
       } else {
         const float alpha = ev.flip * b1;
-        float sa, ca;
+        float sa, va;  // sin(alpha), 1 - cos(alpha)
         if (p.transmit) {
-          sincosf(alpha, &sa, &ca);
+          const float h = sinf(0.5f * alpha);
+          sa = sinf(alpha);
+          va = 2.0f * (h * h);
         } else {
           sa = ev.sine;
-          ca = ev.cosine;
+          va = ev.versine;
         }
+        const float ca = 1.0f - va;
         float sp, cp;
         sincosf(ev.phase + b1_phase, &sp, &cp);
         const float2 e1p = mk(cp, sp);            // e^{i phi}
         const float2 e2p = mul(e1p, e1p);         // e^{2 i phi}
-        const float chh = 0.5f * (1.0f + ca), shh = 0.5f * (1.0f - ca);
+        const float shh = 0.5f * va, chh = 1.0f - shh;
         // T-matrix (reference rotate, _epg_cpu.cpp:2176-2208)
         const float2 t01 = scl(e2p, shh);
         const float2 t02 = mul(mk(0.0f, -sa), e1p);

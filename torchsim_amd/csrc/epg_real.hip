@@ -68,10 +68,12 @@ __global__ void compile_events(const float* __restrict__ duration,
   EventCode code;
   code.dt = duration[e];
   code.flip = flip[e];
-  float s, c;
-  sincosf(code.flip, &s, &c);
-  code.cosine = c;
-  code.sine = s;
+  // 1 - cos(flip) from the half angle: formed as 1 - cos it would carry
+  // cos's rounding (about 6e-8) into a quantity that is small for small
+  // flips, and a train repeats that error on every pulse.
+  const float h = sinf(0.5f * code.flip);
+  code.versine = 2.0f * (h * h);
+  code.sine = sinf(code.flip);
   code.phase = phase ? phase[e] : 0.0f;
   code.ctrl = (uint32_t)action[e] | ((uint32_t)kind[e] << CTRL_KIND_SHIFT);
   code.out = output_index[e];
@@ -220,7 +222,7 @@ __global__ void __launch_bounds__(kPackThreads) pack_ops(const EventCode* __rest
     MacroOp z;
     z.dt = 0.0f;
     z.flip = 0.0f;
-    z.cosine = 1.0f;
+    z.versine = 0.0f;
     z.sine = 0.0f;
     z.ctrl = i >= count ? OP_END : 0u;
     z.out = -1;
@@ -244,7 +246,7 @@ __global__ void __launch_bounds__(kPackThreads) pack_ops(const EventCode* __rest
         op->dt = ev.dt;
       } else if (part == OP_ROTATE) {
         op->flip = ev.flip;
-        op->cosine = ev.cosine;
+        op->versine = ev.versine;
         op->sine = ev.sine;
         op->event = ev.event;
       } else if (part == OP_RECORD) {

@@ -26,7 +26,7 @@ constexpr uint32_t OP_END = 64;
 struct alignas(32) MacroOp {
   float dt;      // relaxation interval, seconds (OP_RELAX)
   float flip;    // radians (OP_ROTATE)
-  float cosine;  // cos(flip)
+  float versine; // 1 - cos(flip)
   float sine;    // sin(flip)
   uint32_t ctrl;
   int32_t out;   // output index (OP_RECORD)
@@ -47,7 +47,7 @@ __device__ __forceinline__ MacroOp load_op(const MacroOp* ops, int64_t i) {
   MacroOp m;
   m.dt = __uint_as_float(w[0]);
   m.flip = __uint_as_float(w[1]);
-  m.cosine = __uint_as_float(w[2]);
+  m.versine = __uint_as_float(w[2]);
   m.sine = __uint_as_float(w[3]);
   m.ctrl = w[4];
   m.out = (int32_t)w[5];
@@ -405,12 +405,12 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   // recovery 1 - e1 on the first lane. The recovery comes from expm1: 1 - e1
   // formed from a rounded e1 loses about five digits when dt << T1, and that
   // error recurs identically on every pulse of a steady train.
-  float k2dt = 0.0f, rec = 0.0f;
+  float k2dt = 0.0f, rec = 0.0f, e1m = 0.0f;
   auto decay = [&](float dt) __attribute__((always_inline)) {
     // dt is wave-uniform: compare bit patterns on the scalar unit
     if (__float_as_uint(dt) == last_dt) return;
     last_dt = __float_as_uint(dt);
-    const float e1m = expm1f(-r1 * dt);
+    e1m = expm1f(-r1 * dt);
     e1 = 1.0f + e1m;
     e2 = expf(-r2 * dt);
     k2dt = k2 * dt;
@@ -471,12 +471,18 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
     if constexpr ((TM & TAN_T1) != 0) tc[ST1][0] -= first ? de1 : 0.0f;
   };
 
-  // cos and sin of the op's flip as this atom plays it.
-  auto angle = [&](const MacroOp& ev, float& cs, float& sn) __attribute__((always_inline)) {
+  // 1 - cos and sin of the op's flip as this atom plays it; with a B1 map
+  // both from the half angle, so the versine keeps its relative accuracy
+  // when the flip is small.
+  auto angle = [&](const MacroOp& ev, float& vs, float& sn) __attribute__((always_inline)) {
     if constexpr (B1MAP) {
-      sincos_reduced(ev.flip * b1, sn, cs);
+      float sh, ch;
+      sincos_reduced(0.5f * (ev.flip * b1), sh, ch);
+      const float s2 = sh + sh;
+      vs = s2 * sh;
+      sn = s2 * ch;
     } else {
-      cs = ev.cosine;
+      vs = ev.versine;
       sn = ev.sine;
     }
   };
@@ -484,14 +490,20 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   // Everything an op does to the planes before its record: a relaxation over
   // dt (kernels without diffusion), then a rotation about x (reference
   // rf_real, _epg_cpu.cpp:2980-3005), an inversion Z -> -eta Z, or a spoil
-  // F+ = F- = 0. Z relaxes first, in place (Z[0] = e1 Z[0] + (1 - e1) with a
-  // single rounding). The transverse decay folds into the rotation, written
-  // with d = p - m as
-  //   u = A d + B z,  p' = E p - u,  m' = E m + u,  z' = C d + D z,
+  // F+ = F- = 0. Z relaxes first, in place, as an increment:
+  // Z' = Z + (e1 - 1)(Z - Z_eq) with e1 - 1 from expm1 (Z_eq = 1 at order
+  // 0 of the first lane, else 0). A rounded e1 is off by up to 6e-8, which
+  // against 1 - e1 ~ TR/T1 ~ 1e-3 is a 1e-4 relative error in the recovery
+  // per pulse, and a train with a long memory (a balanced one) sums it over
+  // hundreds of pulses; the increment keeps e1 - 1 to its own rounding.
+  // The transverse decay folds into the rotation, written with d = p - m as
+  //   u = A d + B z,  p' = E p - u,  m' = E m + u,  z' = C d + z - W z,
   // where a rotation by a has A = e2 (1 - cos)/2, B = sin, C = e2 sin/2,
-  // D = cos, E = e2; an inversion A = B = C = 0, D = -eta; a spoil E = A =
-  // B = C = 0, D = 1; a bare relaxation A = B = C = 0, D = 1 (so p' = e2 p,
-  // exactly rounded). The T2 derivative of the transverse decay is applied
+  // W = 1 - cos (the versine, from the half angle, for the same reason),
+  // E = e2; an inversion A = B = C = 0, W = 1 + eta; a spoil E = A = B = C =
+  // W = 0; a bare relaxation A = B = C = W = 0 (so p' = e2 p, exactly
+  // rounded). The tangent sets take the same coefficients with D = 1 - W.
+  // The T2 derivative of the transverse decay is applied
   // before the scale: a k2 dt and b k2 dt on the T2 tangents of F+ and F-
   // (E multiplies them by e2). Along B1 the rotation angle moves by the flip
   // per unit B1, which adds bA d + bB z to u and bC d + bD z to z' of the B1
@@ -499,23 +511,24 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   // register is read before it is written, so every plane is updated where
   // it lives. With small flips this form keeps p' = p - u close to p, where
   // a sum/difference form would round p and m together.
-  auto apply = [&](auto rc, const MacroOp& m, uint32_t ctl, float cs, float sn) __attribute__((always_inline)) {
+  auto apply = [&](auto rc, const MacroOp& m, uint32_t ctl, float vs, float sn) __attribute__((always_inline)) {
     constexpr int R = decltype(rc)::value;
     const bool relaxing = !DIFF && (ctl & OP_RELAX);
     if (relaxing) decay(m.dt);
-    const float f1 = relaxing ? e1 : 1.0f;
     const float f2 = relaxing ? e2 : 1.0f;
     const float g1 = relaxing ? de1 : 0.0f;
     const float q2 = relaxing ? k2dt : 0.0f;
-    const float r0 = relaxing ? rec : 0.0f;
+    const float h1 = relaxing ? e1m : 0.0f;
     const float dal = (ctl & OP_ROTATE) ? m.flip : 0.0f;
     const bool spoil = (ctl & OP_SPOIL) != 0;
     const bool invert = (ctl & OP_INVERT) != 0;
+    const float cs = 1.0f - vs;
     const float E = spoil ? 0.0f : f2;
-    const float A = f2 * (0.5f * (1.0f - cs));
+    const float A = f2 * (0.5f * vs);
     const float B = sn;
     const float C = f2 * (0.5f * sn);
     const float D = invert ? -eta : cs;
+    const float W = invert ? 1.0f + eta : vs;  // 1 - D
     const float bA = f2 * (0.5f * sn * dal), bB = cs * dal, bC = f2 * (0.5f * cs * dal), bD = -sn * dal;
 
     // longitudinal relaxation and recovery, and the T1/T2 decay derivatives
@@ -529,12 +542,12 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
 #pragma unroll
       for (int k = 0; k < NT; ++k) {
         if ((TM & TAN_T1) != 0 && k == ST1) {
-          tc[k][j] = fmaf(c[j], g1, tc[k][j] * f1);
+          tc[k][j] = fmaf(c[j], g1, fmaf(tc[k][j], h1, tc[k][j]));
         } else {
-          tc[k][j] *= f1;
+          tc[k][j] = fmaf(tc[k][j], h1, tc[k][j]);
         }
       }
-      c[j] = j == 0 ? fmaf(c[j], f1, r0) : c[j] * f1;
+      c[j] = fmaf(j == 0 && first ? c[j] - 1.0f : c[j], h1, c[j]);
     }
     if constexpr ((TM & TAN_T1) != 0) tc[ST1][0] -= first ? g1 : 0.0f;
 
@@ -567,7 +580,7 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
       }
       const float d = pv - mv;
       const float u = fmaf(A, d, B * zv);
-      c[j] = fmaf(C, d, D * zv);
+      c[j] = fmaf(C, d, fmaf(-W, zv, zv));
       b[ib] = scale_add_in_place(mv, E, u);
       a[ia] = scale_sub_in_place(pv, E, u);
     }
@@ -800,13 +813,13 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   // their latency hides behind it.
   MacroOp next = load_op(ops, 0);
   int64_t e = 1;
-  float next_cs, next_sn;
-  angle(next, next_cs, next_sn);
+  float next_vs, next_sn;
+  angle(next, next_vs, next_sn);
   auto segment = [&](auto rc) __attribute__((always_inline)) -> bool {
     bool ended = false;
     for (;;) {
       const MacroOp m = next;
-      const float cs = next_cs, sn = next_sn;
+      const float vs = next_vs, sn = next_sn;
       const uint32_t ctl = m.ctrl;
       if (ctl & OP_END) {
         ended = true;
@@ -830,9 +843,9 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
       }
       // every op runs the plane transform (the identity where it has nothing
       // to apply), and the next op's angle in the same straight-line block;
-      // an op that does not rotate has flip 0, cos 1, sin 0
-      apply(rc, m, ctl, cs, sn);
-      angle(next, next_cs, next_sn);
+      // an op that does not rotate has flip 0, versine 0, sin 0
+      apply(rc, m, ctl, vs, sn);
+      angle(next, next_vs, next_sn);
       if (ctl & OP_RECORD) {
         if (m.out >= 0 && m.out < p.outputs) {
           if constexpr (ACCUM) {
