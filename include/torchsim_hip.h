@@ -109,6 +109,27 @@ int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t even
                            void* signal, void* jacobian, int tile, void* stream);
 
 /*
+ * tsim_simulate_jacobian for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). No counterpart in the reference: it exists for those callers.
+ * Same arguments, checks and results (bitwise) as tsim_simulate_jacobian,
+ * but the compiled event stream lives in a fixed device workspace that the
+ * library holds on every GPU it runs on until the process ends (about
+ * 3.1 MB), so the call allocates and frees nothing. Limit: event_count <=
+ * tsim_noalloc_max_events() (16384), else status 1. Every call on one GPU
+ * shares that GPU's workspace: calls that overlap on one GPU (two streams,
+ * or two host threads) race and must be serialized by the caller.
+ */
+int tsim_simulate_jacobian_noalloc(void* const* packed, int64_t atom_count, int64_t event_count,
+                                   int64_t state_count, int64_t output_count, int features,
+                                   int64_t tissue_stride, const int32_t* columns, int64_t column_count,
+                                   float factor_re, float factor_im, const float* sample_sign,
+                                   void* signal, void* jacobian, int tile, void* stream);
+
+/* Largest event_count tsim_simulate_jacobian_noalloc accepts. */
+int64_t tsim_noalloc_max_events(void);
+
+/*
  * tsim_simulate_jacobian with the samples projected onto a temporal basis
  * inside the kernel: what the reference computes as
  * `Subspace.project(signal)` and `Subspace.project(derivative)` after a full

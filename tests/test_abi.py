@@ -80,3 +80,23 @@ def test_projection_arguments_are_checked_on_the_host():
     status = lib.tsim_simulate_projected(packed, 4, 1, 10, 1, _native.FEATURE_DIFFUSING, 1, cols, 0, 0.0, 1.0, None,
                                          1, 8, None, None, -1, None)
     assert status == 2 and b"diffusion" in lib.tsim_last_error()
+
+
+def test_noalloc_entry_point_rejects_streams_beyond_its_workspace():
+    """The allocation-free entry point holds at most tsim_noalloc_max_events()
+    events; a longer stream is a status 1 with a message, checked on the host
+    before anything reaches a device (the pointers here are never read)."""
+    from torchsim_amd import _native
+
+    if not _native.LIBRARY.exists():
+        pytest.skip("library not built")
+    lib = _native.load()
+    limit = lib.tsim_noalloc_max_events()
+    assert limit == 16384
+    packed = (ctypes.c_void_p * 26)()
+    packed[0] = packed[1] = 16  # t1, t2: never dereferenced on this path
+    cols = (ctypes.c_int32 * 1)(0)
+    status = lib.tsim_simulate_jacobian_noalloc(packed, 8, limit + 1, 10, 4, 0, 1, cols, 1, 0.0, 1.0, None,
+                                                16, 16, -1, None)
+    assert status == 1
+    assert b"at most 16384 events" in lib.tsim_last_error()

@@ -92,6 +92,12 @@ SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
             _P,
         ],
     ),
+    "tsim_simulate_jacobian_noalloc": (
+        _I32,
+        [ctypes.POINTER(_P), _I64, _I64, _I64, _I64, _I32, _I64, ctypes.POINTER(ctypes.c_int32), _I64, _F32, _F32,
+         _P, _P, _P, _I32, _P],
+    ),
+    "tsim_noalloc_max_events": (_I64, []),
     "tsim_simulate_projected": (
         _I32,
         [ctypes.POINTER(_P), _I64, _I64, _I64, _I64, _I32, _I64, ctypes.POINTER(ctypes.c_int32), _I64, _F32, _F32,

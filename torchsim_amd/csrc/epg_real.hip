@@ -844,11 +844,16 @@ int tsim_simulate(void* const* pointers, int64_t pointer_count, int64_t atom_cou
   return run_real(args, 0, features, -1, s);
 }
 
-int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
-                           int64_t state_count, int64_t output_count, int features,
-                           int64_t tissue_stride, const int32_t* columns, int64_t column_count,
-                           float factor_re, float factor_im, const float* sample_sign,
-                           void* signal, void* jacobian, int tile, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// Arguments of a fused signal + Jacobian launch, checked; tm receives the
+// tangent mask.
+int jacobian_args(RealArgs& args, int& tm, void* const* packed, int64_t atom_count, int64_t event_count,
+                  int64_t state_count, int64_t output_count, int features, int64_t tissue_stride,
+                  const int32_t* columns, int64_t column_count, float factor_re, float factor_im,
+                  const float* sample_sign, void* signal, void* jacobian) {
   if (packed == nullptr) {
     set_error("packed buffers are null");
     return BAD_ARGUMENT;
@@ -861,11 +866,10 @@ int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t even
     set_error("Jacobian columns asked for but the Jacobian buffer is null");
     return BAD_ARGUMENT;
   }
-  RealArgs args;
   int st = fill_real_args(args, packed, atom_count, event_count, state_count, output_count,
                           features, tissue_stride);
   if (st != OK) return st;
-  int tm = 0;
+  tm = 0;
   int cols[5];
   if (!tangent_mask(columns, column_count, tm, cols)) return BAD_ARGUMENT;
   if (cols[2] >= 0 && !(features & FEATURE_TRANSMIT)) {
@@ -887,6 +891,63 @@ int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t even
   args.sample_sign = sample_sign;
   args.factor_re = factor_re;
   args.factor_im = factor_im;
+  return OK;
+}
+
+// Workspace of tsim_simulate_jacobian_noalloc: the compiled event codes and
+// macro-ops of one stream of at most kNoAllocEvents events, in a device
+// array that exists for the life of the process (one copy per GPU). Its
+// host-side name is not a device address; each device's address comes from
+// hipGetSymbolAddress once, with that device current.
+constexpr int64_t kNoAllocEvents = 16384;
+constexpr size_t kNoAllocCodeBytes = sizeof(EventCode) * (size_t)kNoAllocEvents;
+constexpr size_t kNoAllocBytes = kNoAllocCodeBytes + sizeof(MacroOp) * (size_t)(5 * kNoAllocEvents + 2);
+static_assert(kNoAllocCodeBytes % 256 == 0, "macro-ops start 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_address[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up on the first call there.
+int scratch_address(unsigned char** out) {
+  int device = -1;
+  int st = hip_status(hipGetDevice(&device), "hipGetDevice");
+  if (st != OK) return st;
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("device %d is beyond the %d this library tracks", device, kMaxDevices);
+    return UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> hold(g_scratch_mutex);
+  if (g_scratch_address[device] == nullptr) {
+    void* address = nullptr;
+    st = hip_status(hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (st != OK) return st;
+    if (address == nullptr) {
+      set_error("hipGetSymbolAddress(scratch) returned no address");
+      return HIP_FAILURE;
+    }
+    g_scratch_address[device] = (unsigned char*)address;
+  }
+  *out = g_scratch_address[device];
+  return OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
+                           int64_t state_count, int64_t output_count, int features,
+                           int64_t tissue_stride, const int32_t* columns, int64_t column_count,
+                           float factor_re, float factor_im, const float* sample_sign,
+                           void* signal, void* jacobian, int tile, void* stream) {
+  RealArgs args;
+  int tm = 0;
+  int st = jacobian_args(args, tm, packed, atom_count, event_count, state_count, output_count, features,
+                         tissue_stride, columns, column_count, factor_re, factor_im, sample_sign, signal, jacobian);
+  if (st != OK) return st;
   hipStream_t s = (hipStream_t)stream;
   Compiled compiled;
   st = compile_packed(packed, event_count, s, compiled);
@@ -894,6 +955,51 @@ int tsim_simulate_jacobian(void* const* packed, int64_t atom_count, int64_t even
   st = compile_ops(event_count, output_count, compiled);
   if (st != OK) return st;
   args.ops = compiled.ops;
+  return run_real(args, tm, features, tile, s);
+}
+
+int64_t tsim_noalloc_max_events(void) { return kNoAllocEvents; }
+
+int tsim_simulate_jacobian_noalloc(void* const* packed, int64_t atom_count, int64_t event_count,
+                                   int64_t state_count, int64_t output_count, int features,
+                                   int64_t tissue_stride, const int32_t* columns, int64_t column_count,
+                                   float factor_re, float factor_im, const float* sample_sign,
+                                   void* signal, void* jacobian, int tile, void* stream) {
+  RealArgs args;
+  int tm = 0;
+  int st = jacobian_args(args, tm, packed, atom_count, event_count, state_count, output_count, features,
+                         tissue_stride, columns, column_count, factor_re, factor_im, sample_sign, signal, jacobian);
+  if (st != OK) return st;
+  if (event_count > kNoAllocEvents) {
+    set_error("tsim_simulate_jacobian_noalloc takes at most %lld events, got %lld",
+              (long long)kNoAllocEvents, (long long)event_count);
+    return BAD_ARGUMENT;
+  }
+  for (int i : {E_DURATION, E_KIND, E_FLIP, E_PHASE, E_ACTION, E_OUTPUT_INDEX}) {
+    if (event_count > 0 && packed[TISSUE_COUNT + i] == nullptr) {
+      set_error("event buffer %d is null", i);
+      return BAD_ARGUMENT;
+    }
+  }
+  unsigned char* base = nullptr;
+  st = scratch_address(&base);
+  if (st != OK) return st;
+  EventCode* codes = (EventCode*)base;
+  MacroOp* ops = (MacroOp*)(base + kNoAllocCodeBytes);
+  hipStream_t s = (hipStream_t)stream;
+  if (event_count > 0) {
+    st = compile_stream((const float*)packed[TISSUE_COUNT + E_DURATION],
+                        (const int32_t*)packed[TISSUE_COUNT + E_KIND], (const float*)packed[TISSUE_COUNT + E_FLIP],
+                        (const float*)packed[TISSUE_COUNT + E_PHASE],
+                        (const uint8_t*)packed[TISSUE_COUNT + E_ACTION],
+                        (const int32_t*)packed[TISSUE_COUNT + E_OUTPUT_INDEX], event_count, codes, s);
+    if (st != OK) return st;
+  }
+  // pack_ops writes every op it emits and the end markers after them, so
+  // nothing a previous call left in the workspace is read.
+  st = pack_stream(codes, event_count, output_count, ops, s);
+  if (st != OK) return st;
+  args.ops = ops;
   return run_real(args, tm, features, tile, s);
 }
 
