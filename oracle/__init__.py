@@ -287,3 +287,138 @@ def reference_complex(
     """
     return _reference_call(events, tissue, states=states, outputs=outputs, features=features,
                            real_axis=-1, pool_kind=2 if exchanging else 0, tangent=tangent, threads=threads)
+
+
+_EVENT_VJP_ORDER = ("flip", "phase", "duration")        # simulate_vjp's gradient slots after the tissue
+_EVENT_JVP_ORDER = ("duration", "flip", "phase")        # simulate_jvp / simulate_vjp_jvp's
+
+
+def reference_vjp(
+    events: dict[str, Any],
+    tissue: dict[str, Any],
+    cotangent: np.ndarray,
+    *,
+    states: int,
+    outputs: int,
+    features: int = FEATURE_ALL,
+    real_axis: int = 1,
+    exchanging: bool = False,
+    tangent: dict[str, Any] | None = None,
+    threads: int = 8,
+) -> dict[str, np.ndarray]:
+    """Reverse mode of the reference's own FP32 kernel (built from source).
+
+    ``cotangent`` is complex (atoms, outputs): its real and imaginary parts
+    seed the two output planes. Without ``tangent`` this is
+    ``simulate_vjp`` (src/torchsim/_epg_cpu.cpp:11150-11416): the gradient
+    of <w, y(x)>. With ``tangent`` (per-atom tissue rows and per-event
+    ``duration``/``flip``/``phase`` rows) it is ``simulate_vjp_jvp``
+    (:11589-11755), and the result is the gradient of <w, J(x) v> -- the
+    forward-over-reverse pass ``_NativeEpgJvp.backward`` takes
+    (sequence/_accelerators.py:1787-1850). Returns the 17 tissue gradients
+    (atoms,) by name and the event gradients ``flip``, ``phase``,
+    ``duration`` (events,), summed over atoms.
+    """
+    module = reference_module()
+    if module is None:
+        raise RuntimeError("oracle/_ref is not built")
+    atoms = max(np.size(v) for v in tissue.values())
+    count = int(np.size(events["kind"]))
+    keep = []
+
+    def f32(value, size):
+        array = np.ascontiguousarray(np.broadcast_to(np.asarray(value, dtype=np.float32), (size,)).copy())
+        keep.append(array)
+        return array
+
+    def arr(value, dtype):
+        array = np.ascontiguousarray(np.asarray(value, dtype=dtype))
+        keep.append(array)
+        return array.ctypes.data
+
+    tissue_ptrs = [f32(tissue.get(n, _TISSUE_DEFAULT.get(n, 0.0)), atoms).ctypes.data for n in _TISSUE_ORDER]
+    event_ptrs = [
+        arr(events["duration"], np.float32), arr(events["kind"], np.int32), arr(events["flip"], np.float32),
+        arr(events.get("phase", np.zeros(count)), np.float32), arr(events["action"], np.uint8),
+        arr(events["output_index"], np.int32), arr(np.zeros(count), np.int32), arr(np.zeros(count), np.float32),
+        arr(np.zeros(count), np.float32),
+    ]
+    w = np.asarray(cotangent).reshape(atoms, outputs)
+    seed = [f32(w.real.reshape(-1), atoms * outputs).ctypes.data, f32(w.imag.reshape(-1), atoms * outputs).ctypes.data]
+    tail = [0] * 13
+    dims = (atoms, 1, count, states, outputs, threads, real_axis, 0.0, 0.0, 1, 1, 0, 1.0, 0, 1.0,
+            2 if exchanging else 0, features, 0, 0, 0)
+
+    def gradient_block(order):
+        tissue_grads = [f32(0.0, atoms) for _ in _TISSUE_ORDER]
+        event_grads = {name: f32(0.0, count) for name in order}
+        pointers = [g.ctypes.data for g in tissue_grads] + [event_grads[n].ctypes.data for n in order]
+        out = {name: g for name, g in zip(_TISSUE_ORDER, tissue_grads)}
+        out.update(event_grads)
+        return pointers, out
+
+    if tangent is None:
+        pointers, grads = gradient_block(_EVENT_VJP_ORDER)
+        module.simulate_vjp((*tissue_ptrs, *event_ptrs, *seed, *pointers, *tail), *dims)
+    else:
+        direction = [f32(tangent.get(n, 0.0), atoms).ctypes.data for n in _TISSUE_ORDER]
+        direction += [f32(tangent.get(n, 0.0), count).ctypes.data for n in _EVENT_JVP_ORDER]
+        value_pointers, _ = gradient_block(_EVENT_JVP_ORDER)
+        pointers, grads = gradient_block(_EVENT_JVP_ORDER)
+        module.simulate_vjp_jvp((*tissue_ptrs, *event_ptrs, *direction, *seed, *value_pointers, *pointers, *tail),
+                                *dims)
+    return {name: value.astype(np.float64) for name, value in grads.items()}
+
+
+def reference_crlb_gradient(
+    events: dict[str, Any],
+    t1: Any,
+    t2: Any,
+    *,
+    states: int,
+    outputs: int,
+    factor: complex = 1j,
+    threads: int = 8,
+) -> tuple[float, np.ndarray, np.ndarray]:
+    """The sequence-design loss sum_atoms tr(F^-1) of (T1, T2, M0) at sigma 1
+    and its gradient with respect to every event's flip and duration, the way
+    the reference's autograd reaches it: Jacobian columns from its FP32
+    kernel (``simulate_jvp``; the M0 column is the signal at M0 = 1), the loss
+    and its cotangents on the columns in FP64, then one forward-over-reverse
+    pass per tissue column (``simulate_vjp_jvp``) and one first-order pass
+    for the M0 column (``simulate_vjp``), summed -- what
+    ``crlb(J).sum().backward()`` runs through ``_NativeEpgJvp.backward`` and
+    ``_NativeEpg.backward`` (optim/_design.py:196-253,
+    sequence/_accelerators.py:1544-1586, 1787-1850).
+
+    ``events`` is a real-subspace stream (the kernels' (0, m0 a0) planes);
+    ``factor`` is the simulator's constant (MRF: 1j), so a public column is
+    factor * i * (its imaginary plane). Returns (loss, d/dflip, d/dduration)
+    with the event gradients (events,).
+    """
+    t1 = np.asarray(t1, dtype=np.float32)
+    t2 = np.asarray(t2, dtype=np.float32)
+    tissue = {"t1": t1, "t2": t2, "m0": np.ones_like(t1)}
+    kw = dict(states=states, outputs=outputs, features=FEATURE_ALL, threads=threads)
+    signal = reference_real(events, tissue, **kw)
+    columns = [reference_real(events, tissue, tangent={name: 1.0}, **kw) for name in ("t1", "t2")] + [signal]
+    scale = complex(factor) * 1j                     # public = scale * plane
+    jac = np.stack(columns, 1).astype(np.float64) * scale
+    rows = np.concatenate((jac.real, jac.imag), -1)   # (atoms, 3, 2 outputs)
+    fisher = rows @ np.transpose(rows, (0, 2, 1))
+    inverse = np.linalg.inv(fisher)
+    loss = float(np.trace(inverse, axis1=1, axis2=2).sum())
+    d_rows = -2.0 * (inverse @ inverse) @ rows        # dL/d rows
+    d_public = d_rows[..., :outputs] + 1j * d_rows[..., outputs:]
+    # <w_pub, scale * plane> over the real pair (re, im) -> cotangent on the plane
+    d_plane = (np.conj(scale) * d_public).real        # real, (atoms, 3, outputs)
+    flip = np.zeros(np.size(events["kind"]))
+    duration = np.zeros_like(flip)
+    for k, name in enumerate(("t1", "t2")):
+        g = reference_vjp(events, tissue, 1j * d_plane[:, k], tangent={name: 1.0}, **kw)
+        flip += g["flip"]
+        duration += g["duration"]
+    g = reference_vjp(events, tissue, 1j * d_plane[:, 2], **kw)
+    flip += g["flip"]
+    duration += g["duration"]
+    return loss, flip, duration

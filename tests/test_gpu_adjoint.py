@@ -20,7 +20,11 @@ from oracle import packed as P
 from tests import _cases
 
 pytestmark = pytest.mark.gpu
-GRAD_TOL = 2e-3  # max|g - g_ref| / max|g_ref|, FP32 adjoint vs FP64
+# max|g - g_ref| / max|g_ref|. The FP32 adjoint meets the FP64 gradient and
+# the reference's FP32 gradient to ~1e-6 (flips 2.7e-7, TRs 1.0e-6 at the
+# full design shape); a dropped term or a wrong sign is off by far more.
+GRAD_TOL = 1e-5
+REF_GRAD_TOL = 1e-5
 
 
 def _design_inputs(pulses, atoms, seed=1):
@@ -94,9 +98,11 @@ def test_gradient_matches_fp64_autograd_reduced():
     rows = torch.cat((jac.real, jac.imag), -1)
     ref_loss = torch.diagonal(torch.linalg.inv(rows @ rows.transpose(-1, -2)), dim1=-2, dim2=-1).sum()
     ref_loss.backward()
-    assert abs(loss - float(ref_loss)) / abs(float(ref_loss)) < 1e-3
-    assert _cases.rel_err(gf, flip.grad.numpy()) < GRAD_TOL
-    assert _cases.rel_err(gt, tr.grad.numpy()) < GRAD_TOL
+    e_flip, e_tr = _cases.rel_err(gf, flip.grad.numpy()), _cases.rel_err(gt, tr.grad.numpy())
+    print(f"reduced design gradient vs FP64 autograd: flips {e_flip:.2e}, TRs {e_tr:.2e}")
+    assert abs(loss - float(ref_loss.detach())) / abs(float(ref_loss.detach())) < 1e-5
+    assert e_flip < GRAD_TOL
+    assert e_tr < GRAD_TOL
     assert gt[-1] == 0.0  # the last TR is never played
 
 
@@ -104,7 +110,7 @@ def test_config2_gradient_vs_fp64_central_differences():
     flips, trs, t1, t2 = _design_inputs(880, 1024)
     loss, gf, gt = _gpu_loss_and_grad(flips, trs, t1, t2)
     ref = _oracle_loss(flips, trs, t1, t2)
-    assert abs(loss - ref) / abs(ref) < 1e-3
+    assert abs(loss - ref) / abs(ref) < 1e-5
     assert gt[-1] == 0.0
     rng = np.random.default_rng(3)
     h = np.rad2deg(1e-4)  # 1e-4 rad in the flip angle
@@ -117,7 +123,32 @@ def test_config2_gradient_vs_fp64_central_differences():
         dn = _oracle_loss(flips - h * vf, trs - h * vt, t1, t2)
         fd = (up - dn) / (2 * h)
         ours = float(gf @ vf + gt @ vt)
-        assert abs(ours - fd) / max(abs(fd), 1e-12) < 5e-3, (trial, ours, fd)
+        print(f"direction {trial}: adjoint {ours:.9e}, FP64 central difference {fd:.9e}")
+        assert abs(ours - fd) / max(abs(fd), 1e-12) < 1e-4, (trial, ours, fd)
+
+
+def test_config2_gradient_matches_the_reference_adjoint():
+    """The full design shape (1,024 atoms x 880 pulses, S = 10): every flip
+    and TR component against the reference's own gradient -- its FP32
+    kernels driven as its autograd drives them (forward-over-reverse per
+    tissue column, tests/golden/make_design_gradient.py), recomputed where
+    oracle/_ref is built, else read from the committed fixture."""
+    from pathlib import Path
+
+    from tests.golden.make_design_gradient import reference_gradient
+
+    flips, trs, t1, t2 = _design_inputs(880, 1024)
+    loss, gf, gt = _gpu_loss_and_grad(flips, trs, t1, t2)
+    if oracle.reference_module() is not None:
+        ref_loss, ref_flip, ref_tr = reference_gradient(flips, trs, t1, t2, threads=16)
+    else:
+        golden = np.load(Path(__file__).resolve().parent / "golden" / "design_gradient_1024x880.npz")
+        ref_loss, ref_flip, ref_tr = float(golden["loss"]), golden["grad_flip"], golden["grad_tr"]
+    e_loss = abs(loss - ref_loss) / ref_loss
+    e_flip, e_tr = _cases.rel_err(gf, ref_flip), _cases.rel_err(gt, ref_tr)
+    print(f"config-2 gradient vs the reference's: loss {e_loss:.2e}, flips {e_flip:.2e}, TRs {e_tr:.2e}")
+    assert e_loss < 1e-5
+    assert e_flip < REF_GRAD_TOL and e_tr < REF_GRAD_TOL
 
 
 def test_checkpoint_length_changes_nothing():
