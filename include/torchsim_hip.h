@@ -157,25 +157,41 @@ int tsim_simulate_projected(void* const* packed, int64_t atom_count, int64_t eve
 
 /*
  * Adjoint of the fused real-subspace system: gradients of a scalar loss with
- * respect to every event's flip angle and duration, given the loss's real
- * cotangents on the recorded m0*a0 and on the Jacobian columns. Replaces the
- * reference's `_epg_cpu.simulate_vjp_jvp` / `simulate_vjp`
- * (src/torchsim/_epg_cpu.cpp:11150-11755; kernels :7028-7314, :5985-6227)
- * that `_NativeEpgJvp.backward` / `_NativeEpg.backward` reach
- * (sequence/_accelerators.py:1544-1586, 1787-1850). The state is
- * checkpointed every segment_events events and recomputed per segment.
+ * respect to every event's flip angle and duration and to each atom's
+ * tissue, given the loss's real cotangents on the recorded m0*a0 and on the
+ * Jacobian columns. Replaces the reference's `_epg_cpu.simulate_vjp`
+ * (first order, when jacobian_cotangent is NULL) and
+ * `_epg_cpu.simulate_vjp_jvp` (forward-over-reverse along the columns)
+ * (src/torchsim/_epg_cpu.cpp:11150-11755; kernels :5985-6227, :7028-7314;
+ * tissue reduction :11366-11395) that `_NativeEpg.backward` /
+ * `_NativeEpgJvp.backward` reach (sequence/_accelerators.py:1544-1586,
+ * 1787-1850). The kernel walks the macro-op stream (one op per MRF pulse);
+ * the dual state is checkpointed every segment_ops ops and recomputed per
+ * segment. op_bound: the number of macro-ops the stream compiles to, or an
+ * upper bound on it (the workspace is sized by it; the kernels read the true
+ * count on the device and never pass the bound); -1 reads the count back
+ * from the device (one stream synchronization).
  *
- * columns:  HOST array of column_count (<= 3) indices among {0 t1, 1 t2, 2 m0}.
+ * columns:  HOST array of column_count (<= 4) indices among {0 t1, 1 t2,
+ *           2 m0, 3 b1}.
  * signal_cotangent:   DEVICE (atom_count, output_count) FP32, or NULL.
  * jacobian_cotangent: DEVICE (atom_count, column_count, output_count), or NULL.
  * grad_flip, grad_duration: DEVICE (event_count) FP32, summed over atoms in a
- *           fixed order (deterministic).
+ *           fixed order (deterministic); events with no rotation (no
+ *           interval) get 0.
+ * tissue_gradients: HOST array of tissue_gradient_count (17, tissue order)
+ *           DEVICE pointers, each NULL or (atom_count) FP32 written with the
+ *           gradient per atom; t1, t2, m0, b1 and inversion_efficiency are
+ *           carried (another non-null entry is status 2); count 0 for none.
+ * No diffusion, off-resonance or flow feature (status 2): those run on the
+ * complex kernel, which has no adjoint.
  */
 int tsim_simulate_vjp_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
                                int64_t state_count, int64_t output_count, int features,
                                int64_t tissue_stride, const int32_t* columns, int64_t column_count,
                                const float* signal_cotangent, const float* jacobian_cotangent,
-                               int64_t segment_events, float* grad_flip, float* grad_duration,
+                               int64_t segment_ops, int64_t op_bound, float* grad_flip, float* grad_duration,
+                               float* const* tissue_gradients, int64_t tissue_gradient_count,
                                void* stream);
 
 /*

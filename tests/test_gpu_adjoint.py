@@ -35,13 +35,13 @@ def _design_inputs(pulses, atoms, seed=1):
     return flips, trs, t1, t2
 
 
-def _gpu_loss_and_grad(flips, trs, t1, t2, states=10, segment_events=256):
+def _gpu_loss_and_grad(flips, trs, t1, t2, states=10, checkpoint_pulses=64):
     from torchsim_amd.optim import crlb
     from torchsim_amd.simulators import MRFSimulator
 
     flip = torch.tensor(flips, dtype=torch.float32, requires_grad=True)
     tr = torch.tensor(trs, dtype=torch.float32, requires_grad=True)
-    seq = MRFSimulator(states=states, segment_events=segment_events)
+    seq = MRFSimulator(states=states, checkpoint_pulses=checkpoint_pulses)
     _, jac = seq.jacobian(("T1", "T2", "M0"), flip=flip, TR=tr, T1=torch.tensor(t1).cuda(),
                           T2=torch.tensor(t2).cuda(), M0=torch.ones(len(t1)).cuda())
     loss = crlb(jac).sum()
@@ -153,9 +153,9 @@ def test_config2_gradient_matches_the_reference_adjoint():
 
 def test_checkpoint_length_changes_nothing():
     flips, trs, t1, t2 = _design_inputs(150, 64)
-    a = _gpu_loss_and_grad(flips, trs, t1, t2, segment_events=256)
-    b = _gpu_loss_and_grad(flips, trs, t1, t2, segment_events=37)
-    c = _gpu_loss_and_grad(flips, trs, t1, t2, segment_events=100000)
+    a = _gpu_loss_and_grad(flips, trs, t1, t2, checkpoint_pulses=64)
+    b = _gpu_loss_and_grad(flips, trs, t1, t2, checkpoint_pulses=9)
+    c = _gpu_loss_and_grad(flips, trs, t1, t2, checkpoint_pulses=100000)
     assert np.array_equal(a[1], b[1]) and np.array_equal(a[1], c[1])
     assert np.array_equal(a[2], b[2]) and np.array_equal(a[2], c[2])
 

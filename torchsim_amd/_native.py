@@ -58,7 +58,7 @@ _F32 = ctypes.c_float
 
 #: Every symbol include/torchsim_hip.h declares, with its ctypes signature.
 #: ABI revision these bindings are written against (``tsim_version``).
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "tsim_last_error": (ctypes.c_char_p, []),
@@ -138,8 +138,11 @@ SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
             _P,
             _P,
             _I64,
+            _I64,
             _P,
             _P,
+            ctypes.POINTER(_P),
+            _I64,
             _P,
         ],
     ),

@@ -108,6 +108,11 @@ int compile_packed(void* const* packed, int64_t event_count, hipStream_t stream,
 // Group compiled events into the real kernel's macro-ops (after compile_packed).
 int compile_ops(int64_t event_count, int64_t outputs, Compiled& out);
 
+// The same grouping into caller-provided ops (op_capacity(events) of them);
+// with op_count, the number of ops (end marker excluded) is written there.
+int pack_stream(const EventCode* codes, int64_t events, int64_t outputs, MacroOp* ops, hipStream_t stream,
+                int64_t* op_count = nullptr);
+
 void set_error(const char* fmt, ...);
 const char* last_error();
 int hip_status(hipError_t err, const char* where);

@@ -156,7 +156,8 @@ __device__ __forceinline__ void for_each_part(const EventCode& ev, F&& f) {
 }
 
 __global__ void __launch_bounds__(kPackThreads) pack_ops(const EventCode* __restrict__ codes, int64_t events,
-                                                         int64_t outputs, MacroOp* __restrict__ ops) {
+                                                         int64_t outputs, MacroOp* __restrict__ ops,
+                                                         int64_t* __restrict__ op_count) {
   __shared__ int s_has[kPackThreads];
   __shared__ int s_first[kPackThreads];
   __shared__ int s_last[kPackThreads];
@@ -227,7 +228,7 @@ __global__ void __launch_bounds__(kPackThreads) pack_ops(const EventCode* __rest
     z.ctrl = i >= count ? OP_END : 0u;
     z.out = -1;
     z.event = -1;
-    z.pad = 0;
+    z.relax_event = -1;
     ops[i] = z;
   }
   __syncthreads();
@@ -244,6 +245,7 @@ __global__ void __launch_bounds__(kPackThreads) pack_ops(const EventCode* __rest
       MacroOp* op = ops + index;
       if (part == OP_RELAX) {
         op->dt = ev.dt;
+        op->relax_event = ev.event;
       } else if (part == OP_ROTATE) {
         op->flip = ev.flip;
         op->versine = ev.versine;
@@ -259,7 +261,8 @@ __global__ void __launch_bounds__(kPackThreads) pack_ops(const EventCode* __rest
     });
   }
   __syncthreads();
-  if (t == 0 && count > 0) ops[0].pad = s_sequential ? (int32_t)OPS_SEQUENTIAL : 0;
+  if (t == 0 && count > 0 && s_sequential) atomicOr(&ops[0].ctrl, OPS_SEQUENTIAL);
+  if (t == 0 && op_count != nullptr) *op_count = count;
 }
 
 }  // namespace
@@ -269,13 +272,14 @@ int compile_ops(int64_t event_count, int64_t outputs, Compiled& out) {
   int st = scratch_alloc((void**)&out.ops, sizeof(MacroOp) * (size_t)capacity, out.stream, "macro-ops");
   if (st != OK) return st;
   hipLaunchKernelGGL(pack_ops, dim3(1), dim3(kPackThreads), 0, out.stream, out.codes,
-                     event_count > 0 ? event_count : 0, outputs, out.ops);
+                     event_count > 0 ? event_count : 0, outputs, out.ops, (int64_t*)nullptr);
   return hip_status(hipGetLastError(), "pack_ops launch");
 }
 
-int pack_stream(const EventCode* codes, int64_t events, int64_t outputs, MacroOp* ops, hipStream_t stream) {
+int pack_stream(const EventCode* codes, int64_t events, int64_t outputs, MacroOp* ops, hipStream_t stream,
+                int64_t* op_count) {
   hipLaunchKernelGGL(pack_ops, dim3(1), dim3(kPackThreads), 0, stream, codes, events > 0 ? events : 0, outputs,
-                     ops);
+                     ops, op_count);
   return hip_status(hipGetLastError(), "pack_ops launch");
 }
 
@@ -805,7 +809,7 @@ extern "C" {
 
 const char* tsim_last_error(void) { return tsim::last_error(); }
 
-int tsim_version(void) { return 1; }
+int tsim_version(void) { return 2; }
 
 int tsim_simulate(void* const* pointers, int64_t pointer_count, int64_t atom_count,
                   int64_t train_count, int64_t event_count, int64_t state_count,

@@ -31,7 +31,7 @@ struct alignas(32) MacroOp {
   uint32_t ctrl;
   int32_t out;   // output index (OP_RECORD)
   int32_t event; // packed position of the rotation's event
-  int32_t pad;   // op 0: stream flags (OPS_SEQUENTIAL)
+  int32_t relax_event;  // packed position of the event whose interval is the relaxation
 };
 
 // Upper bound on the ops E events compile to (each event contributes at most
@@ -52,7 +52,7 @@ __device__ __forceinline__ MacroOp load_op(const MacroOp* ops, int64_t i) {
   m.ctrl = w[4];
   m.out = (int32_t)w[5];
   m.event = (int32_t)w[6];
-  m.pad = (int32_t)w[7];
+  m.relax_event = (int32_t)w[7];
   return m;
 }
 
@@ -108,7 +108,7 @@ struct RealArgs {
 // samples of one row together. This needs the records to arrive in output
 // order 0, 1, 2, ... (the op compiler checks it and marks the stream).
 constexpr int STAGE_BYTES_PER_WAVE = 8192;
-constexpr uint32_t OPS_SEQUENTIAL = 1;  // MacroOp[0].pad: records come in output order
+constexpr uint32_t OPS_SEQUENTIAL = 128;  // in MacroOp[0].ctrl: records come in output order
 
 // Accumulators one atom hands back in normal-equation mode: the cost
 // sum |model - data|^2, J^T r (4), and the upper triangle of J^T J (10),
@@ -591,7 +591,7 @@ __device__ __forceinline__ void real_body(const RealArgs& p, int64_t group) {
   extern __shared__ float2 stage_memory[];
   // Staged writes (outputs) or, in normal-equation mode, staged reads of the
   // measured samples: the same per-atom-row traffic, the other direction.
-  const bool staged = p.stage_k_log2 > 0 && (load_op(ops, 0).pad & OPS_SEQUENTIAL) != 0;
+  const bool staged = p.stage_k_log2 > 0 && (load_op(ops, 0).ctrl & OPS_SEQUENTIAL) != 0;
   const int stage_k = 1 << (p.stage_k_log2 > 0 ? p.stage_k_log2 : 0);
   const int lane = (int)(threadIdx.x & 63);
   const int wave_group = lane / L;

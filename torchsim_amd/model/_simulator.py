@@ -26,6 +26,7 @@ from typing import Any
 import numpy as np
 import torch
 
+from .. import _native
 from ..sequence import _engine
 from ..sequence._events import EventOperators, PackedEvents, compose, pack
 
@@ -57,7 +58,7 @@ PUBLIC_PROPERTIES = {
 #: and the gradient geometry a diffusion coefficient is read through
 #: (reference sequence/_accelerators.py:371-420).
 RUN_SETTINGS = (
-    "states", "repetitions", "device", "tile", "segment_events", "crusher_dephasing_rad", "voxel_size_m",
+    "states", "repetitions", "device", "tile", "checkpoint_pulses", "crusher_dephasing_rad", "voxel_size_m",
 )
 
 #: The apparent diffusion coefficient is given in um^2/ms; the b-factor wants m^2/s.
@@ -285,7 +286,10 @@ class Simulator:
             value = fields["diffusion_um2_per_ms"]
             fields["diffusion_um2_per_ms"] = value * scale if scale else 0.0
         differentiated = tuple(self._tissue_fields({name: held[name] for name in diff}).keys())
-        tissue = _engine.prepare_tissue(fields, device, differentiated)
+        # fields autograd must reach are laid out per atom, like forward-mode ones
+        reversed_fields = tuple(name for name, value in fields.items()
+                                if isinstance(value, torch.Tensor) and value.requires_grad)
+        tissue = _engine.prepare_tissue(fields, device, tuple(dict.fromkeys(differentiated + reversed_fields)))
         states = int(sequence.get("states", self.states) or 10)
         real, flip, samples = _engine.real_subspace(packed, tissue, device)
         factor = self.signal_factor if self.signal_factor is not None else 1.0
@@ -299,20 +303,29 @@ class Simulator:
                                                   columns=differentiated, factor=factor, flip=flip, samples=samples)
             signal, jacobian = self._run(held, sequence, diff)
             return subspace.project(signal), (None if jacobian is None else subspace.project(jacobian))
+        sequence_grad = packed.flip.requires_grad or packed.duration.requires_grad
         if not real:
+            if sequence_grad or reversed_fields:
+                raise NotImplementedError(
+                    "reverse mode runs on the real-subspace kernel: this protocol or tissue (phase-cycled or "
+                    "off-resonant pulses, an exchanging pool) takes the complex kernel, which has no adjoint"
+                )
             from ..sequence import _complex
 
             return _complex.simulate_complex(packed, tissue, states=states, columns=differentiated, factor=factor)
-        if packed.flip.requires_grad or packed.duration.requires_grad:
-            held_grad = [name for name, value in held.items() if isinstance(value, torch.Tensor) and value.requires_grad]
-            if held_grad:
+        if sequence_grad or reversed_fields:
+            unsupported = [name for name in reversed_fields if name not in _engine.GRADIENT_FIELDS]
+            if unsupported:
                 raise NotImplementedError(
-                    f"gradients reach the sequence (flips, timing) only; {held_grad} also require grad"
+                    f"reverse mode reaches T1, T2, M0, B1, the inversion efficiency and the sequence; "
+                    f"{unsupported} also require grad"
                 )
             flip_in = packed.flip if flip is None else flip
+            buffers = [tissue.buffers[_native.TISSUE_INDEX[name]] if name in reversed_fields else None
+                       for name in _engine.GRADIENT_FIELDS]
             signal, jacobian = _engine.FusedRealJacobian.apply(
-                flip_in.to(device), packed.duration.to(device), packed, tissue, states, differentiated, factor,
-                samples, int(sequence.get("segment_events", 256)),
+                flip_in.to(device), packed.duration.to(device), *buffers, packed, tissue, states, differentiated,
+                factor, samples, int(sequence.get("checkpoint_pulses", 64)),
             )
             return signal, (jacobian if differentiated else None)
         return _engine.simulate_real(

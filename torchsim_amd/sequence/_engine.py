@@ -365,6 +365,37 @@ def _tuned_tile(packed: PackedEvents, tissue: PreparedTissue, *, states: int, co
     return _tuning.tile_for(tissue.device, states, tuple(columns), tissue.features, probe)
 
 
+#: Tissue fields the real-subspace adjoint returns gradients for (the
+#: reference's simulate_vjp writes all 17; off-resonance, transmit phase,
+#: diffusion and the pools take the complex path, which has no adjoint here).
+GRADIENT_FIELDS = ("t1_ms", "t2_ms", "m0", "b1", "inversion_efficiency")
+
+
+def op_count(packed: PackedEvents) -> int:
+    """How many macro-ops the library groups the stream into: each event's
+    parts -- its interval (position 0, if nonzero), a pre-shift (3), its
+    rotation or inversion (1) or record (2), a post-shift (3), a spoil (1) or
+    a shift after it (3) -- in order, a new op starting wherever a part's
+    position does not exceed its predecessor's or follows a shift (the rule
+    the op compiler applies on the device)."""
+    kind = packed.kind.cpu().numpy()
+    action = packed.action.cpu().numpy().astype(np.int32)
+    timed = packed.duration.detach().cpu().numpy() != 0
+    position = np.full((kind.size, 5), -1, dtype=np.int8)
+    position[timed, 0] = 0
+    position[(action & 1) != 0, 1] = 3
+    position[kind == 1, 2] = 1
+    position[(kind == 2) & ((action & RECORD) != 0), 2] = 2
+    position[(action & 2) != 0, 3] = 3
+    position[(action & 16) != 0, 4] = 3
+    position[(action & 8) != 0, 4] = 1
+    parts = position.reshape(-1)
+    parts = parts[parts >= 0]
+    if parts.size == 0:
+        return 0
+    return 1 + int(np.count_nonzero((parts[1:] <= parts[:-1]) | (parts[:-1] == 3)))
+
+
 def adjoint_real(
     packed: PackedEvents,
     tissue: PreparedTissue,
@@ -375,11 +406,18 @@ def adjoint_real(
     duration: torch.Tensor,
     signal_cotangent: torch.Tensor | None,
     jacobian_cotangent: torch.Tensor | None,
-    segment_events: int = 256,
-) -> tuple[torch.Tensor, torch.Tensor]:
-    """d(loss)/d(flip) and d(loss)/d(duration), one entry per event, from the
-    loss's real cotangents on the recorded m0*a0 (atoms, outputs) and on the
-    Jacobian columns (atoms, P, outputs)."""
+    checkpoint_pulses: int = 64,
+    tissue_wanted: tuple[str, ...] = (),
+    op_bound: int | None = None,
+) -> tuple[torch.Tensor, torch.Tensor, dict[str, torch.Tensor]]:
+    """d(loss)/d(flip) and d(loss)/d(duration), one entry per event, and
+    d(loss)/d(field) per atom for each field of ``tissue_wanted`` (among
+    GRADIENT_FIELDS), from the loss's real cotangents on the recorded m0*a0
+    (atoms, outputs) and on the Jacobian columns (atoms, P, outputs). The
+    state is checkpointed every ``checkpoint_pulses`` macro-ops (pulses of an
+    MRF train) and recomputed per segment. ``op_bound`` sizes the workspace
+    (default: the stream's op count, worked out on the host; -1 has the
+    library read it back from the device)."""
     device = tissue.device
     _native.require_device(device)
     lib = _native.load()
@@ -389,7 +427,14 @@ def adjoint_real(
     count = packed.event_count
     grad_flip = torch.empty(count, dtype=torch.float32, device=device)
     grad_duration = torch.empty(count, dtype=torch.float32, device=device)
-    ptrs = _native.pointers([*tissue.buffers, *events])
+    grads = {}
+    slots = (ctypes.c_void_p * _native.TISSUE_COUNT)()
+    for name in tissue_wanted:
+        if name not in GRADIENT_FIELDS:
+            raise NotImplementedError(f"the real-subspace adjoint returns no gradient for {name!r}")
+        grads[name] = torch.empty(tissue.atoms, dtype=torch.float32, device=device)
+        slots[_native.TISSUE_INDEX[name]] = grads[name].data_ptr()
+    ptrs = _native.pointers([*(None if b is None else b.detach() for b in tissue.buffers), *events])
     cols = (ctypes.c_int32 * max(1, len(columns)))(*[_COLUMN_INDEX[name] for name in columns])
     ws = None if signal_cotangent is None else signal_cotangent.to(device=device, dtype=torch.float32).contiguous()
     wj = None if jacobian_cotangent is None else jacobian_cotangent.to(device=device, dtype=torch.float32).contiguous()
@@ -406,35 +451,43 @@ def adjoint_real(
             len(columns),
             None if ws is None else ws.data_ptr(),
             None if wj is None else wj.data_ptr(),
-            segment_events,
+            checkpoint_pulses,
+            op_count(packed) if op_bound is None else op_bound,
             grad_flip.data_ptr(),
             grad_duration.data_ptr(),
+            slots,
+            _native.TISSUE_COUNT if grads else 0,
             _native.stream_of(device),
         )
     _native.check(status)
-    return grad_flip, grad_duration
+    return grad_flip, grad_duration, grads
 
 
 class FusedRealJacobian(torch.autograd.Function):
     """Signal and Jacobian of the real subspace, differentiable in the event
-    flips and durations (reference sequence/_accelerators.py:1402-1858: the
-    forward of ``_NativeEpg``/``_NativeEpgJvp`` and their backward, here one
-    fused forward kernel and one checkpointed adjoint kernel)."""
+    flips and durations and in the per-atom T1, T2, M0, B1 and inversion
+    efficiency (reference sequence/_accelerators.py:1402-1858: the forward of
+    ``_NativeEpg``/``_NativeEpgJvp`` and their backward, here one fused
+    forward kernel and one checkpointed adjoint kernel). The tissue inputs
+    are the prepared per-atom buffers the kernels read, so autograd carries
+    their gradients back through the layout to the caller's maps."""
 
     @staticmethod
-    def forward(ctx, flip, duration, packed, tissue, states, columns, factor, samples, segment_events):
+    def forward(ctx, flip, duration, t1, t2, m0, b1, ie, packed, tissue, states, columns, factor, samples,
+                checkpoint_pulses):
         signal, jacobian = simulate_real(
             packed, tissue, states=states, columns=columns, factor=factor, flip=flip,
             samples=samples, duration=duration,
         )
         ctx.save_for_backward(flip, duration)
+        ctx.shapes = [None if v is None else v.shape for v in (t1, t2, m0, b1, ie)]
         ctx.packed = packed
         ctx.tissue = tissue
         ctx.states = states
         ctx.columns = columns
         ctx.factor = complex(factor)
         ctx.samples = samples
-        ctx.segment_events = segment_events
+        ctx.checkpoint_pulses = checkpoint_pulses
         if jacobian is None:
             jacobian = torch.empty((tissue.atoms, 0, packed.output_count), dtype=torch.complex64, device=signal.device)
         return signal, jacobian
@@ -442,6 +495,7 @@ class FusedRealJacobian(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_signal, grad_jacobian):
         flip, duration = ctx.saved_tensors
+        nothing = (None,) * 14
         # every output is c * v with v real and c = factor * i (* sample sign):
         # the loss's derivative along v is Re(conj(c) * grad).
         c = torch.tensor(ctx.factor * 1j, dtype=torch.complex64, device=flip.device)
@@ -453,13 +507,22 @@ class FusedRealJacobian(torch.autograd.Function):
         if grad_jacobian is not None and len(ctx.columns):
             wj = (grad_jacobian * scale).real
         if ws is None and wj is None:
-            return (None,) * 9
-        grad_flip, grad_duration = adjoint_real(
+            return nothing
+        wanted = tuple(name for name, need in zip(GRADIENT_FIELDS, ctx.needs_input_grad[2:7]) if need)
+        grad_flip, grad_duration, grads = adjoint_real(
             ctx.packed, ctx.tissue, states=ctx.states, columns=ctx.columns, flip=flip, duration=duration,
-            signal_cotangent=ws, jacobian_cotangent=wj, segment_events=ctx.segment_events,
+            signal_cotangent=ws, jacobian_cotangent=wj, checkpoint_pulses=ctx.checkpoint_pulses,
+            tissue_wanted=wanted,
         )
+        tissue_out = []
+        for name, shape in zip(GRADIENT_FIELDS, ctx.shapes):
+            g = grads.get(name)
+            if g is not None and shape is not None and g.numel() != torch.Size(shape).numel():
+                g = g.sum().reshape(shape)  # one value for every atom
+            tissue_out.append(g)
         return (
             grad_flip.to(flip.dtype) if ctx.needs_input_grad[0] else None,
             grad_duration.to(duration.dtype) if ctx.needs_input_grad[1] else None,
+            *tissue_out,
             None, None, None, None, None, None, None,
         )
