@@ -282,6 +282,11 @@ typedef struct tsim_batch_request {
  * call (sequence/_accelerators.py:3262-3449).
  * mode: 0 signal + Jacobian (as tsim_simulate_jacobian), 1 normal equations
  * (as tsim_gauss_newton_normal). tile: -1 for the library's choice.
+ * Under HIP graph capture the graph keeps the device address of every buffer
+ * a request names (the 26 packed buffers, sample_sign, signal, jacobian,
+ * data, normal): each must stay allocated, and must not be handed back to an
+ * allocator that could give its memory to another tensor, for as long as the
+ * graph may be replayed; replays read and write them in place.
  */
 int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int mode, int tile,
                         int64_t grid_blocks, void* stream);

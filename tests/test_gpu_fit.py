@@ -57,6 +57,51 @@ def test_normal_equations_match_fp64_oracle():
         assert _cases.rel_err(normal[:, index[pq]], h[:, pq[0], pq[1]]) < 1e-4, pq
 
 
+_TRIANGLE = [(0, 0), (0, 1), (0, 2), (0, 3), (1, 1), (1, 2), (1, 3), (2, 2), (2, 3), (3, 3)]
+
+
+@pytest.mark.parametrize("names", [("M0",), ("T1", "M0"), ("T2", "M0"), ("T1", "T2", "M0"), ("T1", "T2", "B1", "M0")],
+                         ids=lambda n: "-".join(n))
+def test_normal_equations_for_every_column_set(names):
+    """Each set of fitted columns (one to four, B1 with a per-voxel map)
+    against the FP64 oracle; every slot past the fitted columns is exactly 0."""
+    from torchsim_amd.recon import FusedModel
+
+    voxels = 120
+    t1, t2 = _cases.random_tissue(voxels, seed=22)
+    rng = np.random.default_rng(23)
+    maps = {"T1": t1, "T2": t2, "M0": rng.uniform(0.5, 1.0, voxels).astype(np.float32),
+            "B1": rng.uniform(0.8, 1.2, voxels).astype(np.float32)}
+    lower = {"T1": "t1", "T2": "t2", "M0": "m0", "B1": "b1"}
+    seq = _mrf()
+    events = oracle.events_of(seq.packed(**seq.bound))
+    sig, tan = oracle.real(events, **{lower[k]: v for k, v in maps.items()}, states=10, outputs=880,
+                           tangents=[{lower[n]: 1.0} for n in names])
+    model, jac = -sig, -tan
+    data = (model + 0.01 * (rng.normal(size=model.shape) + 1j * rng.normal(size=model.shape))).astype(np.complex64)
+    r = model - data.astype(np.complex128)
+    cost = (np.abs(r) ** 2).sum(-1)
+    g = np.einsum("vps,vs->vp", jac, r.real)
+    h = np.einsum("vps,vqs->vpq", jac, jac)
+
+    known = {k: torch.tensor(v, device="cuda") for k, v in maps.items() if k not in names}
+    op = FusedModel(_mrf(), names, known=known)
+    x = torch.tensor(np.stack([maps[n] for n in names], -1), dtype=torch.float64, device="cuda")
+    normal = op.normal(x, torch.tensor(data).cuda(), torch.arange(voxels, device="cuda")).cpu().numpy()
+    count = len(names)
+    assert _cases.rel_err(normal[:, 0], cost) < 1e-3  # cost of a small residual: cancellation-limited
+    for p in range(4):
+        if p < count:
+            assert _cases.rel_err(normal[:, 1 + p], g[:, p]) < 1e-3, (names, p)
+        else:
+            assert not normal[:, 1 + p].any(), (names, p)
+    for k, (p, q) in enumerate(_TRIANGLE):
+        if q < count:
+            assert _cases.rel_err(normal[:, 5 + k], h[:, p, q]) < 1e-4, (names, p, q)
+        else:
+            assert not normal[:, 5 + k].any(), (names, p, q)
+
+
 def test_damped_solve_matches_numpy():
     from torchsim_amd.recon import solve_damped
 

@@ -172,3 +172,58 @@ def test_reference_property_names_reach_the_complex_kernel():
     want = P.simulate(ev, {"t1": t1, "t2": t2, "fraction": f, "exchange": k, "t1b": 400.0, "t2b": 15.0,
                            "shift": 30.0}, states=4, outputs=100)
     assert _cases.rel_err(theirs.cpu().numpy(), want.numpy()) < 1e-5
+
+
+def test_bssfp_sim_shapes_values_and_jacobian():
+    """bssfp_sim is the reference's closed form (reference
+    tests/functionals/test_bssfp_func.py shapes): FP32 on the GPU against the
+    FP64 fixed-point solve at TE = TR (oracle/bssfp.py), and its forward-mode
+    columns against FP64 central differences of the same closed form."""
+    from oracle import bssfp as oracle_bssfp
+    from torchsim_amd import bssfp_sim
+    from torchsim_amd.simulators.bssfp import steady_state
+
+    flip = np.linspace(5.0, 60.0, 100, dtype=np.float32)
+    dev = "cuda"
+    assert bssfp_sim(5.0, TE=2.0, TR=10.0, T1=1000.0, T2=100.0, device=dev).shape == ()
+    assert bssfp_sim(flip, TE=2.0, TR=10.0, T1=1000.0, T2=100.0, device=dev).shape == (100,)
+    assert bssfp_sim(flip, TE=2.0, TR=10.0, T1=(200, 500, 1000.0), T2=100.0, device=dev).shape == (3, 100)
+    assert bssfp_sim(flip, TE=2.0, TR=10.0, T1=1000.0, T2=100.0, diff="T1", device=dev)[1].shape == (100,)
+    assert bssfp_sim(flip, TE=2.0, TR=10.0, T1=(200, 500, 1000.0), T2=100.0, diff="T1",
+                     device=dev)[1].shape == (3, 100)
+    assert bssfp_sim(flip, TE=2.0, TR=10.0, T1=1000.0, T2=100.0, diff=("T1", "T2"), device=dev)[1].shape == (2, 100)
+    assert bssfp_sim(flip, TE=2.0, TR=10.0, T1=(200, 500, 1000.0), T2=100.0, diff=("T1", "T2"),
+                     device=dev)[1].shape == (3, 2, 100)
+
+    rng = np.random.default_rng(8)
+    t1 = rng.uniform(300, 2500, 16).astype(np.float32)
+    t2 = rng.uniform(20, 200, 16).astype(np.float32)
+    b0 = rng.uniform(-60, 60, 16).astype(np.float32)
+    tr = 6.0
+    sig = bssfp_sim(flip, TE=tr, TR=tr, T1=torch.tensor(t1, device=dev), T2=torch.tensor(t2, device=dev),
+                    B0=torch.tensor(b0, device=dev), phase_inc=180.0).cpu().numpy()
+    ref = np.array([[oracle_bssfp.steady_state(float(t1[v]), float(t2[v]), 1.0, float(b0[v]), 0.0, float(f), tr, tr,
+                                               180.0) for f in flip] for v in range(16)])
+    err = _cases.rel_err(sig, ref)
+    assert err < 1e-5, err
+
+    names = ("T1", "T2", "B0")
+    _, jac = bssfp_sim(flip, TE=2.5, TR=tr, T1=torch.tensor(t1, device=dev), T2=torch.tensor(t2, device=dev),
+                       B0=torch.tensor(b0, device=dev), phase_inc=90.0, diff=names)
+    jac = jac.cpu().numpy()
+    base = {"T1": t1.astype(np.float64), "T2": t2.astype(np.float64), "B0": b0.astype(np.float64)}
+
+    def closed(values):
+        t = lambda a: torch.tensor(a, dtype=torch.float64)[:, None]  # noqa: E731
+        return steady_state(t(values["T1"]), t(values["T2"]), torch.tensor(1.0, dtype=torch.float64), t(values["B0"]),
+                            torch.tensor(0.0, dtype=torch.float64), torch.tensor(flip, dtype=torch.float64),
+                            torch.tensor(tr, dtype=torch.float64), torch.tensor(2.5, dtype=torch.float64),
+                            torch.tensor(90.0, dtype=torch.float64)).numpy()
+
+    for p, name in enumerate(names):
+        step = 1e-4 * np.maximum(np.abs(base[name]), 1.0)
+        up = {**base, name: base[name] + step}
+        down = {**base, name: base[name] - step}
+        fd = (closed(up) - closed(down)) / (2 * step)[:, None]
+        err = _cases.rel_err(jac[:, p], fd)
+        assert err < 1e-4, (name, err)

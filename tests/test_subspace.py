@@ -126,3 +126,18 @@ def test_a_tall_dictionary_takes_the_gram_route_and_agrees():
     projector = subspace.basis.numpy().astype(np.complex128) @ subspace.basis.numpy().conj().T
     np.testing.assert_allclose(projector, basis @ basis.conj().T, atol=1e-4)
     np.testing.assert_allclose(subspace.basis.numpy().conj().T @ subspace.basis.numpy(), np.eye(6), atol=1e-5)
+
+
+def test_the_gram_matrix_summed_in_ragged_blocks_is_the_same(monkeypatch):
+    """The FP64 Gram matrix is summed one block of atoms at a time; blocks of
+    333 atoms over 2,000 (the last one ragged) give the basis of one block."""
+    from torchsim_amd import _subspace
+
+    signals = torch.from_numpy(_echo_train(count=2000).astype(np.complex64))
+    whole = Subspace.fit(signals, 6)
+    monkeypatch.setattr(_subspace, "_GRAM_BLOCK", 333)
+    blocked = Subspace.fit(signals, 6)
+    top = float(whole.singular_values[0])
+    torch.testing.assert_close(blocked.singular_values, whole.singular_values, rtol=0, atol=1e-6 * top)
+    projector = lambda b: b.to(torch.complex128) @ b.to(torch.complex128).mH  # noqa: E731
+    torch.testing.assert_close(projector(blocked.basis), projector(whole.basis), atol=1e-6, rtol=0)

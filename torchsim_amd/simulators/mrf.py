@@ -29,11 +29,11 @@ def _matched(value: Any, reference: torch.Tensor) -> torch.Tensor:
 
 
 class MRFSimulator(Simulator):
-    """An inversion, then a variable flip-angle train whose readouts wind on.
+    """Inversion-prepared MR fingerprinting over a train of N excitations.
 
-    Layout per repetition k: an excitation (flip_k, phase_k), a sample, one
-    unbalanced gradient, and the rest of TR_k; the returned signal is i times
-    the recorded sample.
+    Pulse k plays flip_k at phase_k; its echo is sampled, one spoiler gradient
+    advances every configuration by one order, and the spins relax for the
+    rest of TR_k. The returned signal is the recorded sample times i.
     """
 
     model = SpinPhysics(
