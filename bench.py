@@ -438,6 +438,16 @@ def serve_workload(duration_s: float = 60.0, rate: float = 50.0, seed: int = 0) 
 FP16_MFMA_PEAK_TFLOPS = 2500.0
 
 
+#: Algorithmic flops per configuration order per pulse of the exchanging
+#: pool with tangents along T1 and T2 (complex multiply 6, complex add 2,
+#: real-times-complex 2): relaxation of the value (F+ and F- through the 2x2
+#: complex transverse exponential, 28 each; Z through the real longitudinal
+#: one, 12) = 68; both tangents relaxed the same way (136) plus their sources,
+#: dX F+ and conj(dX) F- added for T2 (64) and dY Z added for T1 (16) = 216;
+#: the RF T-matrix on value and two tangents of both pools (3 x 2 x 54) = 324.
+TWO_POOL_FLOPS_PER_ORDER = 68 + 216 + 324
+
+
 def complex_workload(steps: int = 5, warmup: int = 2, atoms: int = 65536, pulses: int = 750) -> dict:
     """The complex kernel at a serving shape: a two-pool MRF protocol (fast
     fraction, exchange and pool-b relaxation per voxel), 65,536 atoms x 750
@@ -467,8 +477,13 @@ def complex_workload(steps: int = 5, warmup: int = 2, atoms: int = 65536, pulses
     torch.cuda.synchronize()
     wall = (time.perf_counter() - wall0) / steps
     work = atoms * pulses
+    gpu_ms = start.elapsed_time(stop) / steps
+    flops = work * TWO_POOL_FLOPS_PER_ORDER * 10
+    achieved = flops / (gpu_ms * 1e-3) / 1e12
     return {"metric": "atom*pulses/s (two-pool signal+Jacobian)", "value": work / wall, "unit": "atom*pulses/s",
-            "ms_per_step": wall * 1e3, "gpu_ms": start.elapsed_time(stop) / steps, "dtype": "f32",
+            "ms_per_step": wall * 1e3, "gpu_ms": gpu_ms, "dtype": "f32",
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP32_PEAK_TFLOPS, "flops_per_atom_pulse": TWO_POOL_FLOPS_PER_ORDER * 10},
             "config": {"workload": "two-pool MRF, complex kernel", "atoms": atoms, "pulses": pulses, "states": 10,
                        "columns": ["T1", "T2"]}}
 
@@ -590,6 +605,71 @@ def match_workload(steps: int = 3, warmup: int = 1, voxels: int = 65536) -> dict
             "parameters_recovered": hit}
 
 
+def config0_workload(repeats: int = 3, warmup_s: float = 2.0) -> dict:
+    """Plain ``mrf_sim`` at its CPU-runnable size: 1,000 atoms (T1 U[200,3000],
+    T2 U[20,300] with T2 < T1, seed 0) x the 880-pulse train, TR 10 ms, S = 10,
+    signal + d/d(T1, T2). Times the reference's own ``_epg_cpu`` kernel
+    (built from source into oracle/_ref) the way its Jacobian runs it -- one
+    simulate + simulate_jvp per property -- on one thread and on every host
+    core (2 s warm-up, best and median of ``repeats``), and, where a GPU is
+    visible, ``mrf_sim`` on it with the tissue already resident."""
+    import statistics
+
+    import torch
+
+    out: dict = {"metric": "seconds per mrf_sim signal + Jacobian (T1, T2), 1,000 atoms x 880 pulses, S=10",
+                 "unit": "s", "higher_is_better": False, "config": {"workload": "mrf_sim 1,000 x 880, S=10, d/d(T1,T2)"}}
+    t1, t2 = random_tissue(1000, seed=0)
+    train = flips()
+    try:
+        import oracle
+        from torchsim_amd.simulators import MRFSimulator
+
+        module = oracle.reference_module()
+    except Exception:  # pragma: no cover - a broken build is reported as absent
+        module = None
+    if module is not None:
+        events = oracle.events_of(MRFSimulator().packed(flip=torch.tensor(train, dtype=torch.float32), TR=10.0))
+        tissue = {"t1": t1, "t2": t2}
+
+        def once(threads):
+            for name in ("t1", "t2"):
+                oracle.reference_real(events, tissue, states=10, outputs=PULSES, features=0, threads=threads)
+                oracle.reference_real(events, tissue, states=10, outputs=PULSES, features=0, threads=threads,
+                                      tangent={name: np.ones(t1.size, dtype=np.float32)})
+
+        cores = len(os.sched_getaffinity(0))
+        for threads in sorted({1, cores}):
+            begin = time.perf_counter()
+            while time.perf_counter() - begin < warmup_s:
+                once(threads)
+            times = []
+            for _ in range(repeats):
+                start = time.perf_counter()
+                once(threads)
+                times.append(time.perf_counter() - start)
+            out[f"reference_cpu_{threads}_threads"] = {"best_s": min(times), "median_s": statistics.median(times),
+                                                        "atom_pulses_per_s": 1000 * PULSES / min(times)}
+    if torch.cuda.is_available():
+        from torchsim_amd import mrf_sim
+
+        dev = torch.device("cuda", 0)
+        T1, T2 = torch.tensor(t1, device=dev), torch.tensor(t2, device=dev)
+        flip = torch.tensor(train, dtype=torch.float32)
+        for _ in range(3):
+            mrf_sim(flip, 10.0, T1, T2, diff=("T1", "T2"))
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(max(repeats, 10)):
+            start = time.perf_counter()
+            mrf_sim(flip, 10.0, T1, T2, diff=("T1", "T2"))
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - start)
+        out["gpu"] = {"best_s": min(times), "median_s": statistics.median(times),
+                      "atom_pulses_per_s": 1000 * PULSES / min(times)}
+    return out
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__)
     parser.add_argument("--gpus", type=int, default=1)
@@ -600,7 +680,7 @@ def main() -> int:
                         help="state tile index, -1 for the library's cost model, or 'auto' (measured choice)")
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-torch-baseline", action="store_true")
-    parser.add_argument("--workload", choices=("dictionary", "design", "fit", "serve", "match", "complex", "subspace"), default="dictionary")
+    parser.add_argument("--workload", choices=("dictionary", "design", "fit", "serve", "match", "complex", "subspace", "config0"), default="dictionary")
     parser.add_argument("--serve-seconds", type=float, default=60.0)
     parser.add_argument(
         "--calibrate-copy",
@@ -621,6 +701,10 @@ def main() -> int:
         # many concurrent requests: more hardware queues than HIP's default 4
         # (read when the HIP runtime starts, so before torch or the library load)
         os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
+    if args.workload == "config0":
+        print(json.dumps(config0_workload()), flush=True)
+        return 0
 
     import torch
 
