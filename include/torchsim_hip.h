@@ -177,14 +177,22 @@ int tsim_simulate_projected(void* const* packed, int64_t atom_count, int64_t eve
  * signal_cotangent:   DEVICE (atom_count, output_count) FP32, or NULL.
  * jacobian_cotangent: DEVICE (atom_count, column_count, output_count), or NULL.
  * grad_flip, grad_duration: DEVICE (event_count) FP32, summed over atoms in a
- *           fixed order (deterministic); events with no rotation (no
- *           interval) get 0.
+ *           fixed order (deterministic); events with no rotation get a zero
+ *           flip gradient. An event's duration gradient is the derivative
+ *           along the interval before it, also where that interval is zero
+ *           (the reference's convention, _epg_cpu.cpp:6189-6214).
+ * duration_mask: DEVICE (event_count) bytes, or NULL. NULL computes every
+ *           event's duration gradient; otherwise an event whose interval is
+ *           zero gets its duration gradient only where its byte is nonzero
+ *           (0 elsewhere) -- a zero interval the caller will not
+ *           differentiate costs the kernel an op of its own, so a caller
+ *           passes the ones its loss reaches. Timed events always get theirs.
  * tissue_gradients: HOST array of tissue_gradient_count (17, tissue order)
  *           DEVICE pointers, each NULL or (atom_count) FP32 written with the
  *           gradient per atom; t1, t2, m0, b1 and inversion_efficiency are
  *           carried (another non-null entry is status 2); count 0 for none.
  * No diffusion, off-resonance or flow feature (status 2): those run on the
- * complex kernel, which has no adjoint.
+ * complex kernel, whose first-order adjoint is tsim_complex_vjp.
  */
 int tsim_simulate_vjp_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
                                int64_t state_count, int64_t output_count, int features,
@@ -192,7 +200,41 @@ int tsim_simulate_vjp_jacobian(void* const* packed, int64_t atom_count, int64_t 
                                const float* signal_cotangent, const float* jacobian_cotangent,
                                int64_t segment_ops, int64_t op_bound, float* grad_flip, float* grad_duration,
                                float* const* tissue_gradients, int64_t tissue_gradient_count,
-                               void* stream);
+                               const uint8_t* duration_mask, void* stream);
+
+/*
+ * First-order adjoint of the complex kernel (single pool or one exchanging
+ * pool, diffusion, off-resonance, RF and transmit phase): gradients of a
+ * scalar loss with respect to every event's flip, phase and duration and to
+ * each atom's tissue, given the loss's complex cotangent on the signal.
+ * Replaces the reference's `_epg_cpu.simulate_vjp` with real_axis -1
+ * (src/torchsim/_epg_cpu.cpp:4839-5944, entry :11150-11416, tissue reduction
+ * :11366-11395) that `_NativeEpg.backward` reaches for phase-cycled,
+ * off-resonant, exchanging or diffusing protocols
+ * (sequence/_accelerators.py:1544-1586). The state is checkpointed every
+ * segment_events events and recomputed per segment.
+ *
+ * signal_cotangent: DEVICE complex64 (atom_count, output_count): dL/dRe(s) +
+ *           i dL/dIm(s) of the output s = factor * m0 * F+[0] e^{-i phase}
+ *           (both pools' F+[0] summed), the public signal tsim_complex_jacobian
+ *           writes with the same factor.
+ * segment_events: events per checkpoint, 0 for about sqrt(event_count).
+ * grad_flip, grad_phase, grad_duration: DEVICE (event_count) FP32 or NULL,
+ *           summed over atoms in a fixed order (deterministic); an event
+ *           whose flip, phase or interval touches nothing gets 0.
+ * tissue_gradients: HOST array of tissue_gradient_count (17, tissue order)
+ *           DEVICE pointers, each NULL or (atom_count) FP32 written with the
+ *           gradient per atom; t1, t2, m0, b1, b1_phase, b0,
+ *           inversion_efficiency and diffusion (the scaled rate the kernel
+ *           reads) are carried, another non-null entry is status 2; count 0
+ *           for none. Other arguments as tsim_complex_jacobian.
+ * No flow feature (status 2).
+ */
+int tsim_complex_vjp(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
+                     int64_t output_count, int pool_kind, int features, int64_t tissue_stride, float factor_re,
+                     float factor_im, const void* signal_cotangent, int64_t segment_events, float* grad_flip,
+                     float* grad_phase, float* grad_duration, float* const* tissue_gradients,
+                     int64_t tissue_gradient_count, void* stream);
 
 /*
  * Gauss-Newton normal equations, fused: for each voxel, run the real-subspace

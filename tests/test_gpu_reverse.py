@@ -191,13 +191,20 @@ def test_crlb_tissue_gradients_match_the_reference_vjp_jvp():
         assert err < TOL, key
 
 
-def test_reverse_mode_on_the_complex_path_raises():
+def test_what_the_complex_adjoint_does_not_carry_raises():
+    """The complex path's adjoint is first order and reaches no pool-b
+    property: a Jacobian under reverse mode there, or a pool-b map that
+    requires grad, is refused rather than silently cut off."""
     from torchsim_amd.simulators import MRFSimulator
 
     t1 = torch.tensor([800.0, 1200.0], device="cuda", requires_grad=True)
-    with pytest.raises(NotImplementedError):
-        MRFSimulator(flip=torch.full((20,), 30.0), TR=10.0, phases=torch.linspace(0, 90, 20)).simulate(
-            T1=t1, T2=torch.tensor([60.0, 80.0], device="cuda"))
+    t2 = torch.tensor([60.0, 80.0], device="cuda")
+    seq = MRFSimulator(flip=torch.full((20,), 30.0), TR=10.0, phases=torch.linspace(0, 90, 20))
+    with pytest.raises(NotImplementedError, match="forward-over-reverse"):
+        seq.jacobian(("T1", "T2"), T1=t1, T2=t2)
+    fraction = torch.tensor([0.1, 0.2], device="cuda", requires_grad=True)
+    with pytest.raises(NotImplementedError, match="pool_b_fraction"):
+        seq.simulate(T1=t1, T2=t2, poolB_fraction=fraction, poolB_exchange=10.0, poolB_T1=400.0, poolB_T2=15.0)
 
 
 @pytest.mark.parametrize("protocol", ["mrf", "fse", "inverted_balanced"])
@@ -229,3 +236,70 @@ def test_host_op_count_sizes_the_adjoint_exactly(protocol):
     assert torch.equal(f0, f1) and torch.equal(d0, d1)
     assert torch.equal(g0["t1_ms"], g1["t1_ms"]) and torch.equal(g0["t2_ms"], g1["t2_ms"])
     assert float(g0["t1_ms"].abs().max()) > 0
+
+
+def test_an_inversion_time_of_zero_has_its_gradient():
+    """TI = 0 (the MRF default) leaves the first excitation's interval empty;
+    its duration gradient -- the pairing of the adjoint with the relaxation
+    generator there -- still reaches TI, as FP64 autograd says."""
+    from torchsim_amd.simulators import MRFSimulator
+
+    flips, trs, tissue = _inputs(60, 12, seed=51)
+    weight = np.random.default_rng(52).normal(size=(12, 60))
+    ti = torch.tensor(0.0, requires_grad=True)
+    maps = {k: torch.tensor(v, device="cuda") for k, v in tissue.items()}
+    maps["T1"].requires_grad_()
+    sig = MRFSimulator(states=10).simulate(flip=torch.tensor(flips, dtype=torch.float32),
+                                           TR=torch.tensor(trs, dtype=torch.float32), TI=ti, **maps)
+    (sig.real * torch.tensor(weight, dtype=torch.float32, device="cuda")).sum().backward()
+
+    ti64 = torch.tensor(0.0, dtype=torch.float64, requires_grad=True)
+    packed = MRFSimulator().packed(flip=torch.tensor(flips), TR=torch.tensor(trs), TI=ti64)
+    events = {"duration": packed.duration.double(), "flip": packed.flip.double(), "phase": packed.phase.detach(),
+              "kind": packed.kind, "action": packed.action, "output_index": packed.output_index}
+    ref = 1j * P.simulate(events, {_ORACLE_NAME[k]: torch.tensor(v, dtype=torch.float64) for k, v in tissue.items()},
+                          states=10, outputs=60)
+    (ref.real * torch.tensor(weight)).sum().backward()
+    assert float(ti64.grad) != 0.0
+    err = abs(float(ti.grad) - float(ti64.grad)) / abs(float(ti64.grad))
+    print(f"d/dTI: {float(ti.grad):.6e} vs FP64 {float(ti64.grad):.6e} ({err:.1e})")
+    assert err < TOL
+
+
+def test_every_duration_gradient_matches_the_reference_vjp():
+    """With every zero interval asked for (the reference's convention), each
+    event's duration gradient -- the sampled and crusher events' included --
+    is the reference's simulate_vjp's, first order and forward-over-reverse."""
+    if oracle.reference_module() is None:
+        pytest.skip("oracle/_ref is not built")
+    from torchsim_amd.sequence import _engine
+    from torchsim_amd.simulators import MRFSimulator
+
+    flips, trs, tissue = _inputs(80, 16, seed=53)
+    packed = MRFSimulator().packed(flip=torch.tensor(flips, dtype=torch.float32),
+                                   TR=torch.tensor(trs, dtype=torch.float32))
+    events = oracle.events_of(packed)
+    t1, t2 = tissue["T1"], tissue["T2"]
+    prepared = _engine.prepare_tissue({"t1_ms": torch.tensor(t1), "t2_ms": torch.tensor(t2)},
+                                      torch.device("cuda", 0), ("t1_ms", "t2_ms"))
+    rng = np.random.default_rng(54)
+    ws = rng.normal(size=(16, 80))
+    wj = rng.normal(size=(16, 1, 80))
+    # first order
+    _, gd, _ = _engine.adjoint_real(packed, prepared, states=10, columns=(), flip=packed.flip,
+                                    duration=packed.duration, signal_cotangent=torch.tensor(ws),
+                                    jacobian_cotangent=None, zero_intervals=True)
+    ref = oracle.reference_vjp(events, {"t1": t1, "t2": t2}, 1j * ws, states=10, outputs=80)
+    assert np.count_nonzero(ref["duration"][np.asarray(events["duration"]) == 0]) > 0
+    err = _cases.rel_err(gd.cpu().numpy(), ref["duration"])
+    print(f"first-order duration gradient vs reference: {err:.2e}")
+    assert err < 1e-4
+    # forward-over-reverse along T1
+    _, gd, _ = _engine.adjoint_real(packed, prepared, states=10, columns=("t1_ms",), flip=packed.flip,
+                                    duration=packed.duration, signal_cotangent=None,
+                                    jacobian_cotangent=torch.tensor(wj), zero_intervals=True)
+    ref = oracle.reference_vjp(events, {"t1": t1, "t2": t2}, 1j * wj[:, 0], states=10, outputs=80,
+                               tangent={"t1": 1.0})
+    err = _cases.rel_err(gd.cpu().numpy(), ref["duration"])
+    print(f"forward-over-reverse duration gradient vs reference: {err:.2e}")
+    assert err < 1e-4

@@ -58,7 +58,7 @@ _F32 = ctypes.c_float
 
 #: Every symbol include/torchsim_hip.h declares, with its ctypes signature.
 #: ABI revision these bindings are written against (``tsim_version``).
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "tsim_last_error": (ctypes.c_char_p, []),
@@ -144,6 +144,7 @@ SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
             ctypes.POINTER(_P),
             _I64,
             _P,
+            _P,
         ],
     ),
     "tsim_gauss_newton_normal": (
@@ -188,6 +189,11 @@ class BatchRequest(ctypes.Structure):
     ]
 
 
+SIGNATURES["tsim_complex_vjp"] = (
+    _I32,
+    [ctypes.POINTER(_P), _I64, _I64, _I64, _I64, _I32, _I32, _I64, _F32, _F32, _P, _I64, _P, _P, _P,
+     ctypes.POINTER(_P), _I64, _P],
+)
 SIGNATURES["tsim_simulate_batch"] = (_I32, [ctypes.POINTER(BatchRequest), _I64, _I32, _I32, _I64, _P])
 SIGNATURES["tsim_batch_request_size"] = (_I32, [])
 

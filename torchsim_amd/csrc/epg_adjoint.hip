@@ -681,7 +681,8 @@ extern "C" int tsim_simulate_vjp_jacobian(void* const* packed, int64_t atom_coun
                                           int64_t column_count, const float* signal_cotangent,
                                           const float* jacobian_cotangent, int64_t segment_ops, int64_t op_bound,
                                           float* grad_flip, float* grad_duration, float* const* tissue_gradients,
-                                          int64_t tissue_gradient_count, void* stream) {
+                                          int64_t tissue_gradient_count, const uint8_t* duration_mask,
+                                          void* stream) {
   if (packed == nullptr || packed[T_T1] == nullptr || packed[T_T2] == nullptr) {
     set_error("packed buffers with t1 and t2 are required");
     return BAD_ARGUMENT;
@@ -791,7 +792,8 @@ extern "C" int tsim_simulate_vjp_jacobian(void* const* packed, int64_t atom_coun
   if (st != OK) return st;
   MacroOp* ops = (MacroOp*)head;
   int64_t* op_count_device = (int64_t*)(head + sizeof(MacroOp) * (size_t)capacity);
-  st = pack_stream(compiled.codes, event_count, output_count, ops, s, op_count_device);
+  st = pack_stream(compiled.codes, event_count, output_count, ops, s, op_count_device, duration_mask,
+                   duration_mask ? 2 : 1);
   if (st == OK && op_bound < 0) {
     int64_t counted = 0;
     st = hip_status(hipMemcpyAsync(&counted, op_count_device, sizeof(int64_t), hipMemcpyDeviceToHost, s),

@@ -110,8 +110,11 @@ int compile_ops(int64_t event_count, int64_t outputs, Compiled& out);
 
 // The same grouping into caller-provided ops (op_capacity(events) of them);
 // with op_count, the number of ops (end marker excluded) is written there.
+// zero_mode 1 gives every event with a zero interval a relaxation part at
+// dt = 0, 2 each event flagged in zero_mask (DEVICE, one byte per event);
+// the adjoint then returns those intervals' duration gradients.
 int pack_stream(const EventCode* codes, int64_t events, int64_t outputs, MacroOp* ops, hipStream_t stream,
-                int64_t* op_count = nullptr);
+                int64_t* op_count = nullptr, const uint8_t* zero_mask = nullptr, int zero_mode = 0);
 
 void set_error(const char* fmt, ...);
 const char* last_error();

@@ -135,12 +135,15 @@ __device__ __forceinline__ PartSummary combine(const PartSummary& x, const PartS
 }
 
 // Call f(part, position) for each part of event `ev` in order; part is one
-// of OP_RELAX, OP_SHIFT, OP_ROTATE, OP_INVERT, OP_RECORD, OP_SPOIL.
+// of OP_RELAX, OP_SHIFT, OP_ROTATE, OP_INVERT, OP_RECORD, OP_SPOIL. With
+// zero_interval an event whose interval is zero still gets its relaxation
+// part (an identity at dt = 0), so the adjoint returns that interval's
+// duration gradient.
 template <typename F>
-__device__ __forceinline__ void for_each_part(const EventCode& ev, F&& f) {
+__device__ __forceinline__ void for_each_part(const EventCode& ev, bool zero_interval, F&& f) {
   const uint32_t act = ev.ctrl & 0xFFu;
   const int32_t kind = (int32_t)(ev.ctrl >> CTRL_KIND_SHIFT);
-  if (ev.dt != 0.0f) f(OP_RELAX, 0);
+  if (ev.dt != 0.0f || zero_interval) f(OP_RELAX, 0);
   if (act & PRE_SHIFT) f(OP_SHIFT, 3);
   if (kind == KIND_RF) {
     f((act & INVERSION) ? OP_INVERT : OP_ROTATE, 1);
@@ -155,9 +158,12 @@ __device__ __forceinline__ void for_each_part(const EventCode& ev, F&& f) {
   }
 }
 
+// zero_mode: 0 no zero-interval relaxation parts (forward kernels), 1 one
+// for every event, 2 one for each event flagged in zero_mask.
 __global__ void __launch_bounds__(kPackThreads) pack_ops(const EventCode* __restrict__ codes, int64_t events,
                                                          int64_t outputs, MacroOp* __restrict__ ops,
-                                                         int64_t* __restrict__ op_count) {
+                                                         int64_t* __restrict__ op_count,
+                                                         const uint8_t* __restrict__ zero_mask, int zero_mode) {
   __shared__ int s_has[kPackThreads];
   __shared__ int s_first[kPackThreads];
   __shared__ int s_last[kPackThreads];
@@ -172,8 +178,11 @@ __global__ void __launch_bounds__(kPackThreads) pack_ops(const EventCode* __rest
 
   PartSummary mine{0, 0, 0, 0};
   int64_t my_recs = 0;
+  auto zero_interval = [&](int64_t e) -> bool {
+    return zero_mode == 1 || (zero_mode == 2 && zero_mask[e] != 0);
+  };
   for (int64_t e = lo; e < hi; ++e) {
-    for_each_part(codes[e], [&](uint32_t part, int pos) {
+    for_each_part(codes[e], zero_interval(e), [&](uint32_t part, int pos) {
       if (!mine.has) {
         mine.has = 1;
         mine.first = pos;
@@ -238,7 +247,7 @@ __global__ void __launch_bounds__(kPackThreads) pack_ops(const EventCode* __rest
   int64_t index = before.has ? before.cuts : 0;
   for (int64_t e = lo; e < hi; ++e) {
     const EventCode ev = codes[e];
-    for_each_part(ev, [&](uint32_t part, int pos) {
+    for_each_part(ev, zero_interval(e), [&](uint32_t part, int pos) {
       if (has && cut_between(last, pos)) ++index;
       has = 1;
       last = pos;
@@ -272,14 +281,15 @@ int compile_ops(int64_t event_count, int64_t outputs, Compiled& out) {
   int st = scratch_alloc((void**)&out.ops, sizeof(MacroOp) * (size_t)capacity, out.stream, "macro-ops");
   if (st != OK) return st;
   hipLaunchKernelGGL(pack_ops, dim3(1), dim3(kPackThreads), 0, out.stream, out.codes,
-                     event_count > 0 ? event_count : 0, outputs, out.ops, (int64_t*)nullptr);
+                     event_count > 0 ? event_count : 0, outputs, out.ops, (int64_t*)nullptr,
+                     (const uint8_t*)nullptr, 0);
   return hip_status(hipGetLastError(), "pack_ops launch");
 }
 
 int pack_stream(const EventCode* codes, int64_t events, int64_t outputs, MacroOp* ops, hipStream_t stream,
-                int64_t* op_count) {
+                int64_t* op_count, const uint8_t* zero_mask, int zero_mode) {
   hipLaunchKernelGGL(pack_ops, dim3(1), dim3(kPackThreads), 0, stream, codes, events > 0 ? events : 0, outputs,
-                     ops, op_count);
+                     ops, op_count, zero_mask, zero_mode);
   return hip_status(hipGetLastError(), "pack_ops launch");
 }
 
@@ -809,7 +819,7 @@ extern "C" {
 
 const char* tsim_last_error(void) { return tsim::last_error(); }
 
-int tsim_version(void) { return 2; }
+int tsim_version(void) { return 3; }
 
 int tsim_simulate(void* const* pointers, int64_t pointer_count, int64_t atom_count,
                   int64_t train_count, int64_t event_count, int64_t state_count,

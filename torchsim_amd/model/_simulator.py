@@ -58,7 +58,8 @@ PUBLIC_PROPERTIES = {
 #: and the gradient geometry a diffusion coefficient is read through
 #: (reference sequence/_accelerators.py:371-420).
 RUN_SETTINGS = (
-    "states", "repetitions", "device", "tile", "checkpoint_pulses", "crusher_dephasing_rad", "voxel_size_m",
+    "states", "repetitions", "device", "tile", "checkpoint_pulses", "checkpoint_events", "crusher_dephasing_rad",
+    "voxel_size_m",
 )
 
 #: The apparent diffusion coefficient is given in um^2/ms; the b-factor wants m^2/s.
@@ -304,14 +305,35 @@ class Simulator:
             signal, jacobian = self._run(held, sequence, diff)
             return subspace.project(signal), (None if jacobian is None else subspace.project(jacobian))
         sequence_grad = packed.flip.requires_grad or packed.duration.requires_grad
-        if not real:
-            if sequence_grad or reversed_fields:
-                raise NotImplementedError(
-                    "reverse mode runs on the real-subspace kernel: this protocol or tissue (phase-cycled or "
-                    "off-resonant pulses, an exchanging pool) takes the complex kernel, which has no adjoint"
-                )
+        reverse = sequence_grad or bool(reversed_fields) or packed.phase.requires_grad
+        # the real-subspace adjoint carries flips, durations and five tissue
+        # fields; a phase with a gradient leaves the one axis, and any other
+        # field (B0, transmit phase, diffusion) takes the complex adjoint
+        real_reverse = (real and not packed.phase.requires_grad
+                        and all(name in _engine.GRADIENT_FIELDS for name in reversed_fields))
+        if not real or (reverse and not real_reverse):
             from ..sequence import _complex
 
+            if reverse:
+                if differentiated:
+                    raise NotImplementedError(
+                        "a Jacobian under reverse mode (forward-over-reverse) runs on the real-subspace kernel "
+                        "for T1, T2, M0, B1 and the inversion efficiency; this protocol or tissue takes the "
+                        "complex kernel, whose adjoint is first order"
+                    )
+                unsupported = [name for name in reversed_fields if name not in _complex.GRADIENT_FIELDS]
+                if unsupported:
+                    raise NotImplementedError(
+                        f"the complex adjoint reaches {', '.join(_complex.GRADIENT_FIELDS)} and the sequence; "
+                        f"{unsupported} also require grad"
+                    )
+                buffers = [tissue.buffers[_native.TISSUE_INDEX[name]] if name in reversed_fields else None
+                           for name in _complex.GRADIENT_FIELDS]
+                signal = _complex.FusedComplexSignal.apply(
+                    packed.flip.to(device), packed.phase.to(device), packed.duration.to(device), *buffers, packed,
+                    tissue, states, factor, int(sequence.get("checkpoint_events", 0)),
+                )
+                return signal, None
             return _complex.simulate_complex(packed, tissue, states=states, columns=differentiated, factor=factor)
         if sequence_grad or reversed_fields:
             unsupported = [name for name in reversed_fields if name not in _engine.GRADIENT_FIELDS]
@@ -326,6 +348,7 @@ class Simulator:
             signal, jacobian = _engine.FusedRealJacobian.apply(
                 flip_in.to(device), packed.duration.to(device), *buffers, packed, tissue, states, differentiated,
                 factor, samples, int(sequence.get("checkpoint_pulses", 64)),
+                _engine.zero_intervals_reached(packed.duration),
             )
             return signal, (jacobian if differentiated else None)
         return _engine.simulate_real(

@@ -371,7 +371,59 @@ def _tuned_tile(packed: PackedEvents, tissue: PreparedTissue, *, states: int, co
 GRADIENT_FIELDS = ("t1_ms", "t2_ms", "m0", "b1", "inversion_efficiency")
 
 
-def op_count(packed: PackedEvents) -> int:
+def _graph_leaves(tensor: torch.Tensor) -> list[torch.Tensor]:
+    """The leaf tensors requiring grad that ``tensor``'s graph reaches."""
+    # the visited nodes are held so that their ids stay unique
+    leaves, held, seen, todo = [], [], set(), [tensor.grad_fn]
+    while todo:
+        node = todo.pop()
+        if node is None or id(node) in seen:
+            continue
+        held.append(node)
+        seen.add(id(node))
+        variable = getattr(node, "variable", None)
+        if variable is not None:
+            leaves.append(variable)
+        todo.extend(parent for parent, _ in node.next_functions)
+    return leaves
+
+
+def zero_intervals_reached(duration: torch.Tensor) -> np.ndarray:
+    """Which events have a zero interval whose value the caller's graph
+    reaches (a TI of 0 that requires grad, say): those need a duration
+    gradient although their interval is empty. Found as the nonzero entries
+    of J w, J the Jacobian of the durations with respect to the graph's
+    leaves and w positive random weights, by one double backward through
+    the packing; if that cannot run, every zero interval is taken."""
+    values = duration.detach()
+    zero = (values == 0).reshape(-1).cpu().numpy()
+    if not duration.requires_grad or not zero.any():
+        return np.zeros(values.numel(), dtype=bool)
+    leaves = _graph_leaves(duration)
+    if not leaves:
+        return np.zeros(values.numel(), dtype=bool)
+    probe = torch.zeros_like(duration, requires_grad=True)
+    try:
+        grads = torch.autograd.grad(duration, leaves, grad_outputs=probe, create_graph=True, allow_unused=True)
+        generator = torch.Generator().manual_seed(0)
+        total = None
+        for grad in grads:
+            if grad is None or not grad.requires_grad:
+                continue
+            weight = 0.5 + torch.rand(grad.shape, generator=generator, dtype=torch.float64)
+            term = (grad * weight.to(device=grad.device, dtype=grad.dtype)).sum()
+            total = term if total is None else total + term
+        if total is None:
+            return np.zeros(values.numel(), dtype=bool)
+        (reach,) = torch.autograd.grad(total, probe, retain_graph=True, allow_unused=True)
+    except RuntimeError:
+        return zero
+    if reach is None:
+        return np.zeros(values.numel(), dtype=bool)
+    return zero & (reach.detach().reshape(-1).cpu().numpy() != 0)
+
+
+def op_count(packed: PackedEvents, zero_intervals: np.ndarray | None = None) -> int:
     """How many macro-ops the library groups the stream into: each event's
     parts -- its interval (position 0, if nonzero), a pre-shift (3), its
     rotation or inversion (1) or record (2), a post-shift (3), a spoil (1) or
@@ -381,6 +433,8 @@ def op_count(packed: PackedEvents) -> int:
     kind = packed.kind.cpu().numpy()
     action = packed.action.cpu().numpy().astype(np.int32)
     timed = packed.duration.detach().cpu().numpy() != 0
+    if zero_intervals is not None:
+        timed = timed | zero_intervals
     position = np.full((kind.size, 5), -1, dtype=np.int8)
     position[timed, 0] = 0
     position[(action & 1) != 0, 1] = 3
@@ -409,6 +463,7 @@ def adjoint_real(
     checkpoint_pulses: int = 64,
     tissue_wanted: tuple[str, ...] = (),
     op_bound: int | None = None,
+    zero_intervals: np.ndarray | bool | None = None,
 ) -> tuple[torch.Tensor, torch.Tensor, dict[str, torch.Tensor]]:
     """d(loss)/d(flip) and d(loss)/d(duration), one entry per event, and
     d(loss)/d(field) per atom for each field of ``tissue_wanted`` (among
@@ -417,7 +472,10 @@ def adjoint_real(
     state is checkpointed every ``checkpoint_pulses`` macro-ops (pulses of an
     MRF train) and recomputed per segment. ``op_bound`` sizes the workspace
     (default: the stream's op count, worked out on the host; -1 has the
-    library read it back from the device)."""
+    library read it back from the device). ``zero_intervals`` flags the
+    events with a zero interval whose duration gradient is wanted (None:
+    none of them, True: all of them, the reference's convention; every timed
+    event gets its own)."""
     device = tissue.device
     _native.require_device(device)
     lib = _native.load()
@@ -438,6 +496,11 @@ def adjoint_real(
     cols = (ctypes.c_int32 * max(1, len(columns)))(*[_COLUMN_INDEX[name] for name in columns])
     ws = None if signal_cotangent is None else signal_cotangent.to(device=device, dtype=torch.float32).contiguous()
     wj = None if jacobian_cotangent is None else jacobian_cotangent.to(device=device, dtype=torch.float32).contiguous()
+    every = zero_intervals is True
+    if every:
+        zero_intervals = packed.duration.detach().cpu().numpy() == 0
+    flags = np.zeros(count, dtype=np.uint8) if zero_intervals is None else zero_intervals.astype(np.uint8)
+    mask = torch.from_numpy(flags).to(device)
     with torch.cuda.device(device):
         status = lib.tsim_simulate_vjp_jacobian(
             ptrs,
@@ -452,11 +515,12 @@ def adjoint_real(
             None if ws is None else ws.data_ptr(),
             None if wj is None else wj.data_ptr(),
             checkpoint_pulses,
-            op_count(packed) if op_bound is None else op_bound,
+            op_count(packed, zero_intervals) if op_bound is None else op_bound,
             grad_flip.data_ptr(),
             grad_duration.data_ptr(),
             slots,
             _native.TISSUE_COUNT if grads else 0,
+            None if every else mask.data_ptr(),
             _native.stream_of(device),
         )
     _native.check(status)
@@ -474,7 +538,7 @@ class FusedRealJacobian(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, flip, duration, t1, t2, m0, b1, ie, packed, tissue, states, columns, factor, samples,
-                checkpoint_pulses):
+                checkpoint_pulses, zero_intervals):
         signal, jacobian = simulate_real(
             packed, tissue, states=states, columns=columns, factor=factor, flip=flip,
             samples=samples, duration=duration,
@@ -488,6 +552,7 @@ class FusedRealJacobian(torch.autograd.Function):
         ctx.factor = complex(factor)
         ctx.samples = samples
         ctx.checkpoint_pulses = checkpoint_pulses
+        ctx.zero_intervals = zero_intervals
         if jacobian is None:
             jacobian = torch.empty((tissue.atoms, 0, packed.output_count), dtype=torch.complex64, device=signal.device)
         return signal, jacobian
@@ -495,7 +560,7 @@ class FusedRealJacobian(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_signal, grad_jacobian):
         flip, duration = ctx.saved_tensors
-        nothing = (None,) * 14
+        nothing = (None,) * 15
         # every output is c * v with v real and c = factor * i (* sample sign):
         # the loss's derivative along v is Re(conj(c) * grad).
         c = torch.tensor(ctx.factor * 1j, dtype=torch.complex64, device=flip.device)
@@ -512,7 +577,7 @@ class FusedRealJacobian(torch.autograd.Function):
         grad_flip, grad_duration, grads = adjoint_real(
             ctx.packed, ctx.tissue, states=ctx.states, columns=ctx.columns, flip=flip, duration=duration,
             signal_cotangent=ws, jacobian_cotangent=wj, checkpoint_pulses=ctx.checkpoint_pulses,
-            tissue_wanted=wanted,
+            tissue_wanted=wanted, zero_intervals=ctx.zero_intervals if ctx.needs_input_grad[1] else None,
         )
         tissue_out = []
         for name, shape in zip(GRADIENT_FIELDS, ctx.shapes):
@@ -524,5 +589,5 @@ class FusedRealJacobian(torch.autograd.Function):
             grad_flip.to(flip.dtype) if ctx.needs_input_grad[0] else None,
             grad_duration.to(duration.dtype) if ctx.needs_input_grad[1] else None,
             *tissue_out,
-            None, None, None, None, None, None, None,
+            None, None, None, None, None, None, None, None,
         )
