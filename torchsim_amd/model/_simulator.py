@@ -348,7 +348,7 @@ class Simulator:
             signal, jacobian = _engine.FusedRealJacobian.apply(
                 flip_in.to(device), packed.duration.to(device), *buffers, packed, tissue, states, differentiated,
                 factor, samples, int(sequence.get("checkpoint_pulses", 64)),
-                _engine.zero_intervals_reached(packed.duration),
+                _engine.zero_intervals_reached(packed) if packed.duration.requires_grad else None,
             )
             return signal, (jacobian if differentiated else None)
         return _engine.simulate_real(

@@ -388,13 +388,18 @@ def _graph_leaves(tensor: torch.Tensor) -> list[torch.Tensor]:
     return leaves
 
 
-def zero_intervals_reached(duration: torch.Tensor) -> np.ndarray:
+def zero_intervals_reached(packed: PackedEvents) -> np.ndarray:
     """Which events have a zero interval whose value the caller's graph
     reaches (a TI of 0 that requires grad, say): those need a duration
-    gradient although their interval is empty. Found as the nonzero entries
-    of J w, J the Jacobian of the durations with respect to the graph's
-    leaves and w positive random weights, by one double backward through
-    the packing; if that cannot run, every zero interval is taken."""
+    gradient although their interval is empty. A packer that knows says so
+    on the stream; otherwise they are the nonzero entries of J w, J the
+    Jacobian of the durations with respect to the graph's leaves and w
+    positive random weights, found by one double backward through the
+    packing; if that cannot run, every zero interval is taken."""
+    known = packed.__dict__.get("_zero_intervals")
+    if known is not None:
+        return known
+    duration = packed.duration
     values = duration.detach()
     zero = (values == 0).reshape(-1).cpu().numpy()
     if not duration.requires_grad or not zero.any():
@@ -450,6 +455,19 @@ def op_count(packed: PackedEvents, zero_intervals: np.ndarray | None = None) -> 
     return 1 + int(np.count_nonzero((parts[1:] <= parts[:-1]) | (parts[:-1] == 3)))
 
 
+_NO_ZERO_INTERVALS: dict = {}
+
+
+def _no_zero_intervals(count: int, device: torch.device) -> torch.Tensor:
+    """A resident all-zero duration mask of at least ``count`` bytes."""
+    key = str(device)
+    held = _NO_ZERO_INTERVALS.get(key)
+    if held is None or held.numel() < count:
+        held = torch.zeros(max(count, 4096), dtype=torch.uint8, device=device)
+        _NO_ZERO_INTERVALS[key] = held
+    return held
+
+
 def adjoint_real(
     packed: PackedEvents,
     tissue: PreparedTissue,
@@ -499,8 +517,10 @@ def adjoint_real(
     every = zero_intervals is True
     if every:
         zero_intervals = packed.duration.detach().cpu().numpy() == 0
-    flags = np.zeros(count, dtype=np.uint8) if zero_intervals is None else zero_intervals.astype(np.uint8)
-    mask = torch.from_numpy(flags).to(device)
+    if zero_intervals is None or not zero_intervals.any():
+        mask = _no_zero_intervals(count, device)
+    else:
+        mask = torch.from_numpy(zero_intervals.astype(np.uint8)).to(device)
     with torch.cuda.device(device):
         status = lib.tsim_simulate_vjp_jacobian(
             ptrs,

@@ -114,14 +114,21 @@ class MRFSimulator(Simulator):
         clock = np.concatenate([np.zeros(1, dtype=np.float32), np.repeat(stamps, 4)])
         intervals = np.diff(clock, prepend=np.float32(0.0)) * np.float32(1e-6)
         duration = torch.from_numpy(intervals.astype(np.float32))
+        # the zero intervals a graph reaches: only the TRs carry one, onto the
+        # excitations after the first
+        reached = np.zeros(1 + 4 * n, dtype=bool)
         if isinstance(tr, torch.Tensor) and tr.requires_grad:
+            carried = pulse[1:] + turn_at
             live = torch.zeros(1 + 4 * n, dtype=torch.float32).index_put(
-                (pulse[1:] + turn_at,), (spacing_s[:-1] - spacing_s[:-1].detach()).to(torch.float32))
+                (carried,), (spacing_s[:-1] - spacing_s[:-1].detach()).to(torch.float32))
             duration = duration + live
-        return PackedEvents(duration=duration, kind=tiled("kind"), flip=flips, phase=tiled("phase"),
-                            action=tiled("action"), output_index=output_index, shim_index=tiled("shim_index"),
-                            saturation=tiled("saturation"), rf_frequency_hz=tiled("rf_frequency_hz"),
-                            output_count=n)
+            reached[carried.numpy()] = intervals[carried.numpy()] == 0
+        packed = PackedEvents(duration=duration, kind=tiled("kind"), flip=flips, phase=tiled("phase"),
+                              action=tiled("action"), output_index=output_index, shim_index=tiled("shim_index"),
+                              saturation=tiled("saturation"), rf_frequency_hz=tiled("rf_frequency_hz"),
+                              output_count=n)
+        packed.__dict__["_zero_intervals"] = reached
+        return packed
 
     def _pulse_block(self, sequence: dict) -> tuple | None:
         """(head, one pulse's four events, the excitation's and the sample's
