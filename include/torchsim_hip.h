@@ -237,6 +237,31 @@ int tsim_complex_vjp(void* const* packed, int64_t atom_count, int64_t event_coun
                      int64_t tissue_gradient_count, void* stream);
 
 /*
+ * Forward mode along one arbitrary direction: the signal and its directional
+ * derivative for a tangent on any carried tissue property and on every
+ * event's duration, flip and phase. Replaces the reference's
+ * `_epg_cpu.simulate_jvp` (src/torchsim/_epg_cpu.cpp:2469-3005 complex,
+ * 3136-3285 real, entry :11017-11148) that `_NativeEpg.jvp` reaches
+ * (sequence/_accelerators.py:1511-1542); runs the complex state machine,
+ * which is exact on real-subspace inputs too.
+ *
+ * tangents: HOST array of tangent_count (20, or 0 for none) DEVICE pointers in
+ *           the reference's JVP order: the 17 tissue tangents (tissue order;
+ *           t1 and t2 one value per atom, the others as tissue_stride says),
+ *           then duration, flip, phase (one value per event); each NULL for
+ *           zero. t1, t2, m0, b1, b1_phase, b0, inversion_efficiency and
+ *           diffusion are carried; another non-null tissue tangent is
+ *           status 2.
+ * signal:   complex64 (atom_count, output_count) = factor * sample, or NULL.
+ * tangent:  complex64 (atom_count, output_count), its derivative.
+ * Other arguments as tsim_complex_jacobian.
+ */
+int tsim_complex_jvp(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
+                     int64_t output_count, int pool_kind, int features, int64_t tissue_stride, float factor_re,
+                     float factor_im, const float* const* tangents, int64_t tangent_count, void* signal,
+                     void* tangent, void* stream);
+
+/*
  * Gauss-Newton normal equations, fused: for each voxel, run the real-subspace
  * signal and its Jacobian along `columns` and accumulate, at every sample, in
  * FP64: the cost sum |model - data|^2, the gradient J^T r and the upper

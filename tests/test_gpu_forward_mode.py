@@ -1,0 +1,273 @@
+"""Forward mode along arbitrary directions: ``torch.func.jvp`` and
+``torch.autograd.forward_ad`` through ``Simulator.simulate``, along mixed
+tissue directions (real-subspace and complex protocols) and along the
+sequence's flips, TRs and phases, against the same directional derivative of
+the FP64 torch oracle (oracle/packed.py, ``torch.func.jvp``); and the C-ABI
+JVP (tsim_complex_jvp, the reference's ``simulate_jvp`` with its 20 tangent
+slots) against the reference's own FP32 ``simulate_jvp`` where oracle/_ref is
+built (reference sequence/_accelerators.py:1511-1542,
+src/torchsim/_epg_cpu.cpp:2469-3005, 11017-11148)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+from torch.autograd import forward_ad
+
+import oracle
+from oracle import packed as P
+from tests import _cases
+
+pytestmark = pytest.mark.gpu
+TOL = 2e-5  # max|d - d_ref| / max|d_ref|
+
+
+def _train(pulses, seed):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(5.0, 60.0, pulses), rng.uniform(8.0, 15.0, pulses), rng.uniform(0.0, 360.0, pulses)
+
+
+def _events64(flip, tr, phases=None):
+    from torchsim_amd.simulators import MRFSimulator
+
+    kwargs = {} if phases is None else {"phases": phases}
+    packed = MRFSimulator().packed(flip=flip, TR=tr, **kwargs)
+    return {"duration": packed.duration.double(), "flip": packed.flip.double(), "phase": packed.phase.double(),
+            "kind": packed.kind, "action": packed.action, "output_index": packed.output_index}
+
+
+_ORACLE = {"T1": "t1", "T2": "t2", "M0": "m0", "B1": "b1", "inv_efficiency": "ie", "B0": "b0", "B1phase": "b1_phase"}
+
+
+def _tissue(atoms, seed, names):
+    rng = np.random.default_rng(seed)
+    t1, t2 = _cases.random_tissue(atoms, seed=seed)
+    pool = {"T1": t1, "T2": t2, "M0": rng.uniform(0.5, 1.0, atoms), "B1": rng.uniform(0.7, 1.3, atoms),
+            "inv_efficiency": rng.uniform(0.8, 1.0, atoms), "B0": rng.uniform(-30, 30, atoms),
+            "B1phase": rng.uniform(-0.5, 0.5, atoms)}
+    return {k: pool[k].astype(np.float32) for k in names}
+
+
+@pytest.mark.parametrize("phased", [False, True], ids=["real_subspace", "phase_cycled"])
+def test_func_jvp_along_a_mixed_tissue_direction(phased):
+    """torch.func.jvp of simulate along random per-atom directions in every
+    tissue property at once; the real-subspace protocol and a phase-cycled
+    one with B0 and a transmit phase."""
+    from torchsim_amd.simulators import MRFSimulator
+
+    atoms, pulses = 20, 80
+    names = ["T1", "T2", "M0", "B1", "inv_efficiency"] + (["B0", "B1phase"] if phased else [])
+    tissue = _tissue(atoms, 11, names)
+    rng = np.random.default_rng(12)
+    scales = {"T1": 100.0, "T2": 10.0, "M0": 0.1, "B1": 0.1, "inv_efficiency": 0.05, "B0": 5.0, "B1phase": 0.1}
+    direction = {k: (scales[k] * rng.normal(size=atoms)).astype(np.float32) for k in names}
+    flips, trs, phases = _train(pulses, 13)
+    seq = {"flip": torch.tensor(flips, dtype=torch.float32), "TR": torch.tensor(trs, dtype=torch.float32)}
+    if phased:
+        seq["phases"] = torch.tensor(phases, dtype=torch.float32)
+    sim = MRFSimulator(states=10)
+
+    def run(*values):
+        return sim.simulate(**seq, **dict(zip(names, values)))
+
+    primals = tuple(torch.tensor(tissue[k], device="cuda") for k in names)
+    tangents = tuple(torch.tensor(direction[k], device="cuda") for k in names)
+    _, derivative = torch.func.jvp(run, primals, tangents)
+
+    events = _events64(torch.tensor(flips), torch.tensor(trs), torch.tensor(phases) if phased else None)
+
+    def run64(*values):
+        return 1j * P.simulate(events, {_ORACLE[k]: v for k, v in zip(names, values)}, states=10, outputs=pulses)
+
+    _, expect = torch.func.jvp(run64, tuple(torch.tensor(tissue[k], dtype=torch.float64) for k in names),
+                               tuple(torch.tensor(direction[k], dtype=torch.float64) for k in names))
+    err = _cases.rel_err(derivative.cpu().numpy(), expect.numpy())
+    print(f"jvp along {names}: {err:.2e}")
+    assert err < TOL
+
+
+def test_forward_ad_along_flip_tr_and_phase():
+    """Dual numbers on the train's flips, TRs and phases (one direction in
+    all three) through simulate, against the FP64 oracle's jvp."""
+    from torchsim_amd.simulators import MRFSimulator
+
+    atoms, pulses = 16, 70
+    tissue = _tissue(atoms, 21, ["T1", "T2"])
+    flips, trs, phases = _train(pulses, 22)
+    rng = np.random.default_rng(23)
+    d_flip, d_tr, d_phase = rng.normal(size=pulses), rng.normal(size=pulses), 10.0 * rng.normal(size=pulses)
+    maps = {k: torch.tensor(v, device="cuda") for k, v in tissue.items()}
+    with forward_ad.dual_level():
+        flip = forward_ad.make_dual(torch.tensor(flips, dtype=torch.float32), torch.tensor(d_flip, dtype=torch.float32))
+        tr = forward_ad.make_dual(torch.tensor(trs, dtype=torch.float32), torch.tensor(d_tr, dtype=torch.float32))
+        ph = forward_ad.make_dual(torch.tensor(phases, dtype=torch.float32),
+                                  torch.tensor(d_phase, dtype=torch.float32))
+        out = MRFSimulator(states=10).simulate(flip=flip, TR=tr, phases=ph, **maps)
+        derivative = forward_ad.unpack_dual(out).tangent
+    assert derivative is not None
+
+    def run64(f, t, p):
+        events = _events64(f, t, p)
+        return 1j * P.simulate(events, {_ORACLE[k]: torch.tensor(v, dtype=torch.float64) for k, v in tissue.items()},
+                               states=10, outputs=pulses)
+
+    _, expect = torch.func.jvp(run64, (torch.tensor(flips), torch.tensor(trs), torch.tensor(phases)),
+                               (torch.tensor(d_flip), torch.tensor(d_tr), torch.tensor(d_phase)))
+    err = _cases.rel_err(derivative.cpu().numpy(), expect.numpy())
+    print(f"dual flips/TRs/phases: {err:.2e}")
+    assert err < TOL
+
+
+def test_func_jvp_on_an_exchanging_diffusing_protocol():
+    """An exchanging pool and diffusion (crusher geometry declared): the
+    directional derivative along T1, T2, D and B0 together."""
+    from torchsim_amd.model._simulator import damping_scale
+    from torchsim_amd.simulators import MRFSimulator
+
+    atoms, pulses = 12, 60
+    tissue = _tissue(atoms, 31, ["T1", "T2", "B0"])
+    rng = np.random.default_rng(32)
+    tissue["D"] = rng.uniform(0.5, 3.0, atoms).astype(np.float32)
+    direction = {"T1": 50.0 * rng.normal(size=atoms), "T2": 5.0 * rng.normal(size=atoms),
+                 "B0": 3.0 * rng.normal(size=atoms), "D": 0.2 * rng.normal(size=atoms)}
+    fraction = rng.uniform(0.05, 0.3, atoms).astype(np.float32)
+    exchange = rng.uniform(5.0, 20.0, atoms).astype(np.float32)
+    flips, trs, _ = _train(pulses, 33)
+    geometry = {"crusher_dephasing_rad": 8 * np.pi, "voxel_size_m": 0.5e-3}
+    names = list(direction)
+    sim = MRFSimulator(states=10)
+
+    def run(*values):
+        return sim.simulate(flip=torch.tensor(flips, dtype=torch.float32), TR=torch.tensor(trs, dtype=torch.float32),
+                            poolB_fraction=torch.tensor(fraction, device="cuda"),
+                            poolB_exchange=torch.tensor(exchange, device="cuda"), poolB_T1=400.0, poolB_T2=15.0,
+                            **geometry, **dict(zip(names, values)))
+
+    primals = tuple(torch.tensor(tissue[k], device="cuda") for k in names)
+    tangents = tuple(torch.tensor(direction[k], dtype=torch.float32, device="cuda") for k in names)
+    _, derivative = torch.func.jvp(run, primals, tangents)
+    events = _events64(torch.tensor(flips), torch.tensor(trs))
+    scale = damping_scale(geometry)
+
+    def run64(t1, t2, b0, d):
+        return 1j * P.simulate(events, {"t1": t1, "t2": t2, "b0": b0, "diffusion": d * scale,
+                                        "fraction": torch.tensor(fraction), "exchange": torch.tensor(exchange),
+                                        "t1b": 400.0, "t2b": 15.0}, states=10, outputs=pulses)
+
+    _, expect = torch.func.jvp(run64, tuple(torch.tensor(tissue[k], dtype=torch.float64) for k in names),
+                               tuple(torch.tensor(direction[k], dtype=torch.float64) for k in names))
+    err = _cases.rel_err(derivative.cpu().numpy(), expect.numpy())
+    print(f"exchanging + diffusing jvp: {err:.2e}")
+    assert err < TOL
+
+
+def test_c_abi_jvp_matches_the_reference_simulate_jvp():
+    """tsim_complex_jvp with every carried slot seeded (tissue and events) on
+    a phase-cycled, off-resonant protocol, single and exchanging pool, equals
+    the reference's own FP32 simulate_jvp (real_axis -1)."""
+    if oracle.reference_module() is None:
+        pytest.skip("oracle/_ref is not built")
+    from torchsim_amd.sequence import _complex, _engine
+    from torchsim_amd.simulators import MRFSimulator
+
+    atoms, pulses = 14, 60
+    t1, t2 = _cases.random_tissue(atoms, seed=41)
+    rng = np.random.default_rng(42)
+    b0 = rng.uniform(-25, 25, atoms).astype(np.float32)
+    m0 = rng.uniform(0.5, 1.0, atoms).astype(np.float32)
+    flips, trs, phases = _train(pulses, 43)
+    packed = MRFSimulator().packed(flip=torch.tensor(flips, dtype=torch.float32),
+                                   TR=torch.tensor(trs, dtype=torch.float32),
+                                   phases=torch.tensor(phases, dtype=torch.float32))
+    events = oracle.events_of(packed)
+    count = packed.event_count
+    tangent = {"t1": 100.0 * rng.normal(size=atoms), "t2": 10.0 * rng.normal(size=atoms),
+               "m0": 0.1 * rng.normal(size=atoms), "b0": 2.0 * rng.normal(size=atoms),
+               "duration": 1e-3 * rng.normal(size=count), "flip": 0.1 * rng.normal(size=count),
+               "phase": 0.2 * rng.normal(size=count)}
+    tangent = {k: v.astype(np.float32) for k, v in tangent.items()}
+    fraction = rng.uniform(0.05, 0.3, atoms).astype(np.float32)
+    exchange = rng.uniform(5.0, 20.0, atoms).astype(np.float32)
+    for exchanging in (False, True):
+        fields = {"t1_ms": torch.tensor(t1), "t2_ms": torch.tensor(t2), "m0": torch.tensor(m0),
+                  "b0_hz": torch.tensor(b0)}
+        ref_tissue = {"t1": t1, "t2": t2, "m0": m0, "b0": b0}
+        if exchanging:
+            fields.update(pool_b_fraction=torch.tensor(fraction), pool_b_exchange_rate=torch.tensor(exchange),
+                          t1_pool_b_ms=400.0, t2_pool_b_ms=15.0)
+            ref_tissue.update(pool_b_fraction=fraction, pool_b_exchange=exchange, t1_pool_b=400.0, t2_pool_b=15.0)
+        tissue = _engine.prepare_tissue(fields, torch.device("cuda", 0), ("t1_ms", "t2_ms", "m0", "b0_hz"))
+        got = _complex.jvp_complex(
+            packed, tissue, states=10, factor=1.0,
+            tissue_tangents={"t1_ms": torch.tensor(tangent["t1"]), "t2_ms": torch.tensor(tangent["t2"]),
+                             "m0": torch.tensor(tangent["m0"]), "b0_hz": torch.tensor(tangent["b0"])},
+            duration_tangent=torch.tensor(tangent["duration"]), flip_tangent=torch.tensor(tangent["flip"]),
+            phase_tangent=torch.tensor(tangent["phase"])).cpu().numpy()
+        ref = oracle.reference_complex(events, ref_tissue, states=10, outputs=pulses, exchanging=exchanging,
+                                       tangent=tangent)
+        err = _cases.rel_err(got, ref)
+        print(f"C-ABI jvp vs reference simulate_jvp (exchanging={exchanging}): {err:.2e}")
+        assert err < 1e-4
+
+
+def test_a_jacobian_under_forward_mode_is_refused():
+    from torchsim_amd.simulators import MRFSimulator
+
+    t1 = torch.tensor([800.0, 1200.0], device="cuda")
+    sim = MRFSimulator(flip=torch.full((10,), 30.0), TR=10.0)
+    with pytest.raises(NotImplementedError, match="forward-mode"):
+        torch.func.jvp(lambda x: sim.jacobian("T2", T1=x, T2=torch.tensor([60.0, 80.0], device="cuda"))[1],
+                       (t1,), (torch.ones_like(t1),))
+
+
+def test_func_grad_through_simulate():
+    """torch.func.grad of a loss of simulate (functorch's reverse mode) equals
+    plain autograd's, tissue and sequence alike."""
+    from torchsim_amd.simulators import MRFSimulator
+
+    atoms, pulses = 10, 50
+    tissue = _tissue(atoms, 51, ["T1", "T2"])
+    flips, trs, _ = _train(pulses, 52)
+    weight = torch.tensor(np.random.default_rng(53).normal(size=(atoms, pulses)), dtype=torch.float32, device="cuda")
+    sim = MRFSimulator(states=8)
+    T2 = torch.tensor(tissue["T2"], device="cuda")
+
+    def loss(t1, flip):
+        return (sim.simulate(flip=flip, TR=torch.tensor(trs, dtype=torch.float32), T1=t1, T2=T2).real * weight).sum()
+
+    t1 = torch.tensor(tissue["T1"], device="cuda")
+    flip = torch.tensor(flips, dtype=torch.float32)
+    g_t1, g_flip = torch.func.grad(loss, argnums=(0, 1))(t1, flip)
+    t1r, flipr = t1.clone().requires_grad_(), flip.clone().requires_grad_()
+    loss(t1r, flipr).backward()
+    assert torch.allclose(g_t1, t1r.grad, rtol=1e-6, atol=0) and torch.allclose(g_flip, flipr.grad, rtol=1e-6, atol=0)
+
+
+def test_flip_gradient_of_an_alternating_phase_balanced_train():
+    """A balanced train whose pulses alternate in phase runs on the real
+    kernel with signed flips; the gradient reaches the caller's flips through
+    those signs (against FP64 autograd)."""
+    from torchsim_amd.simulators import TrueFISPSimulator
+
+    atoms, shots = 8, 60
+    t1, t2 = _cases.random_tissue(atoms, seed=61)
+    rng = np.random.default_rng(62)
+    weight = rng.normal(size=(atoms, shots)) + 1j * rng.normal(size=(atoms, shots))
+    flips = np.linspace(20.0, 60.0, shots)
+    flip = torch.tensor(flips, dtype=torch.float32, requires_grad=True)
+    sig = TrueFISPSimulator().simulate(flip=flip, TR=5.0, nshots=shots, T1=torch.tensor(t1, device="cuda"),
+                                       T2=torch.tensor(t2, device="cuda"))
+    (sig * torch.tensor(weight, dtype=torch.complex64, device="cuda").conj()).real.sum().backward()
+    assert flip.grad is not None
+
+    flip64 = torch.tensor(flips, dtype=torch.float64, requires_grad=True)
+    packed = TrueFISPSimulator().packed(flip=flip64, TR=5.0, nshots=shots)
+    events = {"duration": packed.duration.double(), "flip": packed.flip.double(), "phase": packed.phase.double(),
+              "kind": packed.kind, "action": packed.action, "output_index": packed.output_index}
+    ref = P.simulate(events, {"t1": torch.tensor(t1, dtype=torch.float64), "t2": torch.tensor(t2, dtype=torch.float64)},
+                     states=1, outputs=shots)
+    (ref * torch.tensor(weight).conj()).real.sum().backward()
+    err = _cases.rel_err(flip.grad.numpy(), flip64.grad.numpy())
+    print(f"alternating-phase flip gradient: {err:.2e}")
+    assert err < TOL

@@ -53,16 +53,11 @@ int clear(float* a, int64_t count, hipStream_t s) {
   return hip_status(hipGetLastError(), "gradient clear");
 }
 
-}  // namespace
-}  // namespace tsim
-
-using namespace tsim;
-
-extern "C" int tsim_complex_vjp(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
-                                int64_t output_count, int pool_kind, int features, int64_t tissue_stride,
-                                float factor_re, float factor_im, const void* signal_cotangent,
-                                int64_t segment_events, float* grad_flip, float* grad_phase, float* grad_duration,
-                                float* const* tissue_gradients, int64_t tissue_gradient_count, void* stream) {
+// Checks and fills what the adjoint and the JVP share (the packed buffers,
+// sizes and feature flags); the event codes come later.
+int fill_common(ComplexAdjointArgs& a, void* const* packed, int64_t atom_count, int64_t event_count,
+                int64_t state_count, int64_t output_count, int pool_kind, int features, int64_t tissue_stride,
+                float factor_re, float factor_im) {
   if (packed == nullptr || packed[T_T1] == nullptr || packed[T_T2] == nullptr) {
     set_error("packed buffers with t1 and t2 are required");
     return BAD_ARGUMENT;
@@ -84,41 +79,10 @@ extern "C" int tsim_complex_vjp(void* const* packed, int64_t atom_count, int64_t
     return BAD_ARGUMENT;
   }
   if (features & FEATURE_MOVING) {
-    set_error("the complex adjoint carries no flow term");
+    set_error("the complex adjoint and JVP carry no flow term");
     return UNSUPPORTED;
   }
-  if (segment_events < 0) {
-    set_error("segment_events must be positive (or 0 for the library's choice)");
-    return BAD_ARGUMENT;
-  }
-  if (signal_cotangent == nullptr && atom_count > 0 && output_count > 0) {
-    set_error("the signal cotangent is required");
-    return BAD_ARGUMENT;
-  }
-  if (tissue_gradient_count != 0 && (tissue_gradients == nullptr || tissue_gradient_count != TISSUE_COUNT)) {
-    set_error("tissue_gradients holds %d pointers (tissue order) or tissue_gradient_count is 0", TISSUE_COUNT);
-    return BAD_ARGUMENT;
-  }
-  ComplexAdjointArgs a;
   memset(&a, 0, sizeof(a));
-  if (tissue_gradient_count == TISSUE_COUNT) {
-    for (int q = 0; q < TISSUE_COUNT; ++q) {
-      if (tissue_gradients[q] == nullptr) continue;
-      switch (q) {
-        case T_T1: a.grad[CG_T1] = tissue_gradients[q]; break;
-        case T_T2: a.grad[CG_T2] = tissue_gradients[q]; break;
-        case T_M0: a.grad[CG_M0] = tissue_gradients[q]; break;
-        case T_B1: a.grad[CG_B1] = tissue_gradients[q]; break;
-        case T_B1_PHASE: a.grad[CG_B1_PHASE] = tissue_gradients[q]; break;
-        case T_B0: a.grad[CG_B0] = tissue_gradients[q]; break;
-        case T_INV_EFF: a.grad[CG_IE] = tissue_gradients[q]; break;
-        case T_DIFFUSION: a.grad[CG_D] = tissue_gradients[q]; break;
-        default:
-          set_error("tissue property %d carries no gradient in the complex adjoint", q);
-          return UNSUPPORTED;
-      }
-    }
-  }
   a.t1 = (const float*)packed[T_T1];
   a.t2 = (const float*)packed[T_T2];
   a.m0 = (const float*)packed[T_M0];
@@ -145,9 +109,77 @@ extern "C" int tsim_complex_vjp(void* const* packed, int64_t atom_count, int64_t
   a.transmit = (features & FEATURE_TRANSMIT) && a.b1 ? 1 : 0;
   a.off_axis = (features & FEATURE_OFF_AXIS) ? 1 : 0;
   a.diffusing = (features & FEATURE_DIFFUSING) && a.diffusion ? 1 : 0;
-  a.cotangent = (const float2*)signal_cotangent;
   a.factor_re = factor_re;
   a.factor_im = factor_im;
+  a.event_count = event_count;
+  return OK;
+}
+
+// The CG slot a tissue property's gradient or tangent takes, or -1.
+int carried_slot(int tissue) {
+  switch (tissue) {
+    case T_T1: return CG_T1;
+    case T_T2: return CG_T2;
+    case T_M0: return CG_M0;
+    case T_B1: return CG_B1;
+    case T_B1_PHASE: return CG_B1_PHASE;
+    case T_B0: return CG_B0;
+    case T_INV_EFF: return CG_IE;
+    case T_DIFFUSION: return CG_D;
+    default: return -1;
+  }
+}
+
+ComplexJvpLaunch pick_complex_jvp(int lanes, bool exch) {
+  switch (lanes) {
+#define TSIM_COMPLEX_JVP_CASE(LANES) \
+  case LANES:                        \
+    return pick_complex_jvp_##LANES(exch);
+    TSIM_COMPLEX_LANES(TSIM_COMPLEX_JVP_CASE)
+#undef TSIM_COMPLEX_JVP_CASE
+    default:
+      return nullptr;
+  }
+}
+
+}  // namespace
+}  // namespace tsim
+
+using namespace tsim;
+
+extern "C" int tsim_complex_vjp(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
+                                int64_t output_count, int pool_kind, int features, int64_t tissue_stride,
+                                float factor_re, float factor_im, const void* signal_cotangent,
+                                int64_t segment_events, float* grad_flip, float* grad_phase, float* grad_duration,
+                                float* const* tissue_gradients, int64_t tissue_gradient_count, void* stream) {
+  ComplexAdjointArgs a;
+  int fill = fill_common(a, packed, atom_count, event_count, state_count, output_count, pool_kind, features,
+                         tissue_stride, factor_re, factor_im);
+  if (fill != OK) return fill;
+  if (segment_events < 0) {
+    set_error("segment_events must be positive (or 0 for the library's choice)");
+    return BAD_ARGUMENT;
+  }
+  if (signal_cotangent == nullptr && atom_count > 0 && output_count > 0) {
+    set_error("the signal cotangent is required");
+    return BAD_ARGUMENT;
+  }
+  if (tissue_gradient_count != 0 && (tissue_gradients == nullptr || tissue_gradient_count != TISSUE_COUNT)) {
+    set_error("tissue_gradients holds %d pointers (tissue order) or tissue_gradient_count is 0", TISSUE_COUNT);
+    return BAD_ARGUMENT;
+  }
+  if (tissue_gradient_count == TISSUE_COUNT) {
+    for (int q = 0; q < TISSUE_COUNT; ++q) {
+      if (tissue_gradients[q] == nullptr) continue;
+      const int slot = carried_slot(q);
+      if (slot < 0) {
+        set_error("tissue property %d carries no gradient in the complex adjoint", q);
+        return UNSUPPORTED;
+      }
+      a.grad[slot] = tissue_gradients[q];
+    }
+  }
+  a.cotangent = (const float2*)signal_cotangent;
 
   hipStream_t s = (hipStream_t)stream;
   int st = OK;
@@ -209,4 +241,68 @@ extern "C" int tsim_complex_vjp(void* const* packed, int64_t atom_count, int64_t
   }
   (void)hipFreeAsync(work, s);
   return st;
+}
+
+extern "C" int tsim_complex_jvp(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
+                                int64_t output_count, int pool_kind, int features, int64_t tissue_stride,
+                                float factor_re, float factor_im, const float* const* tangents,
+                                int64_t tangent_count, void* signal, void* tangent, void* stream) {
+  ComplexJvpArgs q;
+  memset(&q, 0, sizeof(q));
+  int st = fill_common(q.base, packed, atom_count, event_count, state_count, output_count, pool_kind, features,
+                       tissue_stride, factor_re, factor_im);
+  if (st != OK) return st;
+  if (tangent_count != 0 && (tangents == nullptr || tangent_count != TISSUE_COUNT + 3)) {
+    set_error("tangents holds %d pointers (17 tissue, then duration, flip, phase) or tangent_count is 0",
+              TISSUE_COUNT + 3);
+    return BAD_ARGUMENT;
+  }
+  if (tangent == nullptr && atom_count > 0 && output_count > 0) {
+    set_error("the tangent output is required");
+    return BAD_ARGUMENT;
+  }
+  if (tangent_count > 0) {
+    for (int k = 0; k < TISSUE_COUNT; ++k) {
+      if (tangents[k] == nullptr) continue;
+      const int slot = carried_slot(k);
+      if (slot < 0) {
+        set_error("tissue property %d carries no tangent in the complex JVP", k);
+        return UNSUPPORTED;
+      }
+      q.tissue_tangent[slot] = tangents[k];
+    }
+    q.duration_tangent = tangents[TISSUE_COUNT + 0];
+    q.flip_tangent = tangents[TISSUE_COUNT + 1];
+    q.phase_tangent = tangents[TISSUE_COUNT + 2];
+  }
+  q.signal = (float2*)signal;
+  q.tangent = (float2*)tangent;
+  hipStream_t s = (hipStream_t)stream;
+  if (atom_count == 0 || output_count == 0) return OK;
+  // samples no event records stay zero (a fill kernel, not a memset, so a
+  // caller may capture the call in a graph)
+  if (q.signal) st = clear((float*)q.signal, 2 * atom_count * output_count, s);
+  if (st == OK) st = clear((float*)q.tangent, 2 * atom_count * output_count, s);
+  if (st != OK || event_count == 0) return st;
+  const int L = adjoint_lanes(state_count);
+  ComplexJvpLaunch launch = pick_complex_jvp(L, pool_kind == 2);
+  if (!launch) {
+    set_error("no complex JVP kernel for %lld orders", (long long)state_count);
+    return UNSUPPORTED;
+  }
+  const int apw = 64 / L;
+  const int64_t waves = (atom_count + apw - 1) / apw;
+  const int threads = waves < 4LL * 256 ? 64 : 256;
+  const int64_t atoms_per_block = (int64_t)(threads / 64) * apw;
+  const int64_t blocks = (atom_count + atoms_per_block - 1) / atoms_per_block;
+  if (blocks > 0x7FFFFFFF) {
+    set_error("too many atoms for one launch: %lld", (long long)atom_count);
+    return BAD_ARGUMENT;
+  }
+  Compiled compiled;
+  st = compile_packed(packed, event_count, s, compiled);
+  if (st != OK) return st;
+  q.base.events = compiled.codes;
+  launch(q, dim3((unsigned)blocks), dim3(threads), s);
+  return hip_status(hipGetLastError(), "complex_jvp_kernel launch");
 }

@@ -321,6 +321,8 @@ __device__ __forceinline__ void rotate(State<EXCH>& x, const Rotation& T) {
 
 // Re(conj(a) b)
 __device__ __forceinline__ float dotr(float2 a, float2 b) { return fmaf(a.x, b.x, a.y * b.y); }
+// a - b (cx::sub under a name a lane's `sub` index does not hide)
+__device__ __forceinline__ float2 sub_of(float2 a, float2 b) { return mk(a.x - b.x, a.y - b.y); }
 // conj(a) b
 __device__ __forceinline__ float2 cmul_conj(float2 a, float2 b) {
   return mk(fmaf(a.x, b.x, a.y * b.y), fmaf(a.x, b.y, -a.y * b.x));
@@ -681,6 +683,256 @@ __global__ void __launch_bounds__(256) complex_adjoint_kernel(const ComplexAdjoi
     if (first && live) p.grad[q][group] = total;
   }
 }
+
+// Forward mode along one arbitrary direction: a tangent for every carried
+// tissue property (per atom, or one value for all under stride 0) and for
+// every event's duration, flip and phase. It computes the reference's
+// `simulate_jvp` (src/torchsim/_epg_cpu.cpp:2469-3005, entry :11017-11148)
+// for the complex state machine, which is also exact on real-subspace
+// inputs: the state and its tangent walk the events together, each operator
+// applied to the tangent plus the operator's own derivative along the
+// direction applied to the state.
+struct ComplexJvpArgs {
+  ComplexAdjointArgs base;        // events, tissue, sizes, features, factor
+  const float* tissue_tangent[CG_COUNT];  // CG order; null = zero
+  const float* duration_tangent;  // (events) or null
+  const float* flip_tangent;      // (events) or null
+  const float* phase_tangent;     // (events) or null
+  float2* signal;                 // (atoms, outputs) or null
+  float2* tangent;                // (atoms, outputs)
+};
+
+namespace cadj {
+
+// One atom's tangent direction.
+struct Direction {
+  float t1, t2, m0, b1, b1_phase, b0, eta, drate;
+};
+
+// The tangent of one relaxation over interval I (dt) with duration tangent
+// ddt: t <- R t + dR x, then (APPLY) x <- R x. At dt = 0, R is the identity
+// and dR = ddt G, the interval's generator.
+template <bool EXCH, bool APPLY>
+__device__ __forceinline__ void relax_jvp(State<EXCH>& x, State<EXCH>& t, const Interval<EXCH, true>& I,
+                                          const Atom& at, const Direction& d, float dt, float ddt, bool first) {
+  if constexpr (!EXCH) {
+    const float2 tp = scl(I.turn, I.e2), tm = scl(cj(I.turn), I.e2);
+    const float lo = I.e1 * I.dlong;
+    // d ln(tp) and d ln(tm): real parts equal, imaginary parts opposite
+    const float lr = d.t2 * at.k2 * dt - ddt * (at.r2 + at.wt * at.drate) - d.drate * at.wt * dt;
+    const float li = -kTwoPi * (d.b0 * dt + at.b0 * ddt);
+    const float lz = d.t1 * at.k1 * dt - ddt * (at.r1 + at.wl * at.drate) - d.drate * at.wl * dt;
+    const float dom1 = -I.e1 * (d.t1 * at.k1 * dt - ddt * at.r1);
+    t.fp[0] = mul(tp, add(t.fp[0], mul(mk(lr, li), x.fp[0])));
+    t.fm[0] = mul(tm, add(t.fm[0], mul(mk(lr, -li), x.fm[0])));
+    t.z[0] = scl(add(t.z[0], scl(x.z[0], lz)), lo);
+    if (first) t.z[0].x += dom1;
+    if constexpr (APPLY) relax<EXCH, true>(x, I, first);
+  } else {
+    const float2 cx = mk(-at.wt * d.drate * dt, -kTwoPi * d.b0 * dt);
+    const float cy = -at.wl * d.drate * dt;
+    float2 dX[4];
+    float dY[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      dX[i] = add(add(scl(I.dX2[i], d.t2), scl(I.GX[i], ddt)), mul(cx, I.X[i]));
+      dY[i] = d.t1 * I.dY1[i] + ddt * I.GY[i] + cy * I.Y[i];
+    }
+    const float2 a0 = add(add(mul(I.X[0], t.fp[0]), mul(I.X[1], t.fp[1])), add(mul(dX[0], x.fp[0]), mul(dX[1], x.fp[1])));
+    const float2 a1 = add(add(mul(I.X[2], t.fp[0]), mul(I.X[3], t.fp[1])), add(mul(dX[2], x.fp[0]), mul(dX[3], x.fp[1])));
+    const float2 b0 = add(add(mul(cj(I.X[0]), t.fm[0]), mul(cj(I.X[1]), t.fm[1])),
+                          add(mul(cj(dX[0]), x.fm[0]), mul(cj(dX[1]), x.fm[1])));
+    const float2 b1 = add(add(mul(cj(I.X[2]), t.fm[0]), mul(cj(I.X[3]), t.fm[1])),
+                          add(mul(cj(dX[2]), x.fm[0]), mul(cj(dX[3]), x.fm[1])));
+    float2 z0 = add(add(scl(t.z[0], I.Y[0]), scl(t.z[1], I.Y[1])), add(scl(x.z[0], dY[0]), scl(x.z[1], dY[1])));
+    float2 z1 = add(add(scl(t.z[0], I.Y[2]), scl(t.z[1], I.Y[3])), add(scl(x.z[0], dY[2]), scl(x.z[1], dY[3])));
+    if (first) {
+      z0.x += d.t1 * I.drec1[0] + ddt * I.drecdt[0];
+      z1.x += d.t1 * I.drec1[1] + ddt * I.drecdt[1];
+    }
+    t.fp[0] = a0;
+    t.fp[1] = a1;
+    t.fm[0] = b0;
+    t.fm[1] = b1;
+    t.z[0] = z0;
+    t.z[1] = z1;
+    if constexpr (APPLY) relax<EXCH, true>(x, I, first);
+  }
+}
+
+}  // namespace cadj
+
+template <int L, bool EXCH>
+__global__ void __launch_bounds__(256) complex_jvp_kernel(const ComplexJvpArgs q) {
+  using namespace cadj;
+  using S = State<EXCH>;
+  constexpr int NP = S::NP;
+  constexpr int APW = 64 / L;
+  const ComplexAdjointArgs& p = q.base;
+  const int lane = (int)(threadIdx.x & 63);
+  const int wave_group = lane / L;
+  const int sub = lane - wave_group * L;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t group = (tid >> 6) * APW + wave_group;
+  const bool live = wave_group < APW && group < p.atoms;
+  const int64_t atom = live ? group : p.atoms - 1;
+  const bool beyond = sub >= p.states;
+  const bool first = sub == 0;
+  const int64_t ia = atom * p.stride_opt;
+
+  Atom at;
+  at.t1 = p.t1[atom];
+  at.t2 = p.t2[atom];
+  at.r1 = 1000.0f / at.t1;
+  at.r2 = 1000.0f / at.t2;
+  at.k1 = 1000.0f / (at.t1 * at.t1);
+  at.k2 = 1000.0f / (at.t2 * at.t2);
+  at.m0 = p.density ? p.m0[ia] : 1.0f;
+  at.b1 = p.transmit ? p.b1[ia] : 1.0f;
+  at.b1_phase = p.off_axis && p.b1_phase ? p.b1_phase[ia] : 0.0f;
+  at.b0 = p.off_axis && p.b0 ? p.b0[ia] : 0.0f;
+  at.eta = p.inverting ? p.inv_eff[ia] : 1.0f;
+  at.drate = p.diffusing ? p.diffusion[ia] : 0.0f;
+  at.frac = at.kab = at.kba = at.r1b = at.r2b = at.shift_hz = 0.0f;
+  if constexpr (EXCH) {
+    at.frac = p.pool_fraction[ia];
+    const float kex = p.pool_exchange[ia];
+    at.kab = kex * at.frac;
+    at.kba = kex * (1.0f - at.frac);
+    at.r1b = 1000.0f / p.t1_pool[ia];
+    at.r2b = 1000.0f / p.t2_pool[ia];
+    at.shift_hz = p.pool_shift ? p.pool_shift[ia] : 0.0f;
+  }
+  const float order = (float)sub;
+  at.wt = order * order + order + (1.0f / 3.0f);
+  at.wl = order * order;
+  // t1 and t2 tangents are per atom; the others follow the tissue stride
+  auto tan = [&](int k, int64_t index) -> float {
+    return q.tissue_tangent[k] ? q.tissue_tangent[k][index] : 0.0f;
+  };
+  Direction d;
+  d.t1 = tan(CG_T1, atom);
+  d.t2 = tan(CG_T2, atom);
+  d.m0 = tan(CG_M0, ia);
+  d.b1 = tan(CG_B1, ia);
+  d.b1_phase = tan(CG_B1_PHASE, ia);
+  d.b0 = tan(CG_B0, ia);
+  d.eta = tan(CG_IE, ia);
+  d.drate = tan(CG_D, ia);
+
+  S x, t;
+  x.zero();
+  t.zero();
+  if (first) {
+    x.z[0] = mk(EXCH ? 1.0f - at.frac : 1.0f, 0.0f);
+    if constexpr (EXCH) x.z[1] = mk(at.frac, 0.0f);
+  }
+  Interval<EXCH, true> I;
+  Interval<EXCH, true> I0;
+  I0.set(0.0f, at);
+  const float2 factor = mk(p.factor_re, p.factor_im);
+  const int64_t E = p.event_count;
+  for (int64_t e = 0; e < E; ++e) {
+    const EventCode ev = load_event(p.events, e);
+    const uint32_t act = ev.ctrl & 0xFFu;
+    const int32_t kind = (int32_t)(ev.ctrl >> CTRL_KIND_SHIFT);
+    const float ddt = q.duration_tangent ? q.duration_tangent[e] : 0.0f;
+    if (ev.dt != 0.0f) {
+      I.set(ev.dt, at);
+      relax_jvp<EXCH, true>(x, t, I, at, d, ev.dt, ddt, first);
+    } else if (ddt != 0.0f) {
+      relax_jvp<EXCH, false>(x, t, I0, at, d, 0.0f, ddt, first);
+    }
+    if (act & PRE_SHIFT) {
+      shift_all<L, EXCH>(x, sub, lane, beyond);
+      shift_all<L, EXCH>(t, sub, lane, beyond);
+    }
+    if (kind == KIND_RF) {
+      if (act & INVERSION) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          t.z[k] = sub_of(scl(t.z[k], -at.eta), scl(x.z[k], d.eta));
+          x.z[k] = scl(x.z[k], -at.eta);
+        }
+      } else {
+        float sa, va;
+        pulse_angle(ev, at, p.transmit, sa, va);
+        Rotation Tm;
+        Tm.set(sa, va, ev.phase + at.b1_phase);
+        const float dflip = q.flip_tangent ? q.flip_tangent[e] : 0.0f;
+        const float dphase = q.phase_tangent ? q.phase_tangent[e] : 0.0f;
+        const float da = dflip * at.b1 + ev.flip * d.b1;
+        const float dphi = dphase + d.b1_phase;
+        const float ca = Tm.ca, dch = -0.5f * sa, dca = -sa;
+        const float2 e1p = Tm.e1p;
+        const float2 d01 = scl(mul(e1p, e1p), 0.5f * sa);
+        const float2 d02 = mul(mk(0.0f, -ca), e1p);
+        const float2 d10 = cj(d01);
+        const float2 d12 = mul(mk(0.0f, ca), cj(e1p));
+        const float2 d20 = mul(mk(0.0f, -0.5f * ca), cj(e1p));
+        const float2 d21 = mul(mk(0.0f, 0.5f * ca), e1p);
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          const float2 fp = x.fp[k], fm = x.fm[k], z = x.z[k];
+          // along the angle
+          const float2 ap = mac(mac(scl(fp, dch), d01, fm), d02, z);
+          const float2 am = mac(mac(mul(d10, fp), mk(dch, 0.0f), fm), d12, z);
+          const float2 az = mac(mac(mul(d20, fp), d21, fm), mk(dca, 0.0f), z);
+          // along the phase: i (2 t01 fm + t02 z), i (-2 t10 fp - t12 z), i (t21 fm - t20 fp)
+          const float2 qp = add(scl(mul(Tm.t01, fm), 2.0f), mul(Tm.t02, z));
+          const float2 qm = neg(add(scl(mul(Tm.t10, fp), 2.0f), mul(Tm.t12, z)));
+          const float2 qz = sub_of(mul(Tm.t21, fm), mul(Tm.t20, fp));
+          const float2 tp = t.fp[k], tm = t.fm[k], tz = t.z[k];
+          t.fp[k] = add(mac(mac(scl(tp, Tm.chh), Tm.t01, tm), Tm.t02, tz),
+                        add(scl(ap, da), scl(mk(-qp.y, qp.x), dphi)));
+          t.fm[k] = add(mac(mac(mul(Tm.t10, tp), mk(Tm.chh, 0.0f), tm), Tm.t12, tz),
+                        add(scl(am, da), scl(mk(-qm.y, qm.x), dphi)));
+          t.z[k] = add(mac(mac(mul(Tm.t20, tp), Tm.t21, tm), mk(ca, 0.0f), tz),
+                       add(scl(az, da), scl(mk(-qz.y, qz.x), dphi)));
+        }
+        rotate<EXCH>(x, Tm);
+      }
+    } else if (kind == KIND_ADC && (act & RECORD) && ev.out >= 0 && ev.out < p.outputs && first && live) {
+      float sp, cp;
+      sincosf(-ev.phase, &sp, &cp);
+      const float2 fd = mul(factor, mk(cp, sp));
+      float2 total = x.fp[0], ttotal = t.fp[0];
+      if constexpr (EXCH) {
+        total = add(total, x.fp[1]);
+        ttotal = add(ttotal, t.fp[1]);
+      }
+      const float2 sample = mul(fd, scl(total, at.m0));
+      const float dphase = q.phase_tangent ? q.phase_tangent[e] : 0.0f;
+      // d sample = fd (dm0 total + m0 dtotal) - i dphase sample
+      const float2 ds = add(mul(fd, add(scl(total, d.m0), scl(ttotal, at.m0))), scl(mk(sample.y, -sample.x), dphase));
+      const int64_t o = atom * p.outputs + ev.out;
+      if (q.signal) q.signal[o] = sample;
+      q.tangent[o] = ds;
+    }
+    if (act & POST_SHIFT) {
+      shift_all<L, EXCH>(x, sub, lane, beyond);
+      shift_all<L, EXCH>(t, sub, lane, beyond);
+    }
+    if (act & SPOIL_AFTER) {
+#pragma unroll
+      for (int k = 0; k < NP; ++k) x.fp[k] = x.fm[k] = t.fp[k] = t.fm[k] = mk(0.0f, 0.0f);
+    } else if (act & SHIFT_AFTER) {
+      shift_all<L, EXCH>(x, sub, lane, beyond);
+      shift_all<L, EXCH>(t, sub, lane, beyond);
+    }
+  }
+}
+
+using ComplexJvpLaunch = void (*)(const ComplexJvpArgs&, dim3, dim3, hipStream_t);
+
+template <int L, bool EXCH>
+void launch_complex_jvp(const ComplexJvpArgs& a, dim3 grid, dim3 block, hipStream_t s) {
+  hipLaunchKernelGGL((complex_jvp_kernel<L, EXCH>), grid, block, 0, s, a);
+}
+
+#define TSIM_DECLARE_COMPLEX_JVP(LANES) ComplexJvpLaunch pick_complex_jvp_##LANES(bool exch);
+TSIM_COMPLEX_LANES(TSIM_DECLARE_COMPLEX_JVP)
+#undef TSIM_DECLARE_COMPLEX_JVP
 
 using ComplexAdjointLaunch = void (*)(const ComplexAdjointArgs&, dim3, dim3, size_t, hipStream_t);
 

@@ -19,6 +19,7 @@ from __future__ import annotations
 import threading
 from collections import OrderedDict
 from collections.abc import Mapping, Sequence
+import dataclasses
 from copy import copy as shallow_copy
 from dataclasses import dataclass, field
 from typing import Any
@@ -97,7 +98,7 @@ def _protocol_key(kind: type, sequence: Mapping[str, Any]) -> tuple | None:
             continue
         value = sequence[name]
         if isinstance(value, torch.Tensor):
-            if value.requires_grad:
+            if _engine.carries_derivative(value):
                 return None
             held = value.detach().cpu().contiguous()
             parts.append((name, str(held.dtype), tuple(held.shape), held.numpy().tobytes()))
@@ -270,6 +271,30 @@ class Simulator:
             return ()
         return tuple(torch.broadcast_shapes(*shapes))
 
+    def _prepared(self, held: Mapping[str, Any], sequence: Mapping[str, Any], diff: tuple[str, ...] = ()) -> tuple:
+        """The host side of a run: the device, the packed protocol, the tissue
+        fields and their prepared buffers, the state count, and which fields
+        are differentiated (forward-mode columns), require grad, or carry a
+        forward-mode tangent."""
+        device = _target_device({**held, **sequence}, sequence.get("device"))
+        packed = self.packed(**sequence)
+        fields = self._tissue_fields(held)
+        if "diffusion_um2_per_ms" in fields:
+            scale = damping_scale(sequence)
+            value = fields["diffusion_um2_per_ms"]
+            fields["diffusion_um2_per_ms"] = value * scale if scale else 0.0
+        differentiated = tuple(self._tissue_fields({name: held[name] for name in diff}).keys())
+        # fields autograd or a forward-mode tangent must reach are laid out per
+        # atom, like forward-mode ones
+        reversed_fields = tuple(name for name, value in fields.items()
+                                if isinstance(value, torch.Tensor) and value.requires_grad)
+        tangent_fields = tuple(name for name, value in fields.items()
+                               if _engine.carries_derivative(value) and name not in reversed_fields)
+        tissue = _engine.prepare_tissue(fields, device,
+                                        tuple(dict.fromkeys(differentiated + reversed_fields + tangent_fields)))
+        states = int(sequence.get("states", self.states) or 10)
+        return device, packed, fields, tissue, states, differentiated, reversed_fields, tangent_fields
+
     def _run(
         self,
         held: Mapping[str, Any],
@@ -279,70 +304,67 @@ class Simulator:
     ) -> tuple[torch.Tensor, torch.Tensor | None]:
         """Signal (atoms, N) and Jacobian (atoms, P, N) of the recorded samples;
         with a ``subspace``, their projections (atoms, R) and (atoms, P, R)."""
-        device = _target_device({**held, **sequence}, sequence.get("device"))
-        packed = self.packed(**sequence)
-        fields = self._tissue_fields(held)
-        if "diffusion_um2_per_ms" in fields:
-            scale = damping_scale(sequence)
-            value = fields["diffusion_um2_per_ms"]
-            fields["diffusion_um2_per_ms"] = value * scale if scale else 0.0
-        differentiated = tuple(self._tissue_fields({name: held[name] for name in diff}).keys())
-        # fields autograd must reach are laid out per atom, like forward-mode ones
-        reversed_fields = tuple(name for name, value in fields.items()
-                                if isinstance(value, torch.Tensor) and value.requires_grad)
-        tissue = _engine.prepare_tissue(fields, device, tuple(dict.fromkeys(differentiated + reversed_fields)))
-        states = int(sequence.get("states", self.states) or 10)
+        device, packed, fields, tissue, states, differentiated, reversed_fields, tangent_fields = \
+            self._prepared(held, sequence, diff)
         real, flip, samples = _engine.real_subspace(packed, tissue, device)
         factor = self.signal_factor if self.signal_factor is not None else 1.0
+        sequence_live = [_engine.carries_derivative(v) for v in (packed.flip, packed.phase, packed.duration)]
+        reverse = bool(reversed_fields) or any(v.requires_grad for v in (packed.flip, packed.phase, packed.duration))
+        forward_mode = bool(tangent_fields) or (any(sequence_live) and not reverse)
         if subspace is not None:
             if subspace.contrasts != packed.output_count:
                 raise ValueError(f"the subspace spans {subspace.contrasts} contrasts, the sequence records "
                                  f"{packed.output_count}")
-            grad = packed.flip.requires_grad or packed.duration.requires_grad
-            if real and not grad and _engine.projects_fused(tissue, differentiated, subspace.rank):
+            if real and not reverse and not forward_mode and _engine.projects_fused(tissue, differentiated,
+                                                                                     subspace.rank):
                 return _engine.simulate_projected(packed, tissue, subspace.basis, states=states,
                                                   columns=differentiated, factor=factor, flip=flip, samples=samples)
             signal, jacobian = self._run(held, sequence, diff)
             return subspace.project(signal), (None if jacobian is None else subspace.project(jacobian))
-        sequence_grad = packed.flip.requires_grad or packed.duration.requires_grad
-        reverse = sequence_grad or bool(reversed_fields) or packed.phase.requires_grad
-        # the real-subspace adjoint carries flips, durations and five tissue
-        # fields; a phase with a gradient leaves the one axis, and any other
-        # field (B0, transmit phase, diffusion) takes the complex adjoint
-        real_reverse = (real and not packed.phase.requires_grad
-                        and all(name in _engine.GRADIENT_FIELDS for name in reversed_fields))
-        if not real or (reverse and not real_reverse):
-            from ..sequence import _complex
+        from ..sequence import _complex
 
-            if reverse:
-                if differentiated:
-                    raise NotImplementedError(
-                        "a Jacobian under reverse mode (forward-over-reverse) runs on the real-subspace kernel "
-                        "for T1, T2, M0, B1 and the inversion efficiency; this protocol or tissue takes the "
-                        "complex kernel, whose adjoint is first order"
-                    )
-                unsupported = [name for name in reversed_fields if name not in _complex.GRADIENT_FIELDS]
-                if unsupported:
-                    raise NotImplementedError(
-                        f"the complex adjoint reaches {', '.join(_complex.GRADIENT_FIELDS)} and the sequence; "
-                        f"{unsupported} also require grad"
-                    )
-                buffers = [tissue.buffers[_native.TISSUE_INDEX[name]] if name in reversed_fields else None
-                           for name in _complex.GRADIENT_FIELDS]
-                signal = _complex.FusedComplexSignal.apply(
-                    packed.flip.to(device), packed.phase.to(device), packed.duration.to(device), *buffers, packed,
-                    tissue, states, factor, int(sequence.get("checkpoint_events", 0)),
-                )
-                return signal, None
-            return _complex.simulate_complex(packed, tissue, states=states, columns=differentiated, factor=factor)
-        if sequence_grad or reversed_fields:
-            unsupported = [name for name in reversed_fields if name not in _engine.GRADIENT_FIELDS]
+        if not differentiated and (reverse or forward_mode):
+            # the plain signal under reverse or forward mode: one Function whose
+            # backward takes the adjoint that carries what requires grad and
+            # whose jvp follows any direction
+            carried = set(_complex.GRADIENT_FIELDS)
+            unsupported = [name for name in reversed_fields + tangent_fields if name not in carried]
             if unsupported:
                 raise NotImplementedError(
-                    f"reverse mode reaches T1, T2, M0, B1, the inversion efficiency and the sequence; "
-                    f"{unsupported} also require grad"
+                    f"derivatives reach {', '.join(_complex.GRADIENT_FIELDS)} and the sequence's flips, phases "
+                    f"and durations; {unsupported} also carry one"
                 )
-            flip_in = packed.flip if flip is None else flip
+            signs = packed.__dict__.get("_flip_signs") if real else None
+            run = _complex.SignalRun(
+                packed=packed, tissue=tissue, states=states, factor=factor, real=real, flip_signs=signs,
+                samples=samples, checkpoint_pulses=int(sequence.get("checkpoint_pulses", 64)),
+                checkpoint_events=int(sequence.get("checkpoint_events", 0)),
+            )
+            signal = _complex.EpgSignal.apply(
+                packed.flip.to(device), packed.phase.to(device), packed.duration.to(device), *tissue.buffers, run,
+            )
+            return signal, None
+        if differentiated and forward_mode:
+            raise NotImplementedError(
+                "a Jacobian is already a set of forward-mode columns; take torch.func.jvp of simulate() instead"
+            )
+        if not real:
+            if reverse:
+                raise NotImplementedError(
+                    "a Jacobian under reverse mode (forward-over-reverse) runs on the real-subspace kernel "
+                    "for T1, T2, M0, B1 and the inversion efficiency; this protocol or tissue takes the "
+                    "complex kernel, whose adjoint is first order"
+                )
+            return _complex.simulate_complex(packed, tissue, states=states, columns=differentiated, factor=factor)
+        if reverse:
+            unsupported = [name for name in reversed_fields if name not in _engine.GRADIENT_FIELDS]
+            if unsupported or packed.phase.requires_grad:
+                raise NotImplementedError(
+                    f"a Jacobian under reverse mode reaches T1, T2, M0, B1, the inversion efficiency and the "
+                    f"sequence's flips and durations; {unsupported or ['phase']} also require grad"
+                )
+            signs = packed.__dict__.get("_flip_signs")
+            flip_in = packed.flip if signs is None else packed.flip * signs
             buffers = [tissue.buffers[_native.TISSUE_INDEX[name]] if name in reversed_fields else None
                        for name in _engine.GRADIENT_FIELDS]
             signal, jacobian = _engine.FusedRealJacobian.apply(
@@ -369,6 +391,10 @@ class Simulator:
 
     def simulate(self, **values: Any) -> Any:
         """Return the recorded signal, shaped like the caller's property maps."""
+        if _transformed():
+            keys = tuple(name for name, value in values.items() if isinstance(value, torch.Tensor))
+            others = {name: value for name, value in values.items() if name not in keys}
+            return _TransformBridge.apply(self, keys, others, *(values[name] for name in keys))
         backend = _backend(values)
         held, sequence = self._split({**self.bound, **values})
         batch = self._batch(held)
@@ -385,6 +411,11 @@ class Simulator:
             not floating point.
         """
         names = (diff,) if isinstance(diff, str) else tuple(diff)
+        if _transformed():
+            raise NotImplementedError(
+                "a Jacobian is already a set of forward-mode columns; under torch.func take the transform of "
+                "simulate() instead"
+            )
         backend = _backend(values)
         held, sequence = self._split({**self.bound, **values})
         for name in names:
@@ -457,3 +488,143 @@ class Simulator:
             raise NotImplementedError(f"{type(self).__name__} must define linearize")
         signal, jacobian = self._run(held, sequence, names)
         return signal, jacobian
+
+
+def _transformed() -> bool:
+    """Whether a torch.func transform (jvp, grad, vjp) is running: host work
+    (packing a protocol, its real-subspace verdict) cannot read tensors there,
+    so a simulation then runs as one Function of the caller's tensors."""
+    try:
+        return torch._C._functorch.peek_interpreter_stack() is not None
+    except AttributeError:  # pragma: no cover - a torch without functorch
+        return False
+
+
+class _TransformBridge(torch.autograd.Function):
+    """``Simulator.simulate`` as one Function of the caller's tensors, for
+    torch.func transforms: its forward runs the ordinary simulation on plain
+    tensors; its backward runs it again under autograd (the adjoint kernels);
+    its jvp carries the caller's tangents onto the packed events and the
+    prepared tissue buffers (a double backward through those cheap host-side
+    maps) and runs the JVP kernel."""
+
+    @staticmethod
+    def forward(simulator, keys, others, *tensors):
+        return simulator.simulate(**others, **dict(zip(keys, tensors)))
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        simulator, keys, others, *tensors = inputs
+        ctx.simulator, ctx.keys, ctx.others = simulator, keys, others
+        ctx.save_for_backward(*tensors)
+        ctx.save_for_forward(*tensors)
+
+    @staticmethod
+    def backward(ctx, grad):
+        needs = tuple(bool(n) for n in ctx.needs_input_grad[3:])
+        grads = _BridgeGradient.apply(ctx.simulator, ctx.keys, ctx.others, needs, grad, *ctx.saved_tensors)
+        return (None, None, None, *grads)
+
+    @staticmethod
+    def jvp(ctx, _simulator, _keys, _others, *tangents):
+        return _BridgeTangent.apply(ctx.simulator, ctx.keys, ctx.others, len(tangents), *ctx.saved_tensors,
+                                    *tangents)
+
+
+class _BridgeGradient(torch.autograd.Function):
+    """The gradients behind ``_TransformBridge.backward``: the simulation run
+    again under autograd on plain tensors (the adjoint kernels), in a
+    Function's forward where no transform intercepts it."""
+
+    @staticmethod
+    def forward(simulator, keys, others, needs, grad, *tensors):
+        with torch.enable_grad():
+            leaves = [t.detach().requires_grad_(n and torch.is_floating_point(t)) for t, n in zip(tensors, needs)]
+            out = simulator.simulate(**others, **dict(zip(keys, leaves)))
+            wanted = [leaf for leaf in leaves if leaf.requires_grad]
+            found = iter(torch.autograd.grad(out, wanted, grad, allow_unused=True) if wanted else ())
+            grads = [next(found) if leaf.requires_grad else None for leaf in leaves]
+        return tuple(torch.zeros_like(t) if n and g is None else g for t, n, g in zip(tensors, needs, grads))
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        pass
+
+    @staticmethod
+    def backward(ctx, *grads):
+        raise NotImplementedError("a gradient is not differentiated again here")
+
+
+def _directional(outputs: list, leaves: list, tangents: list) -> list:
+    """J v for the map leaves -> outputs by a double backward (the maps are
+    cheap torch operations; a None output or one the leaves do not reach
+    gets None)."""
+    live = [(i, out) for i, out in enumerate(outputs) if isinstance(out, torch.Tensor) and out.requires_grad]
+    result: list = [None] * len(outputs)
+    if not live or not leaves:
+        return result
+    probes = [torch.zeros_like(out, requires_grad=True) for _, out in live]
+    grads = torch.autograd.grad([out for _, out in live], leaves, probes, create_graph=True, allow_unused=True)
+    total = None
+    for grad, tangent in zip(grads, tangents):
+        if grad is None or not grad.requires_grad:
+            continue
+        term = (grad * tangent.to(grad.dtype)).sum()
+        total = term if total is None else total + term
+    if total is None:
+        return result
+    reach = torch.autograd.grad(total, probes, allow_unused=True)
+    for (i, _), value in zip(live, reach):
+        result[i] = value
+    return result
+
+
+class _BridgeTangent(torch.autograd.Function):
+    """The tangent behind ``_TransformBridge.jvp``, a Function of its own so
+    that its forward sees plain tensors."""
+
+    @staticmethod
+    def forward(simulator, keys, others, count, *both):
+        from ..sequence import _complex
+
+        primals, tangents = both[:count], both[count:]
+        if simulator.signal_factor is None:
+            raise NotImplementedError(
+                f"torch.func.jvp of {type(simulator).__name__}, whose signal post-processes the samples: "
+                "use torch.autograd.forward_ad (dual tensors) instead"
+            )
+        with torch.enable_grad():
+            leaves, seeds = [], []
+            values = dict(others)
+            for name, primal, tangent in zip(keys, primals, tangents):
+                if tangent is not None and torch.is_floating_point(primal):
+                    primal = primal.detach().requires_grad_(True)
+                    leaves.append(primal)
+                    seeds.append(tangent)
+                values[name] = primal
+            held, sequence = simulator._split({**simulator.bound, **values})
+            device, packed, _, tissue, states, _, _, _ = simulator._prepared(held, sequence)
+            carried = [_native.TISSUE_INDEX[name] for name in _complex.GRADIENT_FIELDS]
+            targets = [packed.duration, packed.flip, packed.phase] + [tissue.buffers[i] for i in carried]
+            moved = _directional(targets, leaves, seeds)
+        for index, value in zip(carried, moved[3:]):
+            if value is not None and tissue.buffers[index] is None:
+                raise NotImplementedError("a tangent reached a tissue property the simulation leaves out")
+        factor = simulator.signal_factor
+        plain = dataclasses.replace(tissue, buffers=[None if b is None else b.detach() for b in tissue.buffers])
+        tangent = _complex.jvp_complex(
+            packed, plain, states=states, factor=factor,
+            tissue_tangents={name: value for name, value in zip(_complex.GRADIENT_FIELDS, moved[3:]) if value is not None},
+            flip=packed.flip.detach(), phase=packed.phase.detach(), duration=packed.duration.detach(),
+            duration_tangent=moved[0], flip_tangent=moved[1], phase_tangent=moved[2],
+        )
+        batch = simulator._batch(held)
+        return tangent.reshape(*batch, tangent.shape[-1])
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        pass
+
+    @staticmethod
+    def backward(ctx, grad):
+        raise NotImplementedError("a directional derivative is not differentiated again here")

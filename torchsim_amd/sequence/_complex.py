@@ -5,7 +5,9 @@ checkpointed adjoint behind an autograd Function."""
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 
+import numpy as np
 import torch
 
 from .. import _native
@@ -140,47 +142,194 @@ def adjoint_complex(
     return grad_flip, grad_phase, grad_duration, grads
 
 
-class FusedComplexSignal(torch.autograd.Function):
-    """The complex kernel's signal, differentiable in the event flips, phases
-    and durations and in the per-atom fields of GRADIENT_FIELDS (reference
-    sequence/_accelerators.py:1402-1586: ``_NativeEpg``'s forward and its
-    first-order backward through ``simulate_vjp``, here one forward kernel and
-    one checkpointed adjoint kernel). The tissue inputs are the prepared
-    per-atom buffers the kernels read, so autograd carries their gradients
-    back to the caller's maps."""
+def jvp_complex(
+    packed: PackedEvents,
+    tissue: PreparedTissue,
+    *,
+    states: int,
+    factor: complex,
+    tissue_tangents: dict[str, torch.Tensor],
+    flip: torch.Tensor | None = None,
+    phase: torch.Tensor | None = None,
+    duration: torch.Tensor | None = None,
+    flip_tangent: torch.Tensor | None = None,
+    phase_tangent: torch.Tensor | None = None,
+    duration_tangent: torch.Tensor | None = None,
+) -> torch.Tensor:
+    """The directional derivative (atoms, outputs) complex64 of the signal
+    ``factor * sample`` along a tangent on tissue fields (``tissue_tangents``:
+    field name -> a tensor laid out like that field's prepared buffer) and on
+    the events' flips, phases and durations ((events,) each, or None)."""
+    device = tissue.device
+    _native.require_device(device)
+    lib = _native.load()
+    events = _events(packed, device, flip, phase, duration)
+    atoms, outputs = tissue.atoms, packed.output_count
+    tangent = torch.empty((atoms, outputs), dtype=torch.complex64, device=device)
+    if atoms == 0 or outputs == 0:
+        return tangent
+    held = []
+    slots = (ctypes.c_void_p * (_native.TISSUE_COUNT + 3))()
+    for name, value in tissue_tangents.items():
+        if name not in GRADIENT_FIELDS:
+            raise NotImplementedError(f"the complex JVP carries no tangent along {name!r}")
+        buffer = tissue.buffers[_native.TISSUE_INDEX[name]]
+        flat = value.detach().to(device=device, dtype=torch.float32).reshape(-1)
+        if buffer is not None and flat.numel() != buffer.numel():
+            flat = flat.expand(buffer.numel())
+        flat = flat.contiguous()
+        held.append(flat)
+        slots[_native.TISSUE_INDEX[name]] = flat.data_ptr()
+    for offset, value in enumerate((duration_tangent, flip_tangent, phase_tangent)):
+        if value is not None:
+            flat = value.detach().to(device=device, dtype=torch.float32).reshape(-1).contiguous()
+            held.append(flat)
+            slots[_native.TISSUE_COUNT + offset] = flat.data_ptr()
+    ptrs = _native.pointers([*(None if b is None else b.detach() for b in tissue.buffers), *events])
+    with torch.cuda.device(device):
+        status = lib.tsim_complex_jvp(
+            ptrs, atoms, packed.event_count, states, outputs, 2 if tissue.exchanging else 0, tissue.features,
+            tissue.stride, float(complex(factor).real), float(complex(factor).imag), slots,
+            _native.TISSUE_COUNT + 3, None, tangent.data_ptr(), _native.stream_of(device),
+        )
+    _native.check(status)
+    return tangent
+
+
+@dataclasses.dataclass
+class SignalRun:
+    """What a signal evaluation fixes besides its tensors: the protocol, the
+    tissue's layout (its buffers are the Function's inputs), the state count,
+    the output factor, the real-subspace verdict with its flip and sample
+    signs, and the checkpoint spacings of the two adjoints."""
+
+    packed: PackedEvents
+    tissue: PreparedTissue
+    states: int
+    factor: complex
+    real: bool
+    flip_signs: torch.Tensor | None
+    samples: torch.Tensor | None
+    checkpoint_pulses: int = 64
+    checkpoint_events: int = 0
+
+    def with_buffers(self, buffers) -> PreparedTissue:
+        return dataclasses.replace(self.tissue, buffers=list(buffers))
+
+
+class EpgSignal(torch.autograd.Function):
+    """The recorded signal (atoms, outputs), complex64, as a function of the
+    events' flips, phases and durations and of the 17 prepared tissue
+    buffers: the reference's ``_NativeEpg`` (sequence/_accelerators.py:
+    1402-1594) -- forward on the real-subspace kernel where the verdict
+    allows it, else on the complex kernel; ``backward`` through the
+    real-subspace adjoint when what requires grad is among what it carries
+    (flips, durations, T1, T2, M0, B1, inversion efficiency), else through
+    the complex adjoint (adding phases, B1phase, B0, diffusion); ``jvp``
+    along any direction of those inputs through the complex JVP kernel
+    (reference ``_NativeEpg.jvp``, :1511-1542), under ``torch.func.jvp`` and
+    ``torch.autograd.forward_ad`` alike."""
 
     @staticmethod
-    def forward(ctx, flip, phase, duration, t1, t2, m0, b1, b1_phase, b0, ie, diffusion, packed, tissue, states,
-                factor, segment_events):
-        signal, _ = simulate_complex(packed, tissue, states=states, factor=factor, flip=flip, phase=phase,
-                                     duration=duration)
-        ctx.save_for_backward(flip, phase, duration)
-        ctx.shapes = [None if v is None else v.shape for v in (t1, t2, m0, b1, b1_phase, b0, ie, diffusion)]
-        ctx.packed = packed
-        ctx.tissue = tissue
-        ctx.states = states
-        ctx.factor = complex(factor)
-        ctx.segment_events = segment_events
+    def forward(flip, phase, duration, *rest):
+        *buffers, run = rest
+        tissue = run.with_buffers(buffers)
+        if run.real:
+            signed = flip if run.flip_signs is None else flip * run.flip_signs.to(flip.device)
+            from . import _engine
+
+            signal, _ = _engine.simulate_real(run.packed, tissue, states=run.states, factor=run.factor,
+                                              flip=signed, samples=run.samples, duration=duration)
+            return signal
+        signal, _ = simulate_complex(run.packed, tissue, states=run.states, factor=run.factor, flip=flip,
+                                     phase=phase, duration=duration)
         return signal
 
     @staticmethod
+    def setup_context(ctx, inputs, output):
+        flip, phase, duration, *rest = inputs
+        *buffers, run = rest
+        ctx.save_for_backward(flip, phase, duration, *buffers)
+        ctx.save_for_forward(flip, phase, duration, *buffers)
+        ctx.run = run
+
+    @staticmethod
     def backward(ctx, grad_signal):
-        flip, phase, duration = ctx.saved_tensors
-        wanted = tuple(name for name, need in zip(GRADIENT_FIELDS, ctx.needs_input_grad[3:11]) if need)
-        grad_flip, grad_phase, grad_duration, grads = adjoint_complex(
-            ctx.packed, ctx.tissue, states=ctx.states, factor=ctx.factor, cotangent=grad_signal, flip=flip,
-            phase=phase, duration=duration, segment_events=ctx.segment_events, tissue_wanted=wanted,
-        )
-        tissue_out = []
-        for name, shape in zip(GRADIENT_FIELDS, ctx.shapes):
-            g = grads.get(name)
-            if g is not None and shape is not None and g.numel() != torch.Size(shape).numel():
-                g = g.sum().reshape(shape)  # one value for every atom
-            tissue_out.append(g)
+        from . import _engine
+
+        run = ctx.run
+        flip, phase, duration, *buffers = ctx.saved_tensors
+        tissue = run.with_buffers(buffers)
+        needs = ctx.needs_input_grad
+        wanted = tuple(name for name, need in zip(_native.TISSUE_NAMES, needs[3:3 + _native.TISSUE_COUNT]) if need)
+        out_tissue: list = [None] * _native.TISSUE_COUNT
+        shifting = run.states < 2 and bool(((run.packed.action.cpu().numpy().astype(np.int32) & 19) != 0).any())
+        if run.real and not needs[1] and not shifting and all(name in _engine.GRADIENT_FIELDS for name in wanted):
+            # every output is c v, v real, c = factor i (sample sign): the
+            # loss's derivative along v is Re(conj(c) grad)
+            scale = torch.tensor(complex(run.factor) * 1j, dtype=torch.complex64, device=grad_signal.device).conj()
+            if run.samples is not None:
+                scale = scale * run.samples.to(device=grad_signal.device, dtype=torch.complex64)
+            signs = None if run.flip_signs is None else run.flip_signs.to(flip.device)
+            signed = flip if signs is None else flip * signs
+            zero = _engine.zero_intervals_reached(run.packed) if needs[2] else None
+            g_flip, g_duration, grads = _engine.adjoint_real(
+                run.packed, tissue, states=run.states, columns=(), flip=signed, duration=duration,
+                signal_cotangent=(grad_signal * scale).real, jacobian_cotangent=None,
+                checkpoint_pulses=run.checkpoint_pulses, tissue_wanted=wanted, zero_intervals=zero)
+            if signs is not None:
+                g_flip = g_flip * signs
+            g_phase = None
+        else:
+            g_flip, g_phase, g_duration, grads = adjoint_complex(
+                run.packed, tissue, states=run.states, factor=run.factor, cotangent=grad_signal, flip=flip,
+                phase=phase, duration=duration, segment_events=run.checkpoint_events, tissue_wanted=wanted)
+        for name, g in grads.items():
+            index = _native.TISSUE_INDEX[name]
+            buffer = buffers[index]
+            if g is not None and buffer is not None and g.numel() != buffer.numel():
+                g = g.sum().reshape(buffer.shape)  # one value for every atom
+            out_tissue[index] = g
         return (
-            grad_flip.to(flip.dtype) if ctx.needs_input_grad[0] else None,
-            grad_phase.to(phase.dtype) if ctx.needs_input_grad[1] else None,
-            grad_duration.to(duration.dtype) if ctx.needs_input_grad[2] else None,
-            *tissue_out,
-            None, None, None, None, None,
+            g_flip.to(flip.dtype) if needs[0] else None,
+            g_phase.to(phase.dtype) if needs[1] and g_phase is not None else None,
+            g_duration.to(duration.dtype) if needs[2] else None,
+            *out_tissue,
+            None,
         )
+
+    @staticmethod
+    def jvp(ctx, d_flip, d_phase, d_duration, *rest):
+        *d_buffers, _ = rest
+        flip, phase, duration, *buffers = ctx.saved_tensors
+        return _EpgTangent.apply(flip, phase, duration, d_flip, d_phase, d_duration, ctx.run, *buffers, *d_buffers)
+
+
+class _EpgTangent(torch.autograd.Function):
+    """The complex JVP kernel behind ``EpgSignal.jvp``: a Function of its
+    own so that its forward sees plain tensors whichever transform asked."""
+
+    @staticmethod
+    def forward(flip, phase, duration, d_flip, d_phase, d_duration, run, *both):
+        count = _native.TISSUE_COUNT
+        buffers, d_buffers = both[:count], both[count:]
+        tissue = run.with_buffers(buffers)
+        tangents = {}
+        for name, value in zip(_native.TISSUE_NAMES, d_buffers):
+            if value is None:
+                continue
+            if name not in GRADIENT_FIELDS:
+                raise NotImplementedError(f"forward mode carries no tangent along {name!r}")
+            tangents[name] = value
+        return jvp_complex(run.packed, tissue, states=run.states, factor=run.factor, tissue_tangents=tangents,
+                           flip=flip, phase=phase, duration=duration, flip_tangent=d_flip,
+                           phase_tangent=d_phase, duration_tangent=d_duration)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        pass
+
+    @staticmethod
+    def backward(ctx, grad):
+        raise NotImplementedError("a directional derivative is not differentiated again (forward-over-reverse "
+                                  "runs through Simulator.jacobian with reverse mode)")

@@ -17,6 +17,7 @@ from typing import Any
 
 import numpy as np
 import torch
+from torch.autograd import forward_ad
 
 from .. import _native
 from ._events import INVERSION, RECORD, PackedEvents, resident
@@ -67,11 +68,20 @@ class PreparedTissue:
         return self.buffers[0].device
 
 
+def carries_derivative(value: Any) -> bool:
+    """Whether a value carries a derivative: it requires grad, or it holds a
+    forward-mode tangent (a dual number sits at its primal, which may be a
+    property's identity, and its term must not be dropped there)."""
+    if not isinstance(value, torch.Tensor):
+        return False
+    return value.requires_grad or forward_ad.unpack_dual(value).tangent is not None
+
+
 def _at_identity(name: str, value: Any) -> bool:
     if name not in _IDENTITY:
         return False
     if isinstance(value, torch.Tensor):
-        if value.requires_grad:
+        if carries_derivative(value):
             return False
         if value.numel() == 1:
             return float(value.reshape(())) == _IDENTITY[name]
@@ -83,7 +93,7 @@ def _on_device(value: Any, device: torch.device) -> torch.Tensor:
     """A property as a float32 tensor on ``device``; numbers and host scalars
     become a device fill (no host-to-device copy, so no wait on the stream)."""
     if isinstance(value, torch.Tensor):
-        if value.device.type == "cpu" and value.numel() == 1 and not value.requires_grad:
+        if value.device.type == "cpu" and value.numel() == 1 and not carries_derivative(value):
             return torch.full(value.shape, float(value.reshape(())), dtype=torch.float32, device=device)
         return value.to(device=device, dtype=torch.float32)
     if isinstance(value, (int, float, np.floating, np.integer)):
@@ -188,6 +198,7 @@ def real_subspace(packed: PackedEvents, tissue: PreparedTissue,
         else:
             _, flips, samples = turns
             flip = packed.flip.detach().cpu() * flips.to(torch.float32) if bool((flips != 1).any()) else None
+            packed.__dict__["_flip_signs"] = flips.to(torch.float32) if flip is not None else None
             if samples is not None and bool((samples == 1).all()):
                 samples = None
             if samples is not None:
@@ -496,6 +507,13 @@ def adjoint_real(
     event gets its own)."""
     device = tissue.device
     _native.require_device(device)
+    if states < 2:
+        # a stream without shifts never fills order 1, so two orders give the
+        # one-order answer exactly
+        shifts = (packed.action.cpu().numpy().astype(np.int32) & (1 | 2 | 16)) != 0
+        if shifts.any():
+            raise NotImplementedError("the real-subspace adjoint carries at least 2 orders and this stream shifts")
+        states = 2
     lib = _native.load()
     events = list(packed.on(device))
     events[2] = flip.detach().to(device=device, dtype=torch.float32).contiguous()

@@ -22,6 +22,7 @@ from typing import Any
 
 import numpy as np
 import torch
+from torch.autograd import forward_ad
 
 # Bits of the packed action word beyond EventAction's own (reference
 # src/torchsim/_epg_cpu.cpp:36-41; sequence/_accelerators.py EXCITATION=64,
@@ -250,7 +251,7 @@ class PackedEvents:
         the current stream); every call makes the current stream wait for it,
         so callers use the buffers on their own stream without a host wait.
         """
-        if any(value.requires_grad for value in self.buffers()):
+        if any(value.requires_grad or forward_ad.unpack_dual(value).tangent is not None for value in self.buffers()):
             return [value.to(device).contiguous() for value in self.buffers()]
         return resident(self, "events", self.buffers(), device)
 
@@ -335,13 +336,19 @@ def host_values(*arrays: Any) -> tuple[Any, ...]:
     the packed buffers are bit for bit those the tensors would give, at a
     small fraction of the per-operation cost of a layout of a thousand pulses.
     """
-    if any(isinstance(a, torch.Tensor) and a.requires_grad for a in arrays):
+    if any(_carries(a) for a in arrays):
         return arrays
     return tuple(a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else a for a in arrays)
 
 
+def _carries(value: Any) -> bool:
+    """A tensor that requires grad or holds a forward-mode tangent."""
+    return isinstance(value, torch.Tensor) and (
+        value.requires_grad or forward_ad.unpack_dual(value).tangent is not None)
+
+
 def _graphed(values: Sequence[Any]) -> bool:
-    return any(isinstance(value, torch.Tensor) and value.requires_grad for value in values)
+    return any(_carries(value) for value in values)
 
 
 def _stack(values: Sequence[Any]) -> torch.Tensor:

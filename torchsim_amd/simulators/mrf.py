@@ -9,6 +9,7 @@ import numpy as np
 import torch
 
 from ..model._simulator import Simulator, SpinPhysics, _protocol_key
+from ..sequence._engine import carries_derivative
 from ..sequence._events import INVERSION, UNBALANCED, Inversion, PackedEvents, host_values
 
 # one two-pulse template per protocol (everything but the flips and TRs)
@@ -78,8 +79,7 @@ class MRFSimulator(Simulator):
         generic packing.
         """
         flip, tr = sequence.get("flip"), sequence.get("TR")
-        other = [k for k, v in sequence.items()
-                 if k not in ("flip", "TR") and isinstance(v, torch.Tensor) and v.requires_grad]
+        other = [k for k, v in sequence.items() if k not in ("flip", "TR") and carries_derivative(v)]
         ti = sequence.get("TI", 0.0)
         phases = sequence.get("phases", 0.0)
         if (other or flip is None or tr is None or not isinstance(ti, (int, float))
@@ -117,7 +117,7 @@ class MRFSimulator(Simulator):
         # the zero intervals a graph reaches: only the TRs carry one, onto the
         # excitations after the first
         reached = np.zeros(1 + 4 * n, dtype=bool)
-        if isinstance(tr, torch.Tensor) and tr.requires_grad:
+        if carries_derivative(tr):
             carried = pulse[1:] + turn_at
             live = torch.zeros(1 + 4 * n, dtype=torch.float32).index_put(
                 (carried,), (spacing_s[:-1] - spacing_s[:-1].detach()).to(torch.float32))
