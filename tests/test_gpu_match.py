@@ -152,8 +152,9 @@ def test_no_voxels_and_shapes():
     torch.testing.assert_close(volume.indices[..., 0], torch.tensor([[1, 2], [3, 4]], device="cuda"))
     with pytest.raises(ValueError):
         matcher.match(torch.zeros(3, 7, device="cuda"))
-    with pytest.raises(NotImplementedError):
-        _matcher(dictionary, top_k=4)
+    with pytest.raises(ValueError):
+        _matcher(dictionary, top_k=11)  # more candidates than atoms
+    assert _matcher(dictionary, top_k=4).match(dictionary[:2]).indices.shape == (2, 4)
 
 
 def test_matching_in_a_fitted_subspace():
@@ -189,3 +190,86 @@ def test_matching_in_a_fitted_subspace():
     assert (result.indices[:, 0] == pick).float().mean() > 0.3
     with pytest.raises(ValueError, match="rank 10"):
         matcher.from_coefficients(signals)
+
+
+@pytest.mark.parametrize("kind", ["real", "complex"])
+@pytest.mark.parametrize("top_k", [4, 9])
+def test_top_k_beyond_the_fused_kernel(kind, top_k):
+    """top_k above 3 runs on a library GEMM per chunk with an FP64 refine
+    (reference _dictionary.py:120-136, 289-290): same checks as the fused
+    path, chunks smaller than the dictionary and the voxel set."""
+    from torchsim_amd.estimators import DictionaryMatcher
+
+    rng = np.random.default_rng(70 + top_k)
+    atoms, contrasts, voxels = 2003, 33, 301
+    dictionary = rng.standard_normal((atoms, contrasts)).astype(np.float32)
+    if kind == "complex":
+        dictionary = (dictionary + 1j * rng.standard_normal((atoms, contrasts))).astype(np.complex64)
+    pick = rng.integers(0, atoms, voxels)
+    gain = (rng.uniform(0.2, 3.0, voxels) * np.exp(2j * np.pi * rng.uniform(size=voxels))).astype(np.complex64)
+    noise = 0.3 * (rng.standard_normal((voxels, contrasts)) + 1j * rng.standard_normal((voxels, contrasts)))
+    signals = (gain[:, None] * dictionary[pick] + noise).astype(np.complex64)
+    matcher = DictionaryMatcher(dictionary=torch.from_numpy(dictionary).cuda(), top_k=top_k,
+                                query_chunk_size=128, dictionary_chunk_size=700)
+    result = matcher.match(torch.from_numpy(signals).cuda())
+    _check(dictionary, signals, top_k, result)
+
+
+def _fingerprints():
+    """A fingerprinting dictionary over a T1/T2 grid (as the reference's
+    grouped-matching tests build it) and noisy voxels off it."""
+    from torchsim_amd.simulators import MRFSimulator
+
+    generator = torch.Generator().manual_seed(0)
+    flip = 10.0 + 50.0 * torch.rand(200, generator=generator)
+    grid = torch.cartesian_prod(torch.linspace(200.0, 3000.0, 60), torch.linspace(20.0, 300.0, 30))
+    atoms = MRFSimulator(TR=10.0, TI=20.0).simulate(flip=flip, T1=grid[:, 0].cuda(), T2=grid[:, 1].cuda())
+    generator = torch.Generator().manual_seed(7)
+    picked = atoms[::37].cpu()
+    noise = torch.randn(picked.shape, generator=generator, dtype=picked.dtype)
+    return atoms, grid.cuda(), (picked + 0.01 * float(atoms.abs().max()) * noise).cuda()
+
+
+@pytest.mark.parametrize("top_k", [1, 3])
+def test_grouped_matching_matches_the_oracle(top_k):
+    """groups=24 at the default prune: the grouping is the oracle's (same
+    members), and each voxel's answer is the oracle's grouped search."""
+    from torchsim_amd.estimators import DictionaryMatcher
+
+    atoms, grid, measured = _fingerprints()
+    matcher = DictionaryMatcher(dictionary=atoms, parameters=grid, groups=24, top_k=top_k)
+    members, _ = oracle.grouping(atoms.cpu().numpy(), 24)
+    got = [set(m.cpu().tolist()) for m in matcher.grouping.members]
+    assert got == [set(m.tolist()) for m in members]
+    result = matcher.match(measured)
+    indices, scores = oracle.match_grouped(atoms.cpu().numpy(), measured.cpu().numpy(), 24, matcher.prune, top_k)
+    np.testing.assert_array_equal(result.indices.cpu().numpy(), indices)
+    np.testing.assert_allclose(result.scores.cpu().numpy(), scores, rtol=0, atol=1e-6)
+
+
+def test_grouped_matching_finds_what_direct_matching_finds():
+    """Reference tests/estimators/test_grouped.py:81-100: most voxels get the
+    same parameters, and where they differ the error is a grid step."""
+    from torchsim_amd.estimators import DictionaryMatcher
+
+    atoms, grid, measured = _fingerprints()
+    direct = DictionaryMatcher(dictionary=atoms, parameters=grid)(measured)
+    grouped = DictionaryMatcher(dictionary=atoms, parameters=grid, groups=24)(measured)
+    assert float((direct == grouped).all(-1).float().mean()) > 0.8
+    for column in range(grid.shape[-1]):
+        assert float(((grouped[:, column] - direct[:, column]) / direct[:, column]).abs().mean()) < 0.01
+    wide = DictionaryMatcher(dictionary=atoms, parameters=grid, groups=24, prune=0.999999)(measured)
+    torch.testing.assert_close(wide, direct)  # keeping every group is the exhaustive search
+
+
+def test_grouped_matching_in_a_fitted_subspace():
+    """Grouping happens inside the global basis (reference test_grouped.py:
+    215-260): the atoms are clustered as rank-8 coefficients."""
+    from torchsim_amd.estimators import DictionaryMatcher
+
+    atoms, grid, measured = _fingerprints()
+    matcher = DictionaryMatcher(groups=24).fit(atoms, grid, rank=8)
+    assert matcher.grouping.representative.shape == (24, 8)
+    direct = DictionaryMatcher().fit(atoms, grid, rank=8)(measured)
+    grouped = matcher(measured)
+    assert float((direct == grouped).all(-1).float().mean()) > 0.8
