@@ -362,6 +362,24 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
 int tsim_batch_request_size(void);
 
 /*
+ * The uneven-batch launcher of the complex kernel: requests with their own
+ * event streams, state counts (padded to the lanes of the batch's largest),
+ * atom counts, outputs and POOL COUNTS -- a request whose packed buffer
+ * T_POOL_B_FRACTION is set runs the exchanging two-pool body, the others the
+ * single-pool one -- in ONE persistent launch of `grid_blocks` workgroups
+ * (<= 0: one per work block of 4 (64 / lanes) atoms). Each request writes
+ * its signal (times factor) and Jacobian as tsim_complex_jacobian does,
+ * columns among T_T1, T_T2, T_M0, T_B1; all requests of a batch share one
+ * tangent set (the caller groups them). sample_sign, data and normal must be
+ * NULL. Every atom runs the arithmetic it would run alone, so results are
+ * bitwise those of tsim_complex_jacobian. No reference counterpart: the
+ * reference runs one launch per call and pool model
+ * (sequence/_accelerators.py:3262-3449, model/_physics.py:214-262).
+ */
+int tsim_complex_batch(const tsim_batch_request* requests, int64_t count, int64_t grid_blocks,
+                       void* stream);
+
+/*
  * One fixed-iteration Levenberg-Marquardt step of every voxel, in two
  * launches around the normal equations at the candidate (no reference
  * counterpart as entry points: they fuse the per-voxel bookkeeping of

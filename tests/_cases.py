@@ -92,3 +92,50 @@ def phantom(shape=(256, 256, 8), *, classes=6, seed=0, device="cpu"):
         smooth = smooth / smooth.abs().max()
         maps.append((base * (1.0 + 0.05 * smooth)).to(torch.float32).to(device))
     return tuple(maps)
+
+
+def gradient_echo(flips_deg, tr_ms, te_ms, *, spoiled=True, states=48):
+    """A gradient-echo train as a simulator: each pulse, its sample TE later,
+    then ideal spoiling (or one unbalanced gradient), then the rest of TR."""
+    from torchsim_amd.model._simulator import Simulator, SpinPhysics
+    from torchsim_amd.sequence._events import Delay, Dephase, Excitation, Readout, Spoil, module
+
+    class GradientEcho(Simulator):
+        model = SpinPhysics(properties={"T1": "t1_ms", "T2": "t2_ms", "M0": "m0", "B1": "b1"})
+        signal_factor = 1.0
+
+        def layout(self, *, flip, TR, TE, spoiled=True):
+            angles = np.deg2rad(np.atleast_1d(np.asarray(flip, dtype=np.float64)))
+            trs = np.broadcast_to(np.asarray(TR, dtype=np.float64), angles.shape)
+            parts = []
+            for angle, tr in zip(angles, trs):
+                after = Spoil() if spoiled else Dephase()
+                parts.append(module(Excitation(float(angle)), Delay(TE * 1e-3), Readout(0.0), after,
+                                    Delay((tr - TE) * 1e-3), duration_s=tr * 1e-3))
+            return parts
+
+    GradientEcho.states = states
+    return GradientEcho(flip=np.asarray(flips_deg, dtype=np.float64), TR=tr_ms, TE=float(te_ms),
+                        spoiled=bool(spoiled))
+
+
+def sampled_spin_echo(echoes=4, spacing_ms=8.0, offsets_ms=(-2.5, 0.0, 3.5), states=48):
+    """A refocused train (90 deg at a quarter turn, crushed 180s) sampled off
+    the echo centres as well as on them."""
+    from torchsim_amd.model._simulator import Simulator, SpinPhysics
+    from torchsim_amd.sequence._events import Excitation, Readout, Refocusing
+
+    class SampledSpinEcho(Simulator):
+        model = SpinPhysics(properties={"T1": "t1_ms", "T2": "t2_ms", "M0": "m0"})
+        signal_factor = 1.0
+
+        def layout(self, *, echoes, ESP, offsets):
+            parts = [(0.0, Excitation(np.pi / 2, np.pi / 2))]
+            for k in range(echoes):
+                echo = (k + 1) * ESP * 1e-3
+                parts.append((echo - 0.5 * ESP * 1e-3, Refocusing(np.pi, 0.0)))
+                parts += [(echo + offset * 1e-3, Readout(0.0)) for offset in offsets]
+            return parts
+
+    SampledSpinEcho.states = states
+    return SampledSpinEcho(echoes=int(echoes), ESP=float(spacing_ms), offsets=tuple(float(o) for o in offsets_ms))

@@ -2,6 +2,8 @@
 
 from __future__ import annotations
 
+import dataclasses
+
 import numpy as np
 import pytest
 import torch
@@ -48,6 +50,93 @@ def test_batched_launch_matches_single_launches_bitwise():
     for item, (sig, jac) in zip(items, singles):
         assert torch.equal(item.signal, sig)
         assert torch.equal(item.jacobian, jac)
+
+
+def _complex_requests(diff=("T1", "T2", "M0")):
+    """Requests that take the complex kernel, one per feature it adds: phase
+    cycling with off-resonance, an exchanging pool with a chemical shift,
+    diffusion through a crusher geometry with off-resonance, off-resonance
+    alone; 6 to 13
+    orders, 1 to 2049 atoms. The TRs ramp, so the winding is not uniform and
+    the off-resonance stays in the states."""
+    from torchsim_amd.serving import SimulateRequest
+    from torchsim_amd.simulators import MRFSimulator
+
+    rng = np.random.default_rng(50)
+    requests = []
+    for k, (atoms, pulses, states, kind) in enumerate(
+            [(37, 90, 6, "cycled"), (500, 120, 10, "pool"), (1, 60, 13, "diffusion"), (2049, 150, 10, "b0")]):
+        t1, t2 = _cases.random_tissue(atoms, seed=70 + k)
+        props = {"T1": torch.tensor(t1).cuda(), "T2": torch.tensor(t2).cuda(),
+                 "M0": torch.tensor(rng.uniform(0.5, 1.0, atoms), dtype=torch.float32).cuda()}
+        sequence = {"flip": torch.tensor(_cases.benchmark_flips(pulses), dtype=torch.float32),
+                    "TR": torch.linspace(10.0, 14.0, pulses)}
+        if kind == "cycled":
+            sequence["phases"] = torch.tensor((np.arange(pulses) ** 2 * 7.5) % 360.0, dtype=torch.float32)
+        if kind in ("cycled", "b0", "diffusion"):
+            props["B0"] = torch.tensor(rng.uniform(-20.0, 20.0, atoms), dtype=torch.float32).cuda()
+        if kind == "pool":
+            props.update(poolB_fraction=torch.tensor(rng.uniform(0.05, 0.3, atoms), dtype=torch.float32).cuda(),
+                         poolB_exchange=torch.tensor(rng.uniform(5.0, 20.0, atoms), dtype=torch.float32).cuda(),
+                         poolB_T1=400.0, poolB_T2=15.0, poolB_shift=35.0)
+        if kind == "diffusion":
+            props.update(D=torch.tensor(rng.uniform(0.5, 3.0, atoms), dtype=torch.float32).cuda(),
+                         crusher_dephasing_rad=8 * np.pi, voxel_size_m=0.5e-3)
+        requests.append(SimulateRequest(MRFSimulator(states=states, **sequence), props, diff=diff))
+    return requests
+
+
+def test_complex_batch_matches_single_launches_bitwise():
+    """Single-pool, exchanging-pool, diffusing and phase-cycled requests of
+    different sizes and order counts in ONE persistent launch of the complex
+    kernel: every signal and Jacobian column equals its own launch bitwise,
+    with and without Jacobian columns."""
+    from torchsim_amd.serving import BatchItem, launch_batch
+    from torchsim_amd.sequence import _complex, _engine
+
+    for diff in (("T1", "T2", "M0"), ()):
+        items, singles = [], []
+        for request in _complex_requests(diff):
+            sim = request.simulator
+            held, sequence = sim._split({**sim.bound, **request.properties})
+            _, packed, _, tissue, states, columns, _, _, _ = sim._prepared(held, sequence, request.diff)
+            assert not _engine.real_subspace(packed, tissue)[0]
+            atoms, n = tissue.atoms, packed.output_count
+            sig = torch.full((atoms, n), float("nan"), dtype=torch.complex64, device="cuda")
+            jac = torch.full((atoms, len(columns), n), float("nan"), dtype=torch.complex64, device="cuda")
+            items.append(BatchItem(packed, tissue, states, columns, sim.signal_factor, signal=sig, jacobian=jac,
+                                   kernel="complex"))
+            singles.append(_complex.simulate_complex(packed, tissue, states=states, columns=columns,
+                                                     factor=sim.signal_factor))
+        print(f"case: complex batch, columns {diff}", flush=True)
+        assert len({item.variant() for item in items}) == 1
+        launch_batch(items, mode=0, stream=torch.cuda.current_stream(), grid_blocks=7)
+        torch.cuda.synchronize()
+        for item, (sig, jac) in zip(items, singles):
+            assert torch.equal(item.signal, sig)
+            if diff:
+                assert torch.equal(item.jacobian, jac)
+
+
+def test_a_complex_batch_refuses_what_it_does_not_run():
+    from torchsim_amd.serving import BatchItem, launch_batch
+    from torchsim_amd.sequence import _engine
+
+    request = _complex_requests()[0]
+    sim = request.simulator
+    held, sequence = sim._split({**sim.bound, **request.properties})
+    _, packed, _, tissue, states, _, _, _, _ = sim._prepared(held, sequence, ())
+    sig = torch.empty((tissue.atoms, packed.output_count), dtype=torch.complex64, device="cuda")
+    item = BatchItem(packed, tissue, states, ("inversion_efficiency",), 1j, signal=sig, jacobian=sig,
+                     kernel="complex")
+    with pytest.raises(ValueError, match="no Jacobian column"):
+        launch_batch([item], mode=0, stream=torch.cuda.current_stream())
+    with pytest.raises(ValueError, match="mode 0"):
+        launch_batch([dataclasses.replace(item, columns=())], mode=1, stream=torch.cuda.current_stream())
+    real = dataclasses.replace(item, columns=(), kernel="real")
+    with pytest.raises(ValueError, match="one kernel"):
+        launch_batch([dataclasses.replace(item, columns=()), real], mode=0, stream=torch.cuda.current_stream())
+    assert _engine is not None
 
 
 def test_batched_launch_replays_from_a_hip_graph():
@@ -202,8 +291,64 @@ def test_served_results_equal_the_api_and_fits_recover():
     assert float(((maps["T2"].cpu() - torch.tensor(t2)).abs() / torch.tensor(t2)).max()) < 1e-3
 
 
+def test_complex_and_closed_form_requests_same_alone_and_in_the_mix():
+    """bSSFP (closed form), exchanging-pool, diffusing, phase-cycled and
+    off-resonant simulate requests and a two-pool fit, each served alone and
+    all in one mix: the same bits, equal to the simulators' own calls; the
+    complex-kernel requests of one tangent set share launches."""
+    from torchsim_amd.serving import FitRequest, Server, SimulateRequest
+    from torchsim_amd.simulators import bSSFPSimulator
+
+    rng = np.random.default_rng(90)
+    t1 = torch.tensor(rng.uniform(300, 2500, (6, 5)), dtype=torch.float32).cuda()
+    t2 = torch.tensor(rng.uniform(20, 200, (6, 5)), dtype=torch.float32).cuda()
+    bssfp = bSSFPSimulator(flip=torch.linspace(5.0, 60.0, 40), TR=5.0, TE=2.5)
+    requests = [SimulateRequest(bssfp, {"T1": t1, "T2": t2}, diff=("T1", "T2")),
+                *_complex_requests(("T1", "T2")), *_complex_requests(())]
+    mrf = _mrf(80)
+    voxels = 24
+    known = {"poolB_fraction": torch.tensor(rng.uniform(0.05, 0.3, voxels), dtype=torch.float32).cuda(),
+             "poolB_exchange": torch.tensor(rng.uniform(5, 20, voxels), dtype=torch.float32).cuda(),
+             "poolB_T1": 400.0, "poolB_T2": 15.0}
+    pooled = mrf.simulate(T1=torch.tensor(rng.uniform(400, 2000, voxels), dtype=torch.float32).cuda(),
+                          T2=torch.tensor(rng.uniform(40, 200, voxels), dtype=torch.float32).cuda(), **known)
+    fit = FitRequest(mrf, pooled, bounds={"T1": (100.0, 5000.0), "T2": (10.0, 2000.0), "M0": (0.01, 2.0)},
+                     initial={"T1": 1000.0, "T2": 100.0, "M0": 0.8}, known=known, iterations=4)
+
+    def direct(request):
+        sim = request.simulator
+        if request.diff:
+            return sim.jacobian(request.diff, **request.properties)
+        return sim.simulate(**request.properties)
+
+    def same(a, b):
+        if isinstance(a, tuple):
+            return all(torch.equal(x, y) for x, y in zip(a, b))
+        if isinstance(a, dict):
+            return all(torch.equal(a[k], b[k]) for k in ("T1", "T2", "M0"))
+        return torch.equal(a, b)
+
+    alone = []
+    for request in [*requests, fit]:
+        with Server() as server:
+            alone.append(server.submit(request).result(timeout=120))
+    print("case: served alone", flush=True)
+    with Server() as mixed:
+        futures = [mixed.submit(request) for request in [*requests, fit]]
+        together = [f.result(timeout=120) for f in futures]
+        stats = mixed.stats()
+    print("case: served in the mix", stats, flush=True)
+    for k, request in enumerate(requests):
+        assert same(alone[k], together[k]), k
+        assert same(direct(request), together[k]), k
+    assert same(alone[-1], together[-1])
+    assert stats["completed"] == len(requests) + 1
+    assert stats["batched_jobs"] > stats["launches"]
+
+
 def test_unbatched_sequences_are_served_too():
-    """FSE (complex kernel) simulate and fit run through the ordinary path."""
+    """FSE (complex kernel): a simulate request takes the complex batch, a
+    fit the ordinary path."""
     from torchsim_amd.serving import FitRequest, Server, SimulateRequest
     from torchsim_amd.simulators import FSESimulator
 

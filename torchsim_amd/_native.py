@@ -201,6 +201,7 @@ SIGNATURES["tsim_complex_jvp"] = (
 )
 SIGNATURES["tsim_simulate_batch"] = (_I32, [ctypes.POINTER(BatchRequest), _I64, _I32, _I32, _I64, _P])
 SIGNATURES["tsim_batch_request_size"] = (_I32, [])
+SIGNATURES["tsim_complex_batch"] = (_I32, [ctypes.POINTER(BatchRequest), _I64, _I64, _P])
 
 _lib: ctypes.CDLL | None = None
 

@@ -105,6 +105,11 @@ struct Compiled {
 // Compile the reference-format event buffers (packed[17..22]) into EventCodes.
 int compile_packed(void* const* packed, int64_t event_count, hipStream_t stream, Compiled& out);
 
+// The same into a caller's buffer of `events` EventCodes (stream-ordered).
+int compile_stream(const float* duration, const int32_t* kind, const float* flip, const float* phase,
+                   const uint8_t* action, const int32_t* output_index, int64_t events, EventCode* out,
+                   hipStream_t stream);
+
 // Group compiled events into the real kernel's macro-ops (after compile_packed).
 int compile_ops(int64_t event_count, int64_t outputs, Compiled& out);
 
@@ -115,6 +120,11 @@ int compile_ops(int64_t event_count, int64_t outputs, Compiled& out);
 // the adjoint then returns those intervals' duration gradients.
 int pack_stream(const EventCode* codes, int64_t events, int64_t outputs, MacroOp* ops, hipStream_t stream,
                 int64_t* op_count = nullptr, const uint8_t* zero_mask = nullptr, int zero_mode = 0);
+
+// Copy a host table of `bytes` to `device` on stream `s` through pooled
+// pinned staging (no host wait; under graph capture the staging buffer is
+// kept for the graph's replays).
+int upload_table(const void* host, size_t bytes, void* device, hipStream_t s);
 
 void set_error(const char* fmt, ...);
 const char* last_error();

@@ -3,6 +3,9 @@
 #include "epg_complex_impl.h"
 
 #include <cstring>
+#include <vector>
+
+#include "../../include/torchsim_hip.h"
 
 namespace tsim {
 namespace {
@@ -82,6 +85,60 @@ int fill_complex(ComplexArgs& a, void* const* packed, int64_t atoms, int64_t eve
   return OK;
 }
 
+// Output staging: K samples per flush with the padded stream count, one
+// wave's stage within COMPLEX_STAGE_BYTES_PER_WAVE; K < 2 stores directly.
+// Returns the dynamic LDS bytes a block of `threads` needs.
+size_t set_staging(ComplexArgs& a, int lanes, int threads) {
+  a.stage_k_log2 = -1;
+  a.stage_s_log2 = 0;
+  if (a.out_real != nullptr || (a.signal == nullptr && a.jacobian == nullptr)) return 0;
+  int s_log2 = 0;
+  while ((1 << s_log2) < 1 + a.columns) ++s_log2;
+  const int64_t slot = (int64_t)(64 / lanes) * (1 << s_log2) * (int64_t)sizeof(float2);
+  int k_log2 = 0;
+  while (k_log2 < 4 && slot * (2 << k_log2) <= COMPLEX_STAGE_BYTES_PER_WAVE) ++k_log2;
+  if (k_log2 < 1) return 0;
+  a.stage_k_log2 = k_log2;
+  a.stage_s_log2 = s_log2;
+  return (size_t)(threads / 64) * (size_t)slot * ((size_t)1 << k_log2);
+}
+
+ComplexBatchLaunch pick_complex_batch(int lanes, int tm) {
+  switch (lanes) {
+#define TSIM_COMPLEX_BATCH_CASE(LANES) \
+  case LANES:                          \
+    return pick_complex_batch_##LANES(tm);
+    TSIM_COMPLEX_LANES(TSIM_COMPLEX_BATCH_CASE)
+#undef TSIM_COMPLEX_BATCH_CASE
+    default:
+      return nullptr;
+  }
+}
+
+// The tangent mask and column slots of a request's columns (t1, t2, m0, b1).
+int complex_columns(const int32_t* columns, int64_t count, int& tm, int (&cols)[4]) {
+  tm = 0;
+  for (int k = 0; k < 4; ++k) cols[k] = -1;
+  for (int64_t c = 0; c < count; ++c) {
+    int slot;
+    switch (columns[c]) {
+      case T_T1: tm |= CT1; slot = 0; break;
+      case T_T2: tm |= CT2; slot = 1; break;
+      case T_B1: tm |= CB1; slot = 2; break;
+      case T_M0: slot = 3; break;
+      default:
+        set_error("column %lld: property %d carries no complex-kernel tangent", (long long)c, columns[c]);
+        return BAD_ARGUMENT;
+    }
+    if (cols[slot] >= 0) {
+      set_error("property %d appears twice among the Jacobian columns", columns[c]);
+      return BAD_ARGUMENT;
+    }
+    cols[slot] = (int)c;
+  }
+  return OK;
+}
+
 int run_complex(ComplexArgs& a, int tm, bool exch, hipStream_t stream) {
   if (a.atoms == 0 || a.outputs == 0) return OK;
   const int lanes = lanes_for(a.states);
@@ -101,23 +158,7 @@ int run_complex(ComplexArgs& a, int tm, bool exch, hipStream_t stream) {
     set_error("too many atoms for one launch: %lld", (long long)a.atoms);
     return BAD_ARGUMENT;
   }
-  // output staging: K samples per flush with the padded stream count, one
-  // wave's stage within COMPLEX_STAGE_BYTES_PER_WAVE; K < 2 stores directly
-  a.stage_k_log2 = -1;
-  a.stage_s_log2 = 0;
-  size_t shared = 0;
-  if (a.out_real == nullptr && (a.signal != nullptr || a.jacobian != nullptr)) {
-    int s_log2 = 0;
-    while ((1 << s_log2) < 1 + a.columns) ++s_log2;
-    const int64_t slot = (int64_t)(64 / lanes) * (1 << s_log2) * (int64_t)sizeof(float2);
-    int k_log2 = 0;
-    while (k_log2 < 4 && slot * (2 << k_log2) <= COMPLEX_STAGE_BYTES_PER_WAVE) ++k_log2;
-    if (k_log2 >= 1) {
-      a.stage_k_log2 = k_log2;
-      a.stage_s_log2 = s_log2;
-      shared = (size_t)(threads / 64) * (size_t)slot * ((size_t)1 << k_log2);
-    }
-  }
+  const size_t shared = set_staging(a, lanes, threads);
   launch(a, dim3((unsigned)blocks), dim3(threads), shared, stream);
   return hip_status(hipGetLastError(), "complex_kernel launch");
 }
@@ -177,24 +218,9 @@ int tsim_complex_jacobian(void* const* packed, int64_t atom_count, int64_t event
                         features, tissue_stride);
   if (st != OK) return st;
   int tm = 0;
-  int cols[4] = {-1, -1, -1, -1};
-  for (int64_t c = 0; c < column_count; ++c) {
-    int slot;
-    switch (columns[c]) {
-      case T_T1: tm |= CT1; slot = 0; break;
-      case T_T2: tm |= CT2; slot = 1; break;
-      case T_B1: tm |= CB1; slot = 2; break;
-      case T_M0: slot = 3; break;
-      default:
-        set_error("column %lld: property %d carries no complex-kernel tangent", (long long)c, columns[c]);
-        return BAD_ARGUMENT;
-    }
-    if (cols[slot] >= 0) {
-      set_error("property %d appears twice among the Jacobian columns", columns[c]);
-      return BAD_ARGUMENT;
-    }
-    cols[slot] = (int)c;
-  }
+  int cols[4];
+  st = complex_columns(columns, column_count, tm, cols);
+  if (st != OK) return st;
   if (cols[2] >= 0 && !a.transmit) {
     set_error("a B1 column needs the transmit feature and a b1 buffer");
     return BAD_ARGUMENT;
@@ -217,3 +243,130 @@ int tsim_complex_jacobian(void* const* packed, int64_t atom_count, int64_t event
 }
 
 }  // extern "C"
+
+extern "C" int tsim_complex_batch(const tsim_batch_request* requests, int64_t count, int64_t grid_blocks,
+                                  void* stream) {
+  if (count < 0 || (count > 0 && requests == nullptr)) {
+    set_error("a batch needs its request table");
+    return BAD_ARGUMENT;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<ComplexArgs> args;
+  std::vector<int64_t> code_offset;
+  std::vector<int64_t> source;
+  int key_tm = -1, max_states = 0;
+  int64_t total_events = 0;
+  for (int64_t r = 0; r < count; ++r) {
+    const tsim_batch_request& q = requests[r];
+    if (q.atom_count == 0 || q.output_count == 0) continue;
+    const int pool_kind = q.packed[T_POOL_B_FRACTION] != nullptr ? 2 : 0;
+    ComplexArgs a;
+    int st = fill_complex(a, q.packed, q.atom_count, q.event_count, q.state_count, q.output_count, pool_kind,
+                          q.features, q.tissue_stride);
+    if (st != OK) return st;
+    if (q.column_count < 0 || q.column_count > 4) {
+      set_error("request %lld: 0 to 4 columns", (long long)r);
+      return BAD_ARGUMENT;
+    }
+    int tm = 0;
+    int cols[4];
+    st = complex_columns(q.columns, q.column_count, tm, cols);
+    if (st != OK) return st;
+    if (cols[2] >= 0 && !a.transmit) {
+      set_error("request %lld: a B1 column needs the transmit feature and a b1 buffer", (long long)r);
+      return BAD_ARGUMENT;
+    }
+    if (q.sample_sign != nullptr || q.data != nullptr || q.normal != nullptr) {
+      set_error("request %lld: the complex batch writes signal and Jacobian (no sample signs, data or normal "
+                "equations)", (long long)r);
+      return BAD_ARGUMENT;
+    }
+    if (q.column_count > 0 && q.jacobian == nullptr) {
+      set_error("request %lld: Jacobian columns asked for but the Jacobian buffer is null", (long long)r);
+      return BAD_ARGUMENT;
+    }
+    if (key_tm < 0) {
+      key_tm = tm;
+    } else if (tm != key_tm) {
+      set_error("request %lld carries another tangent set than the batch (group requests by tangent set)",
+                (long long)r);
+      return BAD_ARGUMENT;
+    }
+    a.col_t1 = cols[0];
+    a.col_t2 = cols[1];
+    a.col_b1 = cols[2];
+    a.col_m0 = cols[3];
+    a.columns = (int)q.column_count;
+    a.signal = (float2*)q.signal;
+    a.jacobian = q.column_count > 0 ? (float2*)q.jacobian : nullptr;
+    a.factor_re = q.factor_re;
+    a.factor_im = q.factor_im;
+    a.exchanging = pool_kind == 2 ? 1 : 0;
+    max_states = a.states > max_states ? a.states : max_states;
+    code_offset.push_back(total_events);
+    source.push_back(r);
+    total_events += q.event_count;
+    args.push_back(a);
+  }
+  if (args.empty()) return OK;
+  const int lanes = lanes_for(max_states);
+  ComplexBatchLaunch launch = pick_complex_batch(lanes, key_tm);
+  if (!launch) {
+    set_error("no complex batch kernel for %d orders", max_states);
+    return UNSUPPORTED;
+  }
+  const int threads = 256;
+  const int64_t atoms_per_block = (int64_t)(threads / 64) * (64 / lanes);
+  size_t shared = 0;
+  std::vector<ComplexWork> work;
+  for (size_t k = 0; k < args.size(); ++k) {
+    const size_t bytes = set_staging(args[k], lanes, threads);
+    shared = bytes > shared ? bytes : shared;
+    for (int64_t first = 0; first < args[k].atoms; first += atoms_per_block) {
+      ComplexWork w;
+      w.request = (int32_t)k;
+      w.pad = 0;
+      w.first_atom = first;
+      work.push_back(w);
+    }
+  }
+  // one workspace: the compiled events of every request, then the request and
+  // work tables
+  const size_t codes_bytes = sizeof(EventCode) * (size_t)total_events;
+  const size_t a_off = (codes_bytes + 255) & ~(size_t)255;
+  const size_t args_bytes = sizeof(ComplexArgs) * args.size();
+  const size_t w_off = (a_off + args_bytes + 255) & ~(size_t)255;
+  const size_t work_bytes = sizeof(ComplexWork) * work.size();
+  char* space = nullptr;
+  int st = scratch_alloc((void**)&space, w_off + work_bytes, s, "complex batch workspace");
+  if (st != OK) return st;
+  EventCode* codes = (EventCode*)space;
+  for (size_t k = 0; k < args.size() && st == OK; ++k) {
+    const tsim_batch_request& q = requests[source[k]];
+    EventCode* mine = codes + code_offset[k];
+    st = compile_stream((const float*)q.packed[TISSUE_COUNT + E_DURATION],
+                        (const int32_t*)q.packed[TISSUE_COUNT + E_KIND],
+                        (const float*)q.packed[TISSUE_COUNT + E_FLIP],
+                        (const float*)q.packed[TISSUE_COUNT + E_PHASE],
+                        (const uint8_t*)q.packed[TISSUE_COUNT + E_ACTION],
+                        (const int32_t*)q.packed[TISSUE_COUNT + E_OUTPUT_INDEX], q.event_count, mine, s);
+    args[k].events = mine;
+  }
+  if (st == OK) {
+    std::vector<char> tables(w_off + work_bytes - a_off);
+    memcpy(tables.data(), args.data(), args_bytes);
+    memcpy(tables.data() + (w_off - a_off), work.data(), work_bytes);
+    st = upload_table(tables.data(), tables.size(), space + a_off, s);
+  }
+  if (st == OK) {
+    const int64_t blocks = (int64_t)work.size();
+    int64_t grid = grid_blocks > 0 ? grid_blocks : blocks;
+    if (grid > blocks) grid = blocks;
+    if (grid > 0x7FFFFFFF) grid = 0x7FFFFFFF;
+    launch((const ComplexArgs*)(space + a_off), (const ComplexWork*)(space + w_off), blocks, dim3((unsigned)grid),
+           dim3(threads), shared, s);
+    st = hip_status(hipGetLastError(), "complex_batch_kernel launch");
+  }
+  (void)hipFreeAsync(space, s);
+  return st;
+}

@@ -57,6 +57,7 @@ struct ComplexArgs {
   // flush and of the padded stream count; stage_k_log2 < 1 writes directly.
   int stage_k_log2;
   int stage_s_log2;
+  int exchanging;  // one exchanging pool (the batch kernel picks the body per request)
 };
 
 // Bytes of one wave's output stage.
@@ -205,8 +206,9 @@ struct Order {
   float2 z[EXCH ? 2 : 1];
 };
 
+// The atoms of one wave: atom `first + lane / L` on lanes [L (lane / L), ...).
 template <int L, int TM, bool EXCH>
-__global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
+__device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t first) {
   constexpr int NT = cpop(TM);
   constexpr int NTA = NT > 0 ? NT : 1;
   constexpr int NP = EXCH ? 2 : 1;
@@ -221,7 +223,7 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
   const int lane = (int)(threadIdx.x & 63);
   const int wave_group = lane / L;
   const int sub = lane - wave_group * L;
-  const int64_t group = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * APW + wave_group;
+  const int64_t group = first + wave_group;
   const bool live = wave_group < APW && group < p.atoms;
   const int64_t atom = live ? group : p.atoms - 1;
   const bool beyond = sub >= p.states;  // padding order: held at zero
@@ -630,6 +632,64 @@ __global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
   if (staging && (nrec & (stage_k - 1)) != 0) flush((int)(nrec & (stage_k - 1)));
 }
 
+template <int L, int TM, bool EXCH>
+__global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
+  constexpr int APW = 64 / L;
+  complex_atoms<L, TM, EXCH>(p, (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * APW);
+}
+
+// One work block of an uneven batch: a request and its first atom; a block of
+// 256 threads takes 4 (64 / L) atoms from there.
+struct ComplexWork {
+  int32_t request;
+  int32_t pad;
+  int64_t first_atom;
+};
+
+// The uneven-batch launch of the complex kernel: a persistent grid walks the
+// work blocks; each block runs its request's single- or exchanging-pool body
+// (uniform per block), so requests with different pool counts, sequences,
+// state counts (padded to the batch's lanes) and sizes share one launch. Every
+// atom runs the arithmetic it would run alone.
+template <int L, int TM>
+__global__ void __launch_bounds__(256) complex_batch_kernel(const ComplexArgs* __restrict__ args,
+                                                           const ComplexWork* __restrict__ work, int64_t blocks) {
+  constexpr int APW = 64 / L;
+  for (int64_t b = blockIdx.x; b < blocks; b += gridDim.x) {
+    const ComplexWork w = work[b];
+    const ComplexArgs p = args[w.request];
+    const int64_t first = w.first_atom + (int64_t)(threadIdx.x >> 6) * APW;
+    if (p.exchanging) {
+      complex_atoms<L, TM, true>(p, first);
+    } else {
+      complex_atoms<L, TM, false>(p, first);
+    }
+  }
+}
+
+using ComplexBatchLaunch = void (*)(const ComplexArgs*, const ComplexWork*, int64_t, dim3, dim3, size_t,
+                                    hipStream_t);
+
+template <int L, int TM>
+void launch_complex_batch(const ComplexArgs* args, const ComplexWork* work, int64_t blocks, dim3 grid, dim3 block,
+                          size_t shared, hipStream_t stream) {
+  hipLaunchKernelGGL((complex_batch_kernel<L, TM>), grid, block, shared, stream, args, work, blocks);
+}
+
+template <int L>
+ComplexBatchLaunch pick_complex_batch_mask(int tm) {
+  switch (tm & 7) {
+    case 0: return &launch_complex_batch<L, 0>;
+    case 1: return &launch_complex_batch<L, 1>;
+    case 2: return &launch_complex_batch<L, 2>;
+    case 3: return &launch_complex_batch<L, 3>;
+    case 4: return &launch_complex_batch<L, 4>;
+    case 5: return &launch_complex_batch<L, 5>;
+    case 6: return &launch_complex_batch<L, 6>;
+    default: return &launch_complex_batch<L, 7>;
+  }
+}
+
 // Launchers take the dynamic LDS bytes of the output stage.
 using ComplexLaunch = void (*)(const ComplexArgs&, dim3, dim3, size_t, hipStream_t);
 
@@ -660,7 +720,9 @@ ComplexLaunch pick_complex_mask(int tm, bool exch) {
 // Lanes per atom the complex kernel is built for, one translation unit each.
 #define TSIM_COMPLEX_LANES(X) X(4) X(8) X(10) X(12) X(16) X(20) X(32) X(64)
 
-#define TSIM_DECLARE_COMPLEX(LANES) ComplexLaunch pick_complex_##LANES(int tm, bool exch);
+#define TSIM_DECLARE_COMPLEX(LANES)                            \
+  ComplexLaunch pick_complex_##LANES(int tm, bool exch);       \
+  ComplexBatchLaunch pick_complex_batch_##LANES(int tm);
 TSIM_COMPLEX_LANES(TSIM_DECLARE_COMPLEX)
 #undef TSIM_DECLARE_COMPLEX
 

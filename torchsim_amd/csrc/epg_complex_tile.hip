@@ -15,4 +15,8 @@ ComplexLaunch TSIM_CAT(pick_complex_, TSIM_COMPLEX_L)(int tm, bool exch) {
   return pick_complex_mask<TSIM_COMPLEX_L>(tm, exch);
 }
 
+ComplexBatchLaunch TSIM_CAT(pick_complex_batch_, TSIM_COMPLEX_L)(int tm) {
+  return pick_complex_batch_mask<TSIM_COMPLEX_L>(tm);
+}
+
 }  // namespace tsim

@@ -1402,6 +1402,37 @@ void give_staging(const Staged& used, bool landed) {
 
 }  // namespace
 
+namespace tsim {
+
+int upload_table(const void* host, size_t bytes, void* device, hipStream_t s) {
+  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+  int st = hip_status(hipStreamIsCapturing(s, &capture), "hipStreamIsCapturing");
+  if (st != OK) return st;
+  Staged staging;
+  bool have_staging = false;
+  if (capture != hipStreamCaptureStatusNone) {
+    // the graph replays the copy from this buffer: it is never reused
+    st = take_staging_for_capture(bytes, staging);
+  } else {
+    st = take_staging(bytes, staging);
+    have_staging = st == OK;
+  }
+  bool enqueued = false;
+  if (st == OK) {
+    memcpy(staging.host, host, bytes);
+    st = hip_status(hipMemcpyAsync(device, staging.host, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync(table)");
+    enqueued = st == OK;
+  }
+  if (st == OK && have_staging) st = hip_status(hipEventRecord(staging.copied, s), "hipEventRecord");
+  if (have_staging) {
+    if (enqueued && st != OK) (void)hipStreamSynchronize(s);
+    give_staging(staging, st != OK);
+  }
+  return st;
+}
+
+}  // namespace tsim
+
 extern "C" {
 
 typedef struct tsim_batch_request {

@@ -28,7 +28,7 @@ import numpy as np
 import torch
 
 from .. import _native
-from ..sequence import _engine
+from ..sequence import _dephasing, _engine
 from ..sequence._events import EventOperators, PackedEvents, compose, pack
 
 #: Public property names a call may give beside a model's own (reference
@@ -53,7 +53,14 @@ PUBLIC_PROPERTIES = {
     "T1_pool_b": "t1_pool_b_ms",
     "T2_pool_b": "t2_pool_b_ms",
     "pool_b_shift": "pool_b_shift_hz",
+    # the field's spread across the voxel: reaches the samples, not the states
+    # (reference sequence/_parameters.py:168-173, 242)
+    "T2prime": "t2_prime_ms",
 }
+
+#: Properties the sample factor carries rather than the kernels, where the
+#: stream's winding admits it (B0 otherwise stays in the states).
+SAMPLE_FIELDS = ("b0_hz", "t2_prime_ms")
 
 #: Run settings a call may name (reference model/_state_machine.py:97-106),
 #: and the gradient geometry a diffusion coefficient is read through
@@ -271,12 +278,17 @@ class Simulator:
             return ()
         return tuple(torch.broadcast_shapes(*shapes))
 
-    def _prepared(self, held: Mapping[str, Any], sequence: Mapping[str, Any], diff: tuple[str, ...] = ()) -> tuple:
+    def _prepared(self, held: Mapping[str, Any], sequence: Mapping[str, Any], diff: tuple[str, ...] = (),
+                  device: Any = None) -> tuple:
         """The host side of a run: the device, the packed protocol, the tissue
-        fields and their prepared buffers, the state count, and which fields
-        are differentiated (forward-mode columns), require grad, or carry a
-        forward-mode tangent."""
-        device = _target_device({**held, **sequence}, sequence.get("device"))
+        fields and their prepared buffers, the state count, which fields the
+        kernels differentiate (forward-mode columns), which require grad or
+        carry a forward-mode tangent, and the sample factor (a
+        :class:`~torchsim_amd.sequence._dephasing.Dephasing`, or None) that
+        carries B0 and T2' where the stream's winding admits it -- B0 is then
+        held out of the kernels' tissue. ``device`` overrides the one the
+        values name."""
+        device = _target_device({**held, **sequence}, device if device is not None else sequence.get("device"))
         packed = self.packed(**sequence)
         fields = self._tissue_fields(held)
         if "diffusion_um2_per_ms" in fields:
@@ -284,6 +296,9 @@ class Simulator:
             value = fields["diffusion_um2_per_ms"]
             fields["diffusion_um2_per_ms"] = value * scale if scale else 0.0
         differentiated = tuple(self._tissue_fields({name: held[name] for name in diff}).keys())
+        dephasing = self._dephasing(held, packed, fields, differentiated, device)
+        if dephasing is not None:
+            differentiated = tuple(name for name in differentiated if name not in SAMPLE_FIELDS)
         # fields autograd or a forward-mode tangent must reach are laid out per
         # atom, like forward-mode ones
         reversed_fields = tuple(name for name, value in fields.items()
@@ -293,7 +308,38 @@ class Simulator:
         tissue = _engine.prepare_tissue(fields, device,
                                         tuple(dict.fromkeys(differentiated + reversed_fields + tangent_fields)))
         states = int(sequence.get("states", self.states) or 10)
-        return device, packed, fields, tissue, states, differentiated, reversed_fields, tangent_fields
+        return device, packed, fields, tissue, states, differentiated, reversed_fields, tangent_fields, dephasing
+
+    def _dephasing(self, held: Mapping[str, Any], packed: PackedEvents, fields: dict, differentiated: tuple,
+                   device: torch.device) -> _dephasing.Dephasing | None:
+        """Take B0 and T2' out of ``fields`` into a sample factor where they
+        act and the winding admits it (reference
+        sequence/_accelerators.py:647-652); T2' where it does not is refused.
+        A property acts where it carries a derivative, is differentiated, or
+        (B0) is nonzero / (T2') finite anywhere."""
+        t2_prime = fields.pop("t2_prime_ms", None)
+        b0 = fields.get("b0_hz")
+        spreading = t2_prime is not None and (
+            "t2_prime_ms" in differentiated or _engine.carries_derivative(t2_prime)
+            or bool(torch.isfinite(torch.as_tensor(t2_prime)).any()))
+        turning = b0 is not None and ("b0_hz" in differentiated or _engine.carries_derivative(b0)
+                                      or _dephasing.applies(b0))
+        if not spreading and not turning:
+            return None
+        if not _dephasing.winding(packed).analytic:
+            if spreading:
+                _dephasing.refuse_a_spread()
+            return None
+        fields.pop("b0_hz", None)
+        shape = self._batch(held)
+
+        def per_atom(value: Any) -> torch.Tensor:
+            held_value = _engine._on_device(value, device).to(torch.float32)
+            return held_value.expand(shape).reshape(-1)
+
+        return _dephasing.Dephasing(b0_hz=per_atom(b0) if turning else None,
+                                    t2_prime_ms=per_atom(t2_prime) if spreading else None,
+                                    tau_s=_dephasing.unrefocused_s(packed, device))
 
     def _run(
         self,
@@ -304,8 +350,29 @@ class Simulator:
     ) -> tuple[torch.Tensor, torch.Tensor | None]:
         """Signal (atoms, N) and Jacobian (atoms, P, N) of the recorded samples;
         with a ``subspace``, their projections (atoms, R) and (atoms, P, R)."""
-        device, packed, fields, tissue, states, differentiated, reversed_fields, tangent_fields = \
-            self._prepared(held, sequence, diff)
+        prepared = self._prepared(held, sequence, diff)
+        dephasing = prepared[-1]
+        if dephasing is None:
+            return self._run_states(prepared, held, sequence, diff, subspace)
+        if subspace is not None:
+            # the factor differs per atom and sample: project what it made
+            signal, jacobian = self._run(held, sequence, diff)
+            return subspace.project(signal), (None if jacobian is None else subspace.project(jacobian))
+        signal, jacobian = self._run_states(prepared, held, sequence, diff, None)
+        factor = dephasing.factor()
+        signal = signal * factor
+        names = tuple(self._tissue_fields({name: held[name] for name in diff}).keys())
+        if not names:
+            return signal, None
+        kernel_columns = prepared[5]
+        columns = [dephasing.column(signal, name) if name in SAMPLE_FIELDS
+                   else jacobian[:, kernel_columns.index(name)] * factor for name in names]
+        return signal, torch.stack(columns, dim=1)
+
+    def _run_states(self, prepared: tuple, held: Mapping[str, Any], sequence: Mapping[str, Any],
+                    diff: tuple[str, ...], subspace: Any) -> tuple[torch.Tensor, torch.Tensor | None]:
+        """The kernels' part of :meth:`_run`, on its prepared inputs."""
+        device, packed, fields, tissue, states, differentiated, reversed_fields, tangent_fields, _ = prepared
         real, flip, samples = _engine.real_subspace(packed, tissue, device)
         factor = self.signal_factor if self.signal_factor is not None else 1.0
         sequence_live = [_engine.carries_derivative(v) for v in (packed.flip, packed.phase, packed.duration)]
@@ -603,21 +670,35 @@ class _BridgeTangent(torch.autograd.Function):
                     seeds.append(tangent)
                 values[name] = primal
             held, sequence = simulator._split({**simulator.bound, **values})
-            device, packed, _, tissue, states, _, _, _ = simulator._prepared(held, sequence)
+            device, packed, _, tissue, states, _, _, _, dephasing = simulator._prepared(held, sequence)
             carried = [_native.TISSUE_INDEX[name] for name in _complex.GRADIENT_FIELDS]
             targets = [packed.duration, packed.flip, packed.phase] + [tissue.buffers[i] for i in carried]
+            sample_factor = None if dephasing is None else dephasing.factor()
+            if sample_factor is not None:
+                targets += [sample_factor.real, sample_factor.imag]
             moved = _directional(targets, leaves, seeds)
-        for index, value in zip(carried, moved[3:]):
+        reached = moved[3:3 + len(carried)]
+        for index, value in zip(carried, reached):
             if value is not None and tissue.buffers[index] is None:
                 raise NotImplementedError("a tangent reached a tissue property the simulation leaves out")
         factor = simulator.signal_factor
         plain = dataclasses.replace(tissue, buffers=[None if b is None else b.detach() for b in tissue.buffers])
-        tangent = _complex.jvp_complex(
+        both_out = _complex.jvp_complex(
             packed, plain, states=states, factor=factor,
-            tissue_tangents={name: value for name, value in zip(_complex.GRADIENT_FIELDS, moved[3:]) if value is not None},
+            tissue_tangents={name: value for name, value in zip(_complex.GRADIENT_FIELDS, reached) if value is not None},
             flip=packed.flip.detach(), phase=packed.phase.detach(), duration=packed.duration.detach(),
             duration_tangent=moved[0], flip_tangent=moved[1], phase_tangent=moved[2],
+            with_signal=sample_factor is not None,
         )
+        if sample_factor is None:
+            tangent = both_out
+        else:
+            # d(f s) = f ds + s df, df along the same direction
+            signal, tangent = both_out
+            d_real, d_imag = moved[-2], moved[-1]
+            d_factor = torch.complex(torch.zeros_like(sample_factor.real) if d_real is None else d_real,
+                                     torch.zeros_like(sample_factor.imag) if d_imag is None else d_imag)
+            tangent = sample_factor.detach() * tangent + d_factor * signal
         batch = simulator._batch(held)
         return tangent.reshape(*batch, tangent.shape[-1])
 

@@ -155,19 +155,22 @@ def jvp_complex(
     flip_tangent: torch.Tensor | None = None,
     phase_tangent: torch.Tensor | None = None,
     duration_tangent: torch.Tensor | None = None,
-) -> torch.Tensor:
+    with_signal: bool = False,
+) -> torch.Tensor | tuple[torch.Tensor, torch.Tensor]:
     """The directional derivative (atoms, outputs) complex64 of the signal
     ``factor * sample`` along a tangent on tissue fields (``tissue_tangents``:
     field name -> a tensor laid out like that field's prepared buffer) and on
-    the events' flips, phases and durations ((events,) each, or None)."""
+    the events' flips, phases and durations ((events,) each, or None); with
+    ``with_signal``, (signal, tangent) from the same launch."""
     device = tissue.device
     _native.require_device(device)
     lib = _native.load()
     events = _events(packed, device, flip, phase, duration)
     atoms, outputs = tissue.atoms, packed.output_count
     tangent = torch.empty((atoms, outputs), dtype=torch.complex64, device=device)
+    signal = torch.empty_like(tangent) if with_signal else None
     if atoms == 0 or outputs == 0:
-        return tangent
+        return (signal, tangent) if with_signal else tangent
     held = []
     slots = (ctypes.c_void_p * (_native.TISSUE_COUNT + 3))()
     for name, value in tissue_tangents.items():
@@ -190,10 +193,11 @@ def jvp_complex(
         status = lib.tsim_complex_jvp(
             ptrs, atoms, packed.event_count, states, outputs, 2 if tissue.exchanging else 0, tissue.features,
             tissue.stride, float(complex(factor).real), float(complex(factor).imag), slots,
-            _native.TISSUE_COUNT + 3, None, tangent.data_ptr(), _native.stream_of(device),
+            _native.TISSUE_COUNT + 3, None if signal is None else signal.data_ptr(), tangent.data_ptr(),
+            _native.stream_of(device),
         )
     _native.check(status)
-    return tangent
+    return (signal, tangent) if with_signal else tangent
 
 
 @dataclasses.dataclass

@@ -10,8 +10,10 @@ serves many callers on one MI355X:
 * **Continuous batching.** The scheduler works in ticks. At every tick each
   active request offers its next kernel job (a simulate request one, a fit
   one per Gauss-Newton iteration); jobs that run the same kernel variant are
-  packed into ONE uneven-batch launch (``tsim_simulate_batch``) whatever
-  their sizes, sequences and state counts. Requests that arrive while others
+  packed into ONE uneven-batch launch (``tsim_simulate_batch`` for the
+  real-subspace kernel, ``tsim_complex_batch`` for the complex one, which
+  also mixes single and exchanging pools) whatever their sizes, sequences
+  and state counts. Requests that arrive while others
   are on the GPU join at the next tick; nothing waits for a batch to drain.
 * **Overlap.** Each request keeps its own HIP stream for its own work; a
   batched launch runs on a stream of its own that waits on its members'
@@ -26,8 +28,10 @@ serves many callers on one MI355X:
   Gauss-Newton update is per voxel, so a request's result does not depend on
   what it was batched with.
 
-Requests whose kernel is not batchable (complex sequences, diffusion in a
-fit) run on their own stream through the ordinary simulator path.
+Requests whose kernel is not batchable (a simulator with no constant signal
+factor, derivatives the batch does not carry, a fit outside the real
+subspace or with diffusion) run on their own stream through the ordinary
+simulator path.
 """
 
 from __future__ import annotations
@@ -44,10 +48,11 @@ from typing import Any
 import torch
 
 from .. import _native
+from ..model._simulator import Simulator
 from ..recon._bounds import bound_of
 from ..recon._gauss_newton import solve_damped
 from ..sequence import _engine
-from ._batch import BatchItem, launch_batch
+from ._batch import COMPLEX_COLUMNS, BatchItem, launch_batch
 
 
 
@@ -93,7 +98,8 @@ class _Ticket:
 
 
 class _SimulateProgram:
-    """One batched job (real subspace), or the ordinary path on its stream."""
+    """One batched job (real-subspace or complex kernel), or the ordinary
+    path on its stream."""
 
     def __init__(self, request: SimulateRequest, device: torch.device) -> None:
         sim = request.simulator
@@ -102,21 +108,28 @@ class _SimulateProgram:
         self.batch_item: BatchItem | None = None
         self.result: Any = None
         self.launched = False
-        fields = sim._tissue_fields(held)
-        differentiated = tuple(sim._tissue_fields({n: held[n] for n in self.names}))
-        packed = sim.packed(**sequence)
-        batchable = sim.signal_factor is not None and "diffusion_um2_per_ms" not in fields
-        if batchable:
-            tissue = _engine.prepare_tissue(fields, device, differentiated)
-            real, flip, samples = _engine.real_subspace(packed, tissue, device)
-            if real:
-                states = int(sequence.get("states", sim.states) or 10)
-                atoms, n = tissue.atoms, packed.output_count
-                signal = torch.empty((atoms, n), dtype=torch.complex64, device=device)
-                jacobian = torch.empty((atoms, len(differentiated), n), dtype=torch.complex64, device=device)
-                self.batch_item = BatchItem(packed, tissue, states, differentiated, sim.signal_factor,
-                                            flip=flip, samples=samples, signal=signal, jacobian=jacobian)
-                self.batch = sim._batch(held)
+        if sim.signal_factor is None:
+            return
+        _, packed, _, tissue, states, differentiated, reversed_fields, tangent_fields, dephasing = sim._prepared(
+            held, sequence, self.names, device=device)
+        if (reversed_fields or tangent_fields or dephasing is not None or packed.output_count == 0
+                or tissue.atoms == 0):
+            return
+        real, flip, samples = _engine.real_subspace(packed, tissue, device)
+        if not real and any(name not in COMPLEX_COLUMNS for name in differentiated):
+            return
+        atoms, n = tissue.atoms, packed.output_count
+        signal = torch.empty((atoms, n), dtype=torch.complex64, device=device)
+        jacobian = torch.empty((atoms, len(differentiated), n), dtype=torch.complex64, device=device)
+        if real:
+            self.batch_item = BatchItem(packed, tissue, states, differentiated, sim.signal_factor,
+                                        flip=flip, samples=samples, signal=signal, jacobian=jacobian)
+        else:
+            # off-resonance, RF phase, an exchanging pool: the complex kernel's
+            # batch, single and exchanging pools in one launch
+            self.batch_item = BatchItem(packed, tissue, states, differentiated, sim.signal_factor,
+                                        signal=signal, jacobian=jacobian, kernel="complex")
+        self.batch = sim._batch(held)
 
     def job(self) -> tuple[int, BatchItem] | None:
         if self.batch_item is None or self.launched:
@@ -311,6 +324,11 @@ def _request_bytes(request: Any) -> int:
     atoms = 1
     for size in shape:
         atoms *= int(size)
+    if sim.signal_factor is None and type(sim).layout is Simulator.layout:
+        # a closed-form model: one complex value per contrast, no event stream
+        contrasts = max([int(torch.as_tensor(value).numel()) for value in sequence.values()
+                         if isinstance(value, (int, float, list, tuple, torch.Tensor))] or [1])
+        return atoms * (contrasts * 8 * (1 + len(request.diff)) + 5 * 4)
     packed = sim.packed(**sequence)
     return atoms * (packed.output_count * 8 * (1 + len(request.diff)) + 17 * 4) + packed.event_count * 64
 
@@ -578,9 +596,11 @@ class Server:
 
 
 def _tile_key(item: BatchItem) -> int:
-    """Requests of one batch share a tile; group state counts into bands."""
+    """Requests of one batch share a tile; group state counts into bands
+    (for the complex kernel, its lane counts)."""
     s = item.states
-    for band in (4, 8, 10, 12, 16, 20, 24, 32, 40, 48, 64):
+    bands = (4, 8, 10, 12, 16, 20, 32, 64) if item.kernel == "complex" else (4, 8, 10, 12, 16, 20, 24, 32, 40, 48, 64)
+    for band in bands:
         if s <= band:
             return band
     return s
