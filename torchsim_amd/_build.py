@@ -82,12 +82,27 @@ def _compile(hipcc: str, source: Path, defines: list[str], target: Path) -> None
         )
 
 
+_INCLUDE = re.compile(r'^\s*#\s*include\s+"([^"]+)"', re.MULTILINE)
+
+
+def _dependencies(source: Path) -> set[Path]:
+    """The source and every local header it includes, followed recursively."""
+    seen: set[Path] = set()
+    pending = [source]
+    while pending:
+        path = pending.pop()
+        if path in seen or not path.exists():
+            continue
+        seen.add(path)
+        pending += [(path.parent / name).resolve() for name in _INCLUDE.findall(path.read_text())]
+    return seen
+
+
 def _stale(source: Path, target: Path) -> bool:
-    """Whether `target` is older than its source or any header it may include."""
+    """Whether `target` is older than its source or a header it includes."""
     if not target.exists():
         return True
-    newest = max([source.stat().st_mtime] + [h.stat().st_mtime for h in CSRC.glob("*.h")])
-    return target.stat().st_mtime < newest
+    return target.stat().st_mtime < max(path.stat().st_mtime for path in _dependencies(source))
 
 
 def build(jobs: int | None = None, verbose: bool = False, incremental: bool = False) -> Path:
