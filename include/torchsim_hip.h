@@ -237,6 +237,34 @@ int tsim_complex_vjp(void* const* packed, int64_t atom_count, int64_t event_coun
                      int64_t tissue_gradient_count, void* stream);
 
 /*
+ * Forward-over-reverse of the complex kernel: the gradient, with respect to
+ * every event's flip, phase and duration, of a loss that reads the signal
+ * and its Jacobian columns, given their cotangents. Replaces the event part
+ * of _epg_cpu.simulate_vjp_jvp with real_axis -1 (_epg_cpu.cpp:7759-9013,
+ * entry :11572-11755) as `_NativeEpgJvp.backward` reaches it
+ * (sequence/_accelerators.py:1787-1850) for phase-cycled, off-resonant,
+ * exchanging (pool kind 2) and diffusing protocols.
+ * columns: HOST array of column_count (<= 4) indices among T_T1, T_T2,
+ *          T_M0, T_B1 (B1 needs the transmit feature).
+ * signal_cotangent: DEVICE (atom_count, output_count) complex64, or NULL.
+ * jacobian_cotangent: DEVICE (atom_count, column_count, output_count)
+ *          complex64 (the Jacobian as tsim_complex_jacobian writes it).
+ * grad_flip, grad_phase, grad_duration: DEVICE (event_count) FP32, summed
+ *          over atoms in a fixed order; every event gets its duration
+ *          gradient, zero intervals included.
+ * One first-order pass for the signal (an M0 column's cotangent joins it:
+ * the column is the signal over M0) and one forward-over-reverse pass per
+ * T1, T2 or B1 column. Tissue gradients are not carried here (status 2 is
+ * not returned for them: the call has no tissue outputs).
+ */
+int tsim_complex_vjp_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
+                              int64_t state_count, int64_t output_count, int pool_kind, int features,
+                              int64_t tissue_stride, float factor_re, float factor_im,
+                              const void* signal_cotangent, const int32_t* columns, int64_t column_count,
+                              const void* jacobian_cotangent, int64_t segment_events, float* grad_flip,
+                              float* grad_phase, float* grad_duration, void* stream);
+
+/*
  * Forward mode along one arbitrary direction: the signal and its directional
  * derivative for a tangent on any carried tissue property and on every
  * event's duration, flip and phase. Replaces the reference's

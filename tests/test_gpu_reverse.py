@@ -192,15 +192,16 @@ def test_crlb_tissue_gradients_match_the_reference_vjp_jvp():
 
 
 def test_what_the_complex_adjoint_does_not_carry_raises():
-    """The complex path's adjoint is first order and reaches no pool-b
-    property: a Jacobian under reverse mode there, or a pool-b map that
-    requires grad, is refused rather than silently cut off."""
+    """The complex path's forward-over-reverse passes gradients to the
+    sequence only, and its adjoint reaches no pool-b property: a Jacobian
+    under reverse mode with a tissue map that requires grad, or a pool-b map
+    that requires grad, is refused rather than silently cut off."""
     from torchsim_amd.simulators import MRFSimulator
 
     t1 = torch.tensor([800.0, 1200.0], device="cuda", requires_grad=True)
     t2 = torch.tensor([60.0, 80.0], device="cuda")
     seq = MRFSimulator(flip=torch.full((20,), 30.0), TR=10.0, phases=torch.linspace(0, 90, 20))
-    with pytest.raises(NotImplementedError, match="forward-over-reverse"):
+    with pytest.raises(NotImplementedError, match="tissue maps are constants"):
         seq.jacobian(("T1", "T2"), T1=t1, T2=t2)
     fraction = torch.tensor([0.1, 0.2], device="cuda", requires_grad=True)
     with pytest.raises(NotImplementedError, match="pool_b_fraction"):

@@ -1,7 +1,7 @@
 // Host side of the complex adjoint: argument checks, the checkpoint and
 // scratch workspace, the launch, and the fixed-order reduction of the
 // per-wave event partials (declared in include/torchsim_hip.h).
-#include "epg_complex_adjoint_impl.h"
+#include "epg_complex_fovr_impl.h"
 
 #include <cmath>
 #include <cstring>
@@ -28,10 +28,11 @@ int adjoint_lanes(int64_t states) {
   return -1;
 }
 
-// Per-event partials summed over waves in a fixed order.
+// Per-event partials summed over waves in a fixed order (added to what the
+// outputs hold with `accumulate`).
 __global__ void reduce_event_partials(const float* __restrict__ partial, int64_t waves, int64_t events,
                                       float* __restrict__ grad_flip, float* __restrict__ grad_phase,
-                                      float* __restrict__ grad_duration) {
+                                      float* __restrict__ grad_duration, int accumulate) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 3 * events) return;
   float v = 0.0f;
@@ -39,7 +40,25 @@ __global__ void reduce_event_partials(const float* __restrict__ partial, int64_t
   const int64_t e = i / 3;
   const int k = (int)(i % 3);
   float* out = k == 0 ? grad_flip : (k == 1 ? grad_phase : grad_duration);
-  if (out) out[e] = v;
+  if (out) out[e] = accumulate ? out[e] + v : v;
+}
+
+// The signal cotangent an M0 column adds: its cotangent over M0 (the column
+// is the signal over M0), joined to the signal's own (or alone).
+__global__ void join_m0_cotangent(const float2* __restrict__ signal_cotangent, const float2* __restrict__ jacobian_cotangent,
+                                  int64_t columns, int64_t column, const float* __restrict__ m0, int64_t stride,
+                                  int density, int64_t atoms, int64_t outputs, float2* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= atoms * outputs) return;
+  const int64_t atom = i / outputs, o = i - atom * outputs;
+  const float scale = 1.0f / (density ? m0[atom * stride] : 1.0f);
+  const float2 w = jacobian_cotangent[(atom * columns + column) * outputs + o];
+  float2 v = make_float2(w.x * scale, w.y * scale);
+  if (signal_cotangent) {
+    v.x += signal_cotangent[i].x;
+    v.y += signal_cotangent[i].y;
+  }
+  out[i] = v;
 }
 
 __global__ void clear_floats(float* __restrict__ a, int64_t count) {
@@ -127,6 +146,18 @@ int carried_slot(int tissue) {
     case T_INV_EFF: return CG_IE;
     case T_DIFFUSION: return CG_D;
     default: return -1;
+  }
+}
+
+ComplexFovrLaunch pick_complex_fovr(int lanes, bool exch) {
+  switch (lanes) {
+#define TSIM_COMPLEX_FOVR_CASE(LANES) \
+  case LANES:                         \
+    return pick_complex_fovr_##LANES(exch);
+    TSIM_COMPLEX_LANES(TSIM_COMPLEX_FOVR_CASE)
+#undef TSIM_COMPLEX_FOVR_CASE
+    default:
+      return nullptr;
   }
 }
 
@@ -236,7 +267,7 @@ extern "C" int tsim_complex_vjp(void* const* packed, int64_t atom_count, int64_t
   if (st == OK) {
     const int64_t rb = (3 * event_count + 255) / 256;
     hipLaunchKernelGGL(reduce_event_partials, dim3((unsigned)rb), dim3(256), 0, s, (const float*)a.partial, waves,
-                       event_count, grad_flip, grad_phase, grad_duration);
+                       event_count, grad_flip, grad_phase, grad_duration, 0);
     st = hip_status(hipGetLastError(), "reduce_event_partials launch");
   }
   (void)hipFreeAsync(work, s);
@@ -305,4 +336,149 @@ extern "C" int tsim_complex_jvp(void* const* packed, int64_t atom_count, int64_t
   q.base.events = compiled.codes;
   launch(q, dim3((unsigned)blocks), dim3(threads), s);
   return hip_status(hipGetLastError(), "complex_jvp_kernel launch");
+}
+
+extern "C" int tsim_complex_vjp_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
+                                         int64_t state_count, int64_t output_count, int pool_kind, int features,
+                                         int64_t tissue_stride, float factor_re, float factor_im,
+                                         const void* signal_cotangent, const int32_t* columns, int64_t column_count,
+                                         const void* jacobian_cotangent, int64_t segment_events, float* grad_flip,
+                                         float* grad_phase, float* grad_duration, void* stream) {
+  ComplexFovrArgs q;
+  memset(&q, 0, sizeof(q));
+  int st = fill_common(q.base, packed, atom_count, event_count, state_count, output_count, pool_kind, features,
+                       tissue_stride, factor_re, factor_im);
+  if (st != OK) return st;
+  ComplexAdjointArgs& a = q.base;
+  if (column_count < 0 || column_count > 4 || (column_count > 0 && columns == nullptr)) {
+    set_error("between 0 and 4 Jacobian columns, got %lld", (long long)column_count);
+    return BAD_ARGUMENT;
+  }
+  if (segment_events < 0) {
+    set_error("segment_events must be positive (or 0 for the library's choice)");
+    return BAD_ARGUMENT;
+  }
+  const bool sized = atom_count > 0 && output_count > 0;
+  if (column_count > 0 && jacobian_cotangent == nullptr && sized) {
+    set_error("Jacobian columns asked for but the Jacobian cotangent is null");
+    return BAD_ARGUMENT;
+  }
+  int m0_column = -1;
+  int tangent[4];
+  int tangents = 0;
+  int64_t tangent_column[4];
+  for (int64_t c = 0; c < column_count; ++c) {
+    int kind;
+    switch (columns[c]) {
+      case T_T1: kind = CG_T1; break;
+      case T_T2: kind = CG_T2; break;
+      case T_B1: kind = CG_B1; break;
+      case T_M0: kind = CG_M0; break;
+      default:
+        set_error("column %lld: property %d carries no complex-kernel tangent", (long long)c, columns[c]);
+        return BAD_ARGUMENT;
+    }
+    for (int j = 0; j < tangents; ++j)
+      if (tangent[j] == kind) {
+        set_error("property %d appears twice among the Jacobian columns", columns[c]);
+        return BAD_ARGUMENT;
+      }
+    if (kind == CG_M0) {
+      if (m0_column >= 0) {
+        set_error("property %d appears twice among the Jacobian columns", columns[c]);
+        return BAD_ARGUMENT;
+      }
+      m0_column = (int)c;
+      continue;
+    }
+    if (kind == CG_B1 && !a.transmit) {
+      set_error("a B1 column needs the transmit feature and a b1 buffer");
+      return BAD_ARGUMENT;
+    }
+    tangent_column[tangents] = c;
+    tangent[tangents++] = kind;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  for (float* g : {grad_flip, grad_phase, grad_duration})
+    if (st == OK) st = clear(g, event_count, s);
+  if (st != OK || !sized || event_count == 0) return st;
+
+  // the signal's cotangent (with an M0 column's) through the first-order adjoint
+  if (signal_cotangent != nullptr || m0_column >= 0) {
+    void* joined = nullptr;
+    const void* w = signal_cotangent;
+    if (m0_column >= 0) {
+      st = scratch_alloc(&joined, sizeof(float2) * (size_t)(atom_count * output_count), s, "M0 cotangent");
+      if (st != OK) return st;
+      const int64_t n = atom_count * output_count;
+      hipLaunchKernelGGL(join_m0_cotangent, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                         (const float2*)signal_cotangent, (const float2*)jacobian_cotangent, column_count,
+                         (int64_t)m0_column, a.m0, tissue_stride, a.density, atom_count, output_count,
+                         (float2*)joined);
+      st = hip_status(hipGetLastError(), "join_m0_cotangent launch");
+      w = joined;
+    }
+    if (st == OK)
+      st = tsim_complex_vjp(packed, atom_count, event_count, state_count, output_count, pool_kind, features,
+                            tissue_stride, factor_re, factor_im, w, segment_events, grad_flip, grad_phase,
+                            grad_duration, nullptr, 0, stream);
+    if (joined) (void)hipFreeAsync(joined, s);
+    if (st != OK) return st;
+  }
+  if (tangents == 0) return OK;
+
+  const int L = adjoint_lanes(state_count);
+  ComplexFovrLaunch launch = pick_complex_fovr(L, pool_kind == 2);
+  if (!launch) {
+    set_error("no complex forward-over-reverse kernel for %lld orders", (long long)state_count);
+    return UNSUPPORTED;
+  }
+  int64_t segment = segment_events;
+  if (segment == 0) segment = (int64_t)std::ceil(std::sqrt((double)event_count));
+  if (segment > event_count) segment = event_count;
+  const int apw = 64 / L;
+  const int64_t waves_needed = (atom_count + apw - 1) / apw;
+  const int threads = waves_needed < 4LL * 256 ? 64 : 256;
+  const int64_t atoms_per_block = (int64_t)(threads / 64) * apw;
+  const int64_t blocks = (atom_count + atoms_per_block - 1) / atoms_per_block;
+  if (blocks > 0x7FFFFFFF) {
+    set_error("too many atoms for one launch: %lld", (long long)atom_count);
+    return BAD_ARGUMENT;
+  }
+  const int64_t total = blocks * threads;
+  const int64_t waves = total / 64;
+  const int64_t floats = 2 * 6LL * (pool_kind == 2 ? 2 : 1);  // the state and its tangent
+  const int64_t segments = (event_count + segment - 1) / segment;
+  const size_t ck = sizeof(float) * (size_t)(segments * floats * total);
+  const size_t sc = sizeof(float) * (size_t)(segment * floats * total);
+  const size_t pa = sizeof(float) * (size_t)(waves * event_count * 3);
+  Compiled compiled;
+  st = compile_packed(packed, event_count, s, compiled);
+  if (st != OK) return st;
+  char* work = nullptr;
+  st = scratch_alloc((void**)&work, ck + sc + pa, s, "complex forward-over-reverse workspace");
+  if (st != OK) return st;
+  a.events = compiled.codes;
+  a.segment = segment;
+  a.threads_total = total;
+  a.checkpoints = (float*)work;
+  a.scratch = (float*)(work + ck);
+  a.partial = (float*)(work + ck + sc);
+  q.jacobian_cotangent = (const float2*)jacobian_cotangent;
+  q.columns = column_count;
+  const size_t lds = sizeof(float) * (size_t)(threads / 64) * 3 * CFLUSH * 64;
+  const int64_t rb = (3 * event_count + 255) / 256;
+  for (int j = 0; j < tangents && st == OK; ++j) {
+    q.tangent = tangent[j];
+    q.column = tangent_column[j];
+    launch(q, dim3((unsigned)blocks), dim3(threads), lds, s);
+    st = hip_status(hipGetLastError(), "complex_fovr_kernel launch");
+    if (st == OK) {
+      hipLaunchKernelGGL(reduce_event_partials, dim3((unsigned)rb), dim3(256), 0, s, (const float*)a.partial, waves,
+                         event_count, grad_flip, grad_phase, grad_duration, 1);
+      st = hip_status(hipGetLastError(), "reduce_event_partials launch");
+    }
+  }
+  (void)hipFreeAsync(work, s);
+  return st;
 }

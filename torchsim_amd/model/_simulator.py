@@ -22,6 +22,7 @@ from collections.abc import Mapping, Sequence
 import dataclasses
 from copy import copy as shallow_copy
 from dataclasses import dataclass, field
+from types import SimpleNamespace
 from typing import Any
 
 import numpy as np
@@ -417,11 +418,17 @@ class Simulator:
             )
         if not real:
             if reverse:
-                raise NotImplementedError(
-                    "a Jacobian under reverse mode (forward-over-reverse) runs on the real-subspace kernel "
-                    "for T1, T2, M0, B1 and the inversion efficiency; this protocol or tissue takes the "
-                    "complex kernel, whose adjoint is first order"
-                )
+                if reversed_fields:
+                    raise NotImplementedError(
+                        "a Jacobian under reverse mode on the complex kernel passes gradients to the sequence's "
+                        f"flips, phases and durations; tissue maps are constants there ({list(reversed_fields)} "
+                        "require grad)"
+                    )
+                run = SimpleNamespace(packed=packed, tissue=tissue, states=states, factor=factor,
+                                      checkpoint_events=int(sequence.get("checkpoint_events", 0)))
+                signal, jacobian = _complex.ComplexJacobian.apply(
+                    packed.flip.to(device), packed.phase.to(device), packed.duration.to(device), run, differentiated)
+                return signal, (jacobian if differentiated else None)
             return _complex.simulate_complex(packed, tissue, states=states, columns=differentiated, factor=factor)
         if reverse:
             unsupported = [name for name in reversed_fields if name not in _engine.GRADIENT_FIELDS]

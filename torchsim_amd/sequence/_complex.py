@@ -142,6 +142,86 @@ def adjoint_complex(
     return grad_flip, grad_phase, grad_duration, grads
 
 
+def adjoint_complex_jacobian(
+    packed: PackedEvents,
+    tissue: PreparedTissue,
+    *,
+    states: int,
+    factor: complex,
+    columns: tuple[str, ...],
+    signal_cotangent: torch.Tensor | None,
+    jacobian_cotangent: torch.Tensor | None,
+    flip: torch.Tensor | None = None,
+    phase: torch.Tensor | None = None,
+    duration: torch.Tensor | None = None,
+    segment_events: int = 0,
+) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """d(loss)/d(flip), d(phase), d(duration) per event for a loss that reads
+    the complex kernel's signal (atoms, outputs) and Jacobian (atoms, P,
+    outputs) along ``columns`` (among t1_ms, t2_ms, m0, b1), from their
+    cotangents (torch's convention): one first-order pass for the signal and
+    one forward-over-reverse pass per T1, T2 or B1 column."""
+    device = tissue.device
+    _native.require_device(device)
+    for name in columns:
+        if name not in _COLUMN_INDEX:
+            raise NotImplementedError(f"the complex kernel carries no tangent along {name!r}")
+    lib = _native.load()
+    events = _events(packed, device, flip, phase, duration)
+    count = packed.event_count
+    grads = [torch.empty(count, dtype=torch.float32, device=device) for _ in range(3)]
+    ptrs = _native.pointers([*(None if b is None else b.detach() for b in tissue.buffers), *events])
+    cols = (ctypes.c_int32 * max(1, len(columns)))(*[_COLUMN_INDEX[name] for name in columns])
+
+    def held(value):
+        return None if value is None else value.detach().to(device=device, dtype=torch.complex64).contiguous()
+
+    ws, wj = held(signal_cotangent), held(jacobian_cotangent)
+    with torch.cuda.device(device):
+        status = lib.tsim_complex_vjp_jacobian(
+            ptrs, tissue.atoms, count, states, packed.output_count, 2 if tissue.exchanging else 0,
+            tissue.features, tissue.stride, float(complex(factor).real), float(complex(factor).imag),
+            None if ws is None else ws.data_ptr(), cols, len(columns), None if wj is None else wj.data_ptr(),
+            int(segment_events), grads[0].data_ptr(), grads[1].data_ptr(), grads[2].data_ptr(),
+            _native.stream_of(device),
+        )
+    _native.check(status)
+    return grads[0], grads[1], grads[2]
+
+
+class ComplexJacobian(torch.autograd.Function):
+    """The complex kernel's signal and Jacobian as a function of the events'
+    flips, phases and durations, with the reference's forward-over-reverse
+    backward (``_NativeEpgJvp.backward``, sequence/_accelerators.py:
+    1787-1850; ``simulate_vjp_jvp``, _epg_cpu.cpp:7759-9013): what a design
+    loss on a phase-cycled, off-resonant, exchanging or diffusing protocol's
+    Jacobian needs. Tissue maps are constants here."""
+
+    @staticmethod
+    def forward(flip, phase, duration, run, columns):
+        return simulate_complex(run.packed, run.tissue, states=run.states, columns=columns, factor=run.factor,
+                                flip=flip, phase=phase, duration=duration)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        flip, phase, duration, run, columns = inputs
+        ctx.save_for_backward(flip, phase, duration)
+        ctx.run = run
+        ctx.columns = columns
+
+    @staticmethod
+    def backward(ctx, grad_signal, grad_jacobian):
+        flip, phase, duration = ctx.saved_tensors
+        run = ctx.run
+        g_flip, g_phase, g_duration = adjoint_complex_jacobian(
+            run.packed, run.tissue, states=run.states, factor=run.factor, columns=ctx.columns,
+            signal_cotangent=grad_signal, jacobian_cotangent=grad_jacobian if ctx.columns else None,
+            flip=flip, phase=phase, duration=duration, segment_events=run.checkpoint_events)
+        needs = ctx.needs_input_grad
+        return (g_flip.to(flip.dtype) if needs[0] else None, g_phase.to(phase.dtype) if needs[1] else None,
+                g_duration.to(duration.dtype) if needs[2] else None, None, None)
+
+
 def jvp_complex(
     packed: PackedEvents,
     tissue: PreparedTissue,
