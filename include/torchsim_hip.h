@@ -83,6 +83,28 @@ int tsim_complex_jacobian(void* const* packed, int64_t atom_count, int64_t event
                           void* stream);
 
 /*
+ * Signal and Jacobian of a tissue with a semisolid (magnetization-transfer)
+ * pool, complex64 in the public layout. Replaces `_epg_cpu.simulate` +
+ * `_epg_cpu.simulate_jvp` with pool kind 1 and real_axis -1
+ * (src/torchsim/_epg_cpu.cpp:4216-4566, two_pool_step :875-932, the
+ * saturation :4463-4480, the lineshape read :1888-1910; the lineshape table
+ * is the reference's Tail::ABSORPTION slot).
+ * The packed buffers bound_fraction, bound_exchange and t1_bound must be set
+ * (pool_b_fraction must not: three pools are not built), and the event
+ * buffers saturation (1/s per rad^2 of flip) and rf_frequency (Hz).
+ * columns: HOST array of column_count (<= 4) indices among {0 t1, 1 t2,
+ *          2 m0, 3 b1}.
+ * lineshape: DEVICE (lineshape_bins, 2) FP32, value then slope (per Hz) of
+ *          the absorption lineshape at knots lineshape_step Hz apart from
+ *          resonance.
+ */
+int tsim_semisolid_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
+                            int64_t state_count, int64_t output_count, int features, int64_t tissue_stride,
+                            const int32_t* columns, int64_t column_count, float factor_re, float factor_im,
+                            const float* lineshape, int64_t lineshape_bins, float lineshape_step, void* signal,
+                            void* jacobian, void* stream);
+
+/*
  * Real-subspace signal and Jacobian in ONE pass, written as complex64 in the
  * public layout. Replaces the P pairs of `_epg_cpu.simulate` +
  * `_epg_cpu.simulate_jvp` (src/torchsim/_epg_cpu.cpp:10740, 11017) that

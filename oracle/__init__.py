@@ -153,6 +153,8 @@ def events_of(packed: Any) -> dict[str, np.ndarray]:
         "phase": packed.phase.detach().cpu().numpy(),
         "action": packed.action.cpu().numpy(),
         "output_index": packed.output_index.cpu().numpy(),
+        "saturation": packed.saturation.detach().cpu().numpy(),
+        "rf_frequency": packed.rf_frequency_hz.detach().cpu().numpy(),
     }
 
 
@@ -187,6 +189,7 @@ def _reference_call(
     pool_kind: int,
     tangent: dict[str, Any] | None,
     threads: int,
+    lineshape: tuple[Any, Any, float] | None = None,
 ) -> np.ndarray:
     module = reference_module()
     if module is None:
@@ -215,13 +218,19 @@ def _reference_call(
         arr(events["action"], np.uint8),
         arr(events["output_index"], np.int32),
         arr(np.zeros(count), np.int32),
-        arr(np.zeros(count), np.float32),
-        arr(np.zeros(count), np.float32),
+        arr(events.get("saturation", np.zeros(count)), np.float32),
+        arr(events.get("rf_frequency", np.zeros(count)), np.float32),
     ]
     real = np.zeros((atoms, outputs), dtype=np.float32)
     imag = np.zeros((atoms, outputs), dtype=np.float32)
     tail = [0] * 13
-    dims = (atoms, 1, count, states, outputs, threads, real_axis, 0.0, 0.0, 1, 1, 0, 1.0, 0, 1.0, pool_kind,
+    bins, step = 0, 1.0
+    if lineshape is not None:
+        # the absorption table sits in the last tail slot (Tail::ABSORPTION)
+        knots = np.stack((np.asarray(lineshape[0]), np.asarray(lineshape[1])), -1)
+        tail[12] = arr(knots, np.float32)
+        bins, step = int(knots.shape[0]), float(lineshape[2])
+    dims = (atoms, 1, count, states, outputs, threads, real_axis, 0.0, 0.0, 1, 1, 0, 1.0, bins, step, pool_kind,
             features, 0, 0, 0)
     if tangent is None:
         module.simulate((*tissue_ptrs, *event_ptrs, real.ctypes.data, imag.ctypes.data, *tail), *dims)
@@ -271,6 +280,7 @@ def reference_complex(
     exchanging: bool = False,
     tangent: dict[str, Any] | None = None,
     threads: int = 1,
+    lineshape: tuple[Any, Any, float] | None = None,
 ) -> np.ndarray:
     """The reference's own FP32 complex kernel: recorded samples (atoms, outputs), complex.
 
@@ -284,9 +294,14 @@ def reference_complex(
     pool_b_exchange, t1_pool_b, t2_pool_b, pool_b_shift``); ``diffusion`` is
     the rate already scaled to 1/s per unit order squared. ``tangent`` takes
     the same names plus per-event ``duration``, ``flip`` and ``phase``.
+    With ``lineshape`` = (values, slopes, step_hz) the tissue's
+    ``bound_fraction`` / ``bound_exchange`` / ``t1_bound`` run the semisolid
+    pool (pool kind 1), the events' ``saturation`` and ``rf_frequency``
+    driving its saturation.
     """
+    kind = 1 if lineshape is not None else (2 if exchanging else 0)
     return _reference_call(events, tissue, states=states, outputs=outputs, features=features,
-                           real_axis=-1, pool_kind=2 if exchanging else 0, tangent=tangent, threads=threads)
+                           real_axis=-1, pool_kind=kind, tangent=tangent, threads=threads, lineshape=lineshape)
 
 
 _EVENT_VJP_ORDER = ("flip", "phase", "duration")        # simulate_vjp's gradient slots after the tissue

@@ -51,6 +51,7 @@ def simulate(
     *,
     states: int,
     outputs: int,
+    lineshape: tuple[Any, Any, float] | None = None,
     _plant: str | None = None,
 ) -> torch.Tensor:
     """Recorded complex samples (atoms, outputs), complex128.
@@ -59,8 +60,16 @@ def simulate(
     output_index -- numpy or torch, one train. ``tissue``: t1, t2 (ms) and
     optionally m0, b1, b0 (Hz), b1_phase (rad), ie, diffusion (the rate
     already scaled to 1/s per unit order^2), and for an exchanging pool
-    fraction, exchange (1/s), t1b, t2b (ms), shift (Hz). Tissue entries may be
-    tensors requiring grad; flip and duration may be too.
+    fraction, exchange (1/s), t1b, t2b (ms), shift (Hz); for a semisolid
+    (magnetization-transfer) pool bound_fraction, bound_exchange (1/s) and
+    t1_bound (ms), with the events' ``saturation`` (1/s per rad^2) and
+    ``rf_frequency`` (Hz) and ``lineshape`` = (values, slopes, step_hz), the
+    absorption table read by cubic Hermite (reference
+    src/torchsim/_epg_cpu.cpp:875-932, 1888-1910, 4216-4566 with pool kind 1:
+    the bound pool has Z states only, exchanging with the free water's Z and
+    saturated by exp(saturation theta^2 G(rf_frequency - b0)) at every
+    pulse; an inversion leaves it alone). Tissue entries may be tensors
+    requiring grad; flip and duration may be too.
 
     ``_plant`` exists only so the tests can show that their checks fail on a
     plausible mistake: ``"swap_exchange"`` exchanges the two rate constants
@@ -84,6 +93,7 @@ def simulate(
     ie = tt("ie", 1.0) * ones
     rate = tt("diffusion", 0.0) * ones
     exchanging = "fraction" in tissue and tissue["fraction"] is not None
+    semisolid = tissue.get("bound_fraction") is not None
     duration = torch.as_tensor(events["duration"], dtype=_R)
     flip = torch.as_tensor(events["flip"], dtype=_R)
     phase = torch.as_tensor(events["phase"], dtype=_R)
@@ -118,13 +128,48 @@ def simulate(
         z[1] = _at0(z[1], f)
     else:
         z[0] = _at0(z[0], ones)
+    if semisolid:
+        fb = tt("bound_fraction") * ones
+        kb = tt("bound_exchange", 0.0) * ones
+        r1bound = 1000.0 / (tt("t1_bound", 1000.0) * ones)
+        kab_s, kba_s = kb * fb, kb * (1.0 - fb)
+        r1a = 1000.0 / t1
+        gs = torch.stack((torch.stack((-kab_s - r1a, kba_s), -1), torch.stack((kab_s, -kba_s - r1bound), -1)), -2)
+        split = torch.stack((1.0 - fb, fb), -1)
+        z[0] = _at0(z[0], 1.0 - fb)
+        zb = _at0(torch.zeros(shape, dtype=_C), fb)
+        saturation = torch.as_tensor(events["saturation"], dtype=_R)
+        rf_frequency = torch.as_tensor(events["rf_frequency"], dtype=_R)
+        knots = torch.as_tensor(lineshape[0], dtype=_R)
+        slopes = torch.as_tensor(lineshape[1], dtype=_R)
+        step = float(lineshape[2])
+
+        def absorbed(offset):
+            # cubic Hermite between the knots around |offset|, clamped to the table
+            last = knots.numel() - 1
+            scaled = torch.clamp(offset.abs() / step, max=float(last))
+            lower = torch.clamp(torch.floor(scaled), max=float(last - 1))
+            u = scaled - lower
+            i = lower.to(torch.int64)
+            return ((2 * u**3 - 3 * u**2 + 1) * knots[i] + (u**3 - 2 * u**2 + u) * step * slopes[i]
+                    + (-2 * u**3 + 3 * u**2) * knots[i + 1] + (u**3 - u**2) * step * slopes[i + 1])
     samples: list[torch.Tensor | None] = [None] * outputs
     for e in range(len(kind)):
         dt = duration[e]
         damp_t = torch.exp(-(rate * dt)[:, None] * wt[None, :])
         damp_l = torch.exp(-(rate * dt)[:, None] * wl[None, :])
         turn = torch.exp(-2j * torch.pi * b0 * dt)[:, None] * damp_t
-        if not exchanging:
+        if semisolid:
+            e2 = torch.exp(-(1000.0 / t2) * dt)
+            fp[0] = fp[0] * turn * e2[:, None]
+            fm[0] = fm[0] * turn.conj() * e2[:, None]
+            y = torch.linalg.matrix_exp(gs * dt)
+            rec = split - (y @ split.unsqueeze(-1)).squeeze(-1)
+            l0 = y[:, 0, 0, None] * z[0] + y[:, 0, 1, None] * zb
+            l1 = y[:, 1, 0, None] * z[0] + y[:, 1, 1, None] * zb
+            z[0] = l0 * damp_l + _at0(l0, rec[:, 0])
+            zb = l1 * damp_l + _at0(l1, rec[:, 1])
+        elif not exchanging:
             e1 = torch.exp(-(1000.0 / t1) * dt)
             e2 = torch.exp(-(1000.0 / t2) * dt)
             fp[0] = fp[0] * turn * e2[:, None]
@@ -155,6 +200,9 @@ def simulate(
             else:
                 alpha = (flip[e] * b1)[:, None]
                 phi = (phase[e] + b1_phase)[:, None]
+                if semisolid and float(saturation[e]) != 0.0:
+                    g = absorbed(rf_frequency[e] - b0)
+                    zb = zb * torch.exp(saturation[e] * (flip[e] * b1) ** 2 * g)[:, None]
                 ca, sa = torch.cos(alpha), torch.sin(alpha)
                 p1 = torch.exp(1j * phi)
                 p2 = p1 * p1

@@ -1,0 +1,147 @@
+"""The semisolid (magnetization-transfer) pool on the GPU: the kernel's
+signal and Jacobian columns (T1, T2, M0, B1) against the FP64 oracle's
+semisolid pool (pinned on the CPU against the reference's own kernel), on an
+MRF train with off-resonance and a B1 map and on a refocused train, through
+the public simulators and their property names."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from oracle import packed as P
+from tests import _cases
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5    # signal, relative to its largest sample
+JTOL = 1e-4   # Jacobian columns, relative to each column's largest entry
+
+
+def _simulator(name):
+    from torchsim_amd.simulators import FSESimulator, MRFSimulator
+
+    if name == "mrf":
+        return MRFSimulator(flip=torch.tensor(_cases.benchmark_flips(120), dtype=torch.float32), TR=10.0, states=10)
+    # TR far beyond T1: the simulator's recovery factor is one
+    return FSESimulator(flip=torch.full((24,), 150.0), ESP=5.0, TR=1e6, states=10)
+
+
+def _maps(atoms, seed):
+    rng = np.random.default_rng(seed)
+    t1, t2 = _cases.random_tissue(atoms, seed=seed)
+    return {"T1": t1, "T2": t2, "M0": rng.uniform(0.5, 1.0, atoms).astype(np.float32),
+            "B1": rng.uniform(0.8, 1.2, atoms).astype(np.float32),
+            "B0": rng.uniform(-300.0, 300.0, atoms).astype(np.float32),
+            "bound_fraction": rng.uniform(0.05, 0.2, atoms).astype(np.float32),
+            "bound_exchange": rng.uniform(5.0, 40.0, atoms).astype(np.float32),
+            "bound_T1": rng.uniform(300.0, 1200.0, atoms).astype(np.float32)}
+
+
+_ORACLE = {"T1": "t1", "T2": "t2", "M0": "m0", "B1": "b1", "B0": "b0", "bound_fraction": "bound_fraction",
+           "bound_exchange": "bound_exchange", "bound_T1": "t1_bound"}
+
+
+@pytest.mark.parametrize("name", ["mrf", "fse"])
+def test_signal_and_jacobian_against_the_fp64_oracle(name):
+    from torchsim_amd.sequence._lineshape import lineshape_table
+
+    sim = _simulator(name)
+    atoms = 48
+    maps = _maps(atoms, 17)
+    names = ("T1", "T2", "M0", "B1")
+    signal, jacobian = sim.jacobian(names, **{k: torch.tensor(v).cuda() for k, v in maps.items()})
+    packed = sim.packed(**sim.bound)
+    events = oracle.events_of(packed)
+    # both trains wind uniformly and sample where nothing is left unrefocused:
+    # B0 reaches the samples as a factor of one, and the states (and the
+    # lineshape's offset) run without it, as in the reference
+    from torchsim_amd.sequence._dephasing import unrefocused_s, winding
+
+    assert winding(packed).analytic and np.abs(unrefocused_s(packed).double().numpy()).max() == 0.0
+    maps_states = {k: v for k, v in maps.items() if k != "B0"}
+    table = lineshape_table()
+    lineshape = (table.values, table.slopes, table.step)
+    base = {_ORACLE[k]: torch.tensor(v, dtype=torch.float64) for k, v in maps_states.items()}
+    factor = complex(sim.signal_factor) if sim.signal_factor is not None else 1.0
+
+    def run(*values):
+        given = dict(base)
+        given.update({_ORACLE[k]: v for k, v in zip(names, values)})
+        return factor * P.simulate(events, given, states=10, outputs=packed.output_count, lineshape=lineshape)
+
+    primal = tuple(base[_ORACLE[k]] for k in names)
+    expect = run(*primal).numpy()
+    got = signal.cpu().numpy().reshape(atoms, -1)
+    error = np.abs(got - expect).max() / np.abs(expect).max()
+    print(f"{name} signal: {error:.2e}")
+    assert error < TOL
+    for k, n in enumerate(names):
+        tangents = tuple(torch.ones(atoms, dtype=torch.float64) if j == k else torch.zeros(atoms, dtype=torch.float64)
+                         for j in range(len(names)))
+        column = torch.func.jvp(run, primal, tangents)[1].numpy()
+        err = np.abs(jacobian[:, k].cpu().numpy() - column).max() / np.abs(column).max()
+        print(f"{name} d/d{n}: {err:.2e}")
+        assert err < JTOL, n
+
+
+def test_the_kernel_launch_matches_the_oracle_on_a_refocused_train():
+    """The semisolid kernel called directly on the FSE train (the samples
+    before the simulator's recovery factor), with columns T1, T2, M0."""
+    from torchsim_amd.sequence import _engine, _semisolid
+    from torchsim_amd.sequence._lineshape import lineshape_table
+
+    sim = _simulator("fse")
+    packed = sim.packed(**sim.bound)
+    atoms = 40
+    maps = _maps(atoms, 23)
+    fields = {"t1_ms": maps["T1"], "t2_ms": maps["T2"], "m0": maps["M0"], "b0_hz": maps["B0"],
+              "bound_fraction": maps["bound_fraction"], "bound_exchange_rate": maps["bound_exchange"],
+              "t1_bound_ms": maps["bound_T1"]}
+    names = ("t1_ms", "t2_ms", "m0")
+    tissue = _engine.prepare_tissue({k: torch.tensor(v) for k, v in fields.items()}, torch.device("cuda", 0),
+                                    names)
+    assert tissue.semisolid
+    signal, jacobian = _semisolid.simulate_semisolid(packed, tissue, states=10, columns=names)
+    table = lineshape_table()
+    lineshape = (table.values, table.slopes, table.step)
+    keys = {"T1": "t1", "T2": "t2", "M0": "m0", "B0": "b0", "bound_fraction": "bound_fraction",
+            "bound_exchange": "bound_exchange", "bound_T1": "t1_bound"}
+    base = {keys[k]: torch.tensor(v, dtype=torch.float64) for k, v in maps.items() if k in keys}
+    events = oracle.events_of(packed)
+
+    def run(*values):
+        given = dict(base)
+        given.update(dict(zip(("t1", "t2", "m0"), values)))
+        return P.simulate(events, given, states=10, outputs=packed.output_count, lineshape=lineshape)
+
+    primal = (base["t1"], base["t2"], base["m0"])
+    expect = run(*primal).numpy()
+    error = np.abs(signal.cpu().numpy() - expect).max() / np.abs(expect).max()
+    print(f"fse kernel signal: {error:.2e}")
+    assert error < TOL
+    for k in range(3):
+        tangents = tuple(torch.ones(atoms, dtype=torch.float64) if j == k else torch.zeros(atoms, dtype=torch.float64)
+                         for j in range(3))
+        column = torch.func.jvp(run, primal, tangents)[1].numpy()
+        err = np.abs(jacobian[:, k].cpu().numpy() - column).max() / np.abs(column).max()
+        print(f"fse kernel column {names[k]}: {err:.2e}")
+        assert err < JTOL
+
+
+def test_a_zero_bound_fraction_is_the_single_pool_run_bit_for_bit():
+    sim = _simulator("mrf")
+    maps = {k: torch.tensor(v).cuda() for k, v in _maps(30, 29).items() if not k.startswith("bound")}
+    plain = sim.simulate(**maps)
+    gated = sim.simulate(**maps, bound_fraction=0.0, bound_exchange=20.0, bound_T1=500.0)
+    assert torch.equal(plain, gated)
+
+
+def test_derivative_modes_through_the_pool_are_refused():
+    sim = _simulator("mrf")
+    maps = {k: torch.tensor(v).cuda() for k, v in _maps(8, 31).items()}
+    maps["T1"].requires_grad_(True)
+    with pytest.raises(NotImplementedError, match="semisolid"):
+        sim.simulate(**maps)

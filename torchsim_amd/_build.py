@@ -68,6 +68,9 @@ def _units() -> list[tuple[Path, list[str], Path]]:
             for lanes in (4, 8, 10, 12, 16, 20, 32, 64):
                 units.append((source, [f"-DTSIM_COMPLEX_ADJOINT_L={lanes}"],
                               BUILD / f"epg_complex_adjoint_tile_{lanes}.o"))
+        elif source.name == "epg_semisolid_tile.hip":
+            for lanes in (4, 8, 10, 12, 16, 20, 32, 64):
+                units.append((source, [f"-DTSIM_SEMISOLID_L={lanes}"], BUILD / f"epg_semisolid_tile_{lanes}.o"))
         elif source.name == "epg_complex_fovr_tile.hip":
             for lanes in (4, 8, 10, 12, 16, 20, 32, 64):
                 units.append((source, [f"-DTSIM_COMPLEX_FOVR_L={lanes}"], BUILD / f"epg_complex_fovr_tile_{lanes}.o"))

@@ -1,0 +1,163 @@
+// Host side of the semisolid-pool kernel: argument checks, lane-count choice
+// and the C ABI entry point (declared in include/torchsim_hip.h).
+#include "epg_semisolid_impl.h"
+
+#include <cstring>
+
+namespace tsim {
+namespace {
+
+SemisolidLaunch pick_semisolid(int lanes, int tm) {
+  switch (lanes) {
+#define TSIM_SEMISOLID_CASE(LANES) \
+  case LANES:                      \
+    return pick_semisolid_##LANES(tm);
+    TSIM_COMPLEX_LANES(TSIM_SEMISOLID_CASE)
+#undef TSIM_SEMISOLID_CASE
+    default:
+      return nullptr;
+  }
+}
+
+int semisolid_lanes(int64_t states) {
+  for (int l : {4, 8, 10, 12, 16, 20, 32, 64})
+    if (states <= l) return l;
+  return -1;
+}
+
+}  // namespace
+}  // namespace tsim
+
+using namespace tsim;
+
+extern "C" int tsim_semisolid_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
+                                       int64_t state_count, int64_t output_count, int features,
+                                       int64_t tissue_stride, const int32_t* columns, int64_t column_count,
+                                       float factor_re, float factor_im, const float* lineshape,
+                                       int64_t lineshape_bins, float lineshape_step, void* signal, void* jacobian,
+                                       void* stream) {
+  if (packed == nullptr || packed[T_T1] == nullptr || packed[T_T2] == nullptr) {
+    set_error("packed buffers with t1 and t2 are required");
+    return BAD_ARGUMENT;
+  }
+  if (atom_count < 0 || event_count < 0 || state_count < 1 || output_count < 0) {
+    set_error("invalid EPG buffer dimensions");
+    return BAD_ARGUMENT;
+  }
+  if (state_count > 64) {
+    set_error("at most 64 configuration orders are carried, %lld asked", (long long)state_count);
+    return UNSUPPORTED;
+  }
+  if (tissue_stride != 0 && tissue_stride != 1) {
+    set_error("tissue_stride must be 0 or 1");
+    return BAD_ARGUMENT;
+  }
+  if (features & FEATURE_MOVING) {
+    set_error("the semisolid kernel carries no flow term");
+    return UNSUPPORTED;
+  }
+  if (packed[T_POOL_B_FRACTION] != nullptr) {
+    set_error("a semisolid pool beside an exchanging pool (three pools) is not built");
+    return UNSUPPORTED;
+  }
+  if (packed[T_BOUND_FRACTION] == nullptr || packed[T_BOUND_EXCHANGE] == nullptr || packed[T_T1_BOUND] == nullptr) {
+    set_error("the semisolid pool needs its fraction, exchange rate and T1 buffers");
+    return BAD_ARGUMENT;
+  }
+  if (lineshape == nullptr || lineshape_bins < 2 || !(lineshape_step > 0.0f)) {
+    set_error("the lineshape table needs at least two knots and a positive spacing");
+    return BAD_ARGUMENT;
+  }
+  if (column_count < 0 || column_count > 4 || (column_count > 0 && columns == nullptr)) {
+    set_error("between 0 and 4 Jacobian columns, got %lld", (long long)column_count);
+    return BAD_ARGUMENT;
+  }
+  const bool sized = atom_count > 0 && output_count > 0;
+  if (sized && (signal == nullptr || (column_count > 0 && jacobian == nullptr))) {
+    set_error("the signal (and, with columns, the Jacobian) output is required");
+    return BAD_ARGUMENT;
+  }
+  SemisolidArgs a;
+  memset(&a, 0, sizeof(a));
+  a.t1 = (const float*)packed[T_T1];
+  a.t2 = (const float*)packed[T_T2];
+  a.m0 = (const float*)packed[T_M0];
+  a.b1 = (const float*)packed[T_B1];
+  a.b1_phase = (const float*)packed[T_B1_PHASE];
+  a.b0 = (const float*)packed[T_B0];
+  a.inv_eff = (const float*)packed[T_INV_EFF];
+  a.diffusion = (const float*)packed[T_DIFFUSION];
+  a.bound_fraction = (const float*)packed[T_BOUND_FRACTION];
+  a.bound_exchange = (const float*)packed[T_BOUND_EXCHANGE];
+  a.t1_bound = (const float*)packed[T_T1_BOUND];
+  a.saturation = (const float*)packed[TISSUE_COUNT + E_SATURATION];
+  a.rf_frequency = (const float*)packed[TISSUE_COUNT + E_RF_FREQUENCY];
+  if (a.saturation == nullptr || a.rf_frequency == nullptr) {
+    set_error("the saturation and rf_frequency event buffers are required");
+    return BAD_ARGUMENT;
+  }
+  a.lineshape = lineshape;
+  a.lineshape_bins = lineshape_bins;
+  a.lineshape_step = lineshape_step;
+  a.stride_opt = tissue_stride;
+  a.atoms = atom_count;
+  a.outputs = output_count;
+  a.states = (int)state_count;
+  a.density = (features & FEATURE_DENSITY) && a.m0 ? 1 : 0;
+  a.inverting = (features & FEATURE_INVERTING) && a.inv_eff ? 1 : 0;
+  a.transmit = (features & FEATURE_TRANSMIT) && a.b1 ? 1 : 0;
+  a.off_axis = (features & FEATURE_OFF_AXIS) ? 1 : 0;
+  a.diffusing = (features & FEATURE_DIFFUSING) && a.diffusion ? 1 : 0;
+  a.factor_re = factor_re;
+  a.factor_im = factor_im;
+  a.col_t1 = a.col_t2 = a.col_b1 = a.col_m0 = -1;
+  int tm = 0;
+  for (int64_t c = 0; c < column_count; ++c) {
+    int* slot;
+    switch (columns[c]) {
+      case T_T1: tm |= CT1; slot = &a.col_t1; break;
+      case T_T2: tm |= CT2; slot = &a.col_t2; break;
+      case T_B1: tm |= CB1; slot = &a.col_b1; break;
+      case T_M0: slot = &a.col_m0; break;
+      default:
+        set_error("column %lld: property %d carries no semisolid-kernel tangent", (long long)c, columns[c]);
+        return BAD_ARGUMENT;
+    }
+    if (*slot >= 0) {
+      set_error("property %d appears twice among the Jacobian columns", columns[c]);
+      return BAD_ARGUMENT;
+    }
+    *slot = (int)c;
+  }
+  if (a.col_b1 >= 0 && !a.transmit) {
+    set_error("a B1 column needs the transmit feature and a b1 buffer");
+    return BAD_ARGUMENT;
+  }
+  a.columns = (int)column_count;
+  a.signal = (float2*)signal;
+  a.jacobian = column_count > 0 ? (float2*)jacobian : nullptr;
+  if (!sized) return OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int L = semisolid_lanes(state_count);
+  SemisolidLaunch launch = pick_semisolid(L, tm);
+  if (!launch) {
+    set_error("no semisolid kernel for %lld orders", (long long)state_count);
+    return UNSUPPORTED;
+  }
+  Compiled compiled;
+  int st = compile_packed(packed, event_count, s, compiled);
+  if (st != OK) return st;
+  a.events = compiled.codes;
+  a.event_count = event_count;
+  const int apw = 64 / L;
+  const int64_t waves = (atom_count + apw - 1) / apw;
+  const int threads = waves < 4LL * 256 ? 64 : 256;
+  const int64_t atoms_per_block = (int64_t)(threads / 64) * apw;
+  const int64_t blocks = (atom_count + atoms_per_block - 1) / atoms_per_block;
+  if (blocks > 0x7FFFFFFF) {
+    set_error("too many atoms for one launch: %lld", (long long)atom_count);
+    return BAD_ARGUMENT;
+  }
+  launch(a, dim3((unsigned)blocks), dim3(threads), s);
+  return hip_status(hipGetLastError(), "semisolid_kernel launch");
+}

@@ -54,6 +54,11 @@ PUBLIC_PROPERTIES = {
     "T1_pool_b": "t1_pool_b_ms",
     "T2_pool_b": "t2_pool_b_ms",
     "pool_b_shift": "pool_b_shift_hz",
+    # the semisolid (magnetization-transfer) pool (reference
+    # sequence/_parameters.py:117-123, 245-247)
+    "bound_fraction": "bound_fraction",
+    "bound_exchange": "bound_exchange_rate",
+    "bound_T1": "t1_bound_ms",
     # the field's spread across the voxel: reaches the samples, not the states
     # (reference sequence/_parameters.py:168-173, 242)
     "T2prime": "t2_prime_ms",
@@ -391,6 +396,16 @@ class Simulator:
             return subspace.project(signal), (None if jacobian is None else subspace.project(jacobian))
         from ..sequence import _complex
 
+        if tissue.semisolid:
+            if reverse or forward_mode:
+                raise NotImplementedError(
+                    "a tissue with a semisolid pool runs forward with its Jacobian columns (T1, T2, M0, B1); "
+                    "reverse and forward-mode derivatives through it are not built"
+                )
+            from ..sequence import _semisolid
+
+            return _semisolid.simulate_semisolid(packed, tissue, states=states, columns=differentiated,
+                                                 factor=factor)
         if not differentiated and (reverse or forward_mode):
             # the plain signal under reverse or forward mode: one Function whose
             # backward takes the adjoint that carries what requires grad and

@@ -62,6 +62,7 @@ class PreparedTissue:
     features: int
     stride: int  # 1: optional buffers hold one value per atom; 0: one in total
     exchanging: bool
+    semisolid: bool = False  # a magnetization-transfer pool (bound fraction set)
 
     @property
     def device(self) -> torch.device:
@@ -127,6 +128,13 @@ def prepare_tissue(
             live[name] = tensor
             features |= _FEATURE_OF.get(name, 0)
     exchanging = "pool_b_fraction" in live
+    semisolid = "bound_fraction" in live
+    if semisolid:
+        # the pool's exchange rate and T1 take the reference's defaults
+        # (sequence/_simulation.py:212-214) where the caller gave none
+        for name, default in (("bound_exchange_rate", 0.0), ("t1_bound_ms", 1000.0)):
+            if name not in live:
+                live[name] = _on_device(given.get(name, default), device)
     shape = torch.broadcast_shapes(*(value.shape for value in live.values()))
     atoms = 1
     for size in shape:
@@ -149,7 +157,7 @@ def prepare_tissue(
         if features & bit and buffers[index] is None:
             fill = torch.full((1 if stride == 0 else atoms,), _IDENTITY[name], dtype=torch.float32, device=device)
             buffers[index] = fill
-    return PreparedTissue(buffers, torch.Size(shape), atoms, features, stride, exchanging)
+    return PreparedTissue(buffers, torch.Size(shape), atoms, features, stride, exchanging, semisolid)
 
 
 def half_turns(packed: PackedEvents) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor | None] | None:
@@ -188,7 +196,7 @@ def real_subspace(packed: PackedEvents, tissue: PreparedTissue,
     sample signs if so (None where no sign flips). The verdict is worked out
     once per packed stream; with ``device``, the flips and signs come back
     resident there (uploaded once, asynchronously)."""
-    if tissue.exchanging or tissue.features & (_native.FEATURE_OFF_AXIS | _native.FEATURE_MOVING):
+    if tissue.exchanging or tissue.semisolid or tissue.features & (_native.FEATURE_OFF_AXIS | _native.FEATURE_MOVING):
         return False, None, None
     verdict = packed.__dict__.get("_real_subspace")
     if verdict is None:

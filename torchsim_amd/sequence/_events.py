@@ -14,6 +14,8 @@ train packs to the same buffers.
 
 from __future__ import annotations
 
+import math
+
 import threading
 from collections.abc import Callable, Sequence
 from dataclasses import dataclass
@@ -37,6 +39,12 @@ EXCITATION = 64
 REFOCUSING = 128
 
 _TO_US = 1e6
+
+#: The power an ideal hard pulse deposits in a semisolid pool, per flip
+#: squared, in 1/s: -pi int |w|^2 dt / |int w dt|^2 for the reference's ideal
+#: pulse, a unit envelope over 1/(2 pi) s (sequence/_description.py:501-540,
+#: 953-981), which is -2 pi^2. Packed for every pulse.
+IDEAL_SATURATION = -2.0 * math.pi * math.pi
 
 
 class EventType(IntEnum):
@@ -439,7 +447,8 @@ def pack(events: Sequence[SequenceEvent], *, repetitions: int = 1, tr_duration_u
         action=torch.tensor(actions, dtype=torch.uint8),
         output_index=torch.tensor(outputs, dtype=torch.int32),
         shim_index=torch.zeros(count, dtype=torch.int32),
-        saturation=torch.zeros(count, dtype=torch.float32),
+        saturation=torch.tensor([IDEAL_SATURATION if k == EventType.RF else 0.0 for k in kinds],
+                                dtype=torch.float32),
         rf_frequency_hz=torch.zeros(count, dtype=torch.float32),
         output_count=output,
     )
