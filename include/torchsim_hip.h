@@ -105,6 +105,29 @@ int tsim_semisolid_jacobian(void* const* packed, int64_t atom_count, int64_t eve
                             void* jacobian, void* stream);
 
 /*
+ * Signal and Jacobian with slice-profiled pulses: every pulse but an
+ * inversion turns the states through the Cayley-Klein pair (a, b) its shape
+ * performs at the atom's slice position and effective flip flip * b1, read
+ * by cubic Hermite from `profile`, the RF phase applied as b exp(-i phi).
+ * Replaces `_epg_cpu.simulate` / `simulate_jvp` in RfMode::PROFILED with the
+ * table in the Tail::TABLE slot, one shape (src/torchsim/_epg_cpu.cpp:
+ * 2041-2049, 2107-2160, 2273-2304, 4491-4520).
+ * Atoms are voxel-major with `locations` atoms per voxel (atom_count a
+ * multiple of it; atom i sits at position i % locations); the caller
+ * averages each voxel's signal and columns over its positions.
+ * profile: DEVICE (locations, profile_bins, 8) FP32: a, b, da/dtheta,
+ *          db/dtheta as (real, imag) at flips k * profile_step.
+ * A semisolid pool runs too where its packed buffers are set (then the
+ * lineshape is required as for tsim_semisolid_jacobian; else it may be NULL).
+ */
+int tsim_profiled_jacobian(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
+                           int64_t output_count, int features, int64_t tissue_stride, const int32_t* columns,
+                           int64_t column_count, float factor_re, float factor_im, const float* profile,
+                           int64_t locations, int64_t profile_bins, float profile_step, const float* lineshape,
+                           int64_t lineshape_bins, float lineshape_step, void* signal, void* jacobian,
+                           void* stream);
+
+/*
  * Real-subspace signal and Jacobian in ONE pass, written as complex64 in the
  * public layout. Replaces the P pairs of `_epg_cpu.simulate` +
  * `_epg_cpu.simulate_jvp` (src/torchsim/_epg_cpu.cpp:10740, 11017) that

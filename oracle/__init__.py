@@ -190,6 +190,7 @@ def _reference_call(
     tangent: dict[str, Any] | None,
     threads: int,
     lineshape: tuple[Any, Any, float] | None = None,
+    profile: tuple[Any, Any, Any, Any, float] | None = None,
 ) -> np.ndarray:
     module = reference_module()
     if module is None:
@@ -230,8 +231,18 @@ def _reference_call(
         knots = np.stack((np.asarray(lineshape[0]), np.asarray(lineshape[1])), -1)
         tail[12] = arr(knots, np.float32)
         bins, step = int(knots.shape[0]), float(lineshape[2])
-    dims = (atoms, 1, count, states, outputs, threads, real_axis, 0.0, 0.0, 1, 1, 0, 1.0, bins, step, pool_kind,
-            features, 0, 0, 0)
+    locations, profile_bins, profile_step = 1, 0, 1.0
+    if profile is not None:
+        # the table (positions, bins, 8) in Tail::TABLE and every event's
+        # shape row (one shape: 0) in Tail::TABLE_INDEX
+        a, b, da, db = (np.asarray(v, dtype=np.complex128) for v in profile[:4])
+        table = np.stack((a.real, a.imag, b.real, b.imag, da.real, da.imag, db.real, db.imag), -1)
+        tail[5] = arr(table, np.float32)
+        tail[6] = arr(np.zeros(count), np.int32)
+        locations, profile_bins = int(a.shape[0]), int(a.shape[1])
+        profile_step = float(profile[4]) / (profile_bins - 1)
+    dims = (atoms, 1, count, states, outputs, threads, real_axis, 0.0, 0.0, 1, locations, profile_bins,
+            profile_step, bins, step, pool_kind, features, 0, 0, 0)
     if tangent is None:
         module.simulate((*tissue_ptrs, *event_ptrs, real.ctypes.data, imag.ctypes.data, *tail), *dims)
     else:
@@ -281,6 +292,7 @@ def reference_complex(
     tangent: dict[str, Any] | None = None,
     threads: int = 1,
     lineshape: tuple[Any, Any, float] | None = None,
+    profile: tuple[Any, Any, Any, Any, float] | None = None,
 ) -> np.ndarray:
     """The reference's own FP32 complex kernel: recorded samples (atoms, outputs), complex.
 
@@ -297,11 +309,15 @@ def reference_complex(
     With ``lineshape`` = (values, slopes, step_hz) the tissue's
     ``bound_fraction`` / ``bound_exchange`` / ``t1_bound`` run the semisolid
     pool (pool kind 1), the events' ``saturation`` and ``rf_frequency``
-    driving its saturation.
+    driving its saturation. With ``profile`` = (a, b, slope_a, slope_b,
+    theta_max) ((positions, bins) each) the pulses are profiled (RfMode::
+    PROFILED): ``tissue`` then holds one row per (voxel, position),
+    voxel-major.
     """
     kind = 1 if lineshape is not None else (2 if exchanging else 0)
     return _reference_call(events, tissue, states=states, outputs=outputs, features=features,
-                           real_axis=-1, pool_kind=kind, tangent=tangent, threads=threads, lineshape=lineshape)
+                           real_axis=-1, pool_kind=kind, tangent=tangent, threads=threads, lineshape=lineshape,
+                           profile=profile)
 
 
 _EVENT_VJP_ORDER = ("flip", "phase", "duration")        # simulate_vjp's gradient slots after the tissue

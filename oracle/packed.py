@@ -52,6 +52,7 @@ def simulate(
     states: int,
     outputs: int,
     lineshape: tuple[Any, Any, float] | None = None,
+    profile: tuple[Any, Any, Any, Any, float] | None = None,
     _plant: str | None = None,
 ) -> torch.Tensor:
     """Recorded complex samples (atoms, outputs), complex128.
@@ -68,8 +69,13 @@ def simulate(
     src/torchsim/_epg_cpu.cpp:875-932, 1888-1910, 4216-4566 with pool kind 1:
     the bound pool has Z states only, exchanging with the free water's Z and
     saturated by exp(saturation theta^2 G(rf_frequency - b0)) at every
-    pulse; an inversion leaves it alone). Tissue entries may be tensors
-    requiring grad; flip and duration may be too.
+    pulse; an inversion leaves it alone). With ``profile`` = (a, b, slope_a,
+    slope_b, theta_max), each (locations, bins) complex, every pulse but an
+    inversion turns the states through the Cayley-Klein pair read by cubic
+    Hermite at the atom's position (atom % locations) and effective flip,
+    b turned by exp(-i phi) (reference src/torchsim/_epg_cpu.cpp:2041-2049,
+    2107-2160, 2273-2304). Tissue entries may be tensors requiring grad;
+    flip and duration may be too.
 
     ``_plant`` exists only so the tests can show that their checks fail on a
     plausible mistake: ``"swap_exchange"`` exchanges the two rate constants
@@ -153,6 +159,26 @@ def simulate(
             i = lower.to(torch.int64)
             return ((2 * u**3 - 3 * u**2 + 1) * knots[i] + (u**3 - 2 * u**2 + u) * step * slopes[i]
                     + (-2 * u**3 + 3 * u**2) * knots[i + 1] + (u**3 - u**2) * step * slopes[i + 1])
+    if profile is not None:
+        pa, pb, sa_tab, sb_tab = (torch.as_tensor(v, dtype=_C) for v in profile[:4])
+        bins = pa.shape[1]
+        step_t = float(profile[4]) / (bins - 1)
+        where = torch.arange(atoms) % pa.shape[0]
+
+        def pair(theta):
+            # cubic Hermite along the flip at each atom's position, clamped
+            scaled = torch.clamp(theta / step_t, 0.0, float(bins - 1))
+            lower = torch.clamp(torch.floor(scaled), max=float(bins - 2))
+            u = scaled - lower
+            i = lower.to(torch.int64)
+            h = ((2 * u**3 - 3 * u**2 + 1), (u**3 - 2 * u**2 + u) * step_t, (-2 * u**3 + 3 * u**2),
+                 (u**3 - u**2) * step_t)
+
+            def read(values, slopes):
+                return (h[0] * values[where, i] + h[1] * slopes[where, i] + h[2] * values[where, i + 1]
+                        + h[3] * slopes[where, i + 1])
+
+            return read(pa, sa_tab), read(pb, sb_tab)
     samples: list[torch.Tensor | None] = [None] * outputs
     for e in range(len(kind)):
         dt = duration[e]
@@ -203,6 +229,19 @@ def simulate(
                 if semisolid and float(saturation[e]) != 0.0:
                     g = absorbed(rf_frequency[e] - b0)
                     zb = zb * torch.exp(saturation[e] * (flip[e] * b1) ** 2 * g)[:, None]
+                if profile is not None:
+                    a, b = pair(flip[e] * b1)
+                    b = b * torch.exp(-1j * (phase[e] + b1_phase))
+                    a, b = a[:, None], b[:, None]
+                    ca, cb = a.conj(), b.conj()
+                    for q in range(pools):
+                        x, y, w = fp[q], fm[q], z[q]
+                        fp[q] = ca * ca * x - cb * cb * y - 2 * (a * b).conj() * w
+                        fm[q] = -b * b * x + a * a * y - 2 * a * b * w
+                        z[q] = ca * b * x + a * cb * y + (a.abs() ** 2 - b.abs() ** 2) * w
+                    continue_rotation = False
+                else:
+                    continue_rotation = True
                 ca, sa = torch.cos(alpha), torch.sin(alpha)
                 p1 = torch.exp(1j * phi)
                 p2 = p1 * p1
@@ -213,7 +252,7 @@ def simulate(
                 t12 = 1j * sa * p1.conj()
                 t20 = -0.5j * sa * p1.conj()
                 t21 = 0.5j * sa * p1
-                for q in range(pools):
+                for q in range(pools if continue_rotation else 0):
                     a, b, c = fp[q], fm[q], z[q]
                     fp[q] = t00 * a + t01 * b + t02 * c
                     fm[q] = t10 * a + t00 * b + t12 * c

@@ -145,3 +145,82 @@ def test_derivative_modes_through_the_pool_are_refused():
     maps["T1"].requires_grad_(True)
     with pytest.raises(NotImplementedError, match="semisolid"):
         sim.simulate(**maps)
+
+
+# --- slice-profiled pulses ---
+
+
+def _profiled_expected(sim, maps, names, table, lineshape=None):
+    """FP64: the oracle at every tabulated position (one row per voxel and
+    position), averaged over positions, with its forward-mode columns."""
+    packed = sim.packed(**sim.bound)
+    events = oracle.events_of(packed)
+    positions = table.points
+    profile = (table.a.numpy(), table.b.numpy(), table.slope_a.numpy(), table.slope_b.numpy(), table.theta_max)
+    base = {_ORACLE[k]: torch.tensor(np.repeat(v, positions), dtype=torch.float64) for k, v in maps.items()}
+    factor = complex(sim.signal_factor) if sim.signal_factor is not None else 1.0
+    voxels = len(maps["T1"])
+
+    def run(*values):
+        given = dict(base)
+        given.update({_ORACLE[k]: v.repeat_interleave(positions) for k, v in zip(names, values)})
+        rows = P.simulate(events, given, states=10, outputs=packed.output_count, lineshape=lineshape,
+                          profile=profile)
+        return factor * rows.reshape(voxels, positions, -1).mean(1)
+
+    primal = tuple(torch.tensor(maps[k], dtype=torch.float64) for k in names)
+    signal = run(*primal).numpy()
+    columns = []
+    for k in range(len(names)):
+        tangents = tuple(torch.ones(voxels, dtype=torch.float64) if j == k else torch.zeros(voxels, dtype=torch.float64)
+                         for j in range(len(names)))
+        columns.append(torch.func.jvp(run, primal, tangents)[1].numpy())
+    return signal, columns
+
+
+@pytest.mark.parametrize("name", ["mrf", "fse", "fse_semisolid"])
+def test_profiled_pulses_against_the_fp64_oracle(name):
+    """A windowed-sinc pulse worked out at 7 slice positions (the table the
+    oracle reads too): signal and T1/T2/M0/B1 columns of the slice average,
+    with a B1 map (and, for one case, a semisolid pool, whose saturation is
+    the pulse's own deposited power)."""
+    from torchsim_amd.sequence._lineshape import lineshape_table
+    from torchsim_amd.sequence._transition import exact_slice_profile, sinc_pulse, transition_table
+
+    pulse = sinc_pulse(2e-3, 4.0, raster_s=1e-5)
+    profile = exact_slice_profile(7, bins=32)
+    sim = _simulator(name.split("_")[0]).bind(pulse=pulse, across_slice=profile)
+    atoms = 20
+    maps = {k: v for k, v in _maps(atoms, 37).items() if k in ("T1", "T2", "M0", "B1")}
+    lineshape = None
+    if name == "fse_semisolid":
+        extra = _maps(atoms, 38)
+        maps.update({k: extra[k] for k in ("bound_fraction", "bound_exchange", "bound_T1")})
+        table_ls = lineshape_table()
+        lineshape = (table_ls.values, table_ls.slopes, table_ls.step)
+    names = ("T1", "T2", "M0", "B1")
+    signal, jacobian = sim.jacobian(names, **{k: torch.tensor(v).cuda() for k, v in maps.items()})
+    table = transition_table(pulse, profile.positions(), bins=profile.bins, theta_max=profile.theta_max)
+    expect, columns = _profiled_expected(sim, maps, names, table, lineshape)
+    error = np.abs(signal.cpu().numpy() - expect).max() / np.abs(expect).max()
+    print(f"{name} profiled signal: {error:.2e}")
+    assert error < TOL
+    for k, n in enumerate(names):
+        err = np.abs(jacobian[:, k].cpu().numpy() - columns[k]).max() / np.abs(columns[k]).max()
+        print(f"{name} profiled d/d{n}: {err:.2e}")
+        assert err < JTOL, n
+
+
+def test_a_tabulated_hard_pulse_is_the_instant_rotation():
+    """A rectangle without slice selection asked for at the slice centre
+    runs the profiled kernel through its table, and lands on the hard pulse's
+    answer (to the table's interpolation)."""
+    from torchsim_amd.sequence._transition import RfPulse
+
+    sim = _simulator("mrf")
+    maps = {k: torch.tensor(v).cuda() for k, v in _maps(16, 41).items() if k in ("T1", "T2", "M0")}
+    instant = sim.simulate(**maps)
+    tabulated = sim.simulate(**maps, pulse=RfPulse(np.ones(20), 1e-5, 0.0), across_slice=1)
+    error = float((tabulated - instant).abs().max() / instant.abs().max())
+    print(f"hard pulse through its table: {error:.2e}")
+    assert error < 2e-5

@@ -142,3 +142,71 @@ def test_the_oracle_pool_closed_forms():
         sample = P.simulate(wait, {"t1": 900.0, "t2": 70.0, "bound_fraction": f, "bound_exchange": 25.0,
                                    "t1_bound": 400.0}, states=4, outputs=1, lineshape=lineshape).numpy()[0, 0]
         assert abs(sample - (-1j) * (1.0 - f)) < 1e-12
+
+
+# --- slice-profiled pulses ---
+
+
+def test_a_hard_pulse_table_is_the_instant_rotation_and_a_shaped_one_matches_bloch():
+    """A rectangle with no slice selection tabulates to (cos(theta/2),
+    -i sin(theta/2)) at every position with slope (-sin/2, -i cos/2)/1;
+    a windowed sinc under its gradient leaves, at each position, the
+    magnetization an independent Bloch integration (a Rodrigues rotation per
+    sample, FP64) leaves."""
+    from torchsim_amd.sequence._transition import RfPulse, sinc_pulse, transition_table
+
+    theta = np.linspace(0.0, 2 * np.pi, 16)
+    hard = transition_table(RfPulse(np.ones(50), 1e-5, 0.0), np.linspace(-1.0, 1.0, 5), bins=16)
+    assert np.abs(hard.a.numpy() - np.cos(theta / 2)).max() < 1e-6
+    assert np.abs(hard.b.numpy() + 1j * np.sin(theta / 2)).max() < 1e-6
+    assert np.abs(hard.slope_a.numpy() + 0.5 * np.sin(theta / 2)).max() < 1e-6
+    assert np.abs(hard.slope_b.numpy() + 0.5j * np.cos(theta / 2)).max() < 1e-6
+
+    pulse = sinc_pulse(2e-3, 4.0, raster_s=1e-5)
+    positions = np.linspace(-1.0, 1.0, 9)
+    table = transition_table(pulse, positions, bins=32)
+    w = pulse.envelope * pulse.raster_s
+    w = w / w.sum()
+    k = 8  # theta = 8 * 2 pi / 31
+    flip = k * table.step
+    for i, z in enumerate(positions):
+        m = np.array([0.0, 0.0, 1.0])
+        rate = 2 * np.pi * pulse.bandwidth_hz * z * pulse.raster_s
+        for ws in w:
+            field = np.array([(flip * ws).real, (flip * ws).imag, rate])
+            angle = np.linalg.norm(field)
+            n = field / angle
+            m = m * np.cos(angle) + np.cross(n, m) * np.sin(angle) + n * np.dot(n, m) * (1 - np.cos(angle))
+        a, b = complex(table.a[i, k]), complex(table.b[i, k])
+        assert abs(abs(-2 * np.conj(a * b)) - np.hypot(m[0], m[1])) < 2e-6, z
+        assert abs(abs(a) ** 2 - abs(b) ** 2 - m[2]) < 2e-6, z
+    # the slope is the derivative of the same composition
+    fine = transition_table(pulse, positions, bins=32 * 8 - 7)  # knots of the first table plus seven between
+    numeric = (fine.a[:, 8 * k + 1] - fine.a[:, 8 * k - 1]).numpy() / (2 * fine.step)
+    assert np.abs(numeric - table.slope_a[:, k].numpy()).max() < 1e-3
+
+
+@pytest.mark.parametrize("name", ["mrf", "fse"])
+@pytest.mark.parametrize("plant", [None, "hard"])
+def test_the_oracle_profiled_rotation_is_the_reference_kernel(name, plant):
+    """The oracle's profiled pulses against the reference's own kernel in
+    RfMode::PROFILED with the same table (7 positions, B1 per voxel), one row
+    per (voxel, position); the oracle planted with hard pulses misses."""
+    if oracle.reference_module() is None:
+        pytest.skip("oracle/_ref is not built")
+    from torchsim_amd.sequence._transition import sinc_pulse, transition_table
+
+    pulse = sinc_pulse(2e-3, 4.0, raster_s=1e-5)
+    table = transition_table(pulse, np.linspace(-1.0, 1.0, 7), bins=32)
+    profile = (table.a.numpy(), table.b.numpy(), table.slope_a.numpy(), table.slope_b.numpy(), table.theta_max)
+    events = _streams()[name]
+    n = int((events["output_index"] >= 0).sum())
+    t1, t2 = _cases.random_tissue(3, seed=2)
+    rows = {"t1": np.repeat(t1, 7), "t2": np.repeat(t2, 7), "b1": np.repeat([0.9, 1.0, 1.1], 7)}
+    ref = oracle.reference_complex(events, rows, states=10, outputs=n, profile=profile)
+    ours = P.simulate(events, rows, states=10, outputs=n, profile=None if plant else profile).numpy()
+    error = np.abs(ours - ref).max() / np.abs(ref).max()
+    if plant is None:
+        assert error < 2e-5, error
+    else:
+        assert error > 1e-2, error

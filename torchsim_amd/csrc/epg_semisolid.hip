@@ -7,11 +7,11 @@
 namespace tsim {
 namespace {
 
-SemisolidLaunch pick_semisolid(int lanes, int tm) {
+SemisolidLaunch pick_semisolid(int lanes, int tm, bool profiled) {
   switch (lanes) {
 #define TSIM_SEMISOLID_CASE(LANES) \
   case LANES:                      \
-    return pick_semisolid_##LANES(tm);
+    return pick_semisolid_##LANES(tm, profiled);
     TSIM_COMPLEX_LANES(TSIM_SEMISOLID_CASE)
 #undef TSIM_SEMISOLID_CASE
     default:
@@ -28,14 +28,18 @@ int semisolid_lanes(int64_t states) {
 }  // namespace
 }  // namespace tsim
 
-using namespace tsim;
 
-extern "C" int tsim_semisolid_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
-                                       int64_t state_count, int64_t output_count, int features,
-                                       int64_t tissue_stride, const int32_t* columns, int64_t column_count,
-                                       float factor_re, float factor_im, const float* lineshape,
-                                       int64_t lineshape_bins, float lineshape_step, void* signal, void* jacobian,
-                                       void* stream) {
+namespace tsim {
+namespace {
+
+// Checks and launch shared by the two entry points. `bound` asks for the
+// semisolid pool (its buffers and the lineshape required); a profile table
+// makes the pulses profiled.
+int run_shaped(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
+               int64_t output_count, int features, int64_t tissue_stride, const int32_t* columns,
+               int64_t column_count, float factor_re, float factor_im, bool bound, const float* lineshape,
+               int64_t lineshape_bins, float lineshape_step, const float* profile, int64_t locations,
+               int64_t profile_bins, float profile_step, void* signal, void* jacobian, void* stream) {
   if (packed == nullptr || packed[T_T1] == nullptr || packed[T_T2] == nullptr) {
     set_error("packed buffers with t1 and t2 are required");
     return BAD_ARGUMENT;
@@ -53,20 +57,34 @@ extern "C" int tsim_semisolid_jacobian(void* const* packed, int64_t atom_count, 
     return BAD_ARGUMENT;
   }
   if (features & FEATURE_MOVING) {
-    set_error("the semisolid kernel carries no flow term");
+    set_error("the semisolid and profiled kernel carries no flow term");
     return UNSUPPORTED;
   }
   if (packed[T_POOL_B_FRACTION] != nullptr) {
-    set_error("a semisolid pool beside an exchanging pool (three pools) is not built");
+    set_error("an exchanging pool is not built beside a semisolid pool or a profiled pulse");
     return UNSUPPORTED;
   }
-  if (packed[T_BOUND_FRACTION] == nullptr || packed[T_BOUND_EXCHANGE] == nullptr || packed[T_T1_BOUND] == nullptr) {
-    set_error("the semisolid pool needs its fraction, exchange rate and T1 buffers");
-    return BAD_ARGUMENT;
+  if (bound) {
+    if (packed[T_BOUND_FRACTION] == nullptr || packed[T_BOUND_EXCHANGE] == nullptr ||
+        packed[T_T1_BOUND] == nullptr) {
+      set_error("the semisolid pool needs its fraction, exchange rate and T1 buffers");
+      return BAD_ARGUMENT;
+    }
+    if (lineshape == nullptr || lineshape_bins < 2 || !(lineshape_step > 0.0f)) {
+      set_error("the lineshape table needs at least two knots and a positive spacing");
+      return BAD_ARGUMENT;
+    }
   }
-  if (lineshape == nullptr || lineshape_bins < 2 || !(lineshape_step > 0.0f)) {
-    set_error("the lineshape table needs at least two knots and a positive spacing");
-    return BAD_ARGUMENT;
+  if (profile != nullptr) {
+    if (locations < 1 || profile_bins < 2 || !(profile_step > 0.0f)) {
+      set_error("the profile table needs at least one position, two flip knots and a positive spacing");
+      return BAD_ARGUMENT;
+    }
+    if (atom_count % locations != 0) {
+      set_error("atoms come one per slice position: %lld atoms for %lld positions", (long long)atom_count,
+                (long long)locations);
+      return BAD_ARGUMENT;
+    }
   }
   if (column_count < 0 || column_count > 4 || (column_count > 0 && columns == nullptr)) {
     set_error("between 0 and 4 Jacobian columns, got %lld", (long long)column_count);
@@ -87,18 +105,24 @@ extern "C" int tsim_semisolid_jacobian(void* const* packed, int64_t atom_count, 
   a.b0 = (const float*)packed[T_B0];
   a.inv_eff = (const float*)packed[T_INV_EFF];
   a.diffusion = (const float*)packed[T_DIFFUSION];
-  a.bound_fraction = (const float*)packed[T_BOUND_FRACTION];
-  a.bound_exchange = (const float*)packed[T_BOUND_EXCHANGE];
-  a.t1_bound = (const float*)packed[T_T1_BOUND];
+  if (bound) {
+    a.bound_fraction = (const float*)packed[T_BOUND_FRACTION];
+    a.bound_exchange = (const float*)packed[T_BOUND_EXCHANGE];
+    a.t1_bound = (const float*)packed[T_T1_BOUND];
+  }
   a.saturation = (const float*)packed[TISSUE_COUNT + E_SATURATION];
   a.rf_frequency = (const float*)packed[TISSUE_COUNT + E_RF_FREQUENCY];
-  if (a.saturation == nullptr || a.rf_frequency == nullptr) {
+  if (bound && (a.saturation == nullptr || a.rf_frequency == nullptr)) {
     set_error("the saturation and rf_frequency event buffers are required");
     return BAD_ARGUMENT;
   }
   a.lineshape = lineshape;
   a.lineshape_bins = lineshape_bins;
   a.lineshape_step = lineshape_step;
+  a.profile = profile;
+  a.profile_bins = profile_bins;
+  a.profile_step = profile_step;
+  a.locations = profile ? locations : 1;
   a.stride_opt = tissue_stride;
   a.atoms = atom_count;
   a.outputs = output_count;
@@ -120,7 +144,7 @@ extern "C" int tsim_semisolid_jacobian(void* const* packed, int64_t atom_count, 
       case T_B1: tm |= CB1; slot = &a.col_b1; break;
       case T_M0: slot = &a.col_m0; break;
       default:
-        set_error("column %lld: property %d carries no semisolid-kernel tangent", (long long)c, columns[c]);
+        set_error("column %lld: property %d carries no tangent here", (long long)c, columns[c]);
         return BAD_ARGUMENT;
     }
     if (*slot >= 0) {
@@ -139,9 +163,9 @@ extern "C" int tsim_semisolid_jacobian(void* const* packed, int64_t atom_count, 
   if (!sized) return OK;
   hipStream_t s = (hipStream_t)stream;
   const int L = semisolid_lanes(state_count);
-  SemisolidLaunch launch = pick_semisolid(L, tm);
+  SemisolidLaunch launch = pick_semisolid(L, tm, profile != nullptr);
   if (!launch) {
-    set_error("no semisolid kernel for %lld orders", (long long)state_count);
+    set_error("no kernel for %lld orders", (long long)state_count);
     return UNSUPPORTED;
   }
   Compiled compiled;
@@ -160,4 +184,36 @@ extern "C" int tsim_semisolid_jacobian(void* const* packed, int64_t atom_count, 
   }
   launch(a, dim3((unsigned)blocks), dim3(threads), s);
   return hip_status(hipGetLastError(), "semisolid_kernel launch");
+}
+
+}  // namespace
+}  // namespace tsim
+
+using namespace tsim;
+
+extern "C" int tsim_semisolid_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
+                                       int64_t state_count, int64_t output_count, int features,
+                                       int64_t tissue_stride, const int32_t* columns, int64_t column_count,
+                                       float factor_re, float factor_im, const float* lineshape,
+                                       int64_t lineshape_bins, float lineshape_step, void* signal, void* jacobian,
+                                       void* stream) {
+  return run_shaped(packed, atom_count, event_count, state_count, output_count, features, tissue_stride, columns,
+                    column_count, factor_re, factor_im, true, lineshape, lineshape_bins, lineshape_step, nullptr, 1,
+                    0, 1.0f, signal, jacobian, stream);
+}
+
+extern "C" int tsim_profiled_jacobian(void* const* packed, int64_t atom_count, int64_t event_count,
+                                      int64_t state_count, int64_t output_count, int features, int64_t tissue_stride,
+                                      const int32_t* columns, int64_t column_count, float factor_re, float factor_im,
+                                      const float* profile, int64_t locations, int64_t profile_bins,
+                                      float profile_step, const float* lineshape, int64_t lineshape_bins,
+                                      float lineshape_step, void* signal, void* jacobian, void* stream) {
+  if (profile == nullptr) {
+    set_error("the profile table is required");
+    return BAD_ARGUMENT;
+  }
+  const bool bound = packed != nullptr && packed[T_BOUND_FRACTION] != nullptr;
+  return run_shaped(packed, atom_count, event_count, state_count, output_count, features, tissue_stride, columns,
+                    column_count, factor_re, factor_im, bound, lineshape, lineshape_bins, lineshape_step, profile,
+                    locations, profile_bins, profile_step, signal, jacobian, stream);
 }

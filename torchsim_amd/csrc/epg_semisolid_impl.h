@@ -1,6 +1,7 @@
 // The complex EPG state machine with a semisolid (magnetization-transfer)
-// pool, for gfx950: signal and fused Jacobian columns (T1, T2, B1, and M0 by
-// its closed form), complex64 in the public layout.
+// pool and/or slice-profiled pulses, for gfx950: signal and fused Jacobian
+// columns (T1, T2, B1, and M0 by its closed form), complex64 in the public
+// layout.
 //
 // It computes the reference's `simulate_range<INSTANT, SEMISOLID>` and its
 // forward-mode tangents (src/torchsim/_epg_cpu.cpp:4216-4566 with pool kind
@@ -15,6 +16,15 @@
 // exp(s theta^2 G(rf_frequency - b0)), s the pulse's deposited power per
 // flip squared (an event buffer), theta = flip * b1 and G the absorption
 // lineshape read from a table; an inversion leaves the bound pool alone.
+// Without a bound pool the pair reduces to the free water's own Z.
+//
+// A profiled pulse (PROFILED) turns the states through the Cayley-Klein pair
+// (a, b) a shaped pulse performs at the atom's slice position, read by cubic
+// Hermite from a table over (position, effective flip theta = flip * b1) at
+// zero RF phase, the phase applied as b exp(-i phi) (reference
+// rotate_spinor / profile_pair / table_row, src/torchsim/_epg_cpu.cpp:
+// 2041-2049, 2107-2160, 2273-2304, 4491-4520). Atoms are laid out voxel-major
+// with one atom per tabulated position; the caller averages over positions.
 //
 // Layout as the complex kernel: one configuration order per lane, an atom on
 // L adjacent lanes; orders past the state count are padding held at zero.
@@ -44,6 +54,10 @@ struct SemisolidArgs {
   const float* lineshape;  // (bins, 2): value, slope per knot
   int64_t lineshape_bins;
   float lineshape_step;
+  const float* profile;  // (locations, bins, 8): a, b, da/dtheta, db/dtheta as (re, im)
+  int64_t profile_bins;
+  float profile_step;
+  int64_t locations;
   int64_t stride_opt;
   int64_t atoms;
   int64_t outputs;
@@ -86,7 +100,34 @@ struct Order {
 
 }  // namespace mt
 
-template <int L, int TM>
+// The pair (a, b) a profiled pulse performs at position row `loc` and
+// effective flip theta, and its derivative along theta, by cubic Hermite
+// between the knots bracketing theta (clamped to the table).
+__device__ __forceinline__ void profile_pair(const SemisolidArgs& p, int64_t loc, float theta, float2& a, float2& b,
+                                             float2& da, float2& db) {
+  const float h = p.profile_step;
+  const float last = (float)(p.profile_bins - 1);
+  const float scaled = fminf(fmaxf(theta / h, 0.0f), last);
+  const float lower = fminf(floorf(scaled), last - 1.0f);
+  const float u = scaled - lower;
+  const float* near = p.profile + (loc * p.profile_bins + (int64_t)lower) * 8;
+  const float* far = near + 8;
+  const float u2 = u * u, u3 = u2 * u;
+  const float h00 = 2.0f * u3 - 3.0f * u2 + 1.0f, h10 = (u3 - 2.0f * u2 + u) * h;
+  const float h01 = -2.0f * u3 + 3.0f * u2, h11 = (u3 - u2) * h;
+  const float g00 = (6.0f * u2 - 6.0f * u) / h, g10 = 3.0f * u2 - 4.0f * u + 1.0f;
+  const float g01 = (6.0f * u - 6.0f * u2) / h, g11 = 3.0f * u2 - 2.0f * u;
+  auto read = [&](int i, float w00, float w10, float w01, float w11) {
+    return make_float2(w00 * near[i] + w10 * near[i + 4] + w01 * far[i] + w11 * far[i + 4],
+                       w00 * near[i + 1] + w10 * near[i + 5] + w01 * far[i + 1] + w11 * far[i + 5]);
+  };
+  a = read(0, h00, h10, h01, h11);
+  b = read(2, h00, h10, h01, h11);
+  da = read(0, g00, g10, g01, g11);
+  db = read(2, g00, g10, g01, g11);
+}
+
+template <int L, int TM, bool PROFILED>
 __global__ void __launch_bounds__(256) semisolid_kernel(const SemisolidArgs p) {
   using namespace cx;
   using mt::Order;
@@ -115,9 +156,10 @@ __global__ void __launch_bounds__(256) semisolid_kernel(const SemisolidArgs p) {
   const float b0 = p.off_axis && p.b0 ? p.b0[ia] : 0.0f;
   const float eta = p.inverting ? p.inv_eff[ia] : 1.0f;
   const float drate = p.diffusing ? p.diffusion[ia] : 0.0f;
-  const float f = p.bound_fraction[ia];
-  const float kex = p.bound_exchange[ia];
-  const float r1b = 1000.0f / p.t1_bound[ia];
+  // without a bound pool: f = 0, and the pair's second row holds nothing
+  const float f = p.bound_fraction ? p.bound_fraction[ia] : 0.0f;
+  const float kex = p.bound_exchange ? p.bound_exchange[ia] : 0.0f;
+  const float r1b = p.t1_bound ? 1000.0f / p.t1_bound[ia] : r1;
   const float free = 1.0f - f;
   const float kab = kex * f, kba = kex * free;
   const float order = (float)sub;
@@ -232,7 +274,7 @@ __global__ void __launch_bounds__(256) semisolid_kernel(const SemisolidArgs p) {
       } else {
         const float theta = ev.flip * b1;
         // the bound pool absorbs the pulse's power at its offset
-        const float deposit = p.saturation[e];
+        const float deposit = p.bound_fraction ? p.saturation[e] : 0.0f;
         if (deposit != 0.0f) {
           const float g = lineshape_at(p, p.rf_frequency[e] - b0);
           const float absorbed = expf(deposit * theta * theta * g);
@@ -244,6 +286,47 @@ __global__ void __launch_bounds__(256) semisolid_kernel(const SemisolidArgs p) {
           }
           v.zb = scl(v.zb, absorbed);
         }
+        if constexpr (PROFILED) {
+          float2 a, bz, da, dbz;
+          profile_pair(p, atom % p.locations, theta, a, bz, da, dbz);
+          float sp, cp;
+          sincosf(-(ev.phase + b1_phase), &sp, &cp);
+          const float2 spin = mk(cp, sp);
+          const float2 b = mul(bz, spin), db = mul(dbz, spin);
+          // rows (F+, F-, Z) of the spinor's T-matrix (reference rotate_spinor)
+          const float2 ca = cj(a), cb = cj(b);
+          const float2 t00 = mul(ca, ca), t01 = neg(mul(cb, cb)), t02 = scl(cj(mul(a, b)), -2.0f);
+          const float2 t10 = neg(mul(b, b)), t11 = mul(a, a), t12 = scl(mul(a, b), -2.0f);
+          const float2 t20 = mul(ca, b), t21 = mul(a, cb);
+          const float t22 = (a.x * a.x + a.y * a.y) - (b.x * b.x + b.y * b.y);
+          const float2 fp = v.fp, fm = v.fm, z = v.z;
+#pragma unroll
+          for (int k = 0; k < NT; ++k) {
+            const float2 dp = t[k].fp, dm = t[k].fm, dz = t[k].z;
+            float2 np = add(add(mul(t00, dp), mul(t01, dm)), mul(t02, dz));
+            float2 nm = add(add(mul(t10, dp), mul(t11, dm)), mul(t12, dz));
+            float2 nz = add(add(mul(t20, dp), mul(t21, dm)), scl(dz, t22));
+            if ((TM & CB1) != 0 && k == SB) {
+              // d/dB1 = flip * d/dtheta of every entry, by the product rule
+              const float2 cda = cj(da), cdb = cj(db);
+              const float2 u00 = scl(mul(ca, cda), 2.0f), u01 = scl(mul(cb, cdb), -2.0f);
+              const float2 ab = add(mul(da, b), mul(a, db));
+              const float2 u02 = scl(cj(ab), -2.0f);
+              const float2 u10 = scl(mul(b, db), -2.0f), u11 = scl(mul(a, da), 2.0f), u12 = scl(ab, -2.0f);
+              const float2 u20 = add(mul(cda, b), mul(ca, db)), u21 = add(mul(da, cb), mul(a, cdb));
+              const float u22 = 2.0f * (a.x * da.x + a.y * da.y) - 2.0f * (b.x * db.x + b.y * db.y);
+              np = add(np, scl(add(add(mul(u00, fp), mul(u01, fm)), mul(u02, z)), ev.flip));
+              nm = add(nm, scl(add(add(mul(u10, fp), mul(u11, fm)), mul(u12, z)), ev.flip));
+              nz = add(nz, scl(add(add(mul(u20, fp), mul(u21, fm)), scl(z, u22)), ev.flip));
+            }
+            t[k].fp = np;
+            t[k].fm = nm;
+            t[k].z = nz;
+          }
+          v.fp = add(add(mul(t00, fp), mul(t01, fm)), mul(t02, z));
+          v.fm = add(add(mul(t10, fp), mul(t11, fm)), mul(t12, z));
+          v.z = add(add(mul(t20, fp), mul(t21, fm)), scl(z, t22));
+        } else {
         float sa, va;
         if (p.transmit) {
           const float h = sinf(0.5f * theta);
@@ -292,6 +375,7 @@ __global__ void __launch_bounds__(256) semisolid_kernel(const SemisolidArgs p) {
         v.fp = mac(mac(scl(fp, chh), t01, fm), t02, z);
         v.fm = mac(mac(mul(t10, fp), mk(chh, 0.0f), fm), t12, z);
         v.z = mac(mac(mul(t20, fp), t21, fm), mk(ca, 0.0f), z);
+        }
       }
     } else if (kind == KIND_ADC && (act & RECORD) && ev.out >= 0 && ev.out < p.outputs && first && live) {
       float sp, cp;
@@ -333,26 +417,31 @@ __global__ void __launch_bounds__(256) semisolid_kernel(const SemisolidArgs p) {
 
 using SemisolidLaunch = void (*)(const SemisolidArgs&, dim3, dim3, hipStream_t);
 
-template <int L, int TM>
+template <int L, int TM, bool PROFILED>
 void launch_semisolid(const SemisolidArgs& a, dim3 grid, dim3 block, hipStream_t s) {
-  hipLaunchKernelGGL((semisolid_kernel<L, TM>), grid, block, 0, s, a);
+  hipLaunchKernelGGL((semisolid_kernel<L, TM, PROFILED>), grid, block, 0, s, a);
 }
 
-template <int L>
-SemisolidLaunch pick_semisolid_mask(int tm) {
+template <int L, bool PROFILED>
+SemisolidLaunch pick_semisolid_mode(int tm) {
   switch (tm & 7) {
-    case 0: return &launch_semisolid<L, 0>;
-    case 1: return &launch_semisolid<L, 1>;
-    case 2: return &launch_semisolid<L, 2>;
-    case 3: return &launch_semisolid<L, 3>;
-    case 4: return &launch_semisolid<L, 4>;
-    case 5: return &launch_semisolid<L, 5>;
-    case 6: return &launch_semisolid<L, 6>;
-    default: return &launch_semisolid<L, 7>;
+    case 0: return &launch_semisolid<L, 0, PROFILED>;
+    case 1: return &launch_semisolid<L, 1, PROFILED>;
+    case 2: return &launch_semisolid<L, 2, PROFILED>;
+    case 3: return &launch_semisolid<L, 3, PROFILED>;
+    case 4: return &launch_semisolid<L, 4, PROFILED>;
+    case 5: return &launch_semisolid<L, 5, PROFILED>;
+    case 6: return &launch_semisolid<L, 6, PROFILED>;
+    default: return &launch_semisolid<L, 7, PROFILED>;
   }
 }
 
-#define TSIM_DECLARE_SEMISOLID(LANES) SemisolidLaunch pick_semisolid_##LANES(int tm);
+template <int L>
+SemisolidLaunch pick_semisolid_mask(int tm, bool profiled) {
+  return profiled ? pick_semisolid_mode<L, true>(tm) : pick_semisolid_mode<L, false>(tm);
+}
+
+#define TSIM_DECLARE_SEMISOLID(LANES) SemisolidLaunch pick_semisolid_##LANES(int tm, bool profiled);
 TSIM_COMPLEX_LANES(TSIM_DECLARE_SEMISOLID)
 #undef TSIM_DECLARE_SEMISOLID
 

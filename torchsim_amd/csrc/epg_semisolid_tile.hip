@@ -1,5 +1,5 @@
-// One lane count's instantiations of the semisolid-pool kernel (every tangent
-// mask). Built once per lane count with TSIM_SEMISOLID_L defined.
+// One lane count's instantiations of the semisolid-pool / profiled-pulse
+// kernel (every tangent mask, instantaneous or profiled pulses). Built once per lane count with TSIM_SEMISOLID_L defined.
 #include "epg_semisolid_impl.h"
 
 #ifndef TSIM_SEMISOLID_L
@@ -11,6 +11,8 @@
 
 namespace tsim {
 
-SemisolidLaunch TSIM_CAT(pick_semisolid_, TSIM_SEMISOLID_L)(int tm) { return pick_semisolid_mask<TSIM_SEMISOLID_L>(tm); }
+SemisolidLaunch TSIM_CAT(pick_semisolid_, TSIM_SEMISOLID_L)(int tm, bool profiled) {
+  return pick_semisolid_mask<TSIM_SEMISOLID_L>(tm, profiled);
+}
 
 }  // namespace tsim

@@ -30,7 +30,7 @@ import torch
 
 from .. import _native
 from ..sequence import _dephasing, _engine
-from ..sequence._events import EventOperators, PackedEvents, compose, pack
+from ..sequence._events import INVERSION, EventOperators, PackedEvents, compose, pack
 
 #: Public property names a call may give beside a model's own (reference
 #: sequence/_parameters.py PUBLIC_PROPERTIES, the subset this library carries).
@@ -74,6 +74,9 @@ SAMPLE_FIELDS = ("b0_hz", "t2_prime_ms")
 RUN_SETTINGS = (
     "states", "repetitions", "device", "tile", "checkpoint_pulses", "checkpoint_events", "crusher_dephasing_rad",
     "voxel_size_m",
+    # a shaped pulse the layout's pulses play, and where across the slice to
+    # work it out (reference model/_state_machine.py:399, 500-513, 552-553)
+    "pulse", "across_slice",
 )
 
 #: The apparent diffusion coefficient is given in um^2/ms; the b-factor wants m^2/s.
@@ -119,6 +122,10 @@ def _protocol_key(kind: type, sequence: Mapping[str, Any]) -> tuple | None:
             parts.append((name, str(value.dtype), value.shape, np.ascontiguousarray(value).tobytes()))
         elif isinstance(value, (int, float, complex, str, bool, type(None))):
             parts.append((name, type(value).__name__, value))
+        elif hasattr(value, "key") and hasattr(value, "envelope"):
+            parts.append((name, "pulse", value.key))  # a shaped pulse, by its samples
+        elif dataclasses.is_dataclass(value) and getattr(type(value), "__hash__", None) is not None:
+            parts.append((name, type(value).__name__, repr(value)))
         else:
             return None
     return tuple(parts)
@@ -259,7 +266,19 @@ class Simulator:
         events, end_s = compose(*self.layout(**played))
         repetitions = int(sequence.get("repetitions", self.repetitions) or 1)
         tr_us = self.repetition_us(1e6 * end_s, **played) if repetitions > 1 else 0.0
-        return pack(events, repetitions=repetitions, tr_duration_us=tr_us)
+        packed = pack(events, repetitions=repetitions, tr_duration_us=tr_us)
+        pulse = sequence.get("pulse")
+        if pulse is not None:
+            # a shaped pulse deposits its own power in a semisolid pool, and
+            # its area's phase turns every pulse it plays (reference
+            # sequence/_accelerators.py:1016-1025, 1113-1125)
+            rf = packed.kind == 1
+            turned = rf & ((packed.action & INVERSION) == 0)
+            packed.saturation = torch.where(rf, torch.full_like(packed.saturation, pulse.saturation()),
+                                            packed.saturation)
+            if pulse.integral_phase() != 0.0:
+                packed.phase = packed.phase + turned.to(packed.phase.dtype) * pulse.integral_phase()
+        return packed
 
     # -- the two derivative modes -----------------------------------------------
 
@@ -375,11 +394,34 @@ class Simulator:
                    else jacobian[:, kernel_columns.index(name)] * factor for name in names]
         return signal, torch.stack(columns, dim=1)
 
+    @staticmethod
+    def _profile(sequence: Mapping[str, Any]) -> Any:
+        """The transition table of the run's shaped pulse, or None where its
+        pulses turn as flips and phases name: a pulse that is slice-selective
+        or moves its phase asks for one by itself (at the slice centre unless
+        ``across_slice`` says where), and naming positions asks for one for
+        any pulse (reference sequence/_accelerators.py:292-318)."""
+        from ..sequence import _transition
+
+        pulse = sequence.get("pulse")
+        asked = _transition.across_the_slice(sequence.get("across_slice"))
+        if pulse is None:
+            if asked is not None:
+                raise ValueError("across_slice says where to work out a shaped pulse; give the pulse as pulse=")
+            return None
+        if asked is None and not pulse.selective():
+            return None
+        profile = asked or _transition.exact_slice_profile(1)
+        return _transition.transition_table(pulse, profile.positions(), bins=profile.bins,
+                                            theta_max=profile.theta_max)
+
     def _run_states(self, prepared: tuple, held: Mapping[str, Any], sequence: Mapping[str, Any],
                     diff: tuple[str, ...], subspace: Any) -> tuple[torch.Tensor, torch.Tensor | None]:
         """The kernels' part of :meth:`_run`, on its prepared inputs."""
         device, packed, fields, tissue, states, differentiated, reversed_fields, tangent_fields, _ = prepared
+        profile = self._profile(sequence)
         real, flip, samples = _engine.real_subspace(packed, tissue, device)
+        real = real and profile is None
         factor = self.signal_factor if self.signal_factor is not None else 1.0
         sequence_live = [_engine.carries_derivative(v) for v in (packed.flip, packed.phase, packed.duration)]
         reverse = bool(reversed_fields) or any(v.requires_grad for v in (packed.flip, packed.phase, packed.duration))
@@ -396,6 +438,16 @@ class Simulator:
             return subspace.project(signal), (None if jacobian is None else subspace.project(jacobian))
         from ..sequence import _complex
 
+        if profile is not None:
+            if reverse or forward_mode:
+                raise NotImplementedError(
+                    "slice-profiled pulses run forward with their Jacobian columns (T1, T2, M0, B1); reverse and "
+                    "forward-mode derivatives through them are not built"
+                )
+            from ..sequence import _semisolid
+
+            return _semisolid.simulate_profiled(packed, tissue, profile, states=states, columns=differentiated,
+                                                factor=factor)
         if tissue.semisolid:
             if reverse or forward_mode:
                 raise NotImplementedError(

@@ -1,14 +1,18 @@
-"""Launch of the semisolid-pool (magnetization-transfer) kernel.
+"""Launch of the semisolid-pool (magnetization-transfer) and slice-profiled
+pulse kernel.
 
-Reference: the SEMISOLID pool kind of the complex state machine
-(src/torchsim/_epg_cpu.cpp:4216-4566 with pool kind 1) as
-``simulate_native`` reaches it with an absorption table sized by
-``_absorption_table`` (sequence/_simulation.py:80-94, 466-470).
+Reference: the SEMISOLID pool kind and the PROFILED rotation of the complex
+state machine (src/torchsim/_epg_cpu.cpp:4216-4566 with pool kind 1,
+2041-2049, 2107-2160, 2273-2304) as ``simulate_native`` reaches them: an
+absorption table sized by ``_absorption_table`` (sequence/_simulation.py:
+80-94), every voxel copied once per slice position and its signal averaged
+over them (sequence/_accelerators.py:144-158, 730-734).
 """
 
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import threading
 
 import torch
@@ -17,6 +21,7 @@ from .. import _native
 from ._engine import PreparedTissue
 from ._events import PackedEvents
 from ._lineshape import Lineshape, lineshape_reaching
+from ._transition import TransitionTable
 
 _COLUMN_INDEX = {"t1_ms": 0, "t2_ms": 1, "m0": 2, "b1": 3}
 _TABLES: dict = {}
@@ -77,4 +82,67 @@ def simulate_semisolid(
             _native.stream_of(device),
         )
     _native.check(status)
+    return signal, jacobian
+
+
+def _across(tissue: PreparedTissue, locations: int) -> PreparedTissue:
+    """Each atom copied once per slice position, voxel-major (buffers holding
+    one value for every atom stay single)."""
+    if locations == 1:
+        return tissue
+    buffers = [None if b is None else (b.repeat_interleave(locations) if b.numel() == tissue.atoms else b)
+               for b in tissue.buffers]
+    return dataclasses.replace(tissue, buffers=buffers, atoms=tissue.atoms * locations,
+                               shape=torch.Size((tissue.atoms * locations,)))
+
+
+def simulate_profiled(
+    packed: PackedEvents,
+    tissue: PreparedTissue,
+    table: TransitionTable,
+    *,
+    states: int,
+    columns: tuple[str, ...] = (),
+    factor: complex = 1.0,
+) -> tuple[torch.Tensor, torch.Tensor | None]:
+    """Signal (atoms, outputs) and Jacobian (atoms, P, outputs), complex64,
+    with every pulse but an inversion played as the shaped pulse tabulated in
+    ``table``: each atom simulated at every tabulated slice position and
+    averaged (a semisolid pool runs too where the tissue has one)."""
+    device = tissue.device
+    _native.require_device(device)
+    for name in columns:
+        if name not in _COLUMN_INDEX:
+            raise NotImplementedError(f"the profiled kernel carries no tangent along {name!r}")
+    if tissue.exchanging:
+        raise NotImplementedError("an exchanging pool is not built beside a profiled pulse")
+    lib = _native.load()
+    locations = table.points
+    spread = _across(tissue, locations)
+    knots = table.packed(device)
+    lineshape = absorption(packed, tissue) if tissue.semisolid else None
+    shape_knots = _resident(lineshape, device) if lineshape is not None else None
+    atoms, outputs = spread.atoms, packed.output_count
+    signal = torch.zeros((atoms, outputs), dtype=torch.complex64, device=device)
+    jacobian = torch.zeros((atoms, len(columns), outputs), dtype=torch.complex64, device=device) if columns else None
+    if atoms == 0 or outputs == 0:
+        voxels = tissue.atoms
+        return (signal.reshape(voxels, locations, outputs).mean(1),
+                None if jacobian is None else jacobian.reshape(voxels, locations, len(columns), outputs).mean(1))
+    events = [b.detach() for b in packed.on(device)]
+    ptrs = _native.pointers([*(None if b is None else b.detach() for b in spread.buffers), *events])
+    cols = (ctypes.c_int32 * max(1, len(columns)))(*[_COLUMN_INDEX[name] for name in columns])
+    with torch.cuda.device(device):
+        status = lib.tsim_profiled_jacobian(
+            ptrs, atoms, packed.event_count, states, outputs, spread.features, spread.stride, cols, len(columns),
+            float(complex(factor).real), float(complex(factor).imag), knots.data_ptr(), locations, table.bins,
+            float(table.step), None if shape_knots is None else shape_knots.data_ptr(),
+            0 if lineshape is None else lineshape.bins, 1.0 if lineshape is None else float(lineshape.step),
+            signal.data_ptr(), None if jacobian is None else jacobian.data_ptr(), _native.stream_of(device),
+        )
+    _native.check(status)
+    voxels = tissue.atoms
+    signal = signal.reshape(voxels, locations, outputs).mean(1)
+    if jacobian is not None:
+        jacobian = jacobian.reshape(voxels, locations, len(columns), outputs).mean(1)
     return signal, jacobian

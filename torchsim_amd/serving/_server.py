@@ -113,7 +113,7 @@ class _SimulateProgram:
         _, packed, _, tissue, states, differentiated, reversed_fields, tangent_fields, dephasing = sim._prepared(
             held, sequence, self.names, device=device)
         if (reversed_fields or tangent_fields or dephasing is not None or tissue.semisolid
-                or packed.output_count == 0 or tissue.atoms == 0):
+                or sequence.get("pulse") is not None or packed.output_count == 0 or tissue.atoms == 0):
             return
         real, flip, samples = _engine.real_subspace(packed, tissue, device)
         if not real and any(name not in COMPLEX_COLUMNS for name in differentiated):
