@@ -139,12 +139,117 @@ def test_a_zero_bound_fraction_is_the_single_pool_run_bit_for_bit():
     assert torch.equal(plain, gated)
 
 
-def test_derivative_modes_through_the_pool_are_refused():
+def _oracle_run(sim, maps, lineshape, profile=None):
+    """FP64 signal of the oracle as a function of (T1, T2, M0, B1) per voxel
+    (slice-averaged where a transition table is given; B0 left out, as the
+    uniform winding leaves it a factor of one at the samples)."""
+    packed = sim.packed(**sim.bound)
+    events = oracle.events_of(packed)
+    names = ("T1", "T2", "M0", "B1")
+    voxels = len(maps["T1"])
+    positions = 1 if profile is None else profile[0].shape[0]
+    base = {_ORACLE[k]: torch.tensor(np.repeat(v, positions), dtype=torch.float64) for k, v in maps.items()
+            if k != "B0"}
+    factor = complex(sim.signal_factor) if sim.signal_factor is not None else 1.0
+
+    def run(*values):
+        given = dict(base)
+        given.update({_ORACLE[k]: v.repeat_interleave(positions) for k, v in zip(names, values)})
+        rows = P.simulate(events, given, states=10, outputs=packed.output_count, lineshape=lineshape,
+                          profile=profile)
+        return factor * rows.reshape(voxels, positions, -1).mean(1)
+
+    return run, tuple(torch.tensor(maps[k], dtype=torch.float64) for k in names)
+
+
+@pytest.mark.parametrize("name", ["mrf", "fse_profiled"])
+def test_gradients_reach_the_tissue_maps(name):
+    """loss.backward() through the semisolid kernel (and a profiled pulse on
+    top of the pool) reaches T1, T2, M0 and B1: against FP64 autograd of the
+    oracle, each map's gradient within 1e-4 of its largest entry."""
+    from torchsim_amd.sequence._lineshape import lineshape_table
+    from torchsim_amd.sequence._transition import exact_slice_profile, sinc_pulse, transition_table
+
+    sim = _simulator(name.split("_")[0])
+    profile = None
+    if name.endswith("profiled"):
+        pulse, where = sinc_pulse(2e-3, 4.0, raster_s=1e-5), exact_slice_profile(5, bins=32)
+        sim = sim.bind(pulse=pulse, across_slice=where)
+        table = transition_table(pulse, where.positions(), bins=where.bins, theta_max=where.theta_max)
+        profile = (table.a.numpy(), table.b.numpy(), table.slope_a.numpy(), table.slope_b.numpy(), table.theta_max)
+    atoms = 24
+    maps = _maps(atoms, 43)
+    ls = lineshape_table()
+    run, primal = _oracle_run(sim, maps, (ls.values, ls.slopes, ls.step), profile)
+    names = ("T1", "T2", "M0", "B1")
+    leaves = {k: torch.tensor(maps[k]).cuda().requires_grad_(True) for k in names}
+    rest = {k: torch.tensor(v).cuda() for k, v in maps.items() if k not in names}
+    signal = sim.simulate(**leaves, **rest)
+    rng = np.random.default_rng(5)
+    target = rng.standard_normal(tuple(signal.shape)) + 1j * rng.standard_normal(tuple(signal.shape))
+    loss = (signal - torch.tensor(target, dtype=torch.complex64).cuda()).abs().pow(2).sum()
+    loss.backward()
+    leaves64 = tuple(p.clone().requires_grad_(True) for p in primal)
+    expect = run(*leaves64)
+    (expect - torch.tensor(target)).abs().pow(2).sum().backward()
+    for k, leaf in zip(names, leaves64):
+        got = leaves[k].grad.cpu().double().numpy()
+        want = leaf.grad.numpy()
+        err = np.abs(got - want).max() / np.abs(want).max()
+        print(f"{name} dloss/d{k}: {err:.2e}")
+        assert err < JTOL, k
+
+
+def test_forward_mode_along_a_mixed_tissue_direction():
+    """torch.func.jvp and dual tensors along (T1, B1) together through the
+    semisolid kernel, against the FP64 oracle's jvp."""
+    import torch.autograd.forward_ad as fwAD
+    from torchsim_amd.sequence._lineshape import lineshape_table
+
     sim = _simulator("mrf")
+    atoms = 20
+    maps = _maps(atoms, 47)
+    ls = lineshape_table()
+    run, primal = _oracle_run(sim, maps, (ls.values, ls.slopes, ls.step))
+    rng = np.random.default_rng(9)
+    d_t1 = rng.uniform(-50.0, 50.0, atoms).astype(np.float32)
+    d_b1 = rng.uniform(-0.1, 0.1, atoms).astype(np.float32)
+    zero = torch.zeros(atoms, dtype=torch.float64)
+    want = torch.func.jvp(run, primal, (torch.tensor(d_t1, dtype=torch.float64), zero, zero,
+                                        torch.tensor(d_b1, dtype=torch.float64)))[1].numpy()
+    rest = {k: torch.tensor(v).cuda() for k, v in maps.items() if k not in ("T1", "B1")}
+
+    def simulate(t1, b1):
+        return sim.simulate(T1=t1, B1=b1, **rest)
+
+    t1, b1 = torch.tensor(maps["T1"]).cuda(), torch.tensor(maps["B1"]).cuda()
+    _, got = torch.func.jvp(simulate, (t1, b1), (torch.tensor(d_t1).cuda(), torch.tensor(d_b1).cuda()))
+    err = np.abs(got.cpu().numpy() - want).max() / np.abs(want).max()
+    print(f"torch.func.jvp: {err:.2e}")
+    assert err < JTOL
+    with fwAD.dual_level():
+        dual = simulate(fwAD.make_dual(t1, torch.tensor(d_t1).cuda()), fwAD.make_dual(b1, torch.tensor(d_b1).cuda()))
+        tangent = fwAD.unpack_dual(dual).tangent
+    err = np.abs(tangent.cpu().numpy() - want).max() / np.abs(want).max()
+    print(f"dual tensors: {err:.2e}")
+    assert err < JTOL
+
+
+def test_derivatives_the_pool_does_not_carry_are_refused():
+    """The sequence's flips, and the bound pool's own properties, carry no
+    derivative through the semisolid kernel: asking raises."""
+    from torchsim_amd.simulators import MRFSimulator
+
     maps = {k: torch.tensor(v).cuda() for k, v in _maps(8, 31).items()}
-    maps["T1"].requires_grad_(True)
-    with pytest.raises(NotImplementedError, match="semisolid"):
+    flips = torch.tensor(_cases.benchmark_flips(40), dtype=torch.float32, requires_grad=True)
+    sim = MRFSimulator(flip=flips, TR=10.0, states=10)
+    with pytest.raises(NotImplementedError, match="carry none"):
         sim.simulate(**maps)
+    sim = _simulator("mrf")
+    bound = dict(maps)
+    bound["bound_fraction"] = bound["bound_fraction"].clone().requires_grad_(True)
+    with pytest.raises(NotImplementedError, match="bound_fraction"):
+        sim.simulate(**bound)
 
 
 # --- slice-profiled pulses ---

@@ -438,26 +438,24 @@ class Simulator:
             return subspace.project(signal), (None if jacobian is None else subspace.project(jacobian))
         from ..sequence import _complex
 
-        if profile is not None:
-            if reverse or forward_mode:
-                raise NotImplementedError(
-                    "slice-profiled pulses run forward with their Jacobian columns (T1, T2, M0, B1); reverse and "
-                    "forward-mode derivatives through them are not built"
-                )
+        if profile is not None or tissue.semisolid:
             from ..sequence import _semisolid
 
-            return _semisolid.simulate_profiled(packed, tissue, profile, states=states, columns=differentiated,
-                                                factor=factor)
-        if tissue.semisolid:
-            if reverse or forward_mode:
+            if any(sequence_live):
                 raise NotImplementedError(
-                    "a tissue with a semisolid pool runs forward with its Jacobian columns (T1, T2, M0, B1); "
-                    "reverse and forward-mode derivatives through it are not built"
+                    "through a semisolid pool or a profiled pulse derivatives reach the tissue maps "
+                    f"({', '.join(_semisolid.CARRIED)}); the sequence's flips, phases and durations carry none"
                 )
-            from ..sequence import _semisolid
-
-            return _semisolid.simulate_semisolid(packed, tissue, states=states, columns=differentiated,
-                                                 factor=factor)
+            carried = tuple(dict.fromkeys(reversed_fields + tangent_fields))
+            if carried:
+                if differentiated:
+                    raise NotImplementedError(
+                        "a Jacobian through a semisolid pool or a profiled pulse is not differentiated again"
+                    )
+                return _semisolid.carried_signal(packed, tissue, profile, states=states, carried=carried,
+                                                 factor=factor), None
+            return _semisolid.simulate_shaped(packed, tissue, profile, states=states, columns=differentiated,
+                                              factor=factor)
         if not differentiated and (reverse or forward_mode):
             # the plain signal under reverse or forward mode: one Function whose
             # backward takes the adjoint that carries what requires grad and
@@ -757,13 +755,30 @@ class _BridgeTangent(torch.autograd.Function):
                 raise NotImplementedError("a tangent reached a tissue property the simulation leaves out")
         factor = simulator.signal_factor
         plain = dataclasses.replace(tissue, buffers=[None if b is None else b.detach() for b in tissue.buffers])
-        both_out = _complex.jvp_complex(
-            packed, plain, states=states, factor=factor,
-            tissue_tangents={name: value for name, value in zip(_complex.GRADIENT_FIELDS, reached) if value is not None},
-            flip=packed.flip.detach(), phase=packed.phase.detach(), duration=packed.duration.detach(),
-            duration_tangent=moved[0], flip_tangent=moved[1], phase_tangent=moved[2],
-            with_signal=sample_factor is not None,
-        )
+        tissue_tangents = {name: value for name, value in zip(_complex.GRADIENT_FIELDS, reached) if value is not None}
+        dropped = [name for name, b in zip(_native.TISSUE_NAMES, tissue.buffers)
+                   if b is not None and b.requires_grad and name not in _complex.GRADIENT_FIELDS]
+        if dropped:
+            raise NotImplementedError(f"forward mode carries no tangent along {dropped}")
+        profile = simulator._profile(sequence)
+        if profile is not None or tissue.semisolid:
+            from ..sequence import _semisolid
+
+            if any(value is not None for value in moved[:3]):
+                raise NotImplementedError(
+                    "through a semisolid pool or a profiled pulse derivatives reach the tissue maps "
+                    f"({', '.join(_semisolid.CARRIED)}); the sequence's flips, phases and durations carry none"
+                )
+            signal, tangent_states = _semisolid.tangent_of(packed, plain, profile, states=states,
+                                                           tissue_tangents=tissue_tangents, factor=factor)
+            both_out = tangent_states if sample_factor is None else (signal, tangent_states)
+        else:
+            both_out = _complex.jvp_complex(
+                packed, plain, states=states, factor=factor, tissue_tangents=tissue_tangents,
+                flip=packed.flip.detach(), phase=packed.phase.detach(), duration=packed.duration.detach(),
+                duration_tangent=moved[0], flip_tangent=moved[1], phase_tangent=moved[2],
+                with_signal=sample_factor is not None,
+            )
         if sample_factor is None:
             tangent = both_out
         else:

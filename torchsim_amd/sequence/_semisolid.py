@@ -24,6 +24,8 @@ from ._lineshape import Lineshape, lineshape_reaching
 from ._transition import TransitionTable
 
 _COLUMN_INDEX = {"t1_ms": 0, "t2_ms": 1, "m0": 2, "b1": 3}
+#: The tissue maps derivatives reach through these kernels (their columns).
+CARRIED = tuple(_COLUMN_INDEX)
 _TABLES: dict = {}
 _TABLES_LOCK = threading.Lock()
 
@@ -146,3 +148,121 @@ def simulate_profiled(
     if jacobian is not None:
         jacobian = jacobian.reshape(voxels, locations, len(columns), outputs).mean(1)
     return signal, jacobian
+
+
+def simulate_shaped(
+    packed: PackedEvents,
+    tissue: PreparedTissue,
+    table: TransitionTable | None,
+    *,
+    states: int,
+    columns: tuple[str, ...] = (),
+    factor: complex = 1.0,
+) -> tuple[torch.Tensor, torch.Tensor | None]:
+    """:func:`simulate_profiled` where ``table`` names a shaped pulse, else
+    :func:`simulate_semisolid`."""
+    if table is not None:
+        return simulate_profiled(packed, tissue, table, states=states, columns=columns, factor=factor)
+    return simulate_semisolid(packed, tissue, states=states, columns=columns, factor=factor)
+
+
+def refuse_uncarried(names) -> None:
+    """Derivatives through these kernels travel along the tissue maps they
+    write columns for.
+
+    Raises
+    ------
+    NotImplementedError
+        If ``names`` holds anything else.
+    """
+    others = [name for name in names if name not in CARRIED]
+    if others:
+        raise NotImplementedError(
+            f"through a semisolid pool or a profiled pulse derivatives reach {', '.join(CARRIED)}; "
+            f"{others} also carry one"
+        )
+
+
+class ColumnSignal(torch.autograd.Function):
+    """The signal of a semisolid-pool or profiled-pulse run as a function of
+    the tissue maps it was differentiated along, through the Jacobian columns
+    the kernel wrote beside it: ``backward`` contracts the cotangent with the
+    columns (grad_k = Re sum_n g_n conj(J_kn)) and ``jvp`` sums the columns
+    times the tangents (the reference's ``_NativeEpg.backward`` / ``.jvp``,
+    sequence/_accelerators.py:1511-1586, for these pools' tissue inputs). The
+    signal and columns come in computed; the buffers are the maps' prepared
+    per-atom values, in ``names`` order."""
+
+    @staticmethod
+    def forward(signal, jacobian, names, *buffers):
+        return signal.clone()
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        _, jacobian, names, *buffers = inputs
+        ctx.save_for_backward(jacobian)
+        ctx.save_for_forward(jacobian)
+        ctx.shapes = [None if b is None else b.shape for b in buffers]
+
+    @staticmethod
+    def backward(ctx, grad):
+        (jacobian,) = ctx.saved_tensors
+        grads = []
+        for k, (shape, need) in enumerate(zip(ctx.shapes, ctx.needs_input_grad[3:])):
+            if not need or shape is None:
+                grads.append(None)
+                continue
+            g = (grad * jacobian[:, k].conj()).real.sum(-1)
+            # a buffer holding one value for every atom gathers every atom's term
+            grads.append(g.reshape(shape) if g.numel() == shape.numel() else g.sum().reshape(shape))
+        return (None, None, None, *grads)
+
+    @staticmethod
+    def jvp(ctx, _d_signal, _d_jacobian, _names, *tangents):
+        (jacobian,) = ctx.saved_tensors
+        total = None
+        for k, tangent in enumerate(tangents):
+            if tangent is None:
+                continue
+            term = jacobian[:, k] * tangent.reshape(-1, 1).to(jacobian.real.dtype)
+            total = term if total is None else total + term
+        return torch.zeros_like(jacobian[:, 0]) if total is None else total
+
+
+def carried_signal(
+    packed: PackedEvents,
+    tissue: PreparedTissue,
+    table: TransitionTable | None,
+    *,
+    states: int,
+    carried: tuple[str, ...],
+    factor: complex = 1.0,
+) -> torch.Tensor:
+    """The signal with reverse- and forward-mode derivatives along the tissue
+    maps ``carried`` names (among :data:`CARRIED`), through their columns."""
+    refuse_uncarried(carried)
+    plain = dataclasses.replace(tissue, buffers=[None if b is None else b.detach() for b in tissue.buffers])
+    signal, jacobian = simulate_shaped(packed, plain, table, states=states, columns=carried, factor=factor)
+    buffers = [tissue.buffers[_native.TISSUE_INDEX[name]] for name in carried]
+    return ColumnSignal.apply(signal, jacobian, carried, *buffers)
+
+
+def tangent_of(
+    packed: PackedEvents,
+    tissue: PreparedTissue,
+    table: TransitionTable | None,
+    *,
+    states: int,
+    tissue_tangents: dict,
+    factor: complex = 1.0,
+) -> tuple[torch.Tensor, torch.Tensor]:
+    """Signal and its directional derivative along per-atom tissue tangents
+    (names among :data:`CARRIED`): sum_k J_k t_k."""
+    names = tuple(tissue_tangents)
+    refuse_uncarried(names)
+    plain = dataclasses.replace(tissue, buffers=[None if b is None else b.detach() for b in tissue.buffers])
+    signal, jacobian = simulate_shaped(packed, plain, table, states=states, columns=names, factor=factor)
+    tangent = torch.zeros_like(signal)
+    for k, name in enumerate(names):
+        tangent = tangent + jacobian[:, k] * tissue_tangents[name].detach().reshape(-1, 1).to(torch.float32)
+    return signal, tangent
