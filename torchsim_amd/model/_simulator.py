@@ -744,11 +744,20 @@ class _BridgeTangent(torch.autograd.Function):
             held, sequence = simulator._split({**simulator.bound, **values})
             device, packed, _, tissue, states, _, _, _, dephasing = simulator._prepared(held, sequence)
             carried = [_native.TISSUE_INDEX[name] for name in _complex.GRADIENT_FIELDS]
-            targets = [packed.duration, packed.flip, packed.phase] + [tissue.buffers[i] for i in carried]
+            # the other properties too, to refuse a tangent that would be dropped
+            others = [i for i, name in enumerate(_native.TISSUE_NAMES)
+                      if name not in _complex.GRADIENT_FIELDS and tissue.buffers[i] is not None]
+            targets = ([packed.duration, packed.flip, packed.phase] + [tissue.buffers[i] for i in carried]
+                       + [tissue.buffers[i] for i in others])
             sample_factor = None if dephasing is None else dephasing.factor()
             if sample_factor is not None:
                 targets += [sample_factor.real, sample_factor.imag]
             moved = _directional(targets, leaves, seeds)
+        beside = moved[3 + len(carried):3 + len(carried) + len(others)]
+        dropped = [_native.TISSUE_NAMES[i] for i, value in zip(others, beside)
+                   if value is not None and bool((value != 0).any())]
+        if dropped:
+            raise NotImplementedError(f"forward mode carries no tangent along {dropped}")
         reached = moved[3:3 + len(carried)]
         for index, value in zip(carried, reached):
             if value is not None and tissue.buffers[index] is None:
@@ -756,10 +765,6 @@ class _BridgeTangent(torch.autograd.Function):
         factor = simulator.signal_factor
         plain = dataclasses.replace(tissue, buffers=[None if b is None else b.detach() for b in tissue.buffers])
         tissue_tangents = {name: value for name, value in zip(_complex.GRADIENT_FIELDS, reached) if value is not None}
-        dropped = [name for name, b in zip(_native.TISSUE_NAMES, tissue.buffers)
-                   if b is not None and b.requires_grad and name not in _complex.GRADIENT_FIELDS]
-        if dropped:
-            raise NotImplementedError(f"forward mode carries no tangent along {dropped}")
         profile = simulator._profile(sequence)
         if profile is not None or tissue.semisolid:
             from ..sequence import _semisolid
@@ -769,8 +774,10 @@ class _BridgeTangent(torch.autograd.Function):
                     "through a semisolid pool or a profiled pulse derivatives reach the tissue maps "
                     f"({', '.join(_semisolid.CARRIED)}); the sequence's flips, phases and durations carry none"
                 )
+            # buffers laid out together all carry a tangent slot: keep the live ones
+            live = {name: value for name, value in tissue_tangents.items() if bool((value != 0).any())}
             signal, tangent_states = _semisolid.tangent_of(packed, plain, profile, states=states,
-                                                           tissue_tangents=tissue_tangents, factor=factor)
+                                                           tissue_tangents=live, factor=factor)
             both_out = tangent_states if sample_factor is None else (signal, tangent_states)
         else:
             both_out = _complex.jvp_complex(
