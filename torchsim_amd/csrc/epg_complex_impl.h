@@ -275,6 +275,28 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
   // X and dX carry this order's turn and Y and dY its longitudinal damping
   float2 X[4], dX[4];
   float Y[4], dY[4], rec[2], drec[2];
+  // A clock of float32 durations gives one nominal interval a few values a
+  // rounding apart, so consecutive intervals differ often by ~1e-5 of their
+  // length. The operators over dt = base + delta are those over the base
+  // interval times those over delta (each generator commutes with itself),
+  // the latter to second order in |G delta| <= 1e-3 (remainder < 2e-10);
+  // a larger step forms them afresh and becomes the base. The base's
+  // operators before this order's turn and damping are the atom's: they sit
+  // in LDS, one slot per atom of the wave (and one for its idle lanes), the
+  // order's factors over the base in registers. `reach` bounds the
+  // generators' norm for every order of the atom (exchange, relaxation,
+  // shift, B0, diffusion at the top order), so the atom's lanes step alike.
+  constexpr int SLOTS = 64 / L + 1;
+  constexpr int BASE_WORDS = 24;  // X (8), dX/dT2 (8), Y (4), dY/dT1 (4)
+  __shared__ float4 interval_base[EXCH ? 4 * SLOTS * (BASE_WORDS / 4) : 1];
+  float4* const base_slot = interval_base + ((threadIdx.x >> 6) * SLOTS + wave_group) * (BASE_WORDS / 4);
+  float2 turn_b = mk(1.0f, 0.0f);
+  float dlong_b = 1.0f;
+  float base_dt = 0.0f;
+  bool have_base = false;
+  // recovery (Y - I) G^{-1} s, s = ((1-f) R1a, f R1b): u = G^{-1} s and its
+  // T1 derivative du = G^{-1} (ds - dG u), ds = (-(1-f) k1, 0), dG = diag(k1, 0)
+  float u0 = 0.0f, u1 = 0.0f, du0 = 0.0f, du1 = 0.0f, reach = 0.0f;
   if constexpr (EXCH) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -282,6 +304,17 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
       Y[i] = dY[i] = 0.0f;
     }
     rec[0] = rec[1] = drec[0] = drec[1] = 0.0f;
+    const float g00 = -kab - r1, g01 = kba, g10 = kab, g11 = -kba - r1b;
+    const float det = g00 * g11 - g01 * g10;
+    const float s0 = free_frac * r1, s1 = frac * r1b;
+    u0 = (g11 * s0 - g01 * s1) / det;
+    u1 = (-g10 * s0 + g00 * s1) / det;
+    const float w0 = -free_frac * k1 - k1 * u0;
+    du0 = (g11 * w0) / det;
+    du1 = (-g10 * w0) / det;
+    const float top = (float)(L - 1);
+    reach = kex + r1 + r2 + r1b + r2b + 2.0f * 3.14159265358979323846f * (fabsf(shift_hz) + fabsf(b0)) +
+            (top * top + top + (1.0f / 3.0f)) * drate;
   }
 
   // Output staging. Each wave keeps its atoms' last K samples of every
@@ -332,62 +365,167 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
     if (dt != 0.0f) {
       if (dt != last_dt) {
         last_dt = dt;
-        const float bf = drate * dt;
-        const float dtr = expf(-(order * order + order + (1.0f / 3.0f)) * bf);
-        dlong = expf(-(order * order) * bf);
-        float s, c;
-        sincosf(-2.0f * 3.14159265358979323846f * b0 * dt, &s, &c);
-        turn = mk(c * dtr, s * dtr);
-        if constexpr (!EXCH) {
-          // 1 - e1 from expm1: a rounded e1 would cost it five digits when dt << T1
-          const float e1m = expm1f(-r1 * dt);
-          e1 = 1.0f + e1m;
-          om1 = -e1m;
-          e2 = expf(-r2 * dt);
-          de1 = e1 * (k1 * dt);
-          de2 = e2 * (k2 * dt);
-        } else {
-          // transverse generator K - diag(R2) - 2 pi i diag(0, df), times dt
-          const float2 mt[4] = {mk((-kab - r2) * dt, 0.0f), mk(kba * dt, 0.0f), mk(kab * dt, 0.0f),
-                                mk((-kba - r2b) * dt, -2.0f * 3.14159265358979323846f * shift_hz * dt)};
-          const float2 et[4] = {mk(k2 * dt, 0.0f), mk(0.0f, 0.0f), mk(0.0f, 0.0f), mk(0.0f, 0.0f)};
-          expm2<(TM & CT2) != 0>(mt, et, X, dX);
-          // longitudinal generator K - diag(R1)
-          const float2 ml[4] = {mk((-kab - r1) * dt, 0.0f), mk(kba * dt, 0.0f), mk(kab * dt, 0.0f),
-                                mk((-kba - r1b) * dt, 0.0f)};
-          const float2 el[4] = {mk(k1 * dt, 0.0f), mk(0.0f, 0.0f), mk(0.0f, 0.0f), mk(0.0f, 0.0f)};
-          float2 yl[4], dyl[4];
-          expm2<(TM & CT1) != 0>(ml, el, yl, dyl);
+        bool stepped = false;
+        if constexpr (EXCH) {
+          const float d = dt - base_dt;
+          if (have_base && fabsf(d) * reach <= 1e-3f) {
+            stepped = true;
+            // this order's scalar factors over d: diffusion damping, B0 turn
+            const float xo = -(order * order + order + (1.0f / 3.0f)) * drate * d;
+            const float xl = -(order * order) * drate * d;
+            const float yo = -2.0f * 3.14159265358979323846f * b0 * d;
+            const float eo = 1.0f + xo + 0.5f * (xo * xo);
+            const float2 tau = mul(turn_b, mk(eo * (1.0f - 0.5f * (yo * yo)), eo * yo));
+            const float dl = dlong_b * (1.0f + xl + 0.5f * (xl * xl));
+            float2 Xb[4], dXb[4];
+            float Yb[4], dYb[4];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            Y[i] = yl[i].x;
-            dY[i] = dyl[i].x;
-          }
-          // recovery (Y - I) G^{-1} s, s = ((1-f) R1a, f R1b), G the generator
-          const float g00 = -kab - r1, g01 = kba, g10 = kab, g11 = -kba - r1b;
-          const float det = g00 * g11 - g01 * g10;
-          const float s0 = free_frac * r1, s1 = frac * r1b;
-          const float u0 = (g11 * s0 - g01 * s1) / det;
-          const float u1 = (-g10 * s0 + g00 * s1) / det;
-          rec[0] = (Y[0] - 1.0f) * u0 + Y[1] * u1;
-          rec[1] = Y[2] * u0 + (Y[3] - 1.0f) * u1;
-          if constexpr ((TM & CT1) != 0) {
-            // du = G^{-1} (ds - dG u), ds = (-(1-f) k1, 0), dG = diag(k1, 0)
-            const float w0 = -free_frac * k1 - k1 * u0;
-            const float w1 = 0.0f;
-            const float du0 = (g11 * w0 - g01 * w1) / det;
-            const float du1 = (-g10 * w0 + g00 * w1) / det;
-            drec[0] = dY[0] * u0 + dY[1] * u1 + (Y[0] - 1.0f) * du0 + Y[1] * du1;
-            drec[1] = dY[2] * u0 + dY[3] * u1 + Y[2] * du0 + (Y[3] - 1.0f) * du1;
-          }
-          // every event of this interval applies turn X (F+), its conjugate
-          // (F-) and dlong Y (Z): fold the per-order factors in once here
+            for (int i = 0; i < 2; ++i) {
+              const float4 w = base_slot[i];
+              Xb[2 * i] = mk(w.x, w.y);
+              Xb[2 * i + 1] = mk(w.z, w.w);
+            }
+            {
+              const float4 w = base_slot[4];
+              Yb[0] = w.x;
+              Yb[1] = w.y;
+              Yb[2] = w.z;
+              Yb[3] = w.w;
+            }
+            // transverse: E = I + A + A^2/2, A = G d; dE = dA + (dA A + A dA)/2, dA = diag(k2 d, 0)
+            const float a00 = (-kab - r2) * d, a01 = kba * d, a10 = kab * d;
+            const float2 a11 = mk((-kba - r2b) * d, -2.0f * 3.14159265358979323846f * shift_hz * d);
+            const float2 half_tr = mk(1.0f + 0.5f * (a00 + a11.x), 0.5f * a11.y);
+            const float2 e[4] = {mk(1.0f + a00 + 0.5f * (a00 * a00 + a01 * a10), 0.0f), scl(half_tr, a01),
+                                 scl(half_tr, a10),
+                                 add(mk(1.0f + 0.5f * (a01 * a10), 0.0f), add(a11, scl(mul(a11, a11), 0.5f)))};
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            X[i] = mul(turn, X[i]);
-            dX[i] = mul(turn, dX[i]);
-            Y[i] *= dlong;
-            dY[i] *= dlong;
+            for (int r = 0; r < 2; ++r) {
+              const float2 x0 = Xb[2 * r], x1 = Xb[2 * r + 1];
+              X[2 * r] = mul(tau, add(mul(x0, e[0]), mul(x1, e[2])));
+              X[2 * r + 1] = mul(tau, add(mul(x0, e[1]), mul(x1, e[3])));
+            }
+            if constexpr ((TM & CT2) != 0) {
+#pragma unroll
+              for (int i = 0; i < 2; ++i) {
+                const float4 w = base_slot[2 + i];
+                dXb[2 * i] = mk(w.x, w.y);
+                dXb[2 * i + 1] = mk(w.z, w.w);
+              }
+              const float g = k2 * d;
+              const float de00 = g * (1.0f + a00), de01 = 0.5f * g * a01, de10 = 0.5f * g * a10;
+#pragma unroll
+              for (int r = 0; r < 2; ++r) {
+                const float2 x0 = Xb[2 * r], x1 = Xb[2 * r + 1];
+                const float2 y0 = dXb[2 * r], y1 = dXb[2 * r + 1];
+                const float2 c0 = add(add(mul(y0, e[0]), mul(y1, e[2])), add(scl(x0, de00), scl(x1, de10)));
+                const float2 c1 = add(add(mul(y0, e[1]), mul(y1, e[3])), scl(x0, de01));
+                dX[2 * r] = mul(tau, c0);
+                dX[2 * r + 1] = mul(tau, c1);
+              }
+            }
+            // longitudinal, real: the same with B = G d and dB = diag(k1 d, 0)
+            const float b00 = (-kab - r1) * d, b01 = kba * d, b10 = kab * d, b11 = (-kba - r1b) * d;
+            const float hl = 1.0f + 0.5f * (b00 + b11);
+            const float l00 = 1.0f + b00 + 0.5f * (b00 * b00 + b01 * b10), l01 = b01 * hl, l10 = b10 * hl;
+            const float l11 = 1.0f + b11 + 0.5f * (b11 * b11 + b01 * b10);
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+              const float y0 = Yb[2 * r], y1 = Yb[2 * r + 1];
+              Y[2 * r] = y0 * l00 + y1 * l10;
+              Y[2 * r + 1] = y0 * l01 + y1 * l11;
+            }
+            if constexpr ((TM & CT1) != 0) {
+              const float4 w = base_slot[5];
+              dYb[0] = w.x;
+              dYb[1] = w.y;
+              dYb[2] = w.z;
+              dYb[3] = w.w;
+              const float h = k1 * d;
+              const float m00 = h * (1.0f + b00), m01 = 0.5f * h * b01, m10 = 0.5f * h * b10;
+#pragma unroll
+              for (int r = 0; r < 2; ++r) {
+                const float y0 = Yb[2 * r], y1 = Yb[2 * r + 1];
+                const float z0 = dYb[2 * r], z1 = dYb[2 * r + 1];
+                dY[2 * r] = z0 * l00 + z1 * l10 + y0 * m00 + y1 * m10;
+                dY[2 * r + 1] = z0 * l01 + z1 * l11 + y0 * m01;
+              }
+            }
+            // recovery from the atom's Y, then this order's damping
+            rec[0] = (Y[0] - 1.0f) * u0 + Y[1] * u1;
+            rec[1] = Y[2] * u0 + (Y[3] - 1.0f) * u1;
+            if constexpr ((TM & CT1) != 0) {
+              drec[0] = dY[0] * u0 + dY[1] * u1 + (Y[0] - 1.0f) * du0 + Y[1] * du1;
+              drec[1] = dY[2] * u0 + dY[3] * u1 + Y[2] * du0 + (Y[3] - 1.0f) * du1;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              Y[i] *= dl;
+              dY[i] *= dl;
+            }
+          }
+        }
+        if (!stepped) {
+          const float bf = drate * dt;
+          const float dtr = expf(-(order * order + order + (1.0f / 3.0f)) * bf);
+          dlong = expf(-(order * order) * bf);
+          float s, c;
+          sincosf(-2.0f * 3.14159265358979323846f * b0 * dt, &s, &c);
+          turn = mk(c * dtr, s * dtr);
+          if constexpr (!EXCH) {
+            // 1 - e1 from expm1: a rounded e1 would cost it five digits when dt << T1
+            const float e1m = expm1f(-r1 * dt);
+            e1 = 1.0f + e1m;
+            om1 = -e1m;
+            e2 = expf(-r2 * dt);
+            de1 = e1 * (k1 * dt);
+            de2 = e2 * (k2 * dt);
+          } else {
+            // transverse generator K - diag(R2) - 2 pi i diag(0, df), times dt
+            const float2 mt[4] = {mk((-kab - r2) * dt, 0.0f), mk(kba * dt, 0.0f), mk(kab * dt, 0.0f),
+                                  mk((-kba - r2b) * dt, -2.0f * 3.14159265358979323846f * shift_hz * dt)};
+            const float2 et[4] = {mk(k2 * dt, 0.0f), mk(0.0f, 0.0f), mk(0.0f, 0.0f), mk(0.0f, 0.0f)};
+            expm2<(TM & CT2) != 0>(mt, et, X, dX);
+            // longitudinal generator K - diag(R1)
+            const float2 ml[4] = {mk((-kab - r1) * dt, 0.0f), mk(kba * dt, 0.0f), mk(kab * dt, 0.0f),
+                                  mk((-kba - r1b) * dt, 0.0f)};
+            const float2 el[4] = {mk(k1 * dt, 0.0f), mk(0.0f, 0.0f), mk(0.0f, 0.0f), mk(0.0f, 0.0f)};
+            float2 yl[4], dyl[4];
+            expm2<(TM & CT1) != 0>(ml, el, yl, dyl);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              Y[i] = yl[i].x;
+              dY[i] = dyl[i].x;
+            }
+            rec[0] = (Y[0] - 1.0f) * u0 + Y[1] * u1;
+            rec[1] = Y[2] * u0 + (Y[3] - 1.0f) * u1;
+            if constexpr ((TM & CT1) != 0) {
+              drec[0] = dY[0] * u0 + dY[1] * u1 + (Y[0] - 1.0f) * du0 + Y[1] * du1;
+              drec[1] = dY[2] * u0 + dY[3] * u1 + Y[2] * du0 + (Y[3] - 1.0f) * du1;
+            }
+            // the atom's operators become the base (every lane of the atom
+            // writes the same words to its atom's slot and reads them back)
+            base_slot[0] = make_float4(X[0].x, X[0].y, X[1].x, X[1].y);
+            base_slot[1] = make_float4(X[2].x, X[2].y, X[3].x, X[3].y);
+            if constexpr ((TM & CT2) != 0) {
+              base_slot[2] = make_float4(dX[0].x, dX[0].y, dX[1].x, dX[1].y);
+              base_slot[3] = make_float4(dX[2].x, dX[2].y, dX[3].x, dX[3].y);
+            }
+            base_slot[4] = make_float4(Y[0], Y[1], Y[2], Y[3]);
+            if constexpr ((TM & CT1) != 0) base_slot[5] = make_float4(dY[0], dY[1], dY[2], dY[3]);
+            turn_b = turn;
+            dlong_b = dlong;
+            base_dt = dt;
+            have_base = true;
+            // every event of this interval applies turn X (F+), its conjugate
+            // (F-) and dlong Y (Z): fold the per-order factors in once here
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              X[i] = mul(turn, X[i]);
+              dX[i] = mul(turn, dX[i]);
+              Y[i] *= dlong;
+              dY[i] *= dlong;
+            }
           }
         }
       }
