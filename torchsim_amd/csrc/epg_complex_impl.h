@@ -16,6 +16,8 @@
 // interval changes.
 #pragma once
 
+#include <type_traits>
+
 #include "epg_common.h"
 
 namespace tsim {
@@ -99,6 +101,34 @@ __device__ __forceinline__ float2 csqrt(float2 z) {
 __device__ __forceinline__ float2 cdiv(float2 a, float2 b) {
   const float d = b.x * b.x + b.y * b.y;
   return mk((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+
+// One row of a 2x2 operator on a pool pair, m0 f0 + m1 f1 (CONJ: with the
+// entries conjugated); with REAL the entries' imaginary parts are zero and
+// not read. The real form's fused multiply-adds are written out, so every
+// instantiation rounds alike whatever else it computes.
+template <bool REAL, bool CONJ>
+__device__ __forceinline__ float2 row(float2 m0, float2 f0, float2 m1, float2 f1) {
+  if constexpr (REAL) {
+    return mk(fmaf(m1.x, f1.x, m0.x * f0.x), fmaf(m1.x, f1.y, m0.x * f0.y));
+  } else if constexpr (CONJ) {
+    return add(mul(cj(m0), f0), mul(cj(m1), f1));
+  } else {
+    return add(mul(m0, f0), mul(m1, f1));
+  }
+}
+
+// The RF T-matrix on one pool's (F+, F-, Z) for a pulse whose phase is
+// zero: c = cos^2(a/2) and h = sin^2(a/2) couple F+ and F- as real numbers,
+// s = sin(a) couples them to Z as imaginary ones, k = cos(a) keeps Z. With
+// (c, h, s, k) replaced by their derivatives along a, the derivative.
+// The fused multiply-adds are written out (as in `row`).
+__device__ __forceinline__ void rotate_zero_phase(float2 fp, float2 fm, float2 z, float c, float h, float s,
+                                                  float k, float2& np, float2& nm, float2& nz) {
+  np = mk(fmaf(s, z.y, fmaf(h, fm.x, c * fp.x)), fmaf(-s, z.x, fmaf(h, fm.y, c * fp.y)));
+  nm = mk(fmaf(-s, z.y, fmaf(c, fm.x, h * fp.x)), fmaf(s, z.x, fmaf(c, fm.y, h * fp.y)));
+  const float hs = 0.5f * s;
+  nz = mk(fmaf(k, z.x, hs * (fp.y - fm.y)), fmaf(k, z.y, hs * (fm.x - fp.x)));
 }
 
 // The even functions of delta a 2x2 exponential needs, from delta^2:
@@ -249,6 +279,7 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
   }
   const float free_frac = 1.0f - frac;
   const float kab = kex * frac, kba = kex * free_frac;
+  const bool real_transverse = shift_hz == 0.0f && b0 == 0.0f;
   const float order = (float)sub;
 
   Order<EXCH> v;
@@ -554,21 +585,25 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
         if (sub == 0) v.z[0].x += om1;
       } else {
         // transverse: F+ <- (turn X) F+, F- <- conj(turn X) F-; longitudinal
-        // Z <- (dlong Y) Z + recovery
+        // Z <- (dlong Y) Z + recovery. Without B0 and pool shift the
+        // transverse operators are real (the atom's own property, so an atom
+        // takes the same path alone and beside others).
+        auto relax = [&](auto real_tag) __attribute__((always_inline)) {
+          constexpr bool R = decltype(real_tag)::value;
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
-          float2 a0 = add(mul(X[0], t[k].fp[0]), mul(X[1], t[k].fp[1]));
-          float2 a1 = add(mul(X[2], t[k].fp[0]), mul(X[3], t[k].fp[1]));
-          float2 b0v = add(mul(cj(X[0]), t[k].fm[0]), mul(cj(X[1]), t[k].fm[1]));
-          float2 b1v = add(mul(cj(X[2]), t[k].fm[0]), mul(cj(X[3]), t[k].fm[1]));
+          float2 a0 = row<R, false>(X[0], t[k].fp[0], X[1], t[k].fp[1]);
+          float2 a1 = row<R, false>(X[2], t[k].fp[0], X[3], t[k].fp[1]);
+          float2 b0v = row<R, true>(X[0], t[k].fm[0], X[1], t[k].fm[1]);
+          float2 b1v = row<R, true>(X[2], t[k].fm[0], X[3], t[k].fm[1]);
           float z0 = 0.0f, z1 = 0.0f;
           float2 zz0 = add(scl(t[k].z[0], Y[0]), scl(t[k].z[1], Y[1]));
           float2 zz1 = add(scl(t[k].z[0], Y[2]), scl(t[k].z[1], Y[3]));
           if ((TM & CT2) != 0 && k == S2) {
-            a0 = add(a0, add(mul(dX[0], v.fp[0]), mul(dX[1], v.fp[1])));
-            a1 = add(a1, add(mul(dX[2], v.fp[0]), mul(dX[3], v.fp[1])));
-            b0v = add(b0v, add(mul(cj(dX[0]), v.fm[0]), mul(cj(dX[1]), v.fm[1])));
-            b1v = add(b1v, add(mul(cj(dX[2]), v.fm[0]), mul(cj(dX[3]), v.fm[1])));
+            a0 = add(a0, row<R, false>(dX[0], v.fp[0], dX[1], v.fp[1]));
+            a1 = add(a1, row<R, false>(dX[2], v.fp[0], dX[3], v.fp[1]));
+            b0v = add(b0v, row<R, true>(dX[0], v.fm[0], dX[1], v.fm[1]));
+            b1v = add(b1v, row<R, true>(dX[2], v.fm[0], dX[3], v.fm[1]));
           }
           if ((TM & CT1) != 0 && k == S1) {
             zz0 = add(zz0, add(scl(v.z[0], dY[0]), scl(v.z[1], dY[1])));
@@ -587,10 +622,10 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
           t[k].z[0].x += z0;
           t[k].z[1].x += z1;
         }
-        const float2 a0 = add(mul(X[0], v.fp[0]), mul(X[1], v.fp[1]));
-        const float2 a1 = add(mul(X[2], v.fp[0]), mul(X[3], v.fp[1]));
-        const float2 b0v = add(mul(cj(X[0]), v.fm[0]), mul(cj(X[1]), v.fm[1]));
-        const float2 b1v = add(mul(cj(X[2]), v.fm[0]), mul(cj(X[3]), v.fm[1]));
+        const float2 a0 = row<R, false>(X[0], v.fp[0], X[1], v.fp[1]);
+        const float2 a1 = row<R, false>(X[2], v.fp[0], X[3], v.fp[1]);
+        const float2 b0v = row<R, true>(X[0], v.fm[0], X[1], v.fm[1]);
+        const float2 b1v = row<R, true>(X[2], v.fm[0], X[3], v.fm[1]);
         const float2 zz0 = add(scl(v.z[0], Y[0]), scl(v.z[1], Y[1]));
         const float2 zz1 = add(scl(v.z[0], Y[2]), scl(v.z[1], Y[3]));
         v.fp[0] = a0;
@@ -602,6 +637,12 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
         if (sub == 0) {
           v.z[0].x += rec[0];
           v.z[1].x += rec[1];
+        }
+        };
+        if (real_transverse) {
+          relax(std::true_type{});
+        } else {
+          relax(std::false_type{});
         }
       }
     }
@@ -635,11 +676,37 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
           va = ev.versine;
         }
         const float ca = 1.0f - va;
+        const float shh = 0.5f * va, chh = 1.0f - shh;
+        const float dch = -0.5f * sa, dsh = 0.5f * sa;
+        const float dca = -sa;
+        const float dal = ev.flip;  // d alpha / d B1
+        if (ev.phase + b1_phase == 0.0f) {
+          // zero phase: real F+/F- couplings, imaginary Z couplings
+#pragma unroll
+          for (int q = 0; q < NP; ++q) {
+            const float2 fp = v.fp[q], fm = v.fm[q], z = v.z[q];
+#pragma unroll
+            for (int k = 0; k < NT; ++k) {
+              float2 np, nm, nz;
+              rotate_zero_phase(t[k].fp[q], t[k].fm[q], t[k].z[q], chh, shh, sa, ca, np, nm, nz);
+              if ((TM & CB1) != 0 && k == SB) {
+                float2 ep, em, ez;
+                rotate_zero_phase(fp, fm, z, dch, dsh, ca, dca, ep, em, ez);
+                np = add(np, scl(ep, dal));
+                nm = add(nm, scl(em, dal));
+                nz = add(nz, scl(ez, dal));
+              }
+              t[k].fp[q] = np;
+              t[k].fm[q] = nm;
+              t[k].z[q] = nz;
+            }
+            rotate_zero_phase(fp, fm, z, chh, shh, sa, ca, v.fp[q], v.fm[q], v.z[q]);
+          }
+        } else {
         float sp, cp;
         sincosf(ev.phase + b1_phase, &sp, &cp);
         const float2 e1p = mk(cp, sp);            // e^{i phi}
         const float2 e2p = mul(e1p, e1p);         // e^{2 i phi}
-        const float shh = 0.5f * va, chh = 1.0f - shh;
         // T-matrix (reference rotate, _epg_cpu.cpp:2176-2208)
         const float2 t01 = scl(e2p, shh);
         const float2 t02 = mul(mk(0.0f, -sa), e1p);
@@ -648,15 +715,12 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
         const float2 t20 = mul(mk(0.0f, -0.5f * sa), cj(e1p));
         const float2 t21 = mul(mk(0.0f, 0.5f * sa), e1p);
         // derivatives along the angle, per unit angle
-        const float dch = -0.5f * sa, dsh = 0.5f * sa;
         const float2 d01 = scl(e2p, dsh);
         const float2 d02 = mul(mk(0.0f, -ca), e1p);
         const float2 d10 = cj(d01);
         const float2 d12 = mul(mk(0.0f, ca), cj(e1p));
         const float2 d20 = mul(mk(0.0f, -0.5f * ca), cj(e1p));
         const float2 d21 = mul(mk(0.0f, 0.5f * ca), e1p);
-        const float dca = -sa;
-        const float dal = ev.flip;  // d alpha / d B1
 #pragma unroll
         for (int q = 0; q < NP; ++q) {
           const float2 fp = v.fp[q], fm = v.fm[q], z = v.z[q];
@@ -679,6 +743,7 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
           v.fm[q] = mac(mac(mul(t10, fp), mk(chh, 0.0f), fm), t12, z);
           v.z[q] = mac(mac(mul(t20, fp), t21, fm), mk(ca, 0.0f), z);
         }
+        }
       }
     } else if (kind == KIND_ADC && (act & RECORD) && ev.out >= 0 && ev.out < p.outputs) {
       if (staging && ev.out != nrec) {
@@ -687,8 +752,8 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
       }
       if (staging) {
         if (sub == 0 && live) {
-          float sp, cp;
-          sincosf(-ev.phase, &sp, &cp);
+          float sp = 0.0f, cp = 1.0f;
+          if (ev.phase != 0.0f) sincosf(-ev.phase, &sp, &cp);
           const float2 demod = mk(cp, sp);
           float2 total = v.fp[0];
           if constexpr (EXCH) total = add(total, v.fp[1]);
@@ -712,8 +777,8 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
         ++nrec;
         if ((nrec & (stage_k - 1)) == 0) flush(stage_k);
       } else if (sub == 0 && live) {
-        float sp, cp;
-        sincosf(-ev.phase, &sp, &cp);
+        float sp = 0.0f, cp = 1.0f;
+        if (ev.phase != 0.0f) sincosf(-ev.phase, &sp, &cp);
         const float2 demod = mk(cp, sp);
         const int64_t o = ev.out;
         float2 total = v.fp[0];
@@ -770,8 +835,11 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
   if (staging && (nrec & (stage_k - 1)) != 0) flush((int)(nrec & (stage_k - 1)));
 }
 
+// Three waves per SIMD: the exchanging pool's two paths (complex operators,
+// and real ones where B0 and the pool shift vanish) fit 168 VGPRs.
 template <int L, int TM, bool EXCH>
-__global__ void __launch_bounds__(256) complex_kernel(const ComplexArgs p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+complex_kernel(const ComplexArgs p) {
   constexpr int APW = 64 / L;
   complex_atoms<L, TM, EXCH>(p, (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * APW);
 }
@@ -790,8 +858,8 @@ struct ComplexWork {
 // state counts (padded to the batch's lanes) and sizes share one launch. Every
 // atom runs the arithmetic it would run alone.
 template <int L, int TM>
-__global__ void __launch_bounds__(256) complex_batch_kernel(const ComplexArgs* __restrict__ args,
-                                                           const ComplexWork* __restrict__ work, int64_t blocks) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+complex_batch_kernel(const ComplexArgs* __restrict__ args, const ComplexWork* __restrict__ work, int64_t blocks) {
   constexpr int APW = 64 / L;
   for (int64_t b = blockIdx.x; b < blocks; b += gridDim.x) {
     const ComplexWork w = work[b];
