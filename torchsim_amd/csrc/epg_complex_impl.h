@@ -186,26 +186,27 @@ __device__ void expm2(const float2 (&m)[4], const float2 (&e)[4], float2 (&x)[4]
 
 }  // namespace cx
 
-// Cross-lane moves of one value inside an atom's group of L lanes.
-// DPP row moves need every atom inside one 16-lane row: L a divisor of 16.
+// Cross-lane moves of one value inside an atom's group of L lanes: the
+// value of the lane below (from_lower) or above (from_upper). DPP row moves
+// (row_shr:1 / row_shl:1) where every atom sits inside one 16-lane row (L a
+// divisor of 16), whole-wave DPP moves (wave_shr:1 / wave_shl:1, GFX9) where
+// atoms straddle rows; both give 0 past the ends of the row or wave, lanes
+// whose value the shift discards (an atom's order 0 takes conj(F-), its top
+// order drops, and the wave's last lanes are an atom's top or idle).
 template <int L>
 constexpr bool dpp_lanes() { return L <= 16 && 16 % L == 0; }
 
 template <int L>
 __device__ __forceinline__ float from_lower(float v, int lane) {
-  if constexpr (dpp_lanes<L>()) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, true));
-  } else {
-    return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane + 63) & 63) << 2, __float_as_int(v)));
-  }
+  (void)lane;
+  constexpr int ctrl = dpp_lanes<L>() ? 0x111 : 0x138;
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xF, 0xF, true));
 }
 template <int L>
 __device__ __forceinline__ float from_upper(float v, int lane) {
-  if constexpr (dpp_lanes<L>()) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xF, 0xF, true));
-  } else {
-    return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane + 1) & 63) << 2, __float_as_int(v)));
-  }
+  (void)lane;
+  constexpr int ctrl = dpp_lanes<L>() ? 0x101 : 0x130;
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xF, 0xF, true));
 }
 
 // One pool's transverse pair at this lane's order, and the gradient shift
