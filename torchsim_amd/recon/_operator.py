@@ -110,15 +110,40 @@ class FusedModel:
         cols = (ctypes.c_int32 * self.columns)(*[_COLUMN_INDEX[f] for f in self.fields])
         sign = samples
         factor = complex(self.simulator.signal_factor)
-        with torch.cuda.device(device):
-            status = lib.tsim_gauss_newton_normal(
-                _native.pointers([*tissue.buffers, *events]), tissue.atoms, self.packed.event_count, self.states,
-                self.packed.output_count, tissue.features, tissue.stride, cols, self.columns, factor.real, factor.imag,
-                None if sign is None else sign.data_ptr(), measured.data_ptr(), normal.data_ptr(), -1,
-                _native.stream_of(device),
-            )
-        _native.check(status)
+
+        def launch(held: _engine.PreparedTissue, rows: torch.Tensor, out: torch.Tensor, tile: int) -> None:
+            with torch.cuda.device(device):
+                status = lib.tsim_gauss_newton_normal(
+                    _native.pointers([*held.buffers, *events]), held.atoms, self.packed.event_count, self.states,
+                    self.packed.output_count, held.features, held.stride, cols, self.columns, factor.real,
+                    factor.imag, None if sign is None else sign.data_ptr(), rows.data_ptr(), out.data_ptr(), tile,
+                    _native.stream_of(device),
+                )
+            _native.check(status)
+
+        launch(tissue, measured, normal, self._normal_tile(tissue, measured, stride, launch))
         return normal
+
+    def _normal_tile(self, tissue: Any, measured: torch.Tensor, stride: int, launch: Any) -> int:
+        """The state tile of the fused normal equations, measured once per
+        configuration and size class (every tile sums the same arithmetic in
+        the same order, so the choice changes only the speed). A tile with
+        more lanes per voxel fills more of the chip when voxels are few and
+        carries more idle lanes through the per-voxel FP64 sums when they are
+        many, so launches of 65,536 voxels or more are probed on up to
+        262,144 of theirs and kept apart from smaller ones (probed on up to
+        16,384)."""
+        from ..sequence import _tuning
+
+        large = tissue.atoms >= 4 * _tuning.PROBE_ATOMS
+        probe_tissue = _engine.narrow_tissue(tissue, (16 if large else 1) * _tuning.PROBE_ATOMS)
+        if probe_tissue.atoms == 0:
+            return -1
+        rows = measured[:probe_tissue.atoms]
+        out = torch.empty((probe_tissue.atoms, stride), dtype=torch.float64, device=measured.device)
+        return _tuning.tile_for(measured.device, self.states, tuple(self.fields), tissue.features,
+                                lambda tile: launch(probe_tissue, rows, out, tile),
+                                tag="normal equations, " + ("large" if large else "small"))
 
     def _normal_from_jacobian(self, columns: Sequence[torch.Tensor], data: torch.Tensor,
                               live: torch.Tensor | None) -> torch.Tensor:

@@ -62,9 +62,10 @@ def _save() -> None:
         pass
 
 
-def key(device: torch.device, states: int, columns: tuple[str, ...], features: int) -> str:
+def key(device: torch.device, states: int, columns: tuple[str, ...], features: int, tag: str = "") -> str:
     name = torch.cuda.get_device_name(device)
-    return f"{name}|S={states}|cols={','.join(columns)}|f={features & _FEATURES_THAT_MATTER}"
+    text = f"{name}|S={states}|cols={','.join(columns)}|f={features & _FEATURES_THAT_MATTER}"
+    return f"{text}|{tag}" if tag else text
 
 
 def candidates(states: int) -> list[int]:
@@ -99,12 +100,14 @@ def choose(run: Callable[[int], None], tiles: list[int], *, repeats: int = 2) ->
 
 
 def tile_for(device: torch.device, states: int, columns: tuple[str, ...], features: int,
-             probe: Callable[[int], None]) -> int:
+             probe: Callable[[int], None], tag: str = "") -> int:
     """The cached or freshly measured tile for this configuration; ``probe``
-    runs one launch of the configuration on the probe atoms with a given tile."""
+    runs one launch of the configuration on the probe atoms with a given tile;
+    ``tag`` names what else the choice is kept for (a kernel variant, a size
+    class)."""
     with _lock:
         _load()
-        name = key(device, states, columns, features)
+        name = key(device, states, columns, features, tag)
         if name in _choices:
             return _choices[name]
     tiles = candidates(states)
