@@ -57,6 +57,28 @@ def test_normal_equations_match_fp64_oracle():
         assert _cases.rel_err(normal[:, index[pq]], h[:, pq[0], pq[1]]) < 1e-4, pq
 
 
+def test_normal_equations_are_the_same_bits_on_every_tile():
+    """The fused normal equations' state tile is picked by measurement: every
+    covering tile sums the same arithmetic in the same order."""
+    from torchsim_amd.recon import FusedModel
+    from torchsim_amd.sequence import _tuning
+
+    t1, t2 = _cases.random_tissue(300, seed=27)
+    m0 = np.linspace(0.5, 1.0, 300).astype(np.float32)
+    rng = np.random.default_rng(3)
+    data = torch.tensor((rng.normal(size=(300, 880)) + 1j * rng.normal(size=(300, 880))).astype(np.complex64)).cuda()
+    x = torch.tensor(np.stack([t1, t2, m0], -1), dtype=torch.float64, device="cuda")
+    results = {}
+    for tile in _tuning.candidates(10):
+        op = FusedModel(_mrf(), ("T1", "T2", "M0"))
+        op._normal_tile = lambda *args, chosen=tile: chosen
+        results[tile] = op.normal(x, data, torch.arange(300, device="cuda"))
+    first = next(iter(results.values()))
+    assert len(results) > 2
+    for tile, normal in results.items():
+        assert torch.equal(normal, first), tile
+
+
 _TRIANGLE = [(0, 0), (0, 1), (0, 2), (0, 3), (1, 1), (1, 2), (1, 3), (2, 2), (2, 3), (3, 3)]
 
 
