@@ -312,10 +312,11 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
   // length. The operators over dt = base + delta are those over the base
   // interval times those over delta (each generator commutes with itself),
   // the latter to second order in |G delta| <= 1e-3 (remainder < 2e-10);
-  // a larger step forms them afresh and becomes the base. The base's
-  // operators before this order's turn and damping are the atom's: they sit
-  // in LDS, one slot per atom of the wave (and one for its idle lanes), the
-  // order's factors over the base in registers. `reach` bounds the
+  // a larger step forms them afresh and becomes the base. An exchanging
+  // pool's base operators before this order's turn and damping are the
+  // atom's: they sit in LDS, one slot per atom of the wave (and one for its
+  // idle lanes), the order's factors over the base in registers; a single
+  // pool's base factors are a few scalars in registers. `reach` bounds the
   // generators' norm for every order of the atom (exchange, relaxation,
   // shift, B0, diffusion at the top order), so the atom's lanes step alike.
   constexpr int SLOTS = 64 / L + 1;
@@ -329,6 +330,8 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
   // recovery (Y - I) G^{-1} s, s = ((1-f) R1a, f R1b): u = G^{-1} s and its
   // T1 derivative du = G^{-1} (ds - dG u), ds = (-(1-f) k1, 0), dG = diag(k1, 0)
   float u0 = 0.0f, u1 = 0.0f, du0 = 0.0f, du1 = 0.0f, reach = 0.0f;
+  // single pool: the base interval's factors
+  float e1b = 1.0f, om1b = 0.0f, e2b = 1.0f;
   if constexpr (EXCH) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -347,6 +350,9 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
     const float top = (float)(L - 1);
     reach = kex + r1 + r2 + r1b + r2b + 2.0f * 3.14159265358979323846f * (fabsf(shift_hz) + fabsf(b0)) +
             (top * top + top + (1.0f / 3.0f)) * drate;
+  } else {
+    const float top = (float)(L - 1);
+    reach = r1 + r2 + 2.0f * 3.14159265358979323846f * fabsf(b0) + (top * top + top + (1.0f / 3.0f)) * drate;
   }
 
   // Output staging. Each wave keeps its atoms' last K samples of every
@@ -496,6 +502,25 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
               dY[i] *= dl;
             }
           }
+        } else {
+          const float d = dt - base_dt;
+          if (have_base && fabsf(d) * reach <= 1e-3f) {
+            stepped = true;
+            // the base's factors times those over d, to second order
+            const float x1 = -r1 * d, x2 = -r2 * d;
+            const float g1 = x1 + 0.5f * (x1 * x1);  // e^{x1} - 1
+            e1 = fmaf(e1b, g1, e1b);
+            om1 = fmaf(-e1b, g1, om1b);                // 1 - e1 without cancellation
+            e2 = e2b * (1.0f + x2 + 0.5f * (x2 * x2));
+            de1 = e1 * (k1 * dt);
+            de2 = e2 * (k2 * dt);
+            const float xo = -(order * order + order + (1.0f / 3.0f)) * drate * d;
+            const float xl = -(order * order) * drate * d;
+            const float yo = -2.0f * 3.14159265358979323846f * b0 * d;
+            const float eo = 1.0f + xo + 0.5f * (xo * xo);
+            turn = mul(turn_b, mk(eo * (1.0f - 0.5f * (yo * yo)), eo * yo));
+            dlong = dlong_b * (1.0f + xl + 0.5f * (xl * xl));
+          }
         }
         if (!stepped) {
           const float bf = drate * dt;
@@ -512,6 +537,13 @@ __device__ __forceinline__ void complex_atoms(const ComplexArgs& p, int64_t firs
             e2 = expf(-r2 * dt);
             de1 = e1 * (k1 * dt);
             de2 = e2 * (k2 * dt);
+            e1b = e1;
+            om1b = om1;
+            e2b = e2;
+            turn_b = turn;
+            dlong_b = dlong;
+            base_dt = dt;
+            have_base = true;
           } else {
             // transverse generator K - diag(R2) - 2 pi i diag(0, df), times dt
             const float2 mt[4] = {mk((-kab - r2) * dt, 0.0f), mk(kba * dt, 0.0f), mk(kab * dt, 0.0f),

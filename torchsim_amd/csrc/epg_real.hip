@@ -368,19 +368,21 @@ int variant_of(int features, bool accum, int tile, int64_t states) {
   return v;
 }
 
-int default_tile(int states, int tangents, int min_lanes = 1) {
+int default_tile(int states, int tangents, int min_lanes = 1, bool accumulate = false) {
   // Among the tiles covering S orders, weigh the padded orders (work spent on
   // zeros), the in-lane register moves and DPP crossings of a wider atom, and
   // the occupancy the register file leaves: 3 planes * SL orders *
-  // (1 + tangents) floats of state per lane. A projection needs enough lanes
-  // per atom for its basis directions (min_lanes).
+  // (1 + tangents) floats of state per lane, and for the normal equations
+  // their FP64 sums and the staged data on every lane (~40 more). A
+  // projection needs enough lanes per atom for its basis directions
+  // (min_lanes).
   int best = -1;
   float best_cost = 1e30f;
   for (int t = 0; t < kTileCount; ++t) {
     const Tile tile = kTiles[t];
     const int cover = tile.sl * tile.l;
     if (cover < states || tile.l < min_lanes) continue;
-    const int regs = 3 * tile.sl * (1 + tangents);
+    const int regs = 3 * tile.sl * (1 + tangents) + (accumulate ? 40 : 0);
     const float occupancy = regs <= 96 ? 1.0f : (regs <= 150 ? 1.12f : (regs <= 200 ? 1.3f : 1.8f));
     const float cost = (float)cover / (float)states * (1.0f + 0.06f * (float)(tile.l - 1)) * occupancy;
     if (cost < best_cost) {
@@ -1181,7 +1183,7 @@ int tsim_gauss_newton_normal(void* const* packed, int64_t atom_count, int64_t ev
   args.data = (const float2*)data;
   args.normal = normal;
   if (atom_count == 0 || output_count == 0) return OK;
-  if (tile < 0) tile = default_tile(args.states, popcount4(tangent_class(tm)));
+  if (tile < 0) tile = default_tile(args.states, popcount4(tangent_class(tm)), 1, true);
   if (tile < 0 || tile >= kTileCount || kTiles[tile].sl * kTiles[tile].l < args.states) {
     set_error("no state tile covers %d orders", args.states);
     return UNSUPPORTED;
@@ -1521,7 +1523,7 @@ int tsim_simulate_batch(const tsim_batch_request* requests, int64_t count, int m
     args.push_back(a);
   }
   if (args.empty()) return OK;
-  if (tile < 0) tile = default_tile(max_states, popcount4(tangent_class(key_tm)));
+  if (tile < 0) tile = default_tile(max_states, popcount4(tangent_class(key_tm)), 1, mode == 1);
   if (tile < 0 || tile >= kTileCount || kTiles[tile].sl * kTiles[tile].l < max_states) {
     set_error("no state tile covers %d orders", max_states);
     return UNSUPPORTED;
