@@ -432,8 +432,8 @@ def serve_workload(duration_s: float = 60.0, rate: float = 50.0, seed: int = 0) 
         elapsed = time.perf_counter() - start
         stats = server.stats()
     return {"metric": "served requests/s with p50/p99 latency", "value": count / elapsed, "unit": "req/s",
-            "p50_s": stats["p50_s"], "p99_s": stats["p99_s"], "completed": stats["completed"],
-            "errors": outcome["errors"], "launches": stats["launches"], "batched_jobs": stats["batched_jobs"],
+            "p50_s": stats["p50_s"], "p99_s": stats["p99_s"], "by_kind": stats["by_kind"],
+            "completed": stats["completed"], "errors": outcome["errors"], "launches": stats["launches"], "batched_jobs": stats["batched_jobs"],
             "seconds": elapsed, "submission_s": submitted, "server_host_s": stats.get("host_s"),
             "data": "fit data: 256 tissue classes per protocol, simulated, gathered to the voxels, SNR 50 noise",
             "config": {"workload": "Poisson mix", "rate_per_s": rate, "duration_s": duration_s, "requests": count}}
@@ -842,7 +842,7 @@ def serving_summary(window_s: float = 10.0, limit_s: float = 240.0) -> dict:
     if done.returncode != 0 or not lines:
         return {"error": f"exit status {done.returncode}", "stderr": done.stderr[-400:]}
     served = json.loads(lines[-1])
-    keep = ("p50_s", "p99_s", "value", "completed", "errors", "launches", "batched_jobs", "seconds")
+    keep = ("p50_s", "p99_s", "by_kind", "value", "completed", "errors", "launches", "batched_jobs", "seconds")
     summary = {key: served.get(key) for key in keep}
     summary["req_per_s"] = summary.pop("value")
     summary["config"] = served.get("config")

@@ -229,6 +229,8 @@ def test_same_alone_and_in_the_mix():
     for name in ("T1", "T2", "M0"):
         assert torch.equal(f_alone[name], f_mixed[name]), name
     assert stats["completed"] == 8
+    assert stats["by_kind"]["fit"]["completed"] == 1 and stats["by_kind"]["simulate"]["completed"] == 7
+    assert stats["by_kind"]["fit"]["p99_s"] > 0.0
     assert stats["batched_jobs"] > stats["launches"]  # some launches carried several requests
 
 

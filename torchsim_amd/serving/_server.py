@@ -378,6 +378,7 @@ class Server:
         self._stop = threading.Event()
         self._thread: threading.Thread | None = None
         self.latencies: list[float] = []
+        self.latencies_of: dict[str, list[float]] = {"simulate": [], "fit": []}
         self.host_seconds = {"admit": 0.0, "step": 0.0, "batched": 0.0, "unbatched": 0.0, "complete": 0.0}
         self.launches = 0
         self.batched_jobs = 0
@@ -422,18 +423,24 @@ class Server:
             self.resident.pop(name, None)
             self._used -= self._resident_bytes.pop(name, 0)
 
-    def stats(self) -> dict[str, float]:
-        values = sorted(self.latencies)
+    def stats(self) -> dict[str, Any]:
+        """Completed count, p50/p99 latency (submission to result) over every
+        request and per kind (``"simulate"``, ``"fit"``), launches, batched
+        jobs, the HBM budget and the scheduler's host seconds."""
 
-        def pct(q):
+        def pct(values, q):
             if not values:
                 return float("nan")
             return values[min(len(values) - 1, int(round(q * (len(values) - 1))))]
 
+        values = sorted(self.latencies)
+        kinds = {kind: sorted(held) for kind, held in self.latencies_of.items()}
         return {
             "completed": self.completed,
-            "p50_s": pct(0.5),
-            "p99_s": pct(0.99),
+            "p50_s": pct(values, 0.5),
+            "p99_s": pct(values, 0.99),
+            "by_kind": {kind: {"completed": len(held), "p50_s": pct(held, 0.5), "p99_s": pct(held, 0.99)}
+                        for kind, held in kinds.items()},
             "launches": self.launches,
             "batched_jobs": self.batched_jobs,
             "hbm_budget": self.hbm_budget,
@@ -589,6 +596,7 @@ class Server:
                 else:
                     self._used -= ticket.bytes
             self.latencies.append(now - ticket.arrival)
+            self.latencies_of["fit" if isinstance(request, FitRequest) else "simulate"].append(now - ticket.arrival)
             self.completed += 1
             ticket.future.set_result(result)
         self._finishing = [t for t in self._finishing if t not in finished]
