@@ -348,6 +348,47 @@ def test_complex_and_closed_form_requests_same_alone_and_in_the_mix():
     assert stats["batched_jobs"] > stats["launches"]
 
 
+@pytest.mark.skip(reason="fits off the fused path as jobs of the complex kernel's batch are switched off: with "
+                         "them the 60 s serving mix faulted the GPU (memory access) although this test passed; the "
+                         "cause is not found, so those fits run ordinary iterations")
+def test_a_two_pool_fit_in_the_complex_batch_equals_its_ordinary_iterations():
+    """A fit off the fused path (an exchanging pool) runs each iteration's
+    signal + Jacobian as a job of the complex kernel's batch, beside simulate
+    requests of the same tangent set; its maps are the bits of the same
+    Levenberg-Marquardt iterations run on the ordinary path."""
+    from torchsim_amd.serving import FitRequest, Server
+    from torchsim_amd.serving._server import _FitProgram
+
+    rng = np.random.default_rng(91)
+    mrf = _mrf(120)
+    voxels = 40
+    known = {"poolB_fraction": torch.tensor(rng.uniform(0.05, 0.3, voxels), dtype=torch.float32).cuda(),
+             "poolB_exchange": torch.tensor(rng.uniform(5, 20, voxels), dtype=torch.float32).cuda(),
+             "poolB_T1": 400.0, "poolB_T2": 15.0}
+    truth_t1 = torch.tensor(rng.uniform(400, 2000, voxels), dtype=torch.float32).cuda()
+    truth_t2 = torch.tensor(rng.uniform(40, 200, voxels), dtype=torch.float32).cuda()
+    pooled = mrf.simulate(T1=truth_t1, T2=truth_t2, **known)
+    fit = FitRequest(mrf, pooled, bounds={"T1": (100.0, 5000.0), "T2": (10.0, 2000.0), "M0": (0.01, 2.0)},
+                     initial={"T1": 1000.0, "T2": 100.0, "M0": 0.8}, known=known, iterations=6)
+    ordinary = _FitProgram(fit, torch.device("cuda", 0))
+    assert ordinary.complex_batch and not ordinary.fused
+    ordinary.complex_batch = False
+    while not ordinary.finished:
+        ordinary.run_unbatched()
+    expect = ordinary.result
+    with Server() as server:
+        others = [server.submit(request) for request in _complex_requests(("T1", "T2"))]
+        maps = server.submit(fit).result(timeout=120)
+        for f in others:
+            f.result(timeout=120)
+        stats = server.stats()
+    for name in ("T1", "T2", "M0"):
+        assert torch.equal(maps[name], expect[name]), name
+    assert stats["batched_jobs"] > stats["launches"]
+    # and the fit moved toward the truth
+    assert float(((maps["T1"] - truth_t1).abs() / truth_t1).median()) < 0.05
+
+
 def test_unbatched_sequences_are_served_too():
     """FSE (complex kernel): a simulate request takes the complex batch, a
     fit the ordinary path."""
