@@ -94,9 +94,16 @@ def test_complex_batch_matches_single_launches_bitwise():
     from torchsim_amd.serving import BatchItem, launch_batch
     from torchsim_amd.sequence import _complex, _engine
 
-    for diff in (("T1", "T2", "M0"), ()):
+    for diff in (("T1", "T2", "M0"), (), "mixed"):
         items, singles = [], []
-        for request in _complex_requests(diff):
+        if diff == "mixed":
+            # one tangent set (none), stream counts 1 and 2 per atom in one batch:
+            # the waves of a block stage different requests side by side
+            plain, with_m0 = _complex_requests(()), _complex_requests(("M0",))
+            requests = [plain[k] if k % 2 == 0 else with_m0[k] for k in range(len(plain))]
+        else:
+            requests = _complex_requests(diff)
+        for request in requests:
             sim = request.simulator
             held, sequence = sim._split({**sim.bound, **request.properties})
             _, packed, _, tissue, states, columns, _, _, _ = sim._prepared(held, sequence, request.diff)
@@ -114,7 +121,7 @@ def test_complex_batch_matches_single_launches_bitwise():
         torch.cuda.synchronize()
         for item, (sig, jac) in zip(items, singles):
             assert torch.equal(item.signal, sig)
-            if diff:
+            if item.columns:
                 assert torch.equal(item.jacobian, jac)
 
 

@@ -88,11 +88,11 @@ int fill_complex(ComplexArgs& a, void* const* packed, int64_t atoms, int64_t eve
 // Output staging: K samples per flush with the padded stream count, one
 // wave's stage within COMPLEX_STAGE_BYTES_PER_WAVE; K < 2 stores directly.
 // Returns the dynamic LDS bytes a block of `threads` needs.
-size_t set_staging(ComplexArgs& a, int lanes, int threads) {
+size_t set_staging(ComplexArgs& a, int lanes, int threads, int min_s_log2 = 0) {
   a.stage_k_log2 = -1;
   a.stage_s_log2 = 0;
   if (a.out_real != nullptr || (a.signal == nullptr && a.jacobian == nullptr)) return 0;
-  int s_log2 = 0;
+  int s_log2 = min_s_log2;
   while ((1 << s_log2) < 1 + a.columns) ++s_log2;
   const int64_t slot = (int64_t)(64 / lanes) * (1 << s_log2) * (int64_t)sizeof(float2);
   int k_log2 = 0;
@@ -317,10 +317,17 @@ extern "C" int tsim_complex_batch(const tsim_batch_request* requests, int64_t co
   }
   const int threads = 256;
   const int64_t atoms_per_block = (int64_t)(threads / 64) * (64 / lanes);
+  // The waves of a block may work on different requests at once, each in
+  // its own LDS stage region, whose place follows the streams per atom: every
+  // request stages as many streams as the widest of the batch, so the regions
+  // of all requests lie alike.
+  int streams_log2 = 0;
+  for (const ComplexArgs& a : args)
+    while ((1 << streams_log2) < 1 + a.columns) ++streams_log2;
   size_t shared = 0;
   std::vector<ComplexWork> work;
   for (size_t k = 0; k < args.size(); ++k) {
-    const size_t bytes = set_staging(args[k], lanes, threads);
+    const size_t bytes = set_staging(args[k], lanes, threads, streams_log2);
     shared = bytes > shared ? bytes : shared;
     for (int64_t first = 0; first < args[k].atoms; first += atoms_per_block) {
       ComplexWork w;
