@@ -105,6 +105,25 @@ int tsim_semisolid_jacobian(void* const* packed, int64_t atom_count, int64_t eve
                             void* jacobian, void* stream);
 
 /*
+ * First-order adjoint through the semisolid pool (instantaneous pulses):
+ * the gradient of a real loss with respect to every event's flip (rad),
+ * phase (rad) and duration (s), given the loss's cotangent on the signal
+ * tsim_semisolid_jacobian writes (dL/dRe + i dL/dIm per sample). Replaces the
+ * event gradients of `_epg_cpu.simulate_vjp` with pool kind 1 and real_axis
+ * -1 (src/torchsim/_epg_cpu.cpp:4839-5944 via simulate_vjp_range<INSTANT,
+ * SEMISOLID>, :11316-11340); the tissue gradients come from that entry's
+ * Jacobian columns. Buffers and lineshape as for tsim_semisolid_jacobian.
+ * cotangent: DEVICE (atom_count, output_count) complex64.
+ * grad_flip, grad_phase, grad_duration: DEVICE (event_count,) FP32,
+ *          overwritten. Entry states are kept for at most 1 GiB of lanes at a
+ *          time (stream-ordered scratch); more atoms take several launches.
+ */
+int tsim_semisolid_vjp(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
+                       int64_t output_count, int features, int64_t tissue_stride, float factor_re, float factor_im,
+                       const float* lineshape, int64_t lineshape_bins, float lineshape_step, const void* cotangent,
+                       float* grad_flip, float* grad_phase, float* grad_duration, void* stream);
+
+/*
  * Signal and Jacobian with slice-profiled pulses: every pulse but an
  * inversion turns the states through the Cayley-Klein pair (a, b) its shape
  * performs at the atom's slice position and effective flip flip * b1, read

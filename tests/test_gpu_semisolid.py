@@ -236,15 +236,20 @@ def test_forward_mode_along_a_mixed_tissue_direction():
 
 
 def test_derivatives_the_pool_does_not_carry_are_refused():
-    """The sequence's flips, and the bound pool's own properties, carry no
-    derivative through the semisolid kernel: asking raises."""
+    """Forward mode along the sequence, a Jacobian's gradient to the flips,
+    and the bound pool's own properties carry no derivative through the
+    semisolid kernel: asking raises."""
+    import torch.autograd.forward_ad as fwAD
     from torchsim_amd.simulators import MRFSimulator
 
     maps = {k: torch.tensor(v).cuda() for k, v in _maps(8, 31).items()}
     flips = torch.tensor(_cases.benchmark_flips(40), dtype=torch.float32, requires_grad=True)
-    sim = MRFSimulator(flip=flips, TR=10.0, states=10)
-    with pytest.raises(NotImplementedError, match="carry none"):
-        sim.simulate(**maps)
+    with pytest.raises(NotImplementedError, match="forward mode along them"):
+        MRFSimulator(flip=flips, TR=10.0, states=10).jacobian(("T1",), **maps)
+    with fwAD.dual_level():
+        dual = fwAD.make_dual(flips.detach(), torch.ones_like(flips))
+        with pytest.raises(NotImplementedError, match="forward mode along them"):
+            MRFSimulator(flip=dual, TR=10.0, states=10).simulate(**maps)
     sim = _simulator("mrf")
     bound = dict(maps)
     bound["bound_fraction"] = bound["bound_fraction"].clone().requires_grad_(True)
@@ -329,3 +334,132 @@ def test_a_tabulated_hard_pulse_is_the_instant_rotation():
     error = float((tabulated - instant).abs().max() / instant.abs().max())
     print(f"hard pulse through its table: {error:.2e}")
     assert error < 2e-5
+
+
+# --- reverse mode to the sequence through the pool ---
+
+SEQ_TOL = 2e-5  # per gradient, relative to its largest entry
+
+
+def _pool_tissue(atoms, seed):
+    rng = np.random.default_rng(seed)
+    t1, t2 = _cases.random_tissue(atoms, seed=seed)
+    return {"t1": t1, "t2": t2, "m0": rng.uniform(0.5, 1.0, atoms), "b1": rng.uniform(0.8, 1.2, atoms),
+            "b0": rng.uniform(-200.0, 200.0, atoms), "bound_fraction": rng.uniform(0.05, 0.2, atoms),
+            "bound_exchange": rng.uniform(5.0, 40.0, atoms), "t1_bound": rng.uniform(300.0, 1200.0, atoms)}
+
+
+def test_the_adjoint_kernel_against_fp64_autograd_per_event():
+    """tsim_semisolid_vjp on a phase-cycled MRF train with B0, a B1 map, the
+    pulses played 800 Hz off resonance and their saturation scaled 300-fold
+    (so that the bound pool's term moves the flip gradient by percents):
+    every event's flip, phase and duration gradient of Re(sum conj(W) s)
+    against FP64 autograd of the oracle on the same events."""
+    from torchsim_amd.sequence import _engine, _semisolid
+    from torchsim_amd.sequence._lineshape import lineshape_table
+    from torchsim_amd.simulators import MRFSimulator
+
+    atoms, pulses, states = 20, 80, 10
+    rng = np.random.default_rng(5)
+    flips, trs, phases = rng.uniform(5.0, 60.0, pulses), rng.uniform(8.0, 15.0, pulses), rng.uniform(0, 360, pulses)
+    packed = MRFSimulator(states=states).packed(flip=torch.tensor(flips, dtype=torch.float32),
+                                                 TR=torch.tensor(trs, dtype=torch.float32),
+                                                 phases=torch.tensor(phases, dtype=torch.float32))
+    import dataclasses as dc
+
+    packed = dc.replace(packed, saturation=packed.saturation * 300.0,
+                        rf_frequency_hz=torch.where(packed.kind == 1, 800.0, 0.0).to(torch.float32))
+    tissue = _pool_tissue(atoms, 6)
+    fields = {"t1_ms": tissue["t1"], "t2_ms": tissue["t2"], "m0": tissue["m0"], "b1": tissue["b1"],
+              "b0_hz": tissue["b0"], "bound_fraction": tissue["bound_fraction"],
+              "bound_exchange_rate": tissue["bound_exchange"], "t1_bound_ms": tissue["t1_bound"]}
+    prepared = _engine.prepare_tissue({k: torch.tensor(v, dtype=torch.float32) for k, v in fields.items()},
+                                      torch.device("cuda", 0), ())
+    assert prepared.semisolid
+    weight = rng.normal(size=(atoms, packed.output_count)) + 1j * rng.normal(size=(atoms, packed.output_count))
+    signal, _ = _semisolid.simulate_semisolid(packed, prepared, states=states, factor=1j)
+    grads = _semisolid.adjoint_semisolid(packed, prepared, states=states, factor=1j,
+                                         cotangent=torch.tensor(weight, dtype=torch.complex64).cuda())
+    table = lineshape_table()
+    events = oracle.events_of(packed)
+    leaves = {k: torch.tensor(events[k], dtype=torch.float64, requires_grad=True) for k in ("flip", "phase", "duration")}
+    given = dict(events)
+    given.update(leaves)
+    ref = 1j * P.simulate(given, {k: torch.tensor(v, dtype=torch.float64) for k, v in tissue.items()}, states=states,
+                          outputs=packed.output_count, lineshape=(table.values, table.slopes, table.step))
+    error = np.abs(signal.cpu().numpy() - ref.detach().numpy()).max() / np.abs(ref.detach().numpy()).max()
+    print(f"signal: {error:.2e}")
+    assert error < 1e-5
+    (ref * torch.tensor(weight).conj()).real.sum().backward()
+    for name, got in zip(("flip", "phase", "duration"), grads):
+        want = leaves[name].grad.numpy()
+        err = np.abs(got.cpu().numpy() - want).max() / np.abs(want).max()
+        print(f"d/d{name}: {err:.2e}")
+        assert err < SEQ_TOL, name
+    # the bound pool's saturation term matters here: without it the flip gradient misses
+    plain = dict(given)
+    plain["saturation"] = np.zeros_like(events["saturation"])
+    for leaf in leaves.values():
+        leaf.grad = None
+    ref2 = 1j * P.simulate(plain, {k: torch.tensor(v, dtype=torch.float64) for k, v in tissue.items()},
+                           states=states, outputs=packed.output_count,
+                           lineshape=(table.values, table.slopes, table.step))
+    (ref2 * torch.tensor(weight).conj()).real.sum().backward()
+    miss = np.abs(grads[0].cpu().numpy() - leaves["flip"].grad.numpy()).max() / np.abs(leaves["flip"].grad.numpy()).max()
+    print(f"flip gradient against a saturation-free oracle: {miss:.2e}")
+    assert miss > 1e-3
+
+
+@pytest.mark.parametrize("name", ["mrf", "fse"])
+def test_loss_backward_reaches_the_protocol_through_the_pool(name):
+    """loss.backward() through a simulator with a semisolid pool reaches its
+    flips and TRs (MRF: and phases) and the T1/T2 maps, against FP64 autograd
+    of the oracle through the same differentiable packing."""
+    from torchsim_amd.sequence._lineshape import lineshape_table
+    from torchsim_amd.simulators import FSESimulator, MRFSimulator
+
+    atoms = 16
+    rng = np.random.default_rng(11)
+    tissue = _pool_tissue(atoms, 12)
+    tissue.pop("b0")  # B0 in the states only where the winding is not uniform; these trains wind uniformly
+    if name == "mrf":
+        pulses = 60
+        values = {"flip": rng.uniform(5.0, 60.0, pulses), "TR": rng.uniform(8.0, 15.0, pulses),
+                  "phases": rng.uniform(0.0, 360.0, pulses)}
+
+        def build(**kw):
+            return MRFSimulator(states=10, **kw)
+    else:
+        values = {"flip": rng.uniform(120.0, 180.0, 16)}
+
+        def build(**kw):
+            return FSESimulator(ESP=5.0, TR=1e6, states=10, **kw)
+    ours = {k: torch.tensor(v, dtype=torch.float32, requires_grad=True) for k, v in values.items()}
+    maps = {k: torch.tensor(tissue[k], dtype=torch.float32, device="cuda", requires_grad=(k in ("t1", "t2")))
+            for k in tissue}
+    public = {"t1": "T1", "t2": "T2", "m0": "M0", "b1": "B1", "bound_fraction": "bound_fraction",
+              "bound_exchange": "bound_exchange", "t1_bound": "bound_T1"}
+    sim = build(**ours)
+    sig = sim.simulate(**{public[k]: v for k, v in maps.items()})
+    weight = rng.normal(size=tuple(sig.shape)) + 1j * rng.normal(size=tuple(sig.shape))
+    (sig * torch.tensor(weight, dtype=torch.complex64).cuda().conj()).real.sum().backward()
+
+    table = lineshape_table()
+    theirs = {k: torch.tensor(v, dtype=torch.float64, requires_grad=True) for k, v in values.items()}
+    sim64 = build(**theirs)
+    packed = sim64.packed(**sim64.bound)
+    events = oracle.events_of(packed)
+    events.update(duration=packed.duration.double(), flip=packed.flip.double(), phase=packed.phase.double())
+    maps64 = {k: torch.tensor(v, dtype=torch.float64, requires_grad=(k in ("t1", "t2"))) for k, v in tissue.items()}
+    factor = complex(sim64.signal_factor) if sim64.signal_factor is not None else 1.0
+    ref = factor * P.simulate(events, maps64, states=10, outputs=packed.output_count,
+                              lineshape=(table.values, table.slopes, table.step))
+    (ref * torch.tensor(weight).conj()).real.sum().backward()
+    for key in values:
+        err = np.abs(ours[key].grad.numpy() - theirs[key].grad.numpy()).max() / np.abs(theirs[key].grad.numpy()).max()
+        print(f"{name} d/d{key}: {err:.2e}")
+        assert err < SEQ_TOL, key
+    for key in ("t1", "t2"):
+        err = np.abs(maps[key].grad.cpu().numpy() - maps64[key].grad.numpy()).max() / np.abs(maps64[key].grad.numpy()).max()
+        print(f"{name} d/d{key}: {err:.2e}")
+        assert err < JTOL, key

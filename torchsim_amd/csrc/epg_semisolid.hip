@@ -1,6 +1,6 @@
 // Host side of the semisolid-pool kernel: argument checks, lane-count choice
 // and the C ABI entry point (declared in include/torchsim_hip.h).
-#include "epg_semisolid_impl.h"
+#include "epg_semisolid_adjoint_impl.h"
 
 #include <cstring>
 
@@ -32,14 +32,15 @@ int semisolid_lanes(int64_t states) {
 namespace tsim {
 namespace {
 
-// Checks and launch shared by the two entry points. `bound` asks for the
+// Checks and arguments shared by the entry points. `bound` asks for the
 // semisolid pool (its buffers and the lineshape required); a profile table
-// makes the pulses profiled.
-int run_shaped(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
-               int64_t output_count, int features, int64_t tissue_stride, const int32_t* columns,
-               int64_t column_count, float factor_re, float factor_im, bool bound, const float* lineshape,
-               int64_t lineshape_bins, float lineshape_step, const float* profile, int64_t locations,
-               int64_t profile_bins, float profile_step, void* signal, void* jacobian, void* stream) {
+// makes the pulses profiled. Returns the tangent mask in `tm`.
+int fill_shaped(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
+                int64_t output_count, int features, int64_t tissue_stride, const int32_t* columns,
+                int64_t column_count, float factor_re, float factor_im, bool bound, const float* lineshape,
+                int64_t lineshape_bins, float lineshape_step, const float* profile, int64_t locations,
+                int64_t profile_bins, float profile_step, void* signal, void* jacobian, SemisolidArgs& a,
+                int& tm) {
   if (packed == nullptr || packed[T_T1] == nullptr || packed[T_T2] == nullptr) {
     set_error("packed buffers with t1 and t2 are required");
     return BAD_ARGUMENT;
@@ -90,12 +91,6 @@ int run_shaped(void* const* packed, int64_t atom_count, int64_t event_count, int
     set_error("between 0 and 4 Jacobian columns, got %lld", (long long)column_count);
     return BAD_ARGUMENT;
   }
-  const bool sized = atom_count > 0 && output_count > 0;
-  if (sized && (signal == nullptr || (column_count > 0 && jacobian == nullptr))) {
-    set_error("the signal (and, with columns, the Jacobian) output is required");
-    return BAD_ARGUMENT;
-  }
-  SemisolidArgs a;
   memset(&a, 0, sizeof(a));
   a.t1 = (const float*)packed[T_T1];
   a.t2 = (const float*)packed[T_T2];
@@ -135,7 +130,7 @@ int run_shaped(void* const* packed, int64_t atom_count, int64_t event_count, int
   a.factor_re = factor_re;
   a.factor_im = factor_im;
   a.col_t1 = a.col_t2 = a.col_b1 = a.col_m0 = -1;
-  int tm = 0;
+  tm = 0;
   for (int64_t c = 0; c < column_count; ++c) {
     int* slot;
     switch (columns[c]) {
@@ -160,6 +155,26 @@ int run_shaped(void* const* packed, int64_t atom_count, int64_t event_count, int
   a.columns = (int)column_count;
   a.signal = (float2*)signal;
   a.jacobian = column_count > 0 ? (float2*)jacobian : nullptr;
+  return OK;
+}
+
+// Checks and launch of the forward entry points.
+int run_shaped(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
+               int64_t output_count, int features, int64_t tissue_stride, const int32_t* columns,
+               int64_t column_count, float factor_re, float factor_im, bool bound, const float* lineshape,
+               int64_t lineshape_bins, float lineshape_step, const float* profile, int64_t locations,
+               int64_t profile_bins, float profile_step, void* signal, void* jacobian, void* stream) {
+  SemisolidArgs a;
+  int tm = 0;
+  int st = fill_shaped(packed, atom_count, event_count, state_count, output_count, features, tissue_stride, columns,
+                       column_count, factor_re, factor_im, bound, lineshape, lineshape_bins, lineshape_step, profile,
+                       locations, profile_bins, profile_step, signal, jacobian, a, tm);
+  if (st != OK) return st;
+  const bool sized = atom_count > 0 && output_count > 0;
+  if (sized && (signal == nullptr || (column_count > 0 && jacobian == nullptr))) {
+    set_error("the signal (and, with columns, the Jacobian) output is required");
+    return BAD_ARGUMENT;
+  }
   if (!sized) return OK;
   hipStream_t s = (hipStream_t)stream;
   const int L = semisolid_lanes(state_count);
@@ -169,7 +184,7 @@ int run_shaped(void* const* packed, int64_t atom_count, int64_t event_count, int
     return UNSUPPORTED;
   }
   Compiled compiled;
-  int st = compile_packed(packed, event_count, s, compiled);
+  st = compile_packed(packed, event_count, s, compiled);
   if (st != OK) return st;
   a.events = compiled.codes;
   a.event_count = event_count;
@@ -216,4 +231,115 @@ extern "C" int tsim_profiled_jacobian(void* const* packed, int64_t atom_count, i
   return run_shaped(packed, atom_count, event_count, state_count, output_count, features, tissue_stride, columns,
                     column_count, factor_re, factor_im, bound, lineshape, lineshape_bins, lineshape_step, profile,
                     locations, profile_bins, profile_step, signal, jacobian, stream);
+}
+
+namespace tsim {
+namespace {
+
+using SemisolidVjpLaunch = void (*)(const SemisolidVjpArgs&, dim3, dim3, hipStream_t);
+
+template <int L>
+void launch_semisolid_vjp(const SemisolidVjpArgs& a, dim3 grid, dim3 block, hipStream_t s) {
+  hipLaunchKernelGGL((semisolid_vjp_kernel<L>), grid, block, 0, s, a);
+}
+
+SemisolidVjpLaunch pick_semisolid_vjp(int lanes) {
+  switch (lanes) {
+#define TSIM_SEMISOLID_VJP_CASE(LANES) \
+  case LANES:                          \
+    return &launch_semisolid_vjp<LANES>;
+    TSIM_COMPLEX_LANES(TSIM_SEMISOLID_VJP_CASE)
+#undef TSIM_SEMISOLID_VJP_CASE
+    default:
+      return nullptr;
+  }
+}
+
+// Bytes of entry states one launch keeps (the launch covers as many atoms).
+constexpr int64_t kTrajectoryBytes = 1LL << 30;
+
+}  // namespace
+}  // namespace tsim
+
+extern "C" int tsim_semisolid_vjp(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
+                                  int64_t output_count, int features, int64_t tissue_stride, float factor_re,
+                                  float factor_im, const float* lineshape, int64_t lineshape_bins,
+                                  float lineshape_step, const void* cotangent, float* grad_flip, float* grad_phase,
+                                  float* grad_duration, void* stream) {
+  SemisolidArgs a;
+  int tm = 0;
+  int st = fill_shaped(packed, atom_count, event_count, state_count, output_count, features, tissue_stride, nullptr,
+                       0, factor_re, factor_im, true, lineshape, lineshape_bins, lineshape_step, nullptr, 1, 0, 1.0f,
+                       nullptr, nullptr, a, tm);
+  if (st != OK) return st;
+  if (event_count > 0 && (grad_flip == nullptr || grad_phase == nullptr || grad_duration == nullptr)) {
+    set_error("the flip, phase and duration gradient outputs are required");
+    return BAD_ARGUMENT;
+  }
+  if (atom_count > 0 && output_count > 0 && cotangent == nullptr) {
+    set_error("the signal cotangent is required");
+    return BAD_ARGUMENT;
+  }
+  if (event_count == 0) return OK;
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned sum_blocks = (unsigned)((3 * event_count + 255) / 256);
+  if (atom_count == 0 || output_count == 0) {
+    // nothing recorded: every gradient is zero (no waves to sum)
+    hipLaunchKernelGGL(semisolid_sum_partials, dim3(sum_blocks), dim3(256), 0, s, (const float*)nullptr,
+                       (int64_t)0, event_count, grad_flip, grad_phase, grad_duration, 0);
+    return hip_status(hipGetLastError(), "semisolid gradient clear");
+  }
+  const int L = semisolid_lanes(state_count);
+  SemisolidVjpLaunch launch = pick_semisolid_vjp(L);
+  if (!launch) {
+    set_error("no adjoint kernel for %lld orders", (long long)state_count);
+    return UNSUPPORTED;
+  }
+  const int64_t apw = 64 / L;
+  // lanes one launch covers: its entry states within the budget, one wave at least
+  int64_t wave_cap = kTrajectoryBytes / (8 * (int64_t)sizeof(float) * event_count * 64);
+  if (wave_cap < 1) wave_cap = 1;
+  const int64_t waves_all = (atom_count + apw - 1) / apw;
+  const int64_t waves = waves_all < wave_cap ? waves_all : wave_cap;
+  const int64_t threads = waves * 64;
+  Compiled compiled;
+  st = compile_packed(packed, event_count, s, compiled);
+  if (st != OK) return st;
+  a.events = compiled.codes;
+  a.event_count = event_count;
+  float* trajectory = nullptr;
+  float* partial = nullptr;
+  st = scratch_alloc((void**)&trajectory, sizeof(float) * 8 * (size_t)event_count * (size_t)threads, s,
+                     "semisolid adjoint states");
+  if (st != OK) return st;
+  st = scratch_alloc((void**)&partial, sizeof(float) * 3 * (size_t)event_count * (size_t)waves, s,
+                     "semisolid adjoint partials");
+  if (st != OK) {
+    (void)hipFreeAsync(trajectory, s);
+    return st;
+  }
+  SemisolidVjpArgs q;
+  q.p = a;
+  q.cotangent = (const float2*)cotangent;
+  q.trajectory = trajectory;
+  q.partial = partial;
+  q.threads = threads;
+  const int64_t atoms_per_launch = waves * apw;
+  int first = 1;
+  for (int64_t begin = 0; begin < atom_count && st == OK; begin += atoms_per_launch) {
+    const int64_t count = atom_count - begin < atoms_per_launch ? atom_count - begin : atoms_per_launch;
+    const int64_t launch_waves = (count + apw - 1) / apw;
+    q.first_atom = begin;
+    q.launch_atoms = count;
+    launch(q, dim3((unsigned)launch_waves), dim3(64), s);
+    st = hip_status(hipGetLastError(), "semisolid_vjp_kernel launch");
+    if (st != OK) break;
+    hipLaunchKernelGGL(semisolid_sum_partials, dim3(sum_blocks), dim3(256), 0, s, (const float*)partial,
+                       launch_waves, event_count, grad_flip, grad_phase, grad_duration, first ? 0 : 1);
+    st = hip_status(hipGetLastError(), "semisolid gradient sum");
+    first = 0;
+  }
+  (void)hipFreeAsync(partial, s);
+  (void)hipFreeAsync(trajectory, s);
+  return st;
 }

@@ -1,0 +1,386 @@
+// First-order adjoint of the semisolid-pool kernel (instantaneous pulses),
+// for gfx950: the gradient of a real loss with respect to every event's flip
+// angle, phase and duration, given the loss's cotangent on the recorded
+// samples (the reference's simulate_vjp_range<INSTANT, SEMISOLID>,
+// src/torchsim/_epg_cpu.cpp:4839-5944 with pool kind 1 and the dispatch of
+// :11316-11340, for the event gradients `_NativeEpg.backward` hands on,
+// sequence/_accelerators.py:1544-1586).
+//
+// Layout as the forward: one configuration order per lane, an atom on L
+// adjacent lanes. Pass 1 runs the events forward and stores every event's
+// entry state (free F+, F-, Z and bound Z: 8 floats) to scratch,
+// [event][float][lane] so a wave's stores coalesce; pass 2 walks the events
+// last to first, recomputes the event's relaxation and pre-shift from its
+// entry state and applies the transposed operators to the adjoint state
+// lambda: the shift's transpose, T^H for a pulse (and the bound pool's
+// saturation factor, real), conj(tp) / tp / Y^T for the relaxation of
+// F+ / F- / (Z, Z_bound). A launch covers as many atoms as its scratch
+// holds; the caller walks the atoms launch by launch.
+//
+// Conventions. The loss is real; the cotangent w of a complex output s is
+// dL/dRe(s) + i dL/dIm(s), so dL = Re(conj(w) ds), and the adjoint of a state
+// entry x is lambda with dL = Re(conj(lambda) dx). Each event's three
+// gradient terms are summed over the wave's lanes (a butterfly) and stored
+// per wave; a second launch sums the waves in a fixed order.
+#pragma once
+
+#include "epg_semisolid_impl.h"
+
+namespace tsim {
+
+struct SemisolidVjpArgs {
+  SemisolidArgs p;          // the forward's arguments (its outputs unused)
+  const float2* cotangent;  // (atoms, outputs)
+  float* trajectory;        // (events, 8, threads): each event's entry state, this launch's lanes
+  float* partial;           // (waves, events, 3): flip, phase, duration terms per wave
+  int64_t first_atom;       // this launch's atoms: [first_atom, first_atom + launch_atoms)
+  int64_t launch_atoms;
+  int64_t threads;          // lanes of this launch (the trajectory's stride)
+};
+
+namespace mtv {
+
+using namespace cx;
+using mt::Order;
+
+// Re(conj(a) b)
+__device__ __forceinline__ float dotr(float2 a, float2 b) { return fmaf(a.x, b.x, a.y * b.y); }
+
+// One interval on this lane: the free water's transverse factor tp (F+;
+// F- takes its conjugate), the (Z, Z_bound) operator Y with this order's
+// damping, the recovery at order 0, and their derivatives along dt.
+struct Relax {
+  float2 tp, dtp;
+  float Y[4], dY[4];
+  float rec[2], drec[2];
+};
+
+struct Atom {
+  float r1, r2, r1b, kab, kba, f, free, b0, drate, order;
+};
+
+__device__ __forceinline__ void form_relax(const Atom& at, float dt, Relax& R) {
+  const float wt = at.order * at.order + at.order + (1.0f / 3.0f);
+  const float wl = at.order * at.order;
+  const float bf = at.drate * dt;
+  const float dtr = expf(-wt * bf);
+  const float dlong = expf(-wl * bf);
+  float s, c;
+  sincosf(-2.0f * 3.14159265358979323846f * at.b0 * dt, &s, &c);
+  const float e2 = expf(-at.r2 * dt);
+  R.tp = mk(c * dtr * e2, s * dtr * e2);
+  R.dtp = mul(R.tp, mk(-(at.r2 + wt * at.drate), -2.0f * 3.14159265358979323846f * at.b0));
+  const float g00 = -at.kab - at.r1, g01 = at.kba, g10 = at.kab, g11 = -at.kba - at.r1b;
+  const float2 ml[4] = {mk(g00 * dt, 0.0f), mk(g01 * dt, 0.0f), mk(g10 * dt, 0.0f), mk(g11 * dt, 0.0f)};
+  const float2 el[4] = {mk(0.0f, 0.0f), mk(0.0f, 0.0f), mk(0.0f, 0.0f), mk(0.0f, 0.0f)};
+  float2 yl[4], unused[4];
+  expm2<false>(ml, el, yl, unused);
+  const float y0 = yl[0].x, y1 = yl[1].x, y2 = yl[2].x, y3 = yl[3].x;
+  // recovery toward the equilibrium split (1 - f, f); d/dt = -G Y eq
+  R.rec[0] = at.free - (y0 * at.free + y1 * at.f);
+  R.rec[1] = at.f - (y2 * at.free + y3 * at.f);
+  const float q0 = y0 * at.free + y1 * at.f, q1 = y2 * at.free + y3 * at.f;
+  R.drec[0] = -(g00 * q0 + g01 * q1);
+  R.drec[1] = -(g10 * q0 + g11 * q1);
+  R.Y[0] = y0 * dlong;
+  R.Y[1] = y1 * dlong;
+  R.Y[2] = y2 * dlong;
+  R.Y[3] = y3 * dlong;
+  // d(Y dlong)/dt = (G - wl D) Y dlong
+  const float d = wl * at.drate;
+  R.dY[0] = (g00 - d) * R.Y[0] + g01 * R.Y[2];
+  R.dY[1] = (g00 - d) * R.Y[1] + g01 * R.Y[3];
+  R.dY[2] = g10 * R.Y[0] + (g11 - d) * R.Y[2];
+  R.dY[3] = g10 * R.Y[1] + (g11 - d) * R.Y[3];
+}
+
+__device__ __forceinline__ void relax(Order& x, const Relax& R, bool first) {
+  x.fp = mul(x.fp, R.tp);
+  x.fm = mul(x.fm, cj(R.tp));
+  const float2 z = add(scl(x.z, R.Y[0]), scl(x.zb, R.Y[1]));
+  const float2 zb = add(scl(x.z, R.Y[2]), scl(x.zb, R.Y[3]));
+  x.z = z;
+  x.zb = zb;
+  if (first) {
+    x.z.x += R.rec[0];
+    x.zb.x += R.rec[1];
+  }
+}
+
+// The reverse of relax from entry state x: returns the term along dt and
+// turns lambda (the adjoint after the interval) into the one before it.
+__device__ __forceinline__ float relax_reverse(const Relax& R, const Order& x, Order& l, bool first) {
+  float g = dotr(l.fp, mul(R.dtp, x.fp)) + dotr(l.fm, mul(cj(R.dtp), x.fm)) +
+            dotr(l.z, add(scl(x.z, R.dY[0]), scl(x.zb, R.dY[1]))) +
+            dotr(l.zb, add(scl(x.z, R.dY[2]), scl(x.zb, R.dY[3])));
+  if (first) g += l.z.x * R.drec[0] + l.zb.x * R.drec[1];
+  l.fp = mul(cj(R.tp), l.fp);
+  l.fm = mul(R.tp, l.fm);
+  const float2 lz = add(scl(l.z, R.Y[0]), scl(l.zb, R.Y[2]));
+  const float2 lb = add(scl(l.z, R.Y[1]), scl(l.zb, R.Y[3]));
+  l.z = lz;
+  l.zb = lb;
+  return g;
+}
+
+// Transpose of shift_complex: the adjoint of F+ moves down an order (from
+// the lane above, nothing past the last carried order), of F- up; F+[0] =
+// conj(F-[1]) sends conj(lambda_F+[0]) to lambda_F-[1].
+template <int L>
+__device__ __forceinline__ void shift_reverse(Order& l, int sub, int lane, int states) {
+  const float2 p_up = mk(from_upper<L>(l.fp.x, lane), from_upper<L>(l.fp.y, lane));
+  const float2 m_down = mk(from_lower<L>(l.fm.x, lane), from_lower<L>(l.fm.y, lane));
+  const float2 p_down = mk(from_lower<L>(l.fp.x, lane), from_lower<L>(l.fp.y, lane));
+  l.fp = sub + 1 < states ? p_up : mk(0.0f, 0.0f);
+  l.fm = sub == 0 ? mk(0.0f, 0.0f) : m_down;
+  if (sub == 1) l.fm = add(l.fm, cj(p_down));
+  if (sub >= states) l.fm = mk(0.0f, 0.0f);
+}
+
+}  // namespace mtv
+
+template <int L>
+__global__ void __launch_bounds__(64) semisolid_vjp_kernel(const SemisolidVjpArgs q) {
+  using namespace cx;
+  using mt::Order;
+  using mtv::dotr;
+  const SemisolidArgs& p = q.p;
+  constexpr int APW = 64 / L;
+  const int lane = (int)(threadIdx.x & 63);
+  const int wave_group = lane / L;
+  const int sub = lane - wave_group * L;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t wave = tid >> 6;
+  const int64_t local = wave * APW + wave_group;
+  const bool live = wave_group < APW && local < q.launch_atoms;
+  const int64_t atom = q.first_atom + (live ? local : q.launch_atoms - 1);
+  const bool first = sub == 0;
+  const bool beyond = sub >= p.states;
+  const int64_t ia = atom * p.stride_opt;
+
+  mtv::Atom at;
+  const float t1 = p.t1[atom], t2 = p.t2[atom];
+  at.r1 = 1000.0f / t1;
+  at.r2 = 1000.0f / t2;
+  const float m0 = p.density ? p.m0[ia] : 1.0f;
+  const float b1 = p.transmit ? p.b1[ia] : 1.0f;
+  const float b1_phase = p.off_axis && p.b1_phase ? p.b1_phase[ia] : 0.0f;
+  at.b0 = p.off_axis && p.b0 ? p.b0[ia] : 0.0f;
+  const float eta = p.inverting ? p.inv_eff[ia] : 1.0f;
+  at.drate = p.diffusing ? p.diffusion[ia] : 0.0f;
+  at.f = p.bound_fraction ? p.bound_fraction[ia] : 0.0f;
+  const float kex = p.bound_exchange ? p.bound_exchange[ia] : 0.0f;
+  at.r1b = p.t1_bound ? 1000.0f / p.t1_bound[ia] : at.r1;
+  at.free = 1.0f - at.f;
+  at.kab = kex * at.f;
+  at.kba = kex * at.free;
+  at.order = (float)sub;
+  const float mask = live ? 1.0f : 0.0f;
+
+  const int64_t E = p.event_count;
+  const int64_t T = q.threads;
+  float* const traj = q.trajectory + tid;
+  const float2 factor = mk(p.factor_re, p.factor_im);
+
+  // pass 1: forward, storing each event's entry state
+  Order x;
+  x.fp = x.fm = x.z = x.zb = mk(0.0f, 0.0f);
+  if (first) {
+    x.z = mk(at.free, 0.0f);
+    x.zb = mk(at.f, 0.0f);
+  }
+  mtv::Relax R;
+  float last_dt = -1.0f;
+  for (int64_t e = 0; e < E; ++e) {
+    const EventCode ev = load_event(p.events, e);
+    float* slot = traj + e * 8 * T;
+    slot[0] = x.fp.x;
+    slot[T] = x.fp.y;
+    slot[2 * T] = x.fm.x;
+    slot[3 * T] = x.fm.y;
+    slot[4 * T] = x.z.x;
+    slot[5 * T] = x.z.y;
+    slot[6 * T] = x.zb.x;
+    slot[7 * T] = x.zb.y;
+    const uint32_t act = ev.ctrl & 0xFFu;
+    const int32_t kind = (int32_t)(ev.ctrl >> CTRL_KIND_SHIFT);
+    if (ev.dt != 0.0f) {
+      if (ev.dt != last_dt) {
+        last_dt = ev.dt;
+        mtv::form_relax(at, ev.dt, R);
+      }
+      mtv::relax(x, R, first);
+    }
+    if (act & PRE_SHIFT) shift_complex<L>(x.fp, x.fm, sub, lane, beyond);
+    if (kind == KIND_RF) {
+      if (act & INVERSION) {
+        x.z = scl(x.z, -eta);
+      } else {
+        const float theta = ev.flip * b1;
+        const float deposit = p.bound_fraction ? p.saturation[e] : 0.0f;
+        if (deposit != 0.0f) x.zb = scl(x.zb, expf(deposit * theta * theta * lineshape_at(p, p.rf_frequency[e] - at.b0)));
+        float sa, va;
+        if (p.transmit) {
+          const float h = sinf(0.5f * theta);
+          sa = sinf(theta);
+          va = 2.0f * (h * h);
+        } else {
+          sa = ev.sine;
+          va = ev.versine;
+        }
+        const float ca = 1.0f - va;
+        float sp, cp;
+        sincosf(ev.phase + b1_phase, &sp, &cp);
+        const float2 e1p = mk(cp, sp);
+        const float2 e2p = mul(e1p, e1p);
+        const float shh = 0.5f * va, chh = 1.0f - shh;
+        const float2 t01 = scl(e2p, shh), t02 = mul(mk(0.0f, -sa), e1p);
+        const float2 t10 = cj(t01), t12 = mul(mk(0.0f, sa), cj(e1p));
+        const float2 t20 = mul(mk(0.0f, -0.5f * sa), cj(e1p)), t21 = mul(mk(0.0f, 0.5f * sa), e1p);
+        const float2 fp = x.fp, fm = x.fm, z = x.z;
+        x.fp = mac(mac(scl(fp, chh), t01, fm), t02, z);
+        x.fm = mac(mac(mul(t10, fp), mk(chh, 0.0f), fm), t12, z);
+        x.z = mac(mac(mul(t20, fp), t21, fm), mk(ca, 0.0f), z);
+      }
+    }
+    if (act & POST_SHIFT) shift_complex<L>(x.fp, x.fm, sub, lane, beyond);
+    if (act & SPOIL_AFTER) {
+      x.fp = x.fm = mk(0.0f, 0.0f);
+    } else if (act & SHIFT_AFTER) {
+      shift_complex<L>(x.fp, x.fm, sub, lane, beyond);
+    }
+  }
+
+  // pass 2: the events last to first. Every event has a duration gradient,
+  // an untimed one too (the reference's convention): the relaxation at
+  // dt = 0 is the identity and its derivative there the generator.
+  mtv::Relax R0;
+  mtv::form_relax(at, 0.0f, R0);
+  Order l;
+  l.fp = l.fm = l.z = l.zb = mk(0.0f, 0.0f);
+  last_dt = -1.0f;
+  for (int64_t e = E - 1; e >= 0; --e) {
+    const EventCode ev = load_event(p.events, e);
+    const uint32_t act = ev.ctrl & 0xFFu;
+    const int32_t kind = (int32_t)(ev.ctrl >> CTRL_KIND_SHIFT);
+    const float* slot = traj + e * 8 * T;
+    Order x0;
+    x0.fp = mk(slot[0], slot[T]);
+    x0.fm = mk(slot[2 * T], slot[3 * T]);
+    x0.z = mk(slot[4 * T], slot[5 * T]);
+    x0.zb = mk(slot[6 * T], slot[7 * T]);
+    // the state the event's action sees
+    Order xa = x0;
+    if (ev.dt != 0.0f) {
+      if (ev.dt != last_dt) {
+        last_dt = ev.dt;
+        mtv::form_relax(at, ev.dt, R);
+      }
+      mtv::relax(xa, R, first);
+    }
+    if (act & PRE_SHIFT) shift_complex<L>(xa.fp, xa.fm, sub, lane, beyond);
+
+    float g_flip = 0.0f, g_phase = 0.0f, g_dt = 0.0f;
+    if (act & SPOIL_AFTER) {
+      l.fp = l.fm = mk(0.0f, 0.0f);
+    } else if (act & SHIFT_AFTER) {
+      mtv::shift_reverse<L>(l, sub, lane, p.states);
+    }
+    if (act & POST_SHIFT) mtv::shift_reverse<L>(l, sub, lane, p.states);
+    if (kind == KIND_RF) {
+      if (act & INVERSION) {
+        l.z = scl(l.z, -eta);
+      } else {
+        const float theta = ev.flip * b1;
+        const float deposit = p.bound_fraction ? p.saturation[e] : 0.0f;
+        if (deposit != 0.0f) {
+          const float g = lineshape_at(p, p.rf_frequency[e] - at.b0);
+          const float w = expf(deposit * theta * theta * g);
+          g_flip += dotr(l.zb, xa.zb) * w * deposit * 2.0f * theta * b1 * g;
+          l.zb = scl(l.zb, w);
+        }
+        float sa, va;
+        if (p.transmit) {
+          const float h = sinf(0.5f * theta);
+          sa = sinf(theta);
+          va = 2.0f * (h * h);
+        } else {
+          sa = ev.sine;
+          va = ev.versine;
+        }
+        const float ca = 1.0f - va;
+        float sp, cp;
+        sincosf(ev.phase + b1_phase, &sp, &cp);
+        const float2 e1p = mk(cp, sp);
+        const float2 e2p = mul(e1p, e1p);
+        const float shh = 0.5f * va, chh = 1.0f - shh;
+        const float2 t01 = scl(e2p, shh), t02 = mul(mk(0.0f, -sa), e1p);
+        const float2 t10 = cj(t01), t12 = mul(mk(0.0f, sa), cj(e1p));
+        const float2 t20 = mul(mk(0.0f, -0.5f * sa), cj(e1p)), t21 = mul(mk(0.0f, 0.5f * sa), e1p);
+        const float2 fp = xa.fp, fm = xa.fm, z = xa.z;
+        // along the angle (per unit angle; the flip's is b1 times it)
+        const float dch = -0.5f * sa, dca = -sa;
+        const float2 d01 = scl(e2p, 0.5f * sa), d02 = mul(mk(0.0f, -ca), e1p);
+        const float2 d10 = cj(d01), d12 = mul(mk(0.0f, ca), cj(e1p));
+        const float2 d20 = mul(mk(0.0f, -0.5f * ca), cj(e1p)), d21 = mul(mk(0.0f, 0.5f * ca), e1p);
+        const float2 ap = mac(mac(scl(fp, dch), d01, fm), d02, z);
+        const float2 am = mac(mac(mul(d10, fp), mk(dch, 0.0f), fm), d12, z);
+        const float2 az = mac(mac(mul(d20, fp), d21, fm), mk(dca, 0.0f), z);
+        g_flip += b1 * (dotr(l.fp, ap) + dotr(l.fm, am) + dotr(l.z, az));
+        // along the phase: e^{i k phi} entries gain the factor i k
+        const float2 pp = add(mul(mk(0.0f, 2.0f), mul(t01, fm)), mul(mk(0.0f, 1.0f), mul(t02, z)));
+        const float2 pm = add(mul(mk(0.0f, -2.0f), mul(t10, fp)), mul(mk(0.0f, -1.0f), mul(t12, z)));
+        const float2 pz = add(mul(mk(0.0f, -1.0f), mul(t20, fp)), mul(mk(0.0f, 1.0f), mul(t21, fm)));
+        g_phase += dotr(l.fp, pp) + dotr(l.fm, pm) + dotr(l.z, pz);
+        // lambda <- T^H lambda
+        const float2 lp = l.fp, lm = l.fm, lz = l.z;
+        l.fp = add(add(scl(lp, chh), mul(cj(t10), lm)), mul(cj(t20), lz));
+        l.fm = add(add(mul(cj(t01), lp), scl(lm, chh)), mul(cj(t21), lz));
+        l.z = add(add(mul(cj(t02), lp), mul(cj(t12), lm)), scl(lz, ca));
+      }
+    } else if (kind == KIND_ADC && (act & RECORD) && ev.out >= 0 && ev.out < p.outputs && first && live) {
+      float sp, cp;
+      sincosf(-ev.phase, &sp, &cp);
+      const float2 c = scl(mul(factor, mk(cp, sp)), m0);
+      const float2 w = q.cotangent[atom * p.outputs + ev.out];
+      const float2 s = mul(c, xa.fp);
+      g_phase += dotr(w, mk(s.y, -s.x));  // ds/dphase = -i s
+      l.fp = add(l.fp, mul(cj(c), w));
+    }
+    if (act & PRE_SHIFT) mtv::shift_reverse<L>(l, sub, lane, p.states);
+    g_dt += mtv::relax_reverse(ev.dt != 0.0f ? R : R0, x0, l, first);
+
+    g_flip *= mask;
+    g_phase *= mask;
+    g_dt *= mask;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      g_flip += __shfl_xor(g_flip, off, 64);
+      g_phase += __shfl_xor(g_phase, off, 64);
+      g_dt += __shfl_xor(g_dt, off, 64);
+    }
+    if (lane == 0) {
+      float* out = q.partial + (wave * E + e) * 3;
+      out[0] = g_flip;
+      out[1] = g_phase;
+      out[2] = g_dt;
+    }
+  }
+}
+
+// Per-event partials summed over the waves in a fixed order, added to the
+// gradients with `accumulate` (a launch after the first).
+__global__ void semisolid_sum_partials(const float* __restrict__ partial, int64_t waves, int64_t events,
+                                       float* __restrict__ grad_flip, float* __restrict__ grad_phase,
+                                       float* __restrict__ grad_duration, int accumulate) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * events) return;
+  float v = 0.0f;
+  for (int64_t w = 0; w < waves; ++w) v += partial[w * 3 * events + i];
+  const int64_t e = i / 3;
+  const int k = (int)(i % 3);
+  float* out = k == 0 ? grad_flip : (k == 1 ? grad_phase : grad_duration);
+  if (out) out[e] = accumulate ? out[e] + v : v;
+}
+
+}  // namespace tsim

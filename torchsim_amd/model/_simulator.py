@@ -27,6 +27,7 @@ from typing import Any
 
 import numpy as np
 import torch
+from torch.autograd import forward_ad
 
 from .. import _native
 from ..sequence import _dephasing, _engine
@@ -441,10 +442,20 @@ class Simulator:
         if profile is not None or tissue.semisolid:
             from ..sequence import _semisolid
 
+            sequence_grad = any(v.requires_grad for v in (packed.flip, packed.phase, packed.duration))
+            sequence_dual = any(forward_ad.unpack_dual(v).tangent is not None
+                                for v in (packed.flip, packed.phase, packed.duration))
+            if (profile is None and sequence_grad and not sequence_dual and not differentiated
+                    and not tangent_fields):
+                # reverse mode to the flips, phases and durations: the semisolid adjoint
+                return _semisolid.sequence_signal(packed, tissue, states=states,
+                                                  carried=tuple(dict.fromkeys(reversed_fields)), factor=factor), None
             if any(sequence_live):
                 raise NotImplementedError(
-                    "through a semisolid pool or a profiled pulse derivatives reach the tissue maps "
-                    f"({', '.join(_semisolid.CARRIED)}); the sequence's flips, phases and durations carry none"
+                    "through a semisolid pool the sequence's flips, phases and durations carry reverse-mode "
+                    "gradients of the signal; forward mode along them, a Jacobian's gradients, and any derivative "
+                    "through a profiled pulse reach only the tissue maps "
+                    f"({', '.join(_semisolid.CARRIED)})"
                 )
             carried = tuple(dict.fromkeys(reversed_fields + tangent_fields))
             if carried:

@@ -18,6 +18,7 @@ import threading
 import torch
 
 from .. import _native
+from . import _complex
 from ._engine import PreparedTissue
 from ._events import PackedEvents
 from ._lineshape import Lineshape, lineshape_reaching
@@ -55,9 +56,13 @@ def simulate_semisolid(
     states: int,
     columns: tuple[str, ...] = (),
     factor: complex = 1.0,
+    flip: torch.Tensor | None = None,
+    phase: torch.Tensor | None = None,
+    duration: torch.Tensor | None = None,
 ) -> tuple[torch.Tensor, torch.Tensor | None]:
     """Complex64 signal (atoms, outputs) and Jacobian (atoms, P, outputs) of a
-    tissue with a semisolid pool; columns among t1_ms, t2_ms, m0, b1."""
+    tissue with a semisolid pool; columns among t1_ms, t2_ms, m0, b1; the
+    events' flips, phases and durations from the given tensors where given."""
     device = tissue.device
     _native.require_device(device)
     for name in columns:
@@ -73,7 +78,7 @@ def simulate_semisolid(
     jacobian = torch.zeros((atoms, len(columns), outputs), dtype=torch.complex64, device=device) if columns else None
     if atoms == 0 or outputs == 0:
         return signal, jacobian
-    events = [b.detach() for b in packed.on(device)]
+    events = _complex._events(packed, device, flip=flip, phase=phase, duration=duration)
     ptrs = _native.pointers([*(None if b is None else b.detach() for b in tissue.buffers), *events])
     cols = (ctypes.c_int32 * max(1, len(columns)))(*[_COLUMN_INDEX[name] for name in columns])
     with torch.cuda.device(device):
@@ -266,3 +271,107 @@ def tangent_of(
     for k, name in enumerate(names):
         tangent = tangent + jacobian[:, k] * tissue_tangents[name].detach().reshape(-1, 1).to(torch.float32)
     return signal, tangent
+
+
+def adjoint_semisolid(
+    packed: PackedEvents,
+    tissue: PreparedTissue,
+    *,
+    states: int,
+    factor: complex,
+    cotangent: torch.Tensor,
+    flip: torch.Tensor | None = None,
+    phase: torch.Tensor | None = None,
+    duration: torch.Tensor | None = None,
+) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Gradients (events,) FP32 of a real loss with respect to every event's
+    flip, phase and duration through the semisolid pool, given the loss's
+    cotangent (atoms, outputs) on the signal ``simulate_semisolid`` returns."""
+    device = tissue.device
+    _native.require_device(device)
+    if tissue.exchanging:
+        raise NotImplementedError("a semisolid pool beside an exchanging pool (three pools) is not built")
+    lib = _native.load()
+    table = absorption(packed, tissue)
+    knots = _resident(table, device)
+    count = packed.event_count
+    grads = [torch.empty(count, dtype=torch.float32, device=device) for _ in range(3)]
+    events = _complex._events(packed, device, flip=flip, phase=phase, duration=duration)
+    ptrs = _native.pointers([*(None if b is None else b.detach() for b in tissue.buffers), *events])
+    weight = cotangent.detach().to(device=device, dtype=torch.complex64).contiguous()
+    with torch.cuda.device(device):
+        status = lib.tsim_semisolid_vjp(
+            ptrs, tissue.atoms, count, states, packed.output_count, tissue.features, tissue.stride,
+            float(complex(factor).real), float(complex(factor).imag), knots.data_ptr(), table.bins, float(table.step),
+            weight.data_ptr(), *(g.data_ptr() for g in grads), _native.stream_of(device),
+        )
+    _native.check(status)
+    return grads[0], grads[1], grads[2]
+
+
+class SequenceSignal(torch.autograd.Function):
+    """The signal of a tissue with a semisolid pool as a function of the
+    events' flips, phases and durations and of the tissue maps it was
+    differentiated along: ``backward`` takes the event gradients from the
+    semisolid adjoint (``tsim_semisolid_vjp``) and the tissue gradients from
+    the Jacobian columns the forward wrote (as ``ColumnSignal``) -- the
+    reference's ``_NativeEpg.backward`` for this pool
+    (sequence/_accelerators.py:1544-1586). The signal and columns come in
+    computed."""
+
+    @staticmethod
+    def forward(flip, phase, duration, signal, jacobian, run, *buffers):
+        return signal.clone()
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        flip, phase, duration, _, jacobian, run, *buffers = inputs
+        ctx.save_for_backward(flip, phase, duration, *([] if jacobian is None else [jacobian]))
+        ctx.run = run
+        ctx.shapes = [None if b is None else b.shape for b in buffers]
+
+    @staticmethod
+    def backward(ctx, grad):
+        run = ctx.run
+        flip, phase, duration, *held = ctx.saved_tensors
+        needs = ctx.needs_input_grad
+        out: list = [None, None, None, None, None, None]
+        if any(needs[:3]):
+            g_flip, g_phase, g_duration = adjoint_semisolid(
+                run.packed, run.tissue, states=run.states, factor=run.factor, cotangent=grad, flip=flip, phase=phase,
+                duration=duration)
+            out[0] = g_flip.to(flip.device, flip.dtype) if needs[0] else None
+            out[1] = g_phase.to(phase.device, phase.dtype) if needs[1] else None
+            out[2] = g_duration.to(duration.device, duration.dtype) if needs[2] else None
+        grads = []
+        for k, (shape, need) in enumerate(zip(ctx.shapes, needs[6:])):
+            if not need or shape is None:
+                grads.append(None)
+                continue
+            g = (grad * held[0][:, k].conj()).real.sum(-1)
+            grads.append(g.reshape(shape) if g.numel() == shape.numel() else g.sum().reshape(shape))
+        return (*out, *grads)
+
+
+def sequence_signal(
+    packed: PackedEvents,
+    tissue: PreparedTissue,
+    *,
+    states: int,
+    carried: tuple[str, ...],
+    factor: complex = 1.0,
+) -> torch.Tensor:
+    """The signal of a tissue with a semisolid pool with reverse-mode
+    gradients reaching the packed flips, phases and durations (and the tissue
+    maps ``carried`` names, among :data:`CARRIED`)."""
+    from types import SimpleNamespace
+
+    refuse_uncarried(carried)
+    device = tissue.device
+    plain = dataclasses.replace(tissue, buffers=[None if b is None else b.detach() for b in tissue.buffers])
+    flip, phase, duration = (packed.flip.to(device), packed.phase.to(device), packed.duration.to(device))
+    signal, jacobian = simulate_semisolid(packed, plain, states=states, columns=carried, factor=factor,
+                                          flip=flip, phase=phase, duration=duration)
+    run = SimpleNamespace(packed=packed, tissue=plain, states=states, factor=factor)
+    buffers = [tissue.buffers[_native.TISSUE_INDEX[name]] for name in carried]
+    return SequenceSignal.apply(flip, phase, duration, signal, jacobian, run, *buffers)
