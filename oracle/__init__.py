@@ -336,6 +336,7 @@ def reference_vjp(
     exchanging: bool = False,
     tangent: dict[str, Any] | None = None,
     threads: int = 8,
+    lineshape: tuple[Any, Any, float] | None = None,
 ) -> dict[str, np.ndarray]:
     """Reverse mode of the reference's own FP32 kernel (built from source).
 
@@ -348,7 +349,11 @@ def reference_vjp(
     forward-over-reverse pass ``_NativeEpgJvp.backward`` takes
     (sequence/_accelerators.py:1787-1850). Returns the 17 tissue gradients
     (atoms,) by name and the event gradients ``flip``, ``phase``,
-    ``duration`` (events,), summed over atoms.
+    ``duration`` (events,), summed over atoms. With ``lineshape`` =
+    (values, slopes, step_hz) the tissue's bound_fraction / bound_exchange /
+    t1_bound form a semisolid pool (pool kind 1, the table in
+    Tail::ABSORPTION) saturated by the events' ``saturation`` and
+    ``rf_frequency``.
     """
     module = reference_module()
     if module is None:
@@ -371,14 +376,21 @@ def reference_vjp(
     event_ptrs = [
         arr(events["duration"], np.float32), arr(events["kind"], np.int32), arr(events["flip"], np.float32),
         arr(events.get("phase", np.zeros(count)), np.float32), arr(events["action"], np.uint8),
-        arr(events["output_index"], np.int32), arr(np.zeros(count), np.int32), arr(np.zeros(count), np.float32),
-        arr(np.zeros(count), np.float32),
+        arr(events["output_index"], np.int32), arr(np.zeros(count), np.int32),
+        arr(events.get("saturation", np.zeros(count)), np.float32),
+        arr(events.get("rf_frequency", np.zeros(count)), np.float32),
     ]
     w = np.asarray(cotangent).reshape(atoms, outputs)
     seed = [f32(w.real.reshape(-1), atoms * outputs).ctypes.data, f32(w.imag.reshape(-1), atoms * outputs).ctypes.data]
     tail = [0] * 13
-    dims = (atoms, 1, count, states, outputs, threads, real_axis, 0.0, 0.0, 1, 1, 0, 1.0, 0, 1.0,
-            2 if exchanging else 0, features, 0, 0, 0)
+    bins, step = 0, 1.0
+    if lineshape is not None:
+        knots = np.stack((np.asarray(lineshape[0]), np.asarray(lineshape[1])), -1)
+        tail[12] = arr(knots, np.float32)
+        bins, step = int(knots.shape[0]), float(lineshape[2])
+    kind = 1 if lineshape is not None else (2 if exchanging else 0)
+    dims = (atoms, 1, count, states, outputs, threads, real_axis, 0.0, 0.0, 1, 1, 0, 1.0, bins, step,
+            kind, features, 0, 0, 0)
 
     def gradient_block(order):
         tissue_grads = [f32(0.0, atoms) for _ in _TISSUE_ORDER]
