@@ -124,6 +124,20 @@ int tsim_semisolid_vjp(void* const* packed, int64_t atom_count, int64_t event_co
                        float* grad_flip, float* grad_phase, float* grad_duration, void* stream);
 
 /*
+ * First-order adjoint with slice-profiled pulses (and a semisolid pool where
+ * its packed buffers are set): as tsim_semisolid_vjp, the pulses turning the
+ * states as for tsim_profiled_jacobian (the reference's simulate_vjp in
+ * RfMode::PROFILED, src/torchsim/_epg_cpu.cpp:11310-11340). Atoms and the
+ * cotangent are voxel-major with `locations` atoms per voxel, each
+ * position's cotangent the voxel's over `locations` (the caller's average).
+ */
+int tsim_profiled_vjp(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
+                      int64_t output_count, int features, int64_t tissue_stride, float factor_re, float factor_im,
+                      const float* profile, int64_t locations, int64_t profile_bins, float profile_step,
+                      const float* lineshape, int64_t lineshape_bins, float lineshape_step, const void* cotangent,
+                      float* grad_flip, float* grad_phase, float* grad_duration, void* stream);
+
+/*
  * Signal and Jacobian with slice-profiled pulses: every pulse but an
  * inversion turns the states through the Cayley-Klein pair (a, b) its shape
  * performs at the atom's slice position and effective flip flip * b1, read

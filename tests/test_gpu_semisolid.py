@@ -463,3 +463,96 @@ def test_loss_backward_reaches_the_protocol_through_the_pool(name):
         err = np.abs(maps[key].grad.cpu().numpy() - maps64[key].grad.numpy()).max() / np.abs(maps64[key].grad.numpy()).max()
         print(f"{name} d/d{key}: {err:.2e}")
         assert err < JTOL, key
+
+
+@pytest.mark.parametrize("bound", [False, True], ids=["free", "semisolid"])
+def test_the_profiled_adjoint_against_fp64_autograd_per_event(bound):
+    """tsim_profiled_vjp: a windowed sinc worked out at 5 slice positions on
+    a phase-cycled MRF train with a B1 map (and a semisolid pool at 300x
+    saturation): every event's flip, phase and duration gradient of the
+    slice-averaged loss against FP64 autograd of the oracle on the same
+    table."""
+    import dataclasses as dc
+
+    from torchsim_amd.sequence import _engine, _semisolid
+    from torchsim_amd.sequence._lineshape import lineshape_table
+    from torchsim_amd.sequence._transition import sinc_pulse, transition_table
+    from torchsim_amd.simulators import MRFSimulator
+
+    voxels, pulses, states, positions = 8, 50, 10, 5
+    rng = np.random.default_rng(21)
+    packed = MRFSimulator(states=states).packed(flip=torch.tensor(rng.uniform(5.0, 60.0, pulses), dtype=torch.float32),
+                                                 TR=torch.tensor(rng.uniform(8.0, 15.0, pulses), dtype=torch.float32),
+                                                 phases=torch.tensor(rng.uniform(0, 360, pulses), dtype=torch.float32))
+    if bound:
+        packed = dc.replace(packed, saturation=packed.saturation * 300.0)
+    table = transition_table(sinc_pulse(2e-3, 4.0, raster_s=1e-5), np.linspace(-1.0, 1.0, positions), bins=32)
+    tissue = _pool_tissue(voxels, 22)
+    tissue.pop("b0")
+    if not bound:
+        for key in ("bound_fraction", "bound_exchange", "t1_bound"):
+            tissue.pop(key)
+    names = {"t1": "t1_ms", "t2": "t2_ms", "m0": "m0", "b1": "b1", "bound_fraction": "bound_fraction",
+             "bound_exchange": "bound_exchange_rate", "t1_bound": "t1_bound_ms"}
+    prepared = _engine.prepare_tissue({names[k]: torch.tensor(v, dtype=torch.float32) for k, v in tissue.items()},
+                                      torch.device("cuda", 0), ())
+    weight = rng.normal(size=(voxels, packed.output_count)) + 1j * rng.normal(size=(voxels, packed.output_count))
+    signal, _ = _semisolid.simulate_profiled(packed, prepared, table, states=states, factor=1j)
+    grads = _semisolid.adjoint_semisolid(packed, prepared, states=states, factor=1j, table=table,
+                                         cotangent=torch.tensor(weight, dtype=torch.complex64).cuda())
+    events = oracle.events_of(packed)
+    leaves = {k: torch.tensor(events[k], dtype=torch.float64, requires_grad=True) for k in ("flip", "phase", "duration")}
+    given = dict(events)
+    given.update(leaves)
+    rows = {k: torch.tensor(np.repeat(v, positions), dtype=torch.float64) for k, v in tissue.items()}
+    ls = lineshape_table()
+    profile = (table.a.numpy(), table.b.numpy(), table.slope_a.numpy(), table.slope_b.numpy(), table.theta_max)
+    out = 1j * P.simulate(given, rows, states=states, outputs=packed.output_count, profile=profile,
+                          lineshape=(ls.values, ls.slopes, ls.step) if bound else None)
+    ref = out.reshape(voxels, positions, -1).mean(1)
+    error = np.abs(signal.cpu().numpy() - ref.detach().numpy()).max() / np.abs(ref.detach().numpy()).max()
+    print(f"profiled signal: {error:.2e}")
+    assert error < 1e-5
+    (ref * torch.tensor(weight).conj()).real.sum().backward()
+    for name, got in zip(("flip", "phase", "duration"), grads):
+        want = leaves[name].grad.numpy()
+        err = np.abs(got.cpu().numpy() - want).max() / np.abs(want).max()
+        print(f"profiled d/d{name}: {err:.2e}")
+        assert err < SEQ_TOL, name
+
+
+def test_loss_backward_reaches_a_profiled_protocol():
+    """loss.backward() through MRFSimulator(pulse=, across_slice=) reaches the
+    flips and TRs, against FP64 autograd of the oracle on the same table."""
+    from torchsim_amd.sequence._transition import exact_slice_profile, sinc_pulse, transition_table
+    from torchsim_amd.simulators import MRFSimulator
+
+    rng = np.random.default_rng(31)
+    pulses, voxels = 40, 6
+    values = {"flip": rng.uniform(5.0, 60.0, pulses), "TR": rng.uniform(8.0, 15.0, pulses)}
+    pulse, where = sinc_pulse(2e-3, 4.0, raster_s=1e-5), exact_slice_profile(5, bins=32)
+    t1, t2 = _cases.random_tissue(voxels, seed=32)
+    ours = {k: torch.tensor(v, dtype=torch.float32, requires_grad=True) for k, v in values.items()}
+    sim = MRFSimulator(states=10, pulse=pulse, across_slice=where, **ours)
+    sig = sim.simulate(T1=torch.tensor(t1).cuda(), T2=torch.tensor(t2).cuda())
+    weight = rng.normal(size=tuple(sig.shape)) + 1j * rng.normal(size=tuple(sig.shape))
+    (sig * torch.tensor(weight, dtype=torch.complex64).cuda().conj()).real.sum().backward()
+
+    table = transition_table(pulse, where.positions(), bins=where.bins, theta_max=where.theta_max)
+    theirs = {k: torch.tensor(v, dtype=torch.float64, requires_grad=True) for k, v in values.items()}
+    sim64 = MRFSimulator(states=10, pulse=pulse, across_slice=where, **theirs)
+    packed = sim64.packed(**sim64.bound)
+    events = oracle.events_of(packed)
+    events.update(duration=packed.duration.double(), flip=packed.flip.double(), phase=packed.phase.double())
+    positions = table.points
+    rows = {"t1": torch.tensor(np.repeat(t1, positions), dtype=torch.float64),
+            "t2": torch.tensor(np.repeat(t2, positions), dtype=torch.float64)}
+    profile = (table.a.numpy(), table.b.numpy(), table.slope_a.numpy(), table.slope_b.numpy(), table.theta_max)
+    out = complex(sim64.signal_factor) * P.simulate(events, rows, states=10, outputs=packed.output_count,
+                                                   profile=profile)
+    ref = out.reshape(voxels, positions, -1).mean(1)
+    (ref * torch.tensor(weight).conj()).real.sum().backward()
+    for key in values:
+        err = np.abs(ours[key].grad.numpy() - theirs[key].grad.numpy()).max() / np.abs(theirs[key].grad.numpy()).max()
+        print(f"profiled protocol d/d{key}: {err:.2e}")
+        assert err < SEQ_TOL, key

@@ -238,16 +238,16 @@ namespace {
 
 using SemisolidVjpLaunch = void (*)(const SemisolidVjpArgs&, dim3, dim3, hipStream_t);
 
-template <int L>
+template <int L, bool PROFILED>
 void launch_semisolid_vjp(const SemisolidVjpArgs& a, dim3 grid, dim3 block, hipStream_t s) {
-  hipLaunchKernelGGL((semisolid_vjp_kernel<L>), grid, block, 0, s, a);
+  hipLaunchKernelGGL((semisolid_vjp_kernel<L, PROFILED>), grid, block, 0, s, a);
 }
 
-SemisolidVjpLaunch pick_semisolid_vjp(int lanes) {
+SemisolidVjpLaunch pick_semisolid_vjp(int lanes, bool profiled) {
   switch (lanes) {
 #define TSIM_SEMISOLID_VJP_CASE(LANES) \
   case LANES:                          \
-    return &launch_semisolid_vjp<LANES>;
+    return profiled ? &launch_semisolid_vjp<LANES, true> : &launch_semisolid_vjp<LANES, false>;
     TSIM_COMPLEX_LANES(TSIM_SEMISOLID_VJP_CASE)
 #undef TSIM_SEMISOLID_VJP_CASE
     default:
@@ -261,16 +261,19 @@ constexpr int64_t kTrajectoryBytes = 1LL << 30;
 }  // namespace
 }  // namespace tsim
 
-extern "C" int tsim_semisolid_vjp(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
-                                  int64_t output_count, int features, int64_t tissue_stride, float factor_re,
-                                  float factor_im, const float* lineshape, int64_t lineshape_bins,
-                                  float lineshape_step, const void* cotangent, float* grad_flip, float* grad_phase,
-                                  float* grad_duration, void* stream) {
+namespace tsim {
+namespace {
+
+int run_vjp(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count, int64_t output_count,
+            int features, int64_t tissue_stride, float factor_re, float factor_im, bool bound, const float* lineshape,
+            int64_t lineshape_bins, float lineshape_step, const float* profile, int64_t locations,
+            int64_t profile_bins, float profile_step, const void* cotangent, float* grad_flip, float* grad_phase,
+            float* grad_duration, void* stream) {
   SemisolidArgs a;
   int tm = 0;
   int st = fill_shaped(packed, atom_count, event_count, state_count, output_count, features, tissue_stride, nullptr,
-                       0, factor_re, factor_im, true, lineshape, lineshape_bins, lineshape_step, nullptr, 1, 0, 1.0f,
-                       nullptr, nullptr, a, tm);
+                       0, factor_re, factor_im, bound, lineshape, lineshape_bins, lineshape_step, profile, locations,
+                       profile_bins, profile_step, nullptr, nullptr, a, tm);
   if (st != OK) return st;
   if (event_count > 0 && (grad_flip == nullptr || grad_phase == nullptr || grad_duration == nullptr)) {
     set_error("the flip, phase and duration gradient outputs are required");
@@ -290,7 +293,7 @@ extern "C" int tsim_semisolid_vjp(void* const* packed, int64_t atom_count, int64
     return hip_status(hipGetLastError(), "semisolid gradient clear");
   }
   const int L = semisolid_lanes(state_count);
-  SemisolidVjpLaunch launch = pick_semisolid_vjp(L);
+  SemisolidVjpLaunch launch = pick_semisolid_vjp(L, profile != nullptr);
   if (!launch) {
     set_error("no adjoint kernel for %lld orders", (long long)state_count);
     return UNSUPPORTED;
@@ -342,4 +345,33 @@ extern "C" int tsim_semisolid_vjp(void* const* packed, int64_t atom_count, int64
   (void)hipFreeAsync(partial, s);
   (void)hipFreeAsync(trajectory, s);
   return st;
+}
+
+}  // namespace
+}  // namespace tsim
+
+extern "C" int tsim_semisolid_vjp(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
+                                  int64_t output_count, int features, int64_t tissue_stride, float factor_re,
+                                  float factor_im, const float* lineshape, int64_t lineshape_bins,
+                                  float lineshape_step, const void* cotangent, float* grad_flip, float* grad_phase,
+                                  float* grad_duration, void* stream) {
+  return run_vjp(packed, atom_count, event_count, state_count, output_count, features, tissue_stride, factor_re,
+                 factor_im, true, lineshape, lineshape_bins, lineshape_step, nullptr, 1, 0, 1.0f, cotangent, grad_flip,
+                 grad_phase, grad_duration, stream);
+}
+
+extern "C" int tsim_profiled_vjp(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
+                                 int64_t output_count, int features, int64_t tissue_stride, float factor_re,
+                                 float factor_im, const float* profile, int64_t locations, int64_t profile_bins,
+                                 float profile_step, const float* lineshape, int64_t lineshape_bins,
+                                 float lineshape_step, const void* cotangent, float* grad_flip, float* grad_phase,
+                                 float* grad_duration, void* stream) {
+  if (profile == nullptr) {
+    set_error("the profile table is required");
+    return BAD_ARGUMENT;
+  }
+  const bool bound = packed != nullptr && packed[T_BOUND_FRACTION] != nullptr;
+  return run_vjp(packed, atom_count, event_count, state_count, output_count, features, tissue_stride, factor_re,
+                 factor_im, bound, lineshape, lineshape_bins, lineshape_step, profile, locations, profile_bins,
+                 profile_step, cotangent, grad_flip, grad_phase, grad_duration, stream);
 }

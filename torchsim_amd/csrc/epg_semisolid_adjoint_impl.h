@@ -1,10 +1,10 @@
-// First-order adjoint of the semisolid-pool kernel (instantaneous pulses),
+// First-order adjoint of the semisolid-pool / slice-profiled-pulse kernel,
 // for gfx950: the gradient of a real loss with respect to every event's flip
 // angle, phase and duration, given the loss's cotangent on the recorded
-// samples (the reference's simulate_vjp_range<INSTANT, SEMISOLID>,
-// src/torchsim/_epg_cpu.cpp:4839-5944 with pool kind 1 and the dispatch of
-// :11316-11340, for the event gradients `_NativeEpg.backward` hands on,
-// sequence/_accelerators.py:1544-1586).
+// samples (the reference's simulate_vjp_range<INSTANT | PROFILED, SEMISOLID>
+// and simulate_vjp_single_pool<PROFILED>, src/torchsim/_epg_cpu.cpp:
+// 4839-5944 and the dispatch of :11310-11340, for the event gradients
+// `_NativeEpg.backward` hands on, sequence/_accelerators.py:1544-1586).
 //
 // Layout as the forward: one configuration order per lane, an atom on L
 // adjacent lanes. Pass 1 runs the events forward and stores every event's
@@ -137,9 +137,93 @@ __device__ __forceinline__ void shift_reverse(Order& l, int sub, int lane, int s
   if (sub >= states) l.fm = mk(0.0f, 0.0f);
 }
 
+// A pulse's T-matrix on (F+, F-, Z) and its derivative along the effective
+// flip theta: rows [[t00, t01, t02], [t10, t11, t12], [t20, t21, t22]]
+// (t22 real), for an instantaneous pulse from (sin, 1 - cos) of theta and the
+// phase, for a profiled one from the Cayley-Klein pair its table holds at
+// the atom's position (reference rotate / rotate_spinor).
+struct Pulse {
+  float2 t00, t01, t02, t10, t11, t12, t20, t21;
+  float t22;
+  float2 u00, u01, u02, u10, u11, u12, u20, u21;
+  float u22;
+};
+
+template <bool PROFILED>
+__device__ __forceinline__ void form_pulse(const SemisolidArgs& p, const EventCode& ev, int64_t atom, float theta,
+                                           float phase, int transmit, Pulse& T) {
+  if constexpr (PROFILED) {
+    float2 a, bz, da, dbz;
+    profile_pair(p, atom % p.locations, theta, a, bz, da, dbz);
+    float sp, cp;
+    sincosf(-phase, &sp, &cp);
+    const float2 spin = mk(cp, sp);
+    const float2 b = mul(bz, spin), db = mul(dbz, spin);
+    const float2 ca = cj(a), cb = cj(b), cda = cj(da), cdb = cj(db);
+    T.t00 = mul(ca, ca);
+    T.t01 = neg(mul(cb, cb));
+    T.t02 = scl(cj(mul(a, b)), -2.0f);
+    T.t10 = neg(mul(b, b));
+    T.t11 = mul(a, a);
+    T.t12 = scl(mul(a, b), -2.0f);
+    T.t20 = mul(ca, b);
+    T.t21 = mul(a, cb);
+    T.t22 = (a.x * a.x + a.y * a.y) - (b.x * b.x + b.y * b.y);
+    const float2 ab = add(mul(da, b), mul(a, db));
+    T.u00 = scl(mul(ca, cda), 2.0f);
+    T.u01 = scl(mul(cb, cdb), -2.0f);
+    T.u02 = scl(cj(ab), -2.0f);
+    T.u10 = scl(mul(b, db), -2.0f);
+    T.u11 = scl(mul(a, da), 2.0f);
+    T.u12 = scl(ab, -2.0f);
+    T.u20 = add(mul(cda, b), mul(ca, db));
+    T.u21 = add(mul(da, cb), mul(a, cdb));
+    T.u22 = 2.0f * (a.x * da.x + a.y * da.y) - 2.0f * (b.x * db.x + b.y * db.y);
+  } else {
+    float sa, va;
+    if (transmit) {
+      const float h = sinf(0.5f * theta);
+      sa = sinf(theta);
+      va = 2.0f * (h * h);
+    } else {
+      sa = ev.sine;
+      va = ev.versine;
+    }
+    const float ca = 1.0f - va;
+    float sp, cp;
+    sincosf(phase, &sp, &cp);
+    const float2 e1p = mk(cp, sp);
+    const float2 e2p = mul(e1p, e1p);
+    const float shh = 0.5f * va, chh = 1.0f - shh;
+    T.t00 = T.t11 = mk(chh, 0.0f);
+    T.t01 = scl(e2p, shh);
+    T.t02 = mul(mk(0.0f, -sa), e1p);
+    T.t10 = cj(T.t01);
+    T.t12 = mul(mk(0.0f, sa), cj(e1p));
+    T.t20 = mul(mk(0.0f, -0.5f * sa), cj(e1p));
+    T.t21 = mul(mk(0.0f, 0.5f * sa), e1p);
+    T.t22 = ca;
+    T.u00 = T.u11 = mk(-0.5f * sa, 0.0f);
+    T.u01 = scl(e2p, 0.5f * sa);
+    T.u02 = mul(mk(0.0f, -ca), e1p);
+    T.u10 = cj(T.u01);
+    T.u12 = mul(mk(0.0f, ca), cj(e1p));
+    T.u20 = mul(mk(0.0f, -0.5f * ca), cj(e1p));
+    T.u21 = mul(mk(0.0f, 0.5f * ca), e1p);
+    T.u22 = -sa;
+  }
+}
+
+__device__ __forceinline__ void rotate(Order& x, const Pulse& T) {
+  const float2 fp = x.fp, fm = x.fm, z = x.z;
+  x.fp = add(add(mul(T.t00, fp), mul(T.t01, fm)), mul(T.t02, z));
+  x.fm = add(add(mul(T.t10, fp), mul(T.t11, fm)), mul(T.t12, z));
+  x.z = add(add(mul(T.t20, fp), mul(T.t21, fm)), scl(z, T.t22));
+}
+
 }  // namespace mtv
 
-template <int L>
+template <int L, bool PROFILED>
 __global__ void __launch_bounds__(64) semisolid_vjp_kernel(const SemisolidVjpArgs q) {
   using namespace cx;
   using mt::Order;
@@ -219,28 +303,9 @@ __global__ void __launch_bounds__(64) semisolid_vjp_kernel(const SemisolidVjpArg
         const float theta = ev.flip * b1;
         const float deposit = p.bound_fraction ? p.saturation[e] : 0.0f;
         if (deposit != 0.0f) x.zb = scl(x.zb, expf(deposit * theta * theta * lineshape_at(p, p.rf_frequency[e] - at.b0)));
-        float sa, va;
-        if (p.transmit) {
-          const float h = sinf(0.5f * theta);
-          sa = sinf(theta);
-          va = 2.0f * (h * h);
-        } else {
-          sa = ev.sine;
-          va = ev.versine;
-        }
-        const float ca = 1.0f - va;
-        float sp, cp;
-        sincosf(ev.phase + b1_phase, &sp, &cp);
-        const float2 e1p = mk(cp, sp);
-        const float2 e2p = mul(e1p, e1p);
-        const float shh = 0.5f * va, chh = 1.0f - shh;
-        const float2 t01 = scl(e2p, shh), t02 = mul(mk(0.0f, -sa), e1p);
-        const float2 t10 = cj(t01), t12 = mul(mk(0.0f, sa), cj(e1p));
-        const float2 t20 = mul(mk(0.0f, -0.5f * sa), cj(e1p)), t21 = mul(mk(0.0f, 0.5f * sa), e1p);
-        const float2 fp = x.fp, fm = x.fm, z = x.z;
-        x.fp = mac(mac(scl(fp, chh), t01, fm), t02, z);
-        x.fm = mac(mac(mul(t10, fp), mk(chh, 0.0f), fm), t12, z);
-        x.z = mac(mac(mul(t20, fp), t21, fm), mk(ca, 0.0f), z);
+        mtv::Pulse P;
+        mtv::form_pulse<PROFILED>(p, ev, atom, theta, ev.phase + b1_phase, p.transmit, P);
+        mtv::rotate(x, P);
       }
     }
     if (act & POST_SHIFT) shift_complex<L>(x.fp, x.fm, sub, lane, beyond);
@@ -299,44 +364,25 @@ __global__ void __launch_bounds__(64) semisolid_vjp_kernel(const SemisolidVjpArg
           g_flip += dotr(l.zb, xa.zb) * w * deposit * 2.0f * theta * b1 * g;
           l.zb = scl(l.zb, w);
         }
-        float sa, va;
-        if (p.transmit) {
-          const float h = sinf(0.5f * theta);
-          sa = sinf(theta);
-          va = 2.0f * (h * h);
-        } else {
-          sa = ev.sine;
-          va = ev.versine;
-        }
-        const float ca = 1.0f - va;
-        float sp, cp;
-        sincosf(ev.phase + b1_phase, &sp, &cp);
-        const float2 e1p = mk(cp, sp);
-        const float2 e2p = mul(e1p, e1p);
-        const float shh = 0.5f * va, chh = 1.0f - shh;
-        const float2 t01 = scl(e2p, shh), t02 = mul(mk(0.0f, -sa), e1p);
-        const float2 t10 = cj(t01), t12 = mul(mk(0.0f, sa), cj(e1p));
-        const float2 t20 = mul(mk(0.0f, -0.5f * sa), cj(e1p)), t21 = mul(mk(0.0f, 0.5f * sa), e1p);
+        mtv::Pulse P;
+        mtv::form_pulse<PROFILED>(p, ev, atom, theta, ev.phase + b1_phase, p.transmit, P);
         const float2 fp = xa.fp, fm = xa.fm, z = xa.z;
-        // along the angle (per unit angle; the flip's is b1 times it)
-        const float dch = -0.5f * sa, dca = -sa;
-        const float2 d01 = scl(e2p, 0.5f * sa), d02 = mul(mk(0.0f, -ca), e1p);
-        const float2 d10 = cj(d01), d12 = mul(mk(0.0f, ca), cj(e1p));
-        const float2 d20 = mul(mk(0.0f, -0.5f * ca), cj(e1p)), d21 = mul(mk(0.0f, 0.5f * ca), e1p);
-        const float2 ap = mac(mac(scl(fp, dch), d01, fm), d02, z);
-        const float2 am = mac(mac(mul(d10, fp), mk(dch, 0.0f), fm), d12, z);
-        const float2 az = mac(mac(mul(d20, fp), d21, fm), mk(dca, 0.0f), z);
+        // along the effective flip (the flip's is b1 times it)
+        const float2 ap = add(add(mul(P.u00, fp), mul(P.u01, fm)), mul(P.u02, z));
+        const float2 am = add(add(mul(P.u10, fp), mul(P.u11, fm)), mul(P.u12, z));
+        const float2 az = add(add(mul(P.u20, fp), mul(P.u21, fm)), scl(z, P.u22));
         g_flip += b1 * (dotr(l.fp, ap) + dotr(l.fm, am) + dotr(l.z, az));
-        // along the phase: e^{i k phi} entries gain the factor i k
-        const float2 pp = add(mul(mk(0.0f, 2.0f), mul(t01, fm)), mul(mk(0.0f, 1.0f), mul(t02, z)));
-        const float2 pm = add(mul(mk(0.0f, -2.0f), mul(t10, fp)), mul(mk(0.0f, -1.0f), mul(t12, z)));
-        const float2 pz = add(mul(mk(0.0f, -1.0f), mul(t20, fp)), mul(mk(0.0f, 1.0f), mul(t21, fm)));
+        // along the phase: the entries carry e^{i k phi} (k = 2, 1, -2, -1,
+        // -1, 1 off the diagonal) and gain the factor i k
+        const float2 pp = add(mul(mk(0.0f, 2.0f), mul(P.t01, fm)), mul(mk(0.0f, 1.0f), mul(P.t02, z)));
+        const float2 pm = add(mul(mk(0.0f, -2.0f), mul(P.t10, fp)), mul(mk(0.0f, -1.0f), mul(P.t12, z)));
+        const float2 pz = add(mul(mk(0.0f, -1.0f), mul(P.t20, fp)), mul(mk(0.0f, 1.0f), mul(P.t21, fm)));
         g_phase += dotr(l.fp, pp) + dotr(l.fm, pm) + dotr(l.z, pz);
         // lambda <- T^H lambda
         const float2 lp = l.fp, lm = l.fm, lz = l.z;
-        l.fp = add(add(scl(lp, chh), mul(cj(t10), lm)), mul(cj(t20), lz));
-        l.fm = add(add(mul(cj(t01), lp), scl(lm, chh)), mul(cj(t21), lz));
-        l.z = add(add(mul(cj(t02), lp), mul(cj(t12), lm)), scl(lz, ca));
+        l.fp = add(add(mul(cj(P.t00), lp), mul(cj(P.t10), lm)), mul(cj(P.t20), lz));
+        l.fm = add(add(mul(cj(P.t01), lp), mul(cj(P.t11), lm)), mul(cj(P.t21), lz));
+        l.z = add(add(mul(cj(P.t02), lp), mul(cj(P.t12), lm)), scl(lz, P.t22));
       }
     } else if (kind == KIND_ADC && (act & RECORD) && ev.out >= 0 && ev.out < p.outputs && first && live) {
       float sp, cp;

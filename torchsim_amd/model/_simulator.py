@@ -445,17 +445,15 @@ class Simulator:
             sequence_grad = any(v.requires_grad for v in (packed.flip, packed.phase, packed.duration))
             sequence_dual = any(forward_ad.unpack_dual(v).tangent is not None
                                 for v in (packed.flip, packed.phase, packed.duration))
-            if (profile is None and sequence_grad and not sequence_dual and not differentiated
-                    and not tangent_fields):
-                # reverse mode to the flips, phases and durations: the semisolid adjoint
-                return _semisolid.sequence_signal(packed, tissue, states=states,
+            if sequence_grad and not sequence_dual and not differentiated and not tangent_fields:
+                # reverse mode to the flips, phases and durations: the adjoint of this kernel
+                return _semisolid.sequence_signal(packed, tissue, profile, states=states,
                                                   carried=tuple(dict.fromkeys(reversed_fields)), factor=factor), None
             if any(sequence_live):
                 raise NotImplementedError(
-                    "through a semisolid pool the sequence's flips, phases and durations carry reverse-mode "
-                    "gradients of the signal; forward mode along them, a Jacobian's gradients, and any derivative "
-                    "through a profiled pulse reach only the tissue maps "
-                    f"({', '.join(_semisolid.CARRIED)})"
+                    "through a semisolid pool or profiled pulses the sequence's flips, phases and durations carry "
+                    "reverse-mode gradients of the signal; forward mode along them and a Jacobian's gradients reach "
+                    f"only the tissue maps ({', '.join(_semisolid.CARRIED)})"
                 )
             carried = tuple(dict.fromkeys(reversed_fields + tangent_fields))
             if carried:

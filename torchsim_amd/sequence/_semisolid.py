@@ -111,6 +111,9 @@ def simulate_profiled(
     states: int,
     columns: tuple[str, ...] = (),
     factor: complex = 1.0,
+    flip: torch.Tensor | None = None,
+    phase: torch.Tensor | None = None,
+    duration: torch.Tensor | None = None,
 ) -> tuple[torch.Tensor, torch.Tensor | None]:
     """Signal (atoms, outputs) and Jacobian (atoms, P, outputs), complex64,
     with every pulse but an inversion played as the shaped pulse tabulated in
@@ -136,7 +139,7 @@ def simulate_profiled(
         voxels = tissue.atoms
         return (signal.reshape(voxels, locations, outputs).mean(1),
                 None if jacobian is None else jacobian.reshape(voxels, locations, len(columns), outputs).mean(1))
-    events = [b.detach() for b in packed.on(device)]
+    events = _complex._events(packed, device, flip=flip, phase=phase, duration=duration)
     ptrs = _native.pointers([*(None if b is None else b.detach() for b in spread.buffers), *events])
     cols = (ctypes.c_int32 * max(1, len(columns)))(*[_COLUMN_INDEX[name] for name in columns])
     with torch.cuda.device(device):
@@ -163,12 +166,16 @@ def simulate_shaped(
     states: int,
     columns: tuple[str, ...] = (),
     factor: complex = 1.0,
+    flip: torch.Tensor | None = None,
+    phase: torch.Tensor | None = None,
+    duration: torch.Tensor | None = None,
 ) -> tuple[torch.Tensor, torch.Tensor | None]:
     """:func:`simulate_profiled` where ``table`` names a shaped pulse, else
     :func:`simulate_semisolid`."""
+    events = {"flip": flip, "phase": phase, "duration": duration}
     if table is not None:
-        return simulate_profiled(packed, tissue, table, states=states, columns=columns, factor=factor)
-    return simulate_semisolid(packed, tissue, states=states, columns=columns, factor=factor)
+        return simulate_profiled(packed, tissue, table, states=states, columns=columns, factor=factor, **events)
+    return simulate_semisolid(packed, tissue, states=states, columns=columns, factor=factor, **events)
 
 
 def refuse_uncarried(names) -> None:
@@ -283,41 +290,57 @@ def adjoint_semisolid(
     flip: torch.Tensor | None = None,
     phase: torch.Tensor | None = None,
     duration: torch.Tensor | None = None,
+    table: TransitionTable | None = None,
 ) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
     """Gradients (events,) FP32 of a real loss with respect to every event's
-    flip, phase and duration through the semisolid pool, given the loss's
-    cotangent (atoms, outputs) on the signal ``simulate_semisolid`` returns."""
+    flip, phase and duration through the semisolid pool and/or slice-profiled
+    pulses (``table``), given the loss's cotangent (voxels, outputs) on the
+    signal ``simulate_shaped`` returns."""
     device = tissue.device
     _native.require_device(device)
     if tissue.exchanging:
         raise NotImplementedError("a semisolid pool beside an exchanging pool (three pools) is not built")
     lib = _native.load()
-    table = absorption(packed, tissue)
-    knots = _resident(table, device)
+    lineshape = absorption(packed, tissue) if tissue.semisolid else None
+    knots = _resident(lineshape, device) if lineshape is not None else None
     count = packed.event_count
     grads = [torch.empty(count, dtype=torch.float32, device=device) for _ in range(3)]
     events = _complex._events(packed, device, flip=flip, phase=phase, duration=duration)
-    ptrs = _native.pointers([*(None if b is None else b.detach() for b in tissue.buffers), *events])
-    weight = cotangent.detach().to(device=device, dtype=torch.complex64).contiguous()
+    weight = cotangent.detach().to(device=device, dtype=torch.complex64)
+    factor_re, factor_im = float(complex(factor).real), float(complex(factor).imag)
+    shape_args = (None if knots is None else knots.data_ptr(), 0 if lineshape is None else lineshape.bins,
+                  1.0 if lineshape is None else float(lineshape.step))
     with torch.cuda.device(device):
-        status = lib.tsim_semisolid_vjp(
-            ptrs, tissue.atoms, count, states, packed.output_count, tissue.features, tissue.stride,
-            float(complex(factor).real), float(complex(factor).imag), knots.data_ptr(), table.bins, float(table.step),
-            weight.data_ptr(), *(g.data_ptr() for g in grads), _native.stream_of(device),
-        )
+        if table is None:
+            ptrs = _native.pointers([*(None if b is None else b.detach() for b in tissue.buffers), *events])
+            weight = weight.contiguous()
+            status = lib.tsim_semisolid_vjp(
+                ptrs, tissue.atoms, count, states, packed.output_count, tissue.features, tissue.stride, factor_re,
+                factor_im, *shape_args, weight.data_ptr(), *(g.data_ptr() for g in grads), _native.stream_of(device))
+        else:
+            # every voxel at every position; each position's cotangent the voxel's over their count
+            locations = table.points
+            spread = _across(tissue, locations)
+            weight = (weight / locations).repeat_interleave(locations, dim=0).contiguous()
+            profile = table.packed(device)
+            ptrs = _native.pointers([*(None if b is None else b.detach() for b in spread.buffers), *events])
+            status = lib.tsim_profiled_vjp(
+                ptrs, spread.atoms, count, states, packed.output_count, spread.features, spread.stride, factor_re,
+                factor_im, profile.data_ptr(), locations, table.bins, float(table.step), *shape_args,
+                weight.data_ptr(), *(g.data_ptr() for g in grads), _native.stream_of(device))
     _native.check(status)
     return grads[0], grads[1], grads[2]
 
 
 class SequenceSignal(torch.autograd.Function):
-    """The signal of a tissue with a semisolid pool as a function of the
-    events' flips, phases and durations and of the tissue maps it was
-    differentiated along: ``backward`` takes the event gradients from the
-    semisolid adjoint (``tsim_semisolid_vjp``) and the tissue gradients from
-    the Jacobian columns the forward wrote (as ``ColumnSignal``) -- the
-    reference's ``_NativeEpg.backward`` for this pool
-    (sequence/_accelerators.py:1544-1586). The signal and columns come in
-    computed."""
+    """The signal of a tissue with a semisolid pool and/or slice-profiled
+    pulses as a function of the events' flips, phases and durations and of
+    the tissue maps it was differentiated along: ``backward`` takes the event
+    gradients from the adjoint (``tsim_semisolid_vjp`` / ``tsim_profiled_vjp``)
+    and the tissue gradients from the Jacobian columns the forward wrote (as
+    ``ColumnSignal``) -- the reference's ``_NativeEpg.backward`` for these
+    models (sequence/_accelerators.py:1544-1586). The signal and columns come
+    in computed."""
 
     @staticmethod
     def forward(flip, phase, duration, signal, jacobian, run, *buffers):
@@ -339,7 +362,7 @@ class SequenceSignal(torch.autograd.Function):
         if any(needs[:3]):
             g_flip, g_phase, g_duration = adjoint_semisolid(
                 run.packed, run.tissue, states=run.states, factor=run.factor, cotangent=grad, flip=flip, phase=phase,
-                duration=duration)
+                duration=duration, table=run.table)
             out[0] = g_flip.to(flip.device, flip.dtype) if needs[0] else None
             out[1] = g_phase.to(phase.device, phase.dtype) if needs[1] else None
             out[2] = g_duration.to(duration.device, duration.dtype) if needs[2] else None
@@ -356,22 +379,24 @@ class SequenceSignal(torch.autograd.Function):
 def sequence_signal(
     packed: PackedEvents,
     tissue: PreparedTissue,
+    table: TransitionTable | None,
     *,
     states: int,
     carried: tuple[str, ...],
     factor: complex = 1.0,
 ) -> torch.Tensor:
-    """The signal of a tissue with a semisolid pool with reverse-mode
-    gradients reaching the packed flips, phases and durations (and the tissue
-    maps ``carried`` names, among :data:`CARRIED`)."""
+    """The signal of a tissue with a semisolid pool and/or slice-profiled
+    pulses (``table``) with reverse-mode gradients reaching the packed flips,
+    phases and durations (and the tissue maps ``carried`` names, among
+    :data:`CARRIED`)."""
     from types import SimpleNamespace
 
     refuse_uncarried(carried)
     device = tissue.device
     plain = dataclasses.replace(tissue, buffers=[None if b is None else b.detach() for b in tissue.buffers])
     flip, phase, duration = (packed.flip.to(device), packed.phase.to(device), packed.duration.to(device))
-    signal, jacobian = simulate_semisolid(packed, plain, states=states, columns=carried, factor=factor,
-                                          flip=flip, phase=phase, duration=duration)
-    run = SimpleNamespace(packed=packed, tissue=plain, states=states, factor=factor)
+    signal, jacobian = simulate_shaped(packed, plain, table, states=states, columns=carried, factor=factor,
+                                       flip=flip, phase=phase, duration=duration)
+    run = SimpleNamespace(packed=packed, tissue=plain, states=states, factor=factor, table=table)
     buffers = [tissue.buffers[_native.TISSUE_INDEX[name]] for name in carried]
     return SequenceSignal.apply(flip, phase, duration, signal, jacobian, run, *buffers)
