@@ -138,6 +138,26 @@ int tsim_profiled_vjp(void* const* packed, int64_t atom_count, int64_t event_cou
                       float* grad_flip, float* grad_phase, float* grad_duration, void* stream);
 
 /*
+ * Forward mode along the events through the semisolid pool and/or
+ * slice-profiled pulses: the signal's directional derivative for per-event
+ * tangents of the flips (rad), phases (rad) and durations (s) -- the event
+ * rows of `_epg_cpu.simulate_jvp` with pool kind 1 / RfMode::PROFILED
+ * (src/torchsim/_epg_cpu.cpp:11017-11148); the tissue rows are the columns of
+ * tsim_semisolid_jacobian / tsim_profiled_jacobian. Buffers as for those
+ * (profile NULL for instantaneous pulses; a semisolid pool where its packed
+ * buffers are set, then the lineshape required).
+ * d_flip, d_phase, d_duration: DEVICE (event_count,) FP32, each may be NULL.
+ * tangent: DEVICE (atom_count, output_count) complex64; the recorded samples
+ *          are written (the caller clears the rest).
+ */
+int tsim_semisolid_event_jvp(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,
+                             int64_t output_count, int features, int64_t tissue_stride, float factor_re,
+                             float factor_im, const float* profile, int64_t locations, int64_t profile_bins,
+                             float profile_step, const float* lineshape, int64_t lineshape_bins, float lineshape_step,
+                             const float* d_flip, const float* d_phase, const float* d_duration, void* tangent,
+                             void* stream);
+
+/*
  * Signal and Jacobian with slice-profiled pulses: every pulse but an
  * inversion turns the states through the Cayley-Klein pair (a, b) its shape
  * performs at the atom's slice position and effective flip flip * b1, read

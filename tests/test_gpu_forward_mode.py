@@ -118,6 +118,23 @@ def test_forward_ad_along_flip_tr_and_phase():
     print(f"dual flips/TRs/phases: {err:.2e}")
     assert err < TOL
 
+    # the same train bound at construction, under torch.func.jvp and torch.func.grad
+    def bound(f, t, p):
+        return MRFSimulator(states=10, flip=f, TR=t, phases=p).simulate(**maps)
+
+    primal = tuple(torch.tensor(v, dtype=torch.float32) for v in (flips, trs, phases))
+    direction = tuple(torch.tensor(v, dtype=torch.float32) for v in (d_flip, d_tr, d_phase))
+    _, through_func = torch.func.jvp(bound, primal, direction)
+    err = _cases.rel_err(through_func.cpu().numpy(), expect.numpy())
+    print(f"torch.func.jvp along the bound train: {err:.2e}")
+    assert err < TOL
+    weight = torch.tensor(np.random.default_rng(24).normal(size=tuple(expect.shape)), dtype=torch.float32).cuda()
+    grads = torch.func.grad(lambda f, t, p: (bound(f, t, p).real * weight).sum(), argnums=(0, 1, 2))(*primal)
+    directional = sum(float((g * d).sum()) for g, d in zip(grads, direction))
+    want = float((expect.real * weight.double().cpu()).sum())
+    print(f"torch.func.grad along the bound train: {abs(directional - want) / abs(want):.2e}")
+    assert abs(directional - want) < 1e-4 * abs(want)
+
 
 def test_func_jvp_on_an_exchanging_diffusing_protocol():
     """An exchanging pool and diffusion (crusher geometry declared): the

@@ -236,20 +236,20 @@ def test_forward_mode_along_a_mixed_tissue_direction():
 
 
 def test_derivatives_the_pool_does_not_carry_are_refused():
-    """Forward mode along the sequence, a Jacobian's gradient to the flips,
-    and the bound pool's own properties carry no derivative through the
-    semisolid kernel: asking raises."""
+    """A Jacobian's gradient to the flips, a Jacobian along the flips' dual
+    tangents, and the bound pool's own properties carry no derivative
+    through the semisolid kernel: asking raises."""
     import torch.autograd.forward_ad as fwAD
     from torchsim_amd.simulators import MRFSimulator
 
     maps = {k: torch.tensor(v).cuda() for k, v in _maps(8, 31).items()}
     flips = torch.tensor(_cases.benchmark_flips(40), dtype=torch.float32, requires_grad=True)
-    with pytest.raises(NotImplementedError, match="forward mode along them"):
+    with pytest.raises(NotImplementedError, match="not differentiated again"):
         MRFSimulator(flip=flips, TR=10.0, states=10).jacobian(("T1",), **maps)
     with fwAD.dual_level():
         dual = fwAD.make_dual(flips.detach(), torch.ones_like(flips))
-        with pytest.raises(NotImplementedError, match="forward mode along them"):
-            MRFSimulator(flip=dual, TR=10.0, states=10).simulate(**maps)
+        with pytest.raises(NotImplementedError, match="not differentiated again"):
+            MRFSimulator(flip=dual, TR=10.0, states=10).jacobian(("T1",), **maps)
     sim = _simulator("mrf")
     bound = dict(maps)
     bound["bound_fraction"] = bound["bound_fraction"].clone().requires_grad_(True)
@@ -556,3 +556,153 @@ def test_loss_backward_reaches_a_profiled_protocol():
         err = np.abs(ours[key].grad.numpy() - theirs[key].grad.numpy()).max() / np.abs(theirs[key].grad.numpy()).max()
         print(f"profiled protocol d/d{key}: {err:.2e}")
         assert err < SEQ_TOL, key
+
+
+# --- forward mode along the events ---
+
+
+def _oracle_event_jvp(events, rows, states, outputs, directions, **kw):
+    """FP64 forward mode of the oracle along per-event flip, phase and
+    duration directions: (signal, tangent)."""
+    def run(flip, phase, duration):
+        given = dict(events)
+        given.update(flip=flip, phase=phase, duration=duration)
+        return P.simulate(given, rows, states=states, outputs=outputs, **kw)
+
+    primal = tuple(torch.as_tensor(np.asarray(events[k]), dtype=torch.float64) for k in ("flip", "phase", "duration"))
+    tangent = tuple(torch.as_tensor(d, dtype=torch.float64) for d in directions)
+    return torch.func.jvp(run, primal, tangent)
+
+
+@pytest.mark.parametrize("case", ["semisolid", "profiled_free", "profiled_semisolid"])
+def test_the_event_tangent_kernel_against_fp64_forward_mode(case):
+    """tsim_semisolid_event_jvp on a phase-cycled MRF train with a B1 map
+    (semisolid: B0, pulses 800 Hz off resonance, saturation scaled 300-fold;
+    profiled: a windowed sinc at 5 slice positions): the signal's tangent
+    along each of the flips, the phases and the durations alone, and along
+    all three together, against FP64 forward mode of the oracle."""
+    import dataclasses as dc
+
+    from torchsim_amd.sequence import _engine, _semisolid
+    from torchsim_amd.sequence._lineshape import lineshape_table
+    from torchsim_amd.sequence._transition import sinc_pulse, transition_table
+
+    from torchsim_amd.simulators import MRFSimulator
+
+    voxels, pulses, states = 12, 60, 10
+    rng = np.random.default_rng(41)
+    packed = MRFSimulator(states=states).packed(flip=torch.tensor(rng.uniform(5.0, 60.0, pulses), dtype=torch.float32),
+                                                 TR=torch.tensor(rng.uniform(8.0, 15.0, pulses), dtype=torch.float32),
+                                                 phases=torch.tensor(rng.uniform(0, 360, pulses), dtype=torch.float32))
+    bound = case != "profiled_free"
+    profiled = case.startswith("profiled")
+    if bound:
+        packed = dc.replace(packed, saturation=packed.saturation * 300.0)
+    if case == "semisolid":
+        packed = dc.replace(packed, rf_frequency_hz=torch.where(packed.kind == 1, 800.0, 0.0).to(torch.float32))
+    tissue = _pool_tissue(voxels, 42)
+    if profiled:
+        tissue.pop("b0")
+    if not bound:
+        for key in ("bound_fraction", "bound_exchange", "t1_bound"):
+            tissue.pop(key)
+    names = {"t1": "t1_ms", "t2": "t2_ms", "m0": "m0", "b1": "b1", "b0": "b0_hz", "bound_fraction": "bound_fraction",
+             "bound_exchange": "bound_exchange_rate", "t1_bound": "t1_bound_ms"}
+    prepared = _engine.prepare_tissue({names[k]: torch.tensor(v, dtype=torch.float32) for k, v in tissue.items()},
+                                      torch.device("cuda", 0), ())
+    table = transition_table(sinc_pulse(2e-3, 4.0, raster_s=1e-5), np.linspace(-1.0, 1.0, 5), bins=32) if profiled else None
+    positions = 1 if table is None else table.points
+    events = oracle.events_of(packed)
+    rows = {k: torch.tensor(np.repeat(v, positions), dtype=torch.float64) for k, v in tissue.items()}
+    ls = lineshape_table()
+    kw = {"lineshape": (ls.values, ls.slopes, ls.step) if bound else None}
+    if table is not None:
+        kw["profile"] = (table.a.numpy(), table.b.numpy(), table.slope_a.numpy(), table.slope_b.numpy(),
+                         table.theta_max)
+    count = len(events["flip"])
+    each = [rng.normal(size=count), rng.normal(size=count), rng.uniform(-0.5, 0.5, count)]
+    zero = np.zeros(count)
+    trials = {"flip": (each[0], zero, zero), "phase": (zero, each[1], zero), "duration": (zero, zero, each[2]),
+              "all": tuple(each)}
+    for label, directions in trials.items():
+        _, want = _oracle_event_jvp(events, rows, states, packed.output_count, directions, **kw)
+        want = (1j * want).reshape(voxels, positions, -1).mean(1).numpy()
+        got = _semisolid.event_tangent(packed, prepared, table, states=states, factor=1j,
+                                       **{f"{k}_tangent": torch.tensor(d, dtype=torch.float32)
+                                          for k, d in zip(("flip", "phase", "duration"), directions)})
+        err = np.abs(got.cpu().numpy() - want).max() / np.abs(want).max()
+        print(f"{case} along {label}: {err:.2e}")
+        assert err < SEQ_TOL, label
+
+
+@pytest.mark.parametrize("name", ["mrf", "profiled"])
+def test_forward_mode_along_the_protocol_through_the_pool(name):
+    """torch.func.jvp and dual tensors along the flips, TRs and phases of an
+    MRF protocol with a semisolid pool (and, profiled, a windowed sinc across
+    the slice), with a T1 direction beside them, against FP64 forward mode of
+    the oracle through the same differentiable packing."""
+    import torch.autograd.forward_ad as fwAD
+    from torchsim_amd.sequence._lineshape import lineshape_table
+    from torchsim_amd.sequence._transition import exact_slice_profile, sinc_pulse, transition_table
+    from torchsim_amd.simulators import MRFSimulator
+
+    atoms, pulses = 10, 40
+    rng = np.random.default_rng(51)
+    tissue = _pool_tissue(atoms, 52)
+    tissue.pop("b0")
+    extra = {}
+    if name == "profiled":
+        extra = {"pulse": sinc_pulse(2e-3, 4.0, raster_s=1e-5), "across_slice": exact_slice_profile(5, bins=32)}
+    values = {"flip": rng.uniform(5.0, 60.0, pulses), "TR": rng.uniform(8.0, 15.0, pulses),
+              "phases": rng.uniform(0.0, 360.0, pulses)}
+    directions = {"flip": rng.normal(size=pulses), "TR": rng.normal(size=pulses),
+                  "phases": rng.normal(size=pulses) * 5.0}
+    d_t1 = rng.uniform(-50.0, 50.0, atoms)
+    public = {"t1": "T1", "t2": "T2", "m0": "M0", "b1": "B1", "bound_fraction": "bound_fraction",
+              "bound_exchange": "bound_exchange", "t1_bound": "bound_T1"}
+    rest = {public[k]: torch.tensor(v, dtype=torch.float32, device="cuda") for k, v in tissue.items() if k != "t1"}
+
+    def simulate(flip, tr, phases, t1):
+        return MRFSimulator(states=10, flip=flip, TR=tr, phases=phases, **extra).simulate(T1=t1, **rest)
+
+    primal = tuple(torch.tensor(values[k], dtype=torch.float32) for k in values) + (
+        torch.tensor(tissue["t1"], dtype=torch.float32, device="cuda"),)
+    tangent = tuple(torch.tensor(directions[k], dtype=torch.float32) for k in values) + (
+        torch.tensor(d_t1, dtype=torch.float32, device="cuda"),)
+
+    ls = lineshape_table()
+    kw = {"lineshape": (ls.values, ls.slopes, ls.step)}
+    positions = 1
+    if name == "profiled":
+        where = extra["across_slice"]
+        table = transition_table(extra["pulse"], where.positions(), bins=where.bins, theta_max=where.theta_max)
+        positions = table.points
+        kw["profile"] = (table.a.numpy(), table.b.numpy(), table.slope_a.numpy(), table.slope_b.numpy(),
+                         table.theta_max)
+
+    def reference(flip, tr, phases, t1):
+        sim64 = MRFSimulator(states=10, flip=flip, TR=tr, phases=phases, **extra)
+        packed = sim64.packed(**sim64.bound)
+        events = oracle.events_of(packed)
+        events.update(duration=packed.duration.double(), flip=packed.flip.double(), phase=packed.phase.double())
+        rows = {k: torch.tensor(np.repeat(v, positions), dtype=torch.float64) for k, v in tissue.items() if k != "t1"}
+        rows["t1"] = t1.repeat_interleave(positions)
+        out = complex(sim64.signal_factor) * P.simulate(events, rows, states=10, outputs=packed.output_count, **kw)
+        return out.reshape(atoms, positions, -1).mean(1)
+
+    with fwAD.dual_level():
+        want = reference(*(fwAD.make_dual(torch.tensor(values[k], dtype=torch.float64),
+                                          torch.tensor(directions[k], dtype=torch.float64)) for k in values),
+                         fwAD.make_dual(torch.tensor(tissue["t1"], dtype=torch.float64),
+                                        torch.tensor(d_t1, dtype=torch.float64)))
+        want = fwAD.unpack_dual(want).tangent.numpy()
+    _, got = torch.func.jvp(simulate, primal, tangent)
+    err = np.abs(got.cpu().numpy() - want).max() / np.abs(want).max()
+    print(f"{name} torch.func.jvp: {err:.2e}")
+    assert err < SEQ_TOL
+    with fwAD.dual_level():
+        dual = simulate(*(fwAD.make_dual(p, t) for p, t in zip(primal, tangent)))
+        tangent_out = fwAD.unpack_dual(dual).tangent
+    err = np.abs(tangent_out.cpu().numpy() - want).max() / np.abs(want).max()
+    print(f"{name} dual tensors: {err:.2e}")
+    assert err < SEQ_TOL

@@ -213,6 +213,11 @@ SIGNATURES["tsim_profiled_vjp"] = (
     [ctypes.POINTER(_P), _I64, _I64, _I64, _I64, _I32, _I64, _F32, _F32, _P, _I64, _I64, _F32, _P, _I64, _F32, _P, _P,
      _P, _P, _P],
 )
+SIGNATURES["tsim_semisolid_event_jvp"] = (
+    _I32,
+    [ctypes.POINTER(_P), _I64, _I64, _I64, _I64, _I32, _I64, _F32, _F32, _P, _I64, _I64, _F32, _P, _I64, _F32, _P, _P,
+     _P, _P, _P],
+)
 SIGNATURES["tsim_profiled_jacobian"] = (
     _I32,
     [ctypes.POINTER(_P), _I64, _I64, _I64, _I64, _I32, _I64, ctypes.POINTER(ctypes.c_int32), _I64, _F32, _F32, _P,

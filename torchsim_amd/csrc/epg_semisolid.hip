@@ -375,3 +375,79 @@ extern "C" int tsim_profiled_vjp(void* const* packed, int64_t atom_count, int64_
                  factor_im, bound, lineshape, lineshape_bins, lineshape_step, profile, locations, profile_bins,
                  profile_step, cotangent, grad_flip, grad_phase, grad_duration, stream);
 }
+
+namespace tsim {
+namespace {
+
+using EventJvpLaunch = void (*)(const SemisolidEventJvpArgs&, dim3, dim3, hipStream_t);
+
+template <int L, bool PROFILED>
+void launch_event_jvp(const SemisolidEventJvpArgs& a, dim3 grid, dim3 block, hipStream_t s) {
+  hipLaunchKernelGGL((semisolid_event_jvp_kernel<L, PROFILED>), grid, block, 0, s, a);
+}
+
+EventJvpLaunch pick_event_jvp(int lanes, bool profiled) {
+  switch (lanes) {
+#define TSIM_EVENT_JVP_CASE(LANES) \
+  case LANES:                      \
+    return profiled ? &launch_event_jvp<LANES, true> : &launch_event_jvp<LANES, false>;
+    TSIM_COMPLEX_LANES(TSIM_EVENT_JVP_CASE)
+#undef TSIM_EVENT_JVP_CASE
+    default:
+      return nullptr;
+  }
+}
+
+}  // namespace
+}  // namespace tsim
+
+extern "C" int tsim_semisolid_event_jvp(void* const* packed, int64_t atom_count, int64_t event_count,
+                                        int64_t state_count, int64_t output_count, int features,
+                                        int64_t tissue_stride, float factor_re, float factor_im, const float* profile,
+                                        int64_t locations, int64_t profile_bins, float profile_step,
+                                        const float* lineshape, int64_t lineshape_bins, float lineshape_step,
+                                        const float* d_flip, const float* d_phase, const float* d_duration,
+                                        void* tangent, void* stream) {
+  const bool bound = packed != nullptr && packed[T_BOUND_FRACTION] != nullptr;
+  SemisolidArgs a;
+  int tm = 0;
+  int st = fill_shaped(packed, atom_count, event_count, state_count, output_count, features, tissue_stride, nullptr,
+                       0, factor_re, factor_im, bound, lineshape, lineshape_bins, lineshape_step, profile, locations,
+                       profile_bins, profile_step, nullptr, nullptr, a, tm);
+  if (st != OK) return st;
+  if (!bound && profile == nullptr) {
+    set_error("the event tangents run with a semisolid pool or a profile table");
+    return BAD_ARGUMENT;
+  }
+  if (atom_count == 0 || output_count == 0) return OK;
+  if (tangent == nullptr) {
+    set_error("the tangent output is required");
+    return BAD_ARGUMENT;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int L = semisolid_lanes(state_count);
+  EventJvpLaunch launch = pick_event_jvp(L, profile != nullptr);
+  if (!launch) {
+    set_error("no kernel for %lld orders", (long long)state_count);
+    return UNSUPPORTED;
+  }
+  Compiled compiled;
+  st = compile_packed(packed, event_count, s, compiled);
+  if (st != OK) return st;
+  a.events = compiled.codes;
+  a.event_count = event_count;
+  SemisolidEventJvpArgs q;
+  q.p = a;
+  q.d_flip = d_flip;
+  q.d_phase = d_phase;
+  q.d_duration = d_duration;
+  q.tangent = (float2*)tangent;
+  const int64_t apw = 64 / L;
+  const int64_t waves = (atom_count + apw - 1) / apw;
+  if (waves > 0x7FFFFFFF) {
+    set_error("too many atoms for one launch: %lld", (long long)atom_count);
+    return BAD_ARGUMENT;
+  }
+  launch(q, dim3((unsigned)waves), dim3(64), s);
+  return hip_status(hipGetLastError(), "semisolid_event_jvp_kernel launch");
+}

@@ -414,6 +414,155 @@ __global__ void __launch_bounds__(64) semisolid_vjp_kernel(const SemisolidVjpArg
   }
 }
 
+// Forward mode along the events: the directional derivative of the signal
+// for per-event tangents of the flips, phases and durations (the event rows
+// of the reference's simulate_jvp, src/torchsim/_epg_cpu.cpp:11017-11148,
+// for pool kind 1 and RfMode::PROFILED). The tangent state t follows the
+// value x through every operator A and takes A's derivative along the event
+// direction applied to x: dA/d(dt) d_dt for a relaxation, (b1 dT/dtheta
+// d_flip + dT/dphi d_phase) for a pulse (and the saturation factor's flip
+// term), -i s d_phase for a record's demodulation.
+struct SemisolidEventJvpArgs {
+  SemisolidArgs p;
+  const float* d_flip;      // (events,) or null
+  const float* d_phase;     // (events,) or null
+  const float* d_duration;  // (events,) or null
+  float2* tangent;          // (atoms, outputs)
+};
+
+template <int L, bool PROFILED>
+__global__ void __launch_bounds__(64) semisolid_event_jvp_kernel(const SemisolidEventJvpArgs q) {
+  using namespace cx;
+  using mt::Order;
+  const SemisolidArgs& p = q.p;
+  constexpr int APW = 64 / L;
+  const int lane = (int)(threadIdx.x & 63);
+  const int wave_group = lane / L;
+  const int sub = lane - wave_group * L;
+  const int64_t group = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * APW + wave_group;
+  const bool live = wave_group < APW && group < p.atoms;
+  const int64_t atom = live ? group : p.atoms - 1;
+  const bool first = sub == 0;
+  const bool beyond = sub >= p.states;
+  const int64_t ia = atom * p.stride_opt;
+
+  mtv::Atom at;
+  const float t1 = p.t1[atom], t2 = p.t2[atom];
+  at.r1 = 1000.0f / t1;
+  at.r2 = 1000.0f / t2;
+  const float m0 = p.density ? p.m0[ia] : 1.0f;
+  const float b1 = p.transmit ? p.b1[ia] : 1.0f;
+  const float b1_phase = p.off_axis && p.b1_phase ? p.b1_phase[ia] : 0.0f;
+  at.b0 = p.off_axis && p.b0 ? p.b0[ia] : 0.0f;
+  const float eta = p.inverting ? p.inv_eff[ia] : 1.0f;
+  at.drate = p.diffusing ? p.diffusion[ia] : 0.0f;
+  at.f = p.bound_fraction ? p.bound_fraction[ia] : 0.0f;
+  const float kex = p.bound_exchange ? p.bound_exchange[ia] : 0.0f;
+  at.r1b = p.t1_bound ? 1000.0f / p.t1_bound[ia] : at.r1;
+  at.free = 1.0f - at.f;
+  at.kab = kex * at.f;
+  at.kba = kex * at.free;
+  at.order = (float)sub;
+  const float2 factor = mk(p.factor_re, p.factor_im);
+
+  Order x, t;
+  x.fp = x.fm = x.z = x.zb = mk(0.0f, 0.0f);
+  t = x;
+  if (first) {
+    x.z = mk(at.free, 0.0f);
+    x.zb = mk(at.f, 0.0f);
+  }
+  mtv::Relax R, R0;
+  mtv::form_relax(at, 0.0f, R0);
+  float last_dt = -1.0f;
+  for (int64_t e = 0; e < p.event_count; ++e) {
+    const EventCode ev = load_event(p.events, e);
+    const uint32_t act = ev.ctrl & 0xFFu;
+    const int32_t kind = (int32_t)(ev.ctrl >> CTRL_KIND_SHIFT);
+    const float d_dt = q.d_duration ? q.d_duration[e] : 0.0f;
+    // relaxation: t <- R t + dR/d(dt) x d_dt (an untimed event's derivative too)
+    {
+      const mtv::Relax* use = &R0;
+      if (ev.dt != 0.0f) {
+        if (ev.dt != last_dt) {
+          last_dt = ev.dt;
+          mtv::form_relax(at, ev.dt, R);
+        }
+        use = &R;
+      }
+      Order dx;
+      dx.fp = mul(use->dtp, x.fp);
+      dx.fm = mul(cj(use->dtp), x.fm);
+      dx.z = add(scl(x.z, use->dY[0]), scl(x.zb, use->dY[1]));
+      dx.zb = add(scl(x.z, use->dY[2]), scl(x.zb, use->dY[3]));
+      if (first) {
+        dx.z.x += use->drec[0];
+        dx.zb.x += use->drec[1];
+      }
+      if (ev.dt != 0.0f) {
+        mtv::relax(t, R, false);  // the recovery is constant: no tangent term of its own
+        mtv::relax(x, R, first);
+      }
+      t.fp = add(t.fp, scl(dx.fp, d_dt));
+      t.fm = add(t.fm, scl(dx.fm, d_dt));
+      t.z = add(t.z, scl(dx.z, d_dt));
+      t.zb = add(t.zb, scl(dx.zb, d_dt));
+    }
+    if (act & PRE_SHIFT) {
+      shift_complex<L>(x.fp, x.fm, sub, lane, beyond);
+      shift_complex<L>(t.fp, t.fm, sub, lane, beyond);
+    }
+    if (kind == KIND_RF) {
+      if (act & INVERSION) {
+        x.z = scl(x.z, -eta);
+        t.z = scl(t.z, -eta);
+      } else {
+        const float theta = ev.flip * b1;
+        const float d_theta = (q.d_flip ? q.d_flip[e] : 0.0f) * b1;
+        const float d_phi = q.d_phase ? q.d_phase[e] : 0.0f;
+        const float deposit = p.bound_fraction ? p.saturation[e] : 0.0f;
+        if (deposit != 0.0f) {
+          const float g = lineshape_at(p, p.rf_frequency[e] - at.b0);
+          const float w = expf(deposit * theta * theta * g);
+          t.zb = add(scl(t.zb, w), scl(x.zb, w * deposit * 2.0f * theta * g * d_theta));
+          x.zb = scl(x.zb, w);
+        }
+        mtv::Pulse P;
+        mtv::form_pulse<PROFILED>(p, ev, atom, theta, ev.phase + b1_phase, p.transmit, P);
+        const float2 fp = x.fp, fm = x.fm, z = x.z;
+        const float2 ap = add(add(mul(P.u00, fp), mul(P.u01, fm)), mul(P.u02, z));
+        const float2 am = add(add(mul(P.u10, fp), mul(P.u11, fm)), mul(P.u12, z));
+        const float2 az = add(add(mul(P.u20, fp), mul(P.u21, fm)), scl(z, P.u22));
+        const float2 pp = add(mul(mk(0.0f, 2.0f), mul(P.t01, fm)), mul(mk(0.0f, 1.0f), mul(P.t02, z)));
+        const float2 pm = add(mul(mk(0.0f, -2.0f), mul(P.t10, fp)), mul(mk(0.0f, -1.0f), mul(P.t12, z)));
+        const float2 pz = add(mul(mk(0.0f, -1.0f), mul(P.t20, fp)), mul(mk(0.0f, 1.0f), mul(P.t21, fm)));
+        mtv::rotate(t, P);
+        t.fp = add(t.fp, add(scl(ap, d_theta), scl(pp, d_phi)));
+        t.fm = add(t.fm, add(scl(am, d_theta), scl(pm, d_phi)));
+        t.z = add(t.z, add(scl(az, d_theta), scl(pz, d_phi)));
+        mtv::rotate(x, P);
+      }
+    } else if (kind == KIND_ADC && (act & RECORD) && ev.out >= 0 && ev.out < p.outputs && first && live) {
+      float sp, cp;
+      sincosf(-ev.phase, &sp, &cp);
+      const float2 c = scl(mul(factor, mk(cp, sp)), m0);
+      const float d_phi = q.d_phase ? q.d_phase[e] : 0.0f;
+      const float2 s = mul(c, x.fp);
+      q.tangent[atom * p.outputs + ev.out] = add(mul(c, t.fp), scl(mk(s.y, -s.x), d_phi));
+    }
+    if (act & POST_SHIFT) {
+      shift_complex<L>(x.fp, x.fm, sub, lane, beyond);
+      shift_complex<L>(t.fp, t.fm, sub, lane, beyond);
+    }
+    if (act & SPOIL_AFTER) {
+      x.fp = x.fm = t.fp = t.fm = mk(0.0f, 0.0f);
+    } else if (act & SHIFT_AFTER) {
+      shift_complex<L>(x.fp, x.fm, sub, lane, beyond);
+      shift_complex<L>(t.fp, t.fm, sub, lane, beyond);
+    }
+  }
+}
+
 // Per-event partials summed over the waves in a fixed order, added to the
 // gradients with `accumulate` (a launch after the first).
 __global__ void semisolid_sum_partials(const float* __restrict__ partial, int64_t waves, int64_t events,

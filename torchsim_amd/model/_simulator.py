@@ -442,19 +442,16 @@ class Simulator:
         if profile is not None or tissue.semisolid:
             from ..sequence import _semisolid
 
-            sequence_grad = any(v.requires_grad for v in (packed.flip, packed.phase, packed.duration))
-            sequence_dual = any(forward_ad.unpack_dual(v).tangent is not None
-                                for v in (packed.flip, packed.phase, packed.duration))
-            if sequence_grad and not sequence_dual and not differentiated and not tangent_fields:
-                # reverse mode to the flips, phases and durations: the adjoint of this kernel
-                return _semisolid.sequence_signal(packed, tissue, profile, states=states,
-                                                  carried=tuple(dict.fromkeys(reversed_fields)), factor=factor), None
             if any(sequence_live):
-                raise NotImplementedError(
-                    "through a semisolid pool or profiled pulses the sequence's flips, phases and durations carry "
-                    "reverse-mode gradients of the signal; forward mode along them and a Jacobian's gradients reach "
-                    f"only the tissue maps ({', '.join(_semisolid.CARRIED)})"
-                )
+                if differentiated:
+                    raise NotImplementedError(
+                        "a Jacobian through a semisolid pool or a profiled pulse is not differentiated again"
+                    )
+                # derivatives to the flips, phases and durations: the adjoint (reverse mode) and the
+                # event tangents (forward mode) of this kernel
+                return _semisolid.sequence_signal(
+                    packed, tissue, profile, states=states,
+                    carried=tuple(dict.fromkeys(reversed_fields + tangent_fields)), factor=factor), None
             carried = tuple(dict.fromkeys(reversed_fields + tangent_fields))
             if carried:
                 if differentiated:
@@ -540,9 +537,14 @@ class Simulator:
     def simulate(self, **values: Any) -> Any:
         """Return the recorded signal, shaped like the caller's property maps."""
         if _transformed():
-            keys = tuple(name for name, value in values.items() if isinstance(value, torch.Tensor))
-            others = {name: value for name, value in values.items() if name not in keys}
-            return _TransformBridge.apply(self, keys, others, *(values[name] for name in keys))
+            # tensors bound at construction (a protocol's flips, TRs, phases) are
+            # the Function's inputs too, so that a transform's tangents and
+            # cotangents along them reach the kernels
+            given = {**{name: value for name, value in self.bound.items() if isinstance(value, torch.Tensor)},
+                     **values}
+            keys = tuple(name for name, value in given.items() if isinstance(value, torch.Tensor))
+            others = {name: value for name, value in given.items() if name not in keys}
+            return _TransformBridge.apply(self, keys, others, *(given[name] for name in keys))
         backend = _backend(values)
         held, sequence = self._split({**self.bound, **values})
         batch = self._batch(held)
@@ -778,15 +780,12 @@ class _BridgeTangent(torch.autograd.Function):
         if profile is not None or tissue.semisolid:
             from ..sequence import _semisolid
 
-            if any(value is not None for value in moved[:3]):
-                raise NotImplementedError(
-                    "through a semisolid pool or a profiled pulse derivatives reach the tissue maps "
-                    f"({', '.join(_semisolid.CARRIED)}); the sequence's flips, phases and durations carry none"
-                )
             # buffers laid out together all carry a tangent slot: keep the live ones
             live = {name: value for name, value in tissue_tangents.items() if bool((value != 0).any())}
             signal, tangent_states = _semisolid.tangent_of(packed, plain, profile, states=states,
-                                                           tissue_tangents=live, factor=factor)
+                                                           tissue_tangents=live, factor=factor,
+                                                           duration_tangent=moved[0], flip_tangent=moved[1],
+                                                           phase_tangent=moved[2])
             both_out = tangent_states if sample_factor is None else (signal, tangent_states)
         else:
             both_out = _complex.jvp_complex(

@@ -259,6 +259,55 @@ def carried_signal(
     return ColumnSignal.apply(signal, jacobian, carried, *buffers)
 
 
+def event_tangent(
+    packed: PackedEvents,
+    tissue: PreparedTissue,
+    table: TransitionTable | None,
+    *,
+    states: int,
+    factor: complex,
+    flip_tangent: torch.Tensor | None = None,
+    phase_tangent: torch.Tensor | None = None,
+    duration_tangent: torch.Tensor | None = None,
+    flip: torch.Tensor | None = None,
+    phase: torch.Tensor | None = None,
+    duration: torch.Tensor | None = None,
+) -> torch.Tensor:
+    """The signal's directional derivative (voxels, outputs) along per-event
+    tangents of the flips, phases and durations, through the semisolid pool
+    and/or slice-profiled pulses (``table``), by ``tsim_semisolid_event_jvp``."""
+    device = tissue.device
+    _native.require_device(device)
+    if tissue.exchanging:
+        raise NotImplementedError("a semisolid pool beside an exchanging pool (three pools) is not built")
+    lib = _native.load()
+    lineshape = absorption(packed, tissue) if tissue.semisolid else None
+    knots = _resident(lineshape, device) if lineshape is not None else None
+    events = _complex._events(packed, device, flip=flip, phase=phase, duration=duration)
+
+    def on_device(value):
+        return None if value is None else value.detach().to(device=device, dtype=torch.float32).contiguous()
+
+    directions = [on_device(v) for v in (flip_tangent, phase_tangent, duration_tangent)]
+    locations = 1 if table is None else table.points
+    spread = tissue if table is None else _across(tissue, locations)
+    out = torch.zeros((spread.atoms, packed.output_count), dtype=torch.complex64, device=device)
+    profile = None if table is None else table.packed(device)
+    ptrs = _native.pointers([*(None if b is None else b.detach() for b in spread.buffers), *events])
+    with torch.cuda.device(device):
+        status = lib.tsim_semisolid_event_jvp(
+            ptrs, spread.atoms, packed.event_count, states, packed.output_count, spread.features, spread.stride,
+            float(complex(factor).real), float(complex(factor).imag),
+            None if profile is None else profile.data_ptr(), locations, 0 if table is None else table.bins,
+            1.0 if table is None else float(table.step), None if knots is None else knots.data_ptr(),
+            0 if lineshape is None else lineshape.bins, 1.0 if lineshape is None else float(lineshape.step),
+            *(None if d is None else d.data_ptr() for d in directions), out.data_ptr(), _native.stream_of(device))
+    _native.check(status)
+    if locations > 1:
+        out = out.reshape(tissue.atoms, locations, -1).mean(1)
+    return out
+
+
 def tangent_of(
     packed: PackedEvents,
     tissue: PreparedTissue,
@@ -267,9 +316,13 @@ def tangent_of(
     states: int,
     tissue_tangents: dict,
     factor: complex = 1.0,
+    flip_tangent: torch.Tensor | None = None,
+    phase_tangent: torch.Tensor | None = None,
+    duration_tangent: torch.Tensor | None = None,
 ) -> tuple[torch.Tensor, torch.Tensor]:
     """Signal and its directional derivative along per-atom tissue tangents
-    (names among :data:`CARRIED`): sum_k J_k t_k."""
+    (names among :data:`CARRIED`: sum_k J_k t_k) and per-event tangents of the
+    flips, phases and durations."""
     names = tuple(tissue_tangents)
     refuse_uncarried(names)
     plain = dataclasses.replace(tissue, buffers=[None if b is None else b.detach() for b in tissue.buffers])
@@ -277,6 +330,10 @@ def tangent_of(
     tangent = torch.zeros_like(signal)
     for k, name in enumerate(names):
         tangent = tangent + jacobian[:, k] * tissue_tangents[name].detach().reshape(-1, 1).to(torch.float32)
+    if any(d is not None for d in (flip_tangent, phase_tangent, duration_tangent)):
+        tangent = tangent + event_tangent(packed, plain, table, states=states, factor=factor,
+                                          flip_tangent=flip_tangent, phase_tangent=phase_tangent,
+                                          duration_tangent=duration_tangent)
     return signal, tangent
 
 
@@ -350,8 +407,21 @@ class SequenceSignal(torch.autograd.Function):
     def setup_context(ctx, inputs, output):
         flip, phase, duration, _, jacobian, run, *buffers = inputs
         ctx.save_for_backward(flip, phase, duration, *([] if jacobian is None else [jacobian]))
+        ctx.save_for_forward(flip, phase, duration, *([] if jacobian is None else [jacobian]))
         ctx.run = run
         ctx.shapes = [None if b is None else b.shape for b in buffers]
+
+    @staticmethod
+    def jvp(ctx, d_flip, d_phase, d_duration, _d_signal, _d_jacobian, _run, *d_buffers):
+        run = ctx.run
+        flip, phase, duration, *held = ctx.saved_tensors
+        tangent = event_tangent(run.packed, run.tissue, run.table, states=run.states, factor=run.factor,
+                                flip_tangent=d_flip, phase_tangent=d_phase, duration_tangent=d_duration,
+                                flip=flip, phase=phase, duration=duration)
+        for k, d in enumerate(d_buffers):
+            if d is not None:
+                tangent = tangent + held[0][:, k] * d.reshape(-1, 1).to(torch.float32)
+        return tangent
 
     @staticmethod
     def backward(ctx, grad):
