@@ -492,6 +492,90 @@ def complex_workload(steps: int = 5, warmup: int = 2, atoms: int = 65536, pulses
                        "columns": ["T1", "T2"]}}
 
 
+def semisolid_workload(steps: int = 5, warmup: int = 2) -> dict:
+    """The semisolid (magnetization-transfer) pool and slice-profiled pulses
+    on their own kernel, timed per call: an MT MRF dictionary (65,536 atoms x
+    880 pulses, 10 states, signal + d/d(T1, T2, B1)); the same train with a
+    windowed sinc worked out at 5 slice positions (8,192 voxels, signal +
+    d/d(T1, T2)); and, at a design shape (1,024 atoms x 880 pulses, MT),
+    loss.backward() to the flips, TRs and phases (adjoint) and one forward-mode
+    tangent along them (dual tensors)."""
+    import torch
+    import torch.autograd.forward_ad as fwAD
+
+    from torchsim_amd.sequence._transition import exact_slice_profile, sinc_pulse
+    from torchsim_amd.simulators import MRFSimulator
+
+    g = np.random.default_rng(0)
+    device = torch.device("cuda", 0)
+    train = torch.tensor(flips(), dtype=torch.float32)
+    pulses = train.numel()
+
+    def pool(atoms):
+        return {"T1": torch.tensor(g.uniform(300, 2500, atoms), dtype=torch.float32, device=device),
+                "T2": torch.tensor(g.uniform(30, 250, atoms), dtype=torch.float32, device=device),
+                "B1": torch.tensor(g.uniform(0.8, 1.2, atoms), dtype=torch.float32, device=device),
+                "bound_fraction": torch.tensor(g.uniform(0.05, 0.2, atoms), dtype=torch.float32, device=device),
+                "bound_exchange": torch.tensor(g.uniform(5, 40, atoms), dtype=torch.float32, device=device),
+                "bound_T1": torch.tensor(g.uniform(300, 1200, atoms), dtype=torch.float32, device=device)}
+
+    def timed(call):
+        for _ in range(warmup):
+            call()
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        wall0 = time.perf_counter()
+        start.record()
+        for _ in range(steps):
+            call()
+        stop.record()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - wall0) / steps, start.elapsed_time(stop) / steps
+
+    atoms = 65536
+    dictionary_maps = pool(atoms)
+    mt = MRFSimulator(flip=train, TR=10.0, states=10)
+    wall, gpu = timed(lambda: mt.jacobian(("T1", "T2", "B1"), **dictionary_maps))
+    out = {"mt_dictionary": {"atoms": atoms, "pulses": pulses, "states": 10, "columns": ["T1", "T2", "B1"],
+                             "ms_per_call": wall * 1e3, "gpu_ms": gpu, "atom_pulses_per_s": atoms * pulses / wall}}
+
+    voxels, positions = 8192, 5
+    plain = {k: dictionary_maps[k][:voxels] for k in ("T1", "T2")}
+    profiled = MRFSimulator(flip=train, TR=10.0, states=10, pulse=sinc_pulse(2e-3, 4.0, raster_s=1e-5),
+                            across_slice=exact_slice_profile(positions, bins=64))
+    wall, gpu = timed(lambda: profiled.jacobian(("T1", "T2"), **plain))
+    out["profiled_dictionary"] = {"voxels": voxels, "positions": positions, "pulses": pulses, "states": 10,
+                                  "columns": ["T1", "T2"], "ms_per_call": wall * 1e3, "gpu_ms": gpu,
+                                  "atom_pulses_per_s": voxels * positions * pulses / wall}
+
+    design = 1024
+    design_maps = pool(design)
+    tr = torch.full((pulses,), 10.0)
+    phases = torch.tensor(g.uniform(0, 360, pulses), dtype=torch.float32)
+    weight = torch.tensor(g.normal(size=(design, pulses)), dtype=torch.float32, device=device)
+
+    def gradient():
+        leaves = [v.clone().requires_grad_(True) for v in (train, tr, phases)]
+        sig = MRFSimulator(flip=leaves[0], TR=leaves[1], phases=leaves[2], states=10).simulate(**design_maps)
+        (sig.real * weight).sum().backward()
+        return leaves[0].grad
+
+    directions = [torch.tensor(g.normal(size=pulses), dtype=torch.float32) for _ in range(3)]
+
+    def tangent():
+        with fwAD.dual_level():
+            duals = [fwAD.make_dual(v, d) for v, d in zip((train, tr, phases), directions)]
+            sig = MRFSimulator(flip=duals[0], TR=duals[1], phases=duals[2], states=10).simulate(**design_maps)
+            return fwAD.unpack_dual(sig).tangent
+
+    for name, call in (("mt_design_gradient", gradient), ("mt_design_tangent", tangent)):
+        wall, gpu = timed(call)
+        out[name] = {"atoms": design, "pulses": pulses, "states": 10, "ms_per_call": wall * 1e3, "gpu_ms": gpu}
+    return {"metric": "semisolid and profiled kernels, ms per call", "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "MT MRF dictionary, profiled MRF dictionary, MT design gradient and tangent"},
+            **out}
+
+
 def subspace_workload(steps: int = 5, warmup: int = 2, rank: int = 8) -> dict:
     """The config-1 dictionary (1,274,184 atoms x 880 pulses, S=20, signal +
     d/d(T1,T2,B1)) projected onto a rank-8 temporal basis inside the kernel,
@@ -684,7 +768,8 @@ def main() -> int:
                         help="state tile index, -1 for the library's cost model, or 'auto' (measured choice)")
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-torch-baseline", action="store_true")
-    parser.add_argument("--workload", choices=("dictionary", "design", "fit", "serve", "match", "complex", "subspace", "config0"), default="dictionary")
+    parser.add_argument("--workload", choices=("dictionary", "design", "fit", "serve", "match", "complex", "subspace", "semisolid",
+                                                 "config0"), default="dictionary")
     parser.add_argument("--serve-seconds", type=float, default=60.0)
     parser.add_argument("--no-serving", action="store_true",
                         help="skip the short serving run the dictionary workload attaches (p50/p99 latency)")
@@ -733,6 +818,9 @@ def main() -> int:
         return 0
     if args.workload == "subspace":
         print(json.dumps(subspace_workload(steps=args.steps, warmup=args.warmup)), flush=True)
+        return 0
+    if args.workload == "semisolid":
+        print(json.dumps(semisolid_workload(steps=args.steps, warmup=args.warmup)), flush=True)
         return 0
 
     device = torch.device("cuda", 0)

@@ -90,6 +90,47 @@ def test_tiled_packing_of_plain_protocols_matches_generic(pulses, tr, ti, phases
     assert fast.output_count == generic.output_count == pulses
 
 
+@pytest.mark.parametrize("pulses, ti, shaped", [(1, 0.0, False), (5, 0.0, False), (880, 0.0, False), (64, 20.0, False),
+                                               (40, 0.0, True)])
+def test_per_pulse_phases_pack_from_the_template(pulses, ti, shaped):
+    """A phase-cycled protocol (per-pulse float32 phases, plain or carrying
+    a graph, with a shaped pulse too) packs from the template: every buffer
+    equals the event-by-event packing bit for bit, and the phases' and
+    flips' gradients are the generic graph's."""
+    from torchsim_amd.sequence._transition import sinc_pulse
+
+    rng = np.random.default_rng(pulses + 1)
+    flips, trs = rng.uniform(2.0, 70.0, pulses), rng.uniform(8.0, 15.0, pulses)
+    phases = rng.uniform(-360.0, 360.0, pulses)
+    seq = MRFSimulator(states=10)
+    extra = {"pulse": sinc_pulse(2e-3, 4.0, raster_s=1e-5)} if shaped else {}
+    plain = {"flip": torch.tensor(flips, dtype=torch.float32), "TR": torch.tensor(trs, dtype=torch.float32),
+             "phases": torch.tensor(phases, dtype=torch.float32), "TI": ti, **extra}
+    fast = seq._packed(**plain)
+    generic = Simulator._packed(seq, **plain)
+    for field in FIELDS + ("shim_index", "saturation", "rf_frequency_hz"):
+        assert torch.equal(getattr(fast, field), getattr(generic, field)), field
+    assert seq._pulse_block({k: v for k, v in plain.items() if k not in ("flip", "TR")}) is not None
+
+    def leaves():
+        return [torch.tensor(v, dtype=torch.float32, requires_grad=True) for v in (flips, trs, phases)]
+
+    a, b = leaves(), leaves()
+    fast = seq._packed(flip=a[0], TR=a[1], phases=a[2], TI=ti, **extra)
+    generic = Simulator._packed(seq, flip=b[0], TR=b[1], phases=b[2], TI=ti, **extra)
+    for field in FIELDS:
+        assert torch.equal(getattr(fast, field).detach(), getattr(generic, field).detach()), field
+    w, v, u = (torch.randn(fast.duration.numel()) for _ in range(3))
+    ((fast.duration * w).sum() + (fast.flip * v).sum() + (fast.phase * u).sum()).backward()
+    ((generic.duration * w).sum() + (generic.flip * v).sum() + (generic.phase * u).sum()).backward()
+    assert torch.allclose(a[0].grad, b[0].grad, rtol=1e-6, atol=0.0)
+    assert torch.allclose(a[2].grad, b[2].grad, rtol=1e-6, atol=0.0)
+    tr_generic = torch.zeros(pulses) if b[1].grad is None else b[1].grad  # one pulse plays no TR
+    assert float((a[1].grad - tr_generic).abs().max()) <= 1e-5 * float(tr_generic.abs().max())
+    # a pulse's phase reaches its excitation and its sample: d/dphase = (u_rf + u_adc) pi/180
+    assert float(a[2].grad.abs().max()) > 0.0
+
+
 def test_tiled_packing_is_fast():
     import time
 

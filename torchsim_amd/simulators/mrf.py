@@ -74,16 +74,20 @@ class MRFSimulator(Simulator):
         computes, evaluated with the same float32 operations in the same
         order. When the flips and/or TRs carry a graph (sequence design) the
         intervals also carry a term that is zero in value and has
-        d(interval)/d(TR) = 1e-3 s/ms. Anything else (repetitions, a TI or
-        phases that are tensors or per pulse, non-float32 flips) takes the
-        generic packing.
+        d(interval)/d(TR) = 1e-3 s/ms. Per-pulse float32 phases (a
+        phase-cycled protocol, with or without a graph) are added onto the
+        template's phases at the events the template shows them reach, with
+        the sign it shows. Anything else (repetitions, a TI that is a tensor,
+        phases of another dtype, non-float32 flips) takes the generic packing.
         """
         flip, tr = sequence.get("flip"), sequence.get("TR")
-        other = [k for k, v in sequence.items() if k not in ("flip", "TR") and carries_derivative(v)]
+        other = [k for k, v in sequence.items() if k not in ("flip", "TR", "phases") and carries_derivative(v)]
         ti = sequence.get("TI", 0.0)
         phases = sequence.get("phases", 0.0)
+        cycled = isinstance(phases, torch.Tensor)
         if (other or flip is None or tr is None or not isinstance(ti, (int, float))
-                or not isinstance(phases, (int, float))
+                or not (isinstance(phases, (int, float)) or (cycled and phases.dtype == torch.float32
+                                                              and phases.ndim <= 1))
                 or int(sequence.get("repetitions", self.repetitions) or 1) > 1):
             return super()._packed(**sequence)
         angles = torch.deg2rad(torch.atleast_1d(torch.as_tensor(flip)))
@@ -94,7 +98,7 @@ class MRFSimulator(Simulator):
         block = self._pulse_block(sequence)
         if block is None:
             return super()._packed(**sequence)
-        head, body, turn_at, record_at = block
+        head, body, turn_at, record_at, phase_sign = block
 
         def tiled(name: str) -> torch.Tensor:
             return torch.cat([getattr(head, name), getattr(body, name).repeat(n)])
@@ -123,7 +127,13 @@ class MRFSimulator(Simulator):
                 (carried,), (spacing_s[:-1] - spacing_s[:-1].detach()).to(torch.float32))
             duration = duration + live
             reached[carried.numpy()] = intervals[carried.numpy()] == 0
-        packed = PackedEvents(duration=duration, kind=tiled("kind"), flip=flips, phase=tiled("phase"),
+        phase = tiled("phase")
+        if cycled:
+            # the turns as the layout computes them, onto the events they reach
+            turns = torch.deg2rad(_matched(phases, angles))
+            phase = phase + torch.cat([torch.zeros(1, dtype=phase.dtype),
+                                       phase_sign.repeat(n) * turns.repeat_interleave(4).to(phase.dtype)])
+        packed = PackedEvents(duration=duration, kind=tiled("kind"), flip=flips, phase=phase,
                               action=tiled("action"), output_index=output_index, shim_index=tiled("shim_index"),
                               saturation=tiled("saturation"), rf_frequency_hz=tiled("rf_frequency_hz"),
                               output_count=n)
@@ -135,15 +145,31 @@ class MRFSimulator(Simulator):
         place in them) from the generic packing of the protocol with two
         placeholder pulses, or None where that packing is not one inversion
         then identical four-event pulses."""
-        key = _protocol_key(type(self), {k: v for k, v in sequence.items() if k not in ("flip", "TR")})
+        cycled = isinstance(sequence.get("phases"), torch.Tensor)
+        shared = {k: v for k, v in sequence.items() if k not in ("flip", "TR") and not (cycled and k == "phases")}
+        key = _protocol_key(type(self), {**shared, "per_pulse_phases": cycled})
         if key is None:
             return None
         with _TEMPLATE_LOCK:
             if key in _TEMPLATES:
                 return _TEMPLATES[key]
-        placeholder = {**sequence, "flip": torch.ones(2, dtype=torch.float32),
+        placeholder = {**shared, "flip": torch.ones(2, dtype=torch.float32),
                        "TR": torch.ones(2, dtype=torch.float32)}
+        if cycled:
+            placeholder["phases"] = torch.zeros(2, dtype=torch.float32)
         t = Simulator._packed(self, **placeholder)
+        # where a pulse's phase lands in its four events, and with which sign:
+        # the difference a quarter turn makes (none without per-pulse phases)
+        phase_sign = torch.zeros(4, dtype=t.phase.dtype)
+        if cycled:
+            turned = Simulator._packed(self, **{**placeholder, "phases": torch.full((2,), 90.0)})
+            quarter = torch.deg2rad(torch.full((), 90.0))
+            phase_sign = ((turned.phase[1:5] - t.phase[1:5]) / quarter).detach()
+            if not bool(torch.isin(phase_sign, torch.tensor([-1.0, 0.0, 1.0])).all()) or not torch.equal(
+                    turned.phase[5:9] - t.phase[5:9], turned.phase[1:5] - t.phase[1:5]):
+                with _TEMPLATE_LOCK:
+                    _TEMPLATES[key] = None
+                return None
         block = None
         if t.kind.numel() == 9 and t.output_count == 2:
             fields = ("kind", "action", "shim_index", "saturation", "rf_frequency_hz", "phase", "flip")
@@ -155,7 +181,7 @@ class MRFSimulator(Simulator):
                     and int(t.kind[0]) == 1 and bool((t.action[0] & INVERSION) != 0)):
                 head = PackedEvents(*(b[:1].detach() for b in t.buffers()), output_count=0)
                 body = PackedEvents(*(b[1:5].detach() for b in t.buffers()), output_count=0)
-                block = (head, body, int(turning), int(recorded))
+                block = (head, body, int(turning), int(recorded), phase_sign)
         with _TEMPLATE_LOCK:
             _TEMPLATES[key] = block
         return block
