@@ -115,7 +115,7 @@ int tsim_semisolid_jacobian(void* const* packed, int64_t atom_count, int64_t eve
  * Jacobian columns. Buffers and lineshape as for tsim_semisolid_jacobian.
  * cotangent: DEVICE (atom_count, output_count) complex64.
  * grad_flip, grad_phase, grad_duration: DEVICE (event_count,) FP32,
- *          overwritten. Entry states are kept for at most 1 GiB of lanes at a
+ *          overwritten. Entry states are kept for at most 4 GiB of lanes at a
  *          time (stream-ordered scratch); more atoms take several launches.
  */
 int tsim_semisolid_vjp(void* const* packed, int64_t atom_count, int64_t event_count, int64_t state_count,

@@ -324,16 +324,23 @@ __global__ void __launch_bounds__(64) semisolid_vjp_kernel(const SemisolidVjpArg
   Order l;
   l.fp = l.fm = l.z = l.zb = mk(0.0f, 0.0f);
   last_dt = -1.0f;
+  // each event's entry state is read one event ahead, so that the read's
+  // latency overlaps the work on the event before it
+  auto entry_state = [&](int64_t e, Order& o) {
+    const float* slot = traj + e * 8 * T;
+    o.fp = mk(slot[0], slot[T]);
+    o.fm = mk(slot[2 * T], slot[3 * T]);
+    o.z = mk(slot[4 * T], slot[5 * T]);
+    o.zb = mk(slot[6 * T], slot[7 * T]);
+  };
+  Order ahead;
+  entry_state(E - 1, ahead);
   for (int64_t e = E - 1; e >= 0; --e) {
     const EventCode ev = load_event(p.events, e);
     const uint32_t act = ev.ctrl & 0xFFu;
     const int32_t kind = (int32_t)(ev.ctrl >> CTRL_KIND_SHIFT);
-    const float* slot = traj + e * 8 * T;
-    Order x0;
-    x0.fp = mk(slot[0], slot[T]);
-    x0.fm = mk(slot[2 * T], slot[3 * T]);
-    x0.z = mk(slot[4 * T], slot[5 * T]);
-    x0.zb = mk(slot[6 * T], slot[7 * T]);
+    const Order x0 = ahead;
+    if (e > 0) entry_state(e - 1, ahead);
     // the state the event's action sees
     Order xa = x0;
     if (ev.dt != 0.0f) {
