@@ -2,7 +2,10 @@
 signal and Jacobian columns (T1, T2, M0, B1) against the FP64 oracle's
 semisolid pool (pinned on the CPU against the reference's own kernel), on an
 MRF train with off-resonance and a B1 map and on a refocused train, through
-the public simulators and their property names."""
+the public simulators and their property names; slice-profiled pulses; and
+the derivatives along the sequence through both (the adjoint per event,
+the event tangents, loss.backward(), dual tensors and torch.func.jvp)
+against FP64 autograd and forward mode of the oracle."""
 
 from __future__ import annotations
 
